@@ -1,0 +1,161 @@
+"""ctypes binding of the HIP library (include/gpuequiv.h).
+
+The library is built in-tree (``gpuequiv_amd/libgpuequiv_hip.so``) by
+``gpuequiv_amd/build.py``.  There is no fallback: when the library is missing
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+LIB_NAME = "libgpuequiv_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+GEQ_OK = 0
+_KINDS = {1: "NoAdapter", 2: "NoDevice", 3: "BufferMap", 4: "InvalidArg", 5: "OutOfMemory", 6: "Unsupported", 7: "Internal"}
+
+
+class Error(RuntimeError):
+    """gpuequiv::Error (src/error.rs:3-40) plus the C ABI's extra codes."""
+
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.kind = _KINDS.get(code, "Internal")
+        super().__init__(f"{self.kind}: {message}")
+
+
+class geq_conf(C.Structure):
+    _fields_ = [("elements", C.c_uint32), ("max", C.c_uint32)]
+
+
+class geq_graph(C.Structure):
+    _fields_ = [
+        ("conf", geq_conf),
+        ("n_vertices", C.c_uint32),
+        ("n_edges", C.c_uint32),
+        ("row_offsets", C.POINTER(C.c_uint32)),
+        ("column_indices", C.POINTER(C.c_uint32)),
+        ("weights", C.POINTER(C.c_uint32)),
+        ("attacker_pos", C.POINTER(C.c_uint8)),
+        ("rev_row_offsets", C.POINTER(C.c_uint32)),
+        ("rev_column_indices", C.POINTER(C.c_uint32)),
+    ]
+
+
+class geq_energies(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_uint32),
+        ("energy_size", C.c_uint32),
+        ("offsets", C.POINTER(C.c_uint32)),
+        ("data", C.POINTER(C.c_uint32)),
+    ]
+
+
+class geq_stats(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_uint64),
+        ("node_evals", C.c_uint64),
+        ("edge_updates", C.c_uint64),
+        ("alg_bytes", C.c_uint64),
+        ("changed", C.c_uint64),
+        ("solve_ms", C.c_double),
+        ("eval_kernel_ms", C.c_double),
+        ("expand_kernel_ms", C.c_double),
+        ("eval_launches", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+P = C.POINTER
+_SIGS = {
+    "geq_last_error": (C.c_char_p, []),
+    "geq_ctx_create": (C.c_int, [C.c_int, P(C.c_void_p)]),
+    "geq_ctx_destroy": (None, [C.c_void_p]),
+    "geq_ctx_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, P(C.c_int)]),
+    "geq_solve": (C.c_int, [C.c_void_p, P(geq_graph), P(C.c_uint32), C.c_uint32, P(geq_energies), P(geq_energies), P(geq_stats)]),
+    "geq_energies_free": (None, [P(geq_energies)]),
+    "geq_game_upload": (C.c_int, [C.c_void_p, P(geq_graph), P(C.c_void_p)]),
+    "geq_game_solve": (C.c_int, [C.c_void_p, P(C.c_uint32), C.c_uint32, P(geq_energies), C.c_int, P(geq_stats)]),
+    "geq_game_energies": (C.c_int, [C.c_void_p, P(geq_energies)]),
+    "geq_game_free": (None, [C.c_void_p]),
+    "geq_standard_reach": (C.c_int, [P(geq_graph), P(C.c_uint32), P(C.c_uint32)]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"{LIB_PATH} is missing: run gpuequiv_amd/build.py (or __graft_entry__.build())")
+            h = C.CDLL(LIB_PATH)
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(h, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = h
+        return _lib
+
+
+def check(code: int) -> None:
+    if code != GEQ_OK:
+        msg = lib().geq_last_error()
+        raise Error(code, msg.decode() if msg else "")
+
+
+def u32p(a: np.ndarray):
+    return a.ctypes.data_as(P(C.c_uint32))
+
+
+def u8p(a: np.ndarray):
+    return a.ctypes.data_as(P(C.c_uint8))
+
+
+class Context:
+    """Process-wide device context, like GPUCommon::get_gpu (src/energygame/gpuutils.rs:15-31)."""
+
+    _instance = None
+    _ilock = threading.Lock()
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().geq_ctx_create(device, C.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def get(cls) -> "Context":
+        with cls._ilock:
+            if cls._instance is None:
+                cls._instance = Context(int(os.environ.get("GEQ_DEVICE", "0")))
+            return cls._instance
+
+    def info(self) -> tuple:
+        buf = C.create_string_buffer(128)
+        ncu = C.c_int()
+        check(lib().geq_ctx_info(self.handle, buf, 128, C.byref(ncu)))
+        return buf.value.decode(), ncu.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and _lib is not None:
+                _lib.geq_ctx_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def energies_to_numpy(e: geq_energies):
+    n = e.n_nodes
+    offs = np.ctypeslib.as_array(e.offsets, shape=(n + 1,)).copy()
+    tot = int(offs[-1]) * e.energy_size
+    data = np.ctypeslib.as_array(e.data, shape=(max(tot, 1),))[:tot].copy() if tot else np.zeros(0, np.uint32)
+    return offs, data

@@ -1,0 +1,172 @@
+// Device-side primitives of the energy-game solver (gfx950, wave64).
+//
+// Energies of every supported EnergyConf (elements <= 6, max <= 3) are
+// handled in one canonical layout: 6 fields of 2 bits (12-bit point, stored as
+// u16), updates as 6 nibbles (u32).  The 4096 points of {0..3}^6 form one
+// 4096-bit set = 64 lanes x 64 bits: lane l holds the points 64*l .. 64*l+63,
+// i.e. dimensions 0..2 live inside a lane's u64 word and dimensions 3..5 are
+// spread across lanes (strides of 1, 4 and 16 lanes).
+//
+// Upward-closed sets are the solver's currency:
+//  * the attacker merge E(u) <- min(E(u) U U_v inv(E(v), w)) is the minimal
+//    set of the OR of the up-closures (inv_update is monotone);
+//  * the defender combination min{ sup(e_1..e_k) } of
+//    src/energygame/defend_iterative.wgsl:182-267 has the up-closure
+//    AND_k up(inv(E(s_k), w_k)), because x >= sup(a, b) iff x >= a and x >= b.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace geq {
+
+// ---------------------------------------------------------------------------
+// Per-node energy list descriptor: 16 bytes.
+//   h[0]: bits 0..14 = number of energies, bit 15 = list is not sorted
+//   len <= 7 : h[1..7] hold the energies inline
+//   len  > 7 : h[1] | h[2] << 16 = offset into the energy arena (u16 units)
+// ---------------------------------------------------------------------------
+constexpr uint32_t INLINE_CAP = 7;
+constexpr uint32_t LEN_MASK = 0x7fffu;
+constexpr uint32_t UNSORTED = 0x8000u;
+
+struct __align__(16) NodeList {
+    uint16_t h[8];
+};
+
+__device__ __forceinline__ uint32_t nl_len(const uint4& d) { return d.x & LEN_MASK; }
+__device__ __forceinline__ uint32_t nl_off(const uint4& d) { return d.y; }  // h[1] | h[2] << 16
+// inline energy j (0..6) of a descriptor
+__device__ __forceinline__ uint32_t nl_inline(const uint4& d, uint32_t j) {
+    // energies sit at u16 index 1 + j of the 16-byte record
+    uint32_t idx = 1 + j;
+    uint32_t w = (idx < 2) ? d.x : (idx < 4) ? d.y : (idx < 6) ? d.z : d.w;
+    return (idx & 1) ? (w >> 16) : (w & 0xffffu);
+}
+__device__ __forceinline__ uint32_t nl_get(const uint4& d, const uint16_t* __restrict__ arena, uint32_t j) {
+    return nl_len(d) <= INLINE_CAP ? nl_inline(d, j) : (uint32_t)arena[nl_off(d) + j];
+}
+
+// ---------------------------------------------------------------------------
+// inv_update on the canonical layout (shadergen.rs:103-128, attack_update.wgsl:45-61):
+// first every Decrement field i: e[i] = min(e[i] + 1, max); then, in order
+// i = 0..5, every Min field (nibble v >= 2): e[v-2] = max(e[v-2], e[i]),
+// reading values already updated earlier in the sequence.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t inv_update(uint32_t e, uint32_t u, uint32_t maxv) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        uint32_t ui = (u >> (4 * i)) & 0xfu;
+        uint32_t ei = (e >> (2 * i)) & 3u;
+        uint32_t ni = (ui == 1u) ? min(ei + 1u, maxv) : min(ei, maxv);
+        e = (e & ~(3u << (2 * i))) | (ni << (2 * i));
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        uint32_t ui = (u >> (4 * i)) & 0xfu;
+        if (ui > 1u) {
+            uint32_t j = ui - 2u;                 // validated on the host: j < 6
+            uint32_t ei = (e >> (2 * i)) & 3u;
+            uint32_t ej = (e >> (2 * j)) & 3u;
+            uint32_t m = ei > ej ? ei : ej;
+            e = (e & ~(3u << (2 * j))) | (m << (2 * j));
+        }
+    }
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// lanes below this one that are set in `mask`
+__device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Memory ordering between the lanes of one wave through LDS / global memory.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// value of lane (lane_id - delta); garbage for lanes below delta (masked by callers)
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, uint32_t delta, uint32_t lane) {
+    int src = ((int)lane - (int)delta) & 63;
+    uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)v);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t shfl32(uint32_t v, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+
+// inclusive wave prefix sum
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t t = shfl32(v, (lane - d) & 63u);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (uint32_t d = 32; d >= 1; d >>= 1) v += (uint32_t)__shfl_xor((int)v, (int)d, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// 4096-point bitmap operations on one u64 word per lane.
+// Point p = 64 * lane + b has coordinates e_k = (p >> 2k) & 3.
+// ---------------------------------------------------------------------------
+constexpr uint64_t D0_LT3 = 0x7777777777777777ull, D0_LT2 = 0x3333333333333333ull;
+constexpr uint64_t D1_LT3 = 0x0FFF0FFF0FFF0FFFull, D1_LT2 = 0x00FF00FF00FF00FFull;
+constexpr uint64_t D2_LT3 = 0x0000FFFFFFFFFFFFull, D2_LT2 = 0x00000000FFFFFFFFull;
+
+// Upward closure: every point >= some member becomes a member.
+__device__ __forceinline__ uint64_t up_close(uint64_t w, uint32_t lane) {
+    w |= (w & D0_LT3) << 1;
+    w |= (w & D0_LT2) << 2;
+    w |= (w & D1_LT3) << 4;
+    w |= (w & D1_LT2) << 8;
+    w |= (w & D2_LT3) << 16;
+    w |= (w & D2_LT2) << 32;
+    const uint32_t e3 = lane & 3u, e4 = (lane >> 2) & 3u, e5 = lane >> 4;
+    uint64_t t;
+    t = shfl_up64(w, 1, lane);  if (e3 >= 1) w |= t;
+    t = shfl_up64(w, 2, lane);  if (e3 >= 2) w |= t;
+    t = shfl_up64(w, 4, lane);  if (e4 >= 1) w |= t;
+    t = shfl_up64(w, 8, lane);  if (e4 >= 2) w |= t;
+    t = shfl_up64(w, 16, lane); if (e5 >= 1) w |= t;
+    t = shfl_up64(w, 32, lane); if (e5 >= 2) w |= t;
+    return w;
+}
+
+// Minimal points of an upward-closed set: x is minimal iff no x - e_k is a member.
+__device__ __forceinline__ uint64_t minimal_of(uint64_t w, uint32_t lane) {
+    uint64_t nm = ((w & D0_LT3) << 1) | ((w & D1_LT3) << 4) | ((w & D2_LT3) << 16);
+    const uint32_t e3 = lane & 3u, e4 = (lane >> 2) & 3u, e5 = lane >> 4;
+    uint64_t t;
+    t = shfl_up64(w, 1, lane);  if (e3 >= 1) nm |= t;
+    t = shfl_up64(w, 4, lane);  if (e4 >= 1) nm |= t;
+    t = shfl_up64(w, 16, lane); if (e5 >= 1) nm |= t;
+    return w & ~nm;
+}
+
+__device__ __forceinline__ void bm_set(uint32_t* bm, uint32_t p) {
+    atomicOr(&bm[p >> 5], 1u << (p & 31u));
+}
+__device__ __forceinline__ uint64_t bm_word(const uint32_t* bm, uint32_t lane) {
+    return (uint64_t)bm[2 * lane] | ((uint64_t)bm[2 * lane + 1] << 32);
+}
+__device__ __forceinline__ bool bm_test(const uint32_t* bm, uint32_t p) {
+    return (bm[p >> 5] >> (p & 31u)) & 1u;
+}
+
+}  // namespace geq

@@ -1,0 +1,76 @@
+// Host-side runtime of the solver: context, error reporting, device games.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gpuequiv.h"
+#include "solve_kernels.hpp"
+
+namespace geq {
+
+void set_error(const std::string& msg);
+geq_status fail(geq_status s, const std::string& msg);
+
+#define GEQ_HIP(expr, code)                                                                  \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            return ::geq::fail((code), std::string(#expr) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+// Energy configuration accepted by the kernels: elements <= 6, max <= 3.
+struct ConfInfo {
+    uint32_t elements, max;
+    uint32_t ebits, emask, ubits, umask;
+};
+geq_status check_conf(const geq_conf& c, ConfInfo& out);
+// conf packing <-> canonical (2-bit energies, 4-bit updates)
+bool energy_to_canon(const ConfInfo& c, uint32_t packed, uint32_t& canon);
+uint32_t energy_from_canon(const ConfInfo& c, uint32_t canon);
+bool update_to_canon(const ConfInfo& c, uint32_t packed, uint32_t& canon);
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (count ? count : 1));
+    }
+};
+
+}  // namespace geq
+
+struct geq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    int n_cu = 0;
+};
+
+struct geq_game {
+    geq_ctx* ctx = nullptr;
+    geq::ConfInfo conf{};
+    uint32_t V = 0, E = 0;
+    geq::DevBuf<uint32_t> row, rrow, rcol, wl0, wl1, stamp, seed, lens, offs, out;
+    geq::DevBuf<uint2> edges;
+    geq::DevBuf<uint8_t> atk, chg;
+    geq::DevBuf<uint4> cur, nxt;
+    geq::DevBuf<uint16_t> arena;
+    geq::DevBuf<geq::Ctl> ctl;
+    geq::DevBuf<unsigned long long> stats;
+    geq::Ctl* h_ctl = nullptr;        // pinned
+    uint32_t arena_cap = 0;
+    std::vector<uint8_t> h_atk;       // kept for standard_reach / reach validation
+    ~geq_game();
+};

@@ -1,0 +1,302 @@
+// Fixpoint kernels of the energy-game solver: one wave per worklist node.
+#include "solve_kernels.hpp"
+
+namespace geq {
+
+// Write the members of a 4096-point bitmap (one u64 word per lane) as an
+// ascending list: inline into the node's record when it fits, else into the
+// arena.  Returns the list length (uniform).  On arena exhaustion sets
+// ctl->overflow and returns ~0u.
+__device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint4* rec, uint16_t* arena, Ctl* ctl,
+                                                 uint32_t arena_cap) {
+    const uint32_t c = (uint32_t)__popcll(M);
+    const uint32_t incl = wave_incl_scan(c, lane);
+    const uint32_t total = uniform(shfl32(incl, 63));
+    uint32_t p = incl - c;
+    if (total <= INLINE_CAP) {
+        uint16_t* h = reinterpret_cast<uint16_t*>(rec);
+        uint64_t m = M;
+        while (m) {
+            uint32_t b = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            h[1 + p] = (uint16_t)(64u * lane + b);
+            p++;
+        }
+        if (lane == 0) h[0] = (uint16_t)total;
+        return total;
+    }
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&ctl->arena_top, total);
+    base = uniform(shfl32(base, 0));
+    if ((uint64_t)base + total > arena_cap) {
+        if (lane == 0) atomicOr(&ctl->overflow, 1u);
+        return ~0u;
+    }
+    uint64_t m = M;
+    while (m) {
+        uint32_t b = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        arena[base + p] = (uint16_t)(64u * lane + b);
+        p++;
+    }
+    if (lane == 0) *rec = make_uint4(total, base, 0u, 0u);
+    return total;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_eval(EvalArgs a) {
+    __shared__ uint32_t s_bm[WAVES][3][128];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wid = threadIdx.x >> 6;
+    uint32_t* A = s_bm[wid][0];   // union (attacker) bitmap
+    uint32_t* B = s_bm[wid][1];   // one successor's images (defender)
+    uint32_t* O = s_bm[wid][2];   // old list of the node
+    const uint32_t maxv = a.maxv;
+    const uint16_t* __restrict__ arena = a.arena;
+
+    __shared__ unsigned long long s_stats[4];
+    if (threadIdx.x < 4) s_stats[threadIdx.x] = 0ull;
+    __syncthreads();
+    uint64_t st_evals = 0, st_edges = 0, st_bytes_lane = 0, st_bytes_node = 0, st_changed = 0;
+    const uint32_t gstride = gridDim.x * WAVES;
+    for (uint32_t idx = uniform(blockIdx.x * WAVES + wid); idx < a.n; idx += gstride) {
+        const uint32_t u = uniform(a.wl[idx]);
+        const uint32_t r0 = uniform(a.row[u]);
+        const uint32_t deg = uniform(a.row[u + 1]) - r0;
+        const bool isatk = uniform(a.atk[u]) != 0;
+        const uint4 own = a.cur[u];
+        const uint32_t own_len = uniform(nl_len(own));
+
+        A[lane] = 0u; A[lane + 64] = 0u;
+        O[lane] = 0u; O[lane + 64] = 0u;
+        wave_sync();
+        for (uint32_t j = lane; j < own_len; j += 64) {
+            uint32_t e = nl_get(own, arena, j);
+            bm_set(O, e);
+            if (isatk) bm_set(A, inv_update(e, 0u, maxv));
+        }
+
+        bool changed = false;
+        bool evaluated = true;
+        uint32_t new_len = 0;
+        if (isatk) {
+            // ---- attacker: OR of the inverse-updated successor lists + own list
+            for (uint32_t base = 0; base < deg; base += 64) {
+                const uint32_t k = base + lane;
+                uint32_t len = 0, w = 0;
+                uint4 d = make_uint4(0, 0, 0, 0);
+                if (k < deg) {
+                    uint2 ed = a.edges[r0 + k];
+                    w = ed.y;
+                    d = a.cur[ed.x];
+                    len = nl_len(d);
+                    st_bytes_lane += 8u + 4u * len;
+                }
+                if (len <= INLINE_CAP) {
+                    for (uint32_t j = 0; j < len; j++) bm_set(A, inv_update(nl_inline(d, j), w, maxv));
+                }
+                uint64_t longm = __ballot(len > INLINE_CAP);
+                while (longm) {
+                    const uint32_t s = (uint32_t)__builtin_ctzll(longm);
+                    longm &= longm - 1;
+                    const uint32_t lw = uniform(shfl32(w, s));
+                    const uint32_t loff = uniform(shfl32(d.y, s));
+                    const uint32_t llen = uniform(shfl32(len, s));
+                    for (uint32_t j = lane; j < llen; j += 64) bm_set(A, inv_update(arena[loff + j], lw, maxv));
+                }
+            }
+            wave_sync();
+            const uint64_t W = up_close(bm_word(A, lane), lane);
+            const uint64_t M = minimal_of(W, lane);
+            const uint64_t Ow = bm_word(O, lane);
+            // changed iff a minimal budget is not already one of the node's own
+            // energies (attack_minimize.wgsl:208-214: a survivor from a successor slice)
+            changed = __ballot((M & ~Ow) != 0ull) != 0ull;
+            if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
+        } else {
+            // ---- defender: skipped while any successor has no energies (src/energygame.rs:789-792)
+            bool any_empty = false;
+            for (uint32_t base = 0; base < deg; base += 64) {
+                const uint32_t k = base + lane;
+                uint32_t len = 1;
+                if (k < deg) len = nl_len(a.cur[a.edges[r0 + k].x]);
+                any_empty |= (__ballot(len == 0) != 0ull);
+            }
+            if (any_empty || deg == 0) {
+                evaluated = false;
+            } else if (deg == 1) {
+                // A single successor is copied, not minimised
+                // (defend_iterative.wgsl:301-313 and the empty fold loop).
+                const uint2 ed = a.edges[r0];
+                const uint32_t w = uniform(ed.y);
+                const uint4 d = a.cur[ed.x];
+                const uint32_t len = uniform(nl_len(d));
+                st_bytes_lane += (lane == 0) ? 8u + 4u * len : 0u;
+                bool all_in = true;
+                for (uint32_t j = lane; j < len; j += 64) all_in &= bm_test(O, inv_update(nl_get(d, arena, j), w, maxv));
+                const bool same = (len == own_len) && (__ballot(!all_in) == 0ull);
+                changed = !same;
+                if (changed) {
+                    uint4* rec = &a.nxt[u];
+                    if (len <= INLINE_CAP) {
+                        uint16_t* h = reinterpret_cast<uint16_t*>(rec);
+                        if (lane < len) h[1 + lane] = (uint16_t)inv_update(nl_inline(d, lane), w, maxv);
+                        if (lane == 0) h[0] = (uint16_t)(len | (len > 1 ? UNSORTED : 0u));
+                        new_len = len;
+                    } else {
+                        uint32_t base = 0;
+                        if (lane == 0) base = atomicAdd(&a.ctl->arena_top, len);
+                        base = uniform(shfl32(base, 0));
+                        if ((uint64_t)base + len > a.arena_cap) {
+                            if (lane == 0) atomicOr(&a.ctl->overflow, 1u);
+                            new_len = ~0u;
+                        } else {
+                            for (uint32_t j = lane; j < len; j += 64)
+                                a.arena[base + j] = (uint16_t)inv_update(arena[nl_off(d) + j], w, maxv);
+                            if (lane == 0) *rec = make_uint4(len | UNSORTED, base, 0u, 0u);
+                            new_len = len;
+                        }
+                    }
+                }
+            } else {
+                // AND of the up-closures of every successor's inverse-updated list
+                uint64_t acc = ~0ull;
+                for (uint32_t base = 0; base < deg; base += 64) {
+                    const uint32_t k = base + lane;
+                    uint32_t w = 0;
+                    uint4 d = make_uint4(0, 0, 0, 0);
+                    if (k < deg) {
+                        uint2 ed = a.edges[r0 + k];
+                        w = ed.y;
+                        d = a.cur[ed.x];
+                        st_bytes_lane += 8u + 4u * nl_len(d);
+                    }
+                    const uint32_t nk = min(64u, deg - base);
+                    for (uint32_t s = 0; s < nk; s++) {
+                        const uint32_t ws = uniform(shfl32(w, s));
+                        uint4 ds;
+                        ds.x = uniform(shfl32(d.x, s));
+                        ds.y = uniform(shfl32(d.y, s));
+                        ds.z = uniform(shfl32(d.z, s));
+                        ds.w = uniform(shfl32(d.w, s));
+                        const uint32_t ls = nl_len(ds);
+                        B[lane] = 0u; B[lane + 64] = 0u;
+                        wave_sync();
+                        for (uint32_t j = lane; j < ls; j += 64) bm_set(B, inv_update(nl_get(ds, arena, j), ws, maxv));
+                        wave_sync();
+                        acc &= up_close(bm_word(B, lane), lane);
+                        wave_sync();
+                    }
+                }
+                const uint64_t M = minimal_of(acc, lane);
+                const uint64_t Ow = bm_word(O, lane);
+                const uint32_t total = uniform(wave_sum((uint32_t)__popcll(M)));
+                // equality test of src/energygame.rs:929 (same size, new within old)
+                const bool same = (total == own_len) && (__ballot((M & ~Ow) != 0ull) == 0ull);
+                changed = !same;
+                if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
+            }
+        }
+        if (new_len == ~0u) changed = false;   // arena overflow: the host re-runs the round
+        if (lane == 0) {
+            a.chg[idx] = changed ? 1 : 0;
+            if (evaluated) {
+                st_evals++;
+                st_edges += deg;
+                st_bytes_node += 9u + 4u * own_len + (changed ? 4u * new_len : 0u);
+                st_changed += changed ? 1 : 0;
+            }
+        }
+        wave_sync();
+    }
+    // statistics: LDS reduction, one global atomic per counter and block
+    if (st_bytes_lane) atomicAdd(&s_stats[2], (unsigned long long)st_bytes_lane);
+    if (lane == 0) {
+        atomicAdd(&s_stats[0], (unsigned long long)st_evals);
+        atomicAdd(&s_stats[1], (unsigned long long)st_edges);
+        atomicAdd(&s_stats[2], (unsigned long long)st_bytes_node);
+        atomicAdd(&s_stats[3], (unsigned long long)st_changed);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+}
+
+// Commit changed nodes (cur <- nxt) and append their not-yet-queued
+// predecessors to the next worklist (GPURunner::changed_nodes,
+// src/energygame.rs:1467-1479).  Appends are staged per wave in LDS and
+// flushed with one atomic per 192+ entries.
+__global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
+    __shared__ uint32_t s_stage[WAVES][256];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wid = threadIdx.x >> 6;
+    uint32_t* stage = s_stage[wid];
+    if (a.ctl->overflow) return;   // the round is re-run after the arena grows
+    uint32_t cnt = 0;
+    auto flush = [&]() {
+        wave_sync();
+        uint32_t base = 0;
+        if (lane == 0 && cnt) base = atomicAdd(&a.ctl->wl_n[a.out_slot], cnt);
+        base = uniform(shfl32(base, 0));
+        for (uint32_t j = lane; j < cnt; j += 64) a.out[base + j] = stage[j];
+        cnt = 0;
+        wave_sync();
+    };
+    const uint32_t gstride = gridDim.x * WAVES * 64;
+    for (uint32_t b = (blockIdx.x * WAVES + wid) * 64; b < a.n; b += gstride) {
+        const uint32_t i = b + lane;
+        uint32_t u = 0;
+        bool f = false;
+        if (i < a.n) {
+            u = a.list[i];
+            f = a.chg ? (a.chg[i] != 0) : true;
+        }
+        if (f && a.nxt) a.cur[u] = a.nxt[u];
+        uint64_t mask = __ballot(f);
+        while (mask) {
+            const uint32_t s = (uint32_t)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t v = uniform(shfl32(u, s));
+            const uint32_t r0 = uniform(a.rrow[v]), r1 = uniform(a.rrow[v + 1]);
+            for (uint32_t k0 = r0; k0 < r1; k0 += 64) {
+                const uint32_t k = k0 + lane;
+                bool win = false;
+                uint32_t w = 0;
+                if (k < r1) {
+                    w = a.rcol[k];
+                    win = atomicExch(&a.stamp[w], a.stamp_val) != a.stamp_val;
+                }
+                const uint64_t bal = __ballot(win);
+                if (win) stage[cnt + mbcnt(bal)] = w;
+                cnt += (uint32_t)__popcll(bal);
+                if (cnt > 192) flush();
+            }
+        }
+    }
+    flush();
+}
+
+__global__ void k_list_lens(const uint4* cur, uint32_t n, uint32_t* lens) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) lens[v] = nl_len(cur[v]);
+}
+
+// Copy every node's list into the packed output, ascending.  Lists produced
+// by the single-successor defender copy keep the successor's order and are
+// sorted here (insertion sort; such lists are short antichain images).
+__global__ void k_gather(const uint4* cur, const uint16_t* arena, uint32_t n, const uint32_t* offs, uint32_t* out) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint4 d = cur[v];
+    const uint32_t len = nl_len(d);
+    uint32_t* o = out + offs[v];
+    for (uint32_t j = 0; j < len; j++) o[j] = nl_get(d, arena, j);
+    if (d.x & UNSORTED) {
+        for (uint32_t i = 1; i < len; i++) {
+            uint32_t x = o[i];
+            uint32_t j = i;
+            while (j > 0 && o[j - 1] > x) { o[j] = o[j - 1]; j--; }
+            o[j] = x;
+        }
+    }
+}
+
+}  // namespace geq

@@ -1,0 +1,99 @@
+"""Seeded synthetic workloads (numpy PCG64; same seed -> same bytes everywhere).
+
+* ``random_game``  - standalone energy game: Bernoulli attacker positions,
+  geometric out-degree on {0,1,...} with the given mean, uniform targets, and
+  updates drawn as identity / one Decrement / one Min(j) (the update shapes of
+  graphgen/graphgen.py:14-27, restated with j != i).
+* ``random_lts`` / ``powerlaw_lts`` - labelled transition systems.
+* ``game_batch``  - many random games with log-uniform sizes.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .energy import EnergyConf
+from .energygame import GameGraph
+
+
+def random_update_words(rng: np.random.Generator, n: int, p_ident: float = 0.5, p_dec: float = 0.35,
+                        elements: int = 6) -> np.ndarray:
+    """Packed STANDARD updates (4-bit fields): identity, a Decrement at one
+    uniform component, or Min(j) at component i with j uniform over the others
+    (1-based, as Upd::Min)."""
+    r = rng.random(n)
+    comp = rng.integers(0, elements, n)
+    other = rng.integers(0, elements - 1, n)
+    other = other + (other >= comp)           # uniform over the other components, 0-based
+    w = np.zeros(n, dtype=np.uint32)
+    dec = (r >= p_ident) & (r < p_ident + p_dec)
+    mn = r >= p_ident + p_dec
+    w[dec] = (np.uint32(1) << (4 * comp[dec]).astype(np.uint32))
+    # Min(k) with k = other + 1 is packed as k + 1 = other + 2
+    w[mn] = ((other[mn] + 2).astype(np.uint32) << (4 * comp[mn]).astype(np.uint32))
+    return w
+
+
+def random_game(n_vertices: int, seed: int, p_attacker: float = 0.5, mean_degree: float = 4.0,
+                p_ident: float = 0.5, p_dec: float = 0.35) -> GameGraph:
+    """Standalone game: out-degree Geometric on {0,1,...} with the given mean,
+    targets uniform, self-loops and multi-edges allowed (GameGraph::new allows
+    both, src/energygame.rs:141-194)."""
+    rng = np.random.default_rng(seed)
+    V = int(n_vertices)
+    atk = (rng.random(V) < p_attacker).astype(np.uint8)
+    p = 1.0 / (mean_degree + 1.0)
+    deg = rng.geometric(p, V) - 1
+    row = np.zeros(V + 1, dtype=np.uint64)
+    np.cumsum(deg, out=row[1:])
+    E = int(row[-1])
+    col = rng.integers(0, V, E, dtype=np.uint32)
+    w = random_update_words(rng, E, p_ident, p_dec)
+    return GameGraph(EnergyConf.STANDARD, row.astype(np.uint32), col, w.reshape(E, 1), atk)
+
+
+def game_batch(n_games: int, seed: int, v_min: int = 1000, v_max: int = 200000):
+    """``n_games`` random games with log-uniform position counts in [v_min, v_max]."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(v_min), np.log(v_max), n_games)).astype(np.int64)
+    seeds = rng.integers(0, 2**63 - 1, n_games)
+    return [random_game(int(s), int(sd)) for s, sd in zip(sizes, seeds)]
+
+
+def random_lts(n_states: int, n_transitions: int, n_labels: int, seed: int):
+    """Uniform sources, targets and labels.  Returns (n, src, dst, lbl) edge lists."""
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n_states, n_transitions, dtype=np.uint32)
+    dst = rng.integers(0, n_states, n_transitions, dtype=np.uint32)
+    lbl = rng.integers(0, n_labels, n_transitions).astype(np.int32)
+    return n_states, src, dst, lbl
+
+
+def powerlaw_lts(n_states: int, n_transitions: int, n_labels: int, seed: int, alpha: float = 2.1, max_deg: int = 200,
+                 zipf_s: float = 1.2, p_tau: float = 0.1):
+    """Power-law out-degrees (alpha, capped, rescaled to the exact transition
+    count), uniform targets, Zipf(s) label frequencies over labels 1..n-1 and
+    label 0 ("i", src/lib.rs:376) with probability p_tau."""
+    rng = np.random.default_rng(seed)
+    u = rng.random(n_states)
+    raw = np.minimum(np.floor((1 - u) ** (-1.0 / (alpha - 1.0))), max_deg)
+    deg = np.maximum(0, np.round(raw * (n_transitions / raw.sum()))).astype(np.int64)
+    deg = np.minimum(deg, max_deg)
+    diff = n_transitions - int(deg.sum())
+    while diff != 0:
+        idx = rng.integers(0, n_states, abs(diff))
+        if diff > 0:
+            ok = deg[idx] < max_deg
+            np.add.at(deg, idx[ok], 1)
+        else:
+            ok = deg[idx] > 0
+            np.add.at(deg, idx[ok], -1)
+        deg = np.clip(deg, 0, max_deg)
+        diff = n_transitions - int(deg.sum())
+    src = np.repeat(np.arange(n_states, dtype=np.uint32), deg)
+    dst = rng.integers(0, n_states, n_transitions, dtype=np.uint32)
+    ranks = np.arange(1, n_labels, dtype=np.float64)
+    pz = ranks ** (-zipf_s)
+    pz /= pz.sum()
+    lbl = (rng.choice(n_labels - 1, n_transitions, p=pz) + 1).astype(np.int32)
+    lbl[rng.random(n_transitions) < p_tau] = 0
+    return n_states, src, dst, lbl

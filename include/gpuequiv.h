@@ -1,0 +1,134 @@
+/*
+ * gpuequiv on MI355X (gfx950): C ABI of the energy-game solver.
+ *
+ * This is the drop-in boundary for the spectroscopy path of the Rust crate
+ * Gobbel2000/gpuequiv v1.0.0.  It replaces the crate's wgpu device / buffer /
+ * pipeline layer (src/energygame/gpuutils.rs, the private GPURunner,
+ * AttackShader and DefendIterShader of src/energygame.rs and the four WGSL
+ * kernels).  A Rust GPURunner replacement would bind these functions with
+ * `extern "C"`; the binding is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *  - All arrays are plain host pointers + sizes; the caller keeps ownership of
+ *    inputs.  Outputs allocated by the library are released with the matching
+ *    geq_*_free function.
+ *  - Every function returns a geq_status; on failure geq_last_error() returns
+ *    a thread-local message.  No function aborts or throws across the ABI.
+ *  - Energies and updates use the crate's bit packing (src/energy.rs:36-78,
+ *    :143-162, :543-565).  Energies are returned sorted ascending per node as
+ *    packed words (the crate's EnergyArray compares order-insensitively,
+ *    src/energy.rs:334-369, so this is its canonical form).
+ *  - Supported energy configurations: elements <= 6 and max <= 3, which
+ *    includes EnergyConf::STANDARD (the conf gamebuild uses,
+ *    src/gamebuild.rs:152).  Others return GEQ_ERR_UNSUPPORTED.
+ */
+#ifndef GPUEQUIV_H
+#define GPUEQUIV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes.  1..3 mirror gpuequiv::Error::{NoAdapter, NoDevice, BufferMap}
+ * (src/error.rs:3-8); the others replace panics of the reference on invalid
+ * input or resource exhaustion. */
+typedef enum {
+    GEQ_OK = 0,
+    GEQ_ERR_NO_ADAPTER = 1,    /* no GPU / invalid device ordinal          */
+    GEQ_ERR_NO_DEVICE = 2,     /* device present but could not be set up   */
+    GEQ_ERR_TRANSFER = 3,      /* host<->device copy failed (BufferMap)    */
+    GEQ_ERR_INVALID_ARG = 4,   /* malformed graph / energies / updates     */
+    GEQ_ERR_OUT_OF_MEMORY = 5, /* device allocation or budget exhausted    */
+    GEQ_ERR_UNSUPPORTED = 6,   /* energy configuration outside the kernels */
+    GEQ_ERR_INTERNAL = 7
+} geq_status;
+
+typedef struct geq_ctx geq_ctx;
+typedef struct geq_game geq_game;
+
+/* EnergyConf (src/energy.rs:36-47). */
+typedef struct {
+    uint32_t elements;
+    uint32_t max;
+} geq_conf;
+
+/* GameGraph public fields (src/energygame.rs:120-138), borrowed for the call.
+ * weights holds n_edges * update_size packed words.  rev_* may be NULL, in
+ * which case the reverse CSR is derived (GameGraph::make_reverse, :235-250). */
+typedef struct {
+    geq_conf conf;
+    uint32_t n_vertices;
+    uint32_t n_edges;
+    const uint32_t* row_offsets;      /* n_vertices + 1 */
+    const uint32_t* column_indices;   /* n_edges        */
+    const uint32_t* weights;          /* n_edges * update_size */
+    const uint8_t* attacker_pos;      /* n_vertices, 0 / 1 */
+    const uint32_t* rev_row_offsets;  /* n_vertices + 1 or NULL */
+    const uint32_t* rev_column_indices; /* n_edges or NULL */
+} geq_graph;
+
+/* Per-node energy lists (EnergyGame::energies, src/energygame.rs:329-338):
+ * node v owns data[offsets[v] * energy_size .. offsets[v+1] * energy_size). */
+typedef struct {
+    uint32_t n_nodes;
+    uint32_t energy_size;
+    uint32_t* offsets;   /* n_nodes + 1, in energies */
+    uint32_t* data;      /* packed words */
+} geq_energies;
+
+/* Counters of one solve.  A node evaluation is one attacker update or one
+ * defender combination in some round; an edge update is one edge relaxed by
+ * such an evaluation.  alg_bytes follows the per-evaluation byte count
+ * 9 + sum_v (4 + 4*update_size + 4*|E(v)|) + 4*(|E_old| + |E_new| if written). */
+typedef struct {
+    uint64_t rounds;
+    uint64_t node_evals;
+    uint64_t edge_updates;
+    uint64_t alg_bytes;
+    uint64_t changed;
+    double solve_ms;        /* device time of the fixpoint loop (hipEvents)        */
+    double eval_kernel_ms;  /* sum of k_eval durations, when profiling was asked   */
+    double expand_kernel_ms;/* sum of k_expand durations, when profiling was asked */
+    uint64_t eval_launches;
+} geq_stats;
+
+/* ---- context (GPUCommon::get_gpu, src/energygame/gpuutils.rs:9-61) ---- */
+geq_status geq_ctx_create(int device, geq_ctx** out);
+void geq_ctx_destroy(geq_ctx* ctx);
+const char* geq_last_error(void);
+/* HIP device name / CU count of the context's device. */
+geq_status geq_ctx_info(geq_ctx* ctx, char* name, size_t name_len, int* n_cu);
+
+/* ---- one-shot solve: EnergyGame::run (src/energygame.rs:379-383) ----
+ * to_reach: the nodes initialised with the zero energy (EnergyGame::with_reach,
+ * :355-365; standard_reach :347-352 picks the defender dead ends).
+ * initial: optional explicit starting energies (NULL = with_reach semantics).
+ * The result is written to *out (free with geq_energies_free). */
+geq_status geq_solve(geq_ctx* ctx, const geq_graph* graph, const uint32_t* to_reach, uint32_t n_reach,
+                     const geq_energies* initial, geq_energies* out, geq_stats* stats);
+
+void geq_energies_free(geq_energies* e);
+
+/* ---- device-resident game (the pieces of geq_solve, for repeated solves) ----
+ * geq_game_upload copies the graph to HBM; geq_game_solve resets the energies
+ * (to_reach / initial as above) and runs the fixpoint with every input already
+ * resident; geq_game_energies downloads the result. */
+geq_status geq_game_upload(geq_ctx* ctx, const geq_graph* graph, geq_game** out);
+geq_status geq_game_solve(geq_game* game, const uint32_t* to_reach, uint32_t n_reach, const geq_energies* initial,
+                          int profile_kernels, geq_stats* stats);
+geq_status geq_game_energies(geq_game* game, geq_energies* out);
+void geq_game_free(geq_game* game);
+
+/* ---- helpers (no device needed) ---- */
+/* standard_reach: defender positions without successors (:347-352). Writes at
+ * most n_vertices ids into out, returns the count through *n_out. */
+geq_status geq_standard_reach(const geq_graph* graph, uint32_t* out, uint32_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUEQUIV_H */

@@ -1,0 +1,163 @@
+"""GPU parity of the fixpoint solver (EnergyGame.run through the C ABI) against
+the reference's golden games, the kernel fixtures and the oracle."""
+import numpy as np
+import pytest
+
+from gpuequiv_amd import EnergyArray, EnergyConf, EnergyGame, GameGraph, Update, Upd, update
+from gpuequiv_amd import workloads
+from gpuequiv_amd.energygame import energies_from_flat
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+STD = EnergyConf.STANDARD
+
+
+def _run(graph, reach=None, energies=None):
+    game = EnergyGame.standard_reach(graph) if reach is None else EnergyGame.with_reach(graph, reach)
+    if energies is not None:
+        game.energies = energies
+    out = game.run()
+    return game, out
+
+
+def _canon(arrays):
+    return [tuple(sorted(int(x) for x in a.array.reshape(-1))) for a in arrays]
+
+
+# ---------------------------------------------------------------- tests/energygame.rs
+@pytest.mark.parametrize("name", ["simple", "simple_one_dimension", "fork", "combinations"])
+def test_golden_games(name):
+    d = next(g for g in H.load("energygame_games.json")["games"] if g["name"] == name)
+    print("golden game", name, flush=True)
+    _, out = _run(H.fixture_game(d))
+    assert out == H.expected_arrays(d["expected"])
+
+
+def test_wide_game():
+    d = H.load("energygame_games.json")["wide_game"]
+    graph, expected = H.wide(d["single"], d["double"])
+    _, out = _run(graph)
+    assert out == H.expected_arrays(expected)
+
+
+def test_simple_one_dimension_by_set_weights():
+    # tests/energygame.rs:73-104 rewrites the weights of the built graph in place
+    d = next(g for g in H.load("energygame_games.json")["games"] if g["name"] == "simple")
+    exp = next(g for g in H.load("energygame_games.json")["games"] if g["name"] == "simple_one_dimension")
+    graph = H.fixture_game(d)
+    for i in range(graph.n_edges()):
+        graph.weights.set(i, Update.new([Upd.Decrement], STD))
+    _, out = _run(graph)
+    assert out == H.expected_arrays(exp["expected"])
+
+
+# ---------------------------------------------------------------- src/energygame/tests.rs
+def _defender_over(lists, weights, conf=STD):
+    """Defender 0 with one attacker successor per list; the successors hold the
+    lists as their energies and are the nodes to reach, so node 0 is visited."""
+    k = len(lists)
+    edges = [(0, i + 1, w) for i, w in enumerate(weights)]
+    graph = GameGraph.new(k + 1, edges, [False] + [True] * k, conf)
+    energies = [EnergyArray.empty(conf)] + [EnergyArray.from_conf(l, conf) for l in lists]
+    game = EnergyGame.with_reach(graph, list(range(1, k + 1)))
+    game.energies = energies
+    return game.run()[0]
+
+
+def test_defend_four_successors():
+    fx = H.load("kernel_fixtures.json")["four_successors"]
+    got = _defender_over(fx["successors"], [update(*w) for w in fx["weights"]])
+    assert got == EnergyArray.from_conf(fx["expected"], STD)
+
+
+def test_defend_antichain_141():
+    fx = H.load("kernel_fixtures.json")["antichain"]["energies"]
+    got = _defender_over([fx] * 4, [update(0)] * 4)
+    assert got.n_energies() == 141
+    assert got == EnergyArray.from_conf(fx, STD)
+
+
+def test_defend_mixed_nodes():
+    # src/energygame/tests.rs:345-483: three defenders solved in the same launch
+    fx = H.load("kernel_fixtures.json")
+    ac = fx["antichain"]["energies"]
+    ac2 = [[v + 1 for v in e] for e in ac]
+    four = fx["four_successors"]
+    conf = STD
+    lists = [ac] * 4 + [ac2] * 4 + four["successors"]
+    n = 3 + len(lists)
+    edges = []
+    heads = [0, 1, 2]
+    ws = [[update(0)] * 4, [update(0)] * 4, [update(*w) for w in four["weights"]]]
+    s = 3
+    for h, wl in zip(heads, ws):
+        for w in wl:
+            edges.append((h, s, w))
+            s += 1
+    atk = [False, False, False] + [True] * len(lists)
+    graph = GameGraph.new(n, edges, atk, conf)
+    game = EnergyGame.with_reach(graph, list(range(3, n)))
+    game.energies = [EnergyArray.empty(conf)] * 3 + [EnergyArray.from_conf(l, conf) for l in lists]
+    out = game.run()
+    assert out[0] == EnergyArray.from_conf(ac, conf)
+    assert out[1] == EnergyArray.from_conf(ac2, conf)
+    assert out[2] == EnergyArray.from_conf(four["expected"], conf)
+
+
+# ---------------------------------------------------------------- parity with the oracle
+@pytest.mark.parametrize("V,seed", [(50, 1), (200, 2), (1000, 3), (5000, 4), (20000, 5), (100000, 42)])
+def test_random_games_match_oracle(V, seed):
+    g = workloads.random_game(V, seed)
+    print("random game", V, seed, flush=True)
+    reach = H.standard_reach(g)
+    offs, data, st = H.oracle_solve_game(g, reach)
+    game, out = _run(g)
+    assert _canon(out) == H.canon_lists(offs, data)
+    s = game.last_stats
+    # the same Jacobi rounds: identical schedule counters
+    assert s["rounds"] == st["rounds"]
+    assert s["node_evals"] == st["node_evals"]
+    assert s["edge_updates"] == st["edge_updates"]
+    assert s["alg_bytes"] == st["alg_bytes"]
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_random_games_varied_shapes(seed):
+    # dense, decrement-heavy games build long antichains (arena lists > 7)
+    g = workloads.random_game(3000, seed, mean_degree=8.0, p_ident=0.2, p_dec=0.6)
+    reach = H.standard_reach(g)
+    offs, data, _ = H.oracle_solve_game(g, reach)
+    _, out = _run(g)
+    assert _canon(out) == H.canon_lists(offs, data)
+
+
+def test_initial_energies_and_empty_reach():
+    d = next(g for g in H.load("energygame_games.json")["games"] if g["name"] == "combinations")
+    graph = H.fixture_game(d)
+    game = EnergyGame.with_reach(graph, [])
+    out = game.run()
+    assert all(a.is_empty() for a in out)
+
+
+def test_small_confs_match_oracle():
+    # EnergyConf{elements: 4, max: 2}: 2-bit fields, saturation at 2
+    conf = EnergyConf(4, 2)
+    rng = np.random.default_rng(3)
+    V = 400
+    edges = []
+    for v in range(V):
+        for _ in range(rng.integers(0, 5)):
+            i = int(rng.integers(0, 4))
+            u = [0] * 4
+            r = rng.random()
+            if r < 0.4:
+                u[i] = -1
+            elif r < 0.6:
+                u[i] = int(rng.integers(1, 5))
+            edges.append((v, int(rng.integers(0, V)), update(*u)))
+    atk = (rng.random(V) < 0.5).tolist()
+    g = GameGraph.new(V, edges, atk, conf)
+    reach = H.standard_reach(g)
+    offs, data, _ = H.oracle_solve_game(g, reach)
+    _, out = _run(g)
+    assert _canon(out) == H.canon_lists(offs, data)
