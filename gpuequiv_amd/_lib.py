@@ -85,6 +85,9 @@ _SIGS = {
     "geq_game_energies": (C.c_int, [C.c_void_p, P(geq_energies)]),
     "geq_game_free": (None, [C.c_void_p]),
     "geq_standard_reach": (C.c_int, [P(geq_graph), P(C.c_uint32), P(C.c_uint32)]),
+    "geq_batch_upload": (C.c_int, [C.c_void_p, C.c_uint32, P(geq_graph), P(C.c_void_p)]),
+    "geq_batch_solve": (C.c_int, [C.c_void_p, P(P(C.c_uint32)), P(C.c_uint32), C.c_int, P(geq_stats)]),
+    "geq_batch_energies": (C.c_int, [C.c_void_p, P(geq_energies)]),
 }
 
 _lib = None

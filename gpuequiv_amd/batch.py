@@ -1,0 +1,87 @@
+"""Batched solving: many independent energy games in one device-resident solve.
+
+Games are laid out back to back in HBM and share one device worklist, so every
+round evaluates the frontier of all games together (geq_batch_* in
+include/gpuequiv.h).  Each game's result equals its standalone result.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from .energygame import GameGraph, energies_from_flat
+
+
+def standard_reach(graph: GameGraph) -> np.ndarray:
+    deg = np.diff(graph.row_offsets.astype(np.int64))
+    return np.ascontiguousarray(np.nonzero((graph.attacker_pos == 0) & (deg == 0))[0], dtype=np.uint32)
+
+
+class GameBatch:
+    """Device-resident batch of games (standard_reach seeding per game)."""
+
+    def __init__(self, graphs: Sequence[GameGraph], reach: Sequence[np.ndarray] | None = None):
+        self.graphs = list(graphs)
+        self.ctx = _lib.Context.get()
+        self._keep: list = []
+        arr = (_lib.geq_graph * len(self.graphs))()
+        for i, g in enumerate(self.graphs):
+            arr[i] = g._c_struct(self._keep)
+        self._structs = arr
+        h = C.c_void_p()
+        _lib.check(_lib.lib().geq_batch_upload(self.ctx.handle, len(self.graphs), arr, C.byref(h)))
+        self.handle = h
+        self.reach = [np.ascontiguousarray(r, dtype=np.uint32) for r in
+                      (reach if reach is not None else [standard_reach(g) for g in self.graphs])]
+        self._reach_ptrs = (C.POINTER(C.c_uint32) * len(self.reach))(*[_lib.u32p(r) for r in self.reach])
+        self._reach_n = np.array([len(r) for r in self.reach], dtype=np.uint32)
+        self.last_stats: dict | None = None
+
+    def solve(self, profile: bool = False) -> dict:
+        st = _lib.geq_stats()
+        _lib.check(_lib.lib().geq_batch_solve(self.handle, self._reach_ptrs, _lib.u32p(self._reach_n),
+                                              1 if profile else 0, C.byref(st)))
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def energies(self) -> list:
+        n = len(self.graphs)
+        outs = (_lib.geq_energies * n)()
+        _lib.check(_lib.lib().geq_batch_energies(self.handle, outs))
+        res = []
+        try:
+            for i in range(n):
+                o, d = _lib.energies_to_numpy(outs[i])
+                res.append(energies_from_flat(o, d, self.graphs[i].conf))
+        finally:
+            for i in range(n):
+                _lib.lib().geq_energies_free(C.byref(outs[i]))
+        return res
+
+    def raw_energies(self) -> list:
+        """Per game (offsets, data) numpy arrays, canonical order."""
+        n = len(self.graphs)
+        outs = (_lib.geq_energies * n)()
+        _lib.check(_lib.lib().geq_batch_energies(self.handle, outs))
+        res = []
+        try:
+            for i in range(n):
+                res.append(_lib.energies_to_numpy(outs[i]))
+        finally:
+            for i in range(n):
+                _lib.lib().geq_energies_free(C.byref(outs[i]))
+        return res
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            _lib.lib().geq_game_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,6 +1,7 @@
 // C ABI of the solver (include/gpuequiv.h): context, game upload, the
 // device-resident fixpoint loop and result download.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -72,6 +73,7 @@ bool update_to_canon(const ConfInfo& c, uint32_t packed, uint32_t& canon) {
 using namespace geq;
 
 geq_game::~geq_game() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (h_ctl) (void)hipHostFree(h_ctl);
 }
 
@@ -140,12 +142,14 @@ void geq_energies_free(geq_energies* e) {
     e->n_nodes = 0;
 }
 
-geq_status geq_game_upload(geq_ctx* ctx, const geq_graph* g, geq_game** out) {
-    if (!ctx || !g || !out) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
-    *out = nullptr;
-    ConfInfo ci;
-    geq_status st = check_conf(g->conf, ci);
-    if (st != GEQ_OK) return st;
+}  // extern "C"
+
+namespace {
+
+// Validate one graph and append it to the merged host arrays at node offset vb / edge offset eb.
+geq_status append_graph(const geq_graph* g, const ConfInfo& ci, uint32_t vb, uint64_t eb, std::vector<uint32_t>& row,
+                        std::vector<uint2>& edges, std::vector<uint32_t>& rrow, std::vector<uint32_t>& rcol,
+                        std::vector<uint8_t>& atk) {
     const uint32_t V = g->n_vertices, E = g->n_edges;
     if (V == 0) return fail(GEQ_ERR_INVALID_ARG, "empty game graph");
     if (!g->row_offsets || !g->attacker_pos || (E && (!g->column_indices || !g->weights)))
@@ -153,31 +157,67 @@ geq_status geq_game_upload(geq_ctx* ctx, const geq_graph* g, geq_game** out) {
     if (g->row_offsets[0] != 0 || g->row_offsets[V] != E) return fail(GEQ_ERR_INVALID_ARG, "row_offsets must start at 0 and end at n_edges");
     for (uint32_t v = 0; v < V; v++)
         if (g->row_offsets[v + 1] < g->row_offsets[v]) return fail(GEQ_ERR_INVALID_ARG, "row_offsets not monotone");
-    std::vector<uint2> edges(E);
+    for (uint32_t v = 0; v < V; v++) row[vb + v + 1] = (uint32_t)(eb + g->row_offsets[v + 1]);
     for (uint32_t k = 0; k < E; k++) {
         uint32_t c = g->column_indices[k];
         if (c >= V) return fail(GEQ_ERR_INVALID_ARG, "column index out of range at edge " + std::to_string(k));
         uint32_t w;
         if (!update_to_canon(ci, g->weights[k], w)) return fail(GEQ_ERR_INVALID_ARG, "invalid update at edge " + std::to_string(k));
-        edges[k] = make_uint2(c, w);
+        edges[eb + k] = make_uint2(vb + c, w);
     }
-    // reverse CSR: given, or derived as GameGraph::make_reverse does
-    std::vector<uint32_t> rrow(V + 1, 0), rcol(E);
+    for (uint32_t v = 0; v < V; v++) atk[vb + v] = g->attacker_pos[v] ? 1 : 0;
+    // reverse CSR: given, or derived as GameGraph::make_reverse does (:235-250)
     if (g->rev_row_offsets && g->rev_column_indices) {
         if (g->rev_row_offsets[0] != 0 || g->rev_row_offsets[V] != E) return fail(GEQ_ERR_INVALID_ARG, "rev_row_offsets malformed");
-        for (uint32_t v = 0; v <= V; v++) rrow[v] = g->rev_row_offsets[v];
         for (uint32_t v = 0; v < V; v++)
-            if (rrow[v + 1] < rrow[v]) return fail(GEQ_ERR_INVALID_ARG, "rev_row_offsets not monotone");
+            if (g->rev_row_offsets[v + 1] < g->rev_row_offsets[v]) return fail(GEQ_ERR_INVALID_ARG, "rev_row_offsets not monotone");
+        for (uint32_t v = 0; v < V; v++) rrow[vb + v + 1] = (uint32_t)(eb + g->rev_row_offsets[v + 1]);
         for (uint32_t k = 0; k < E; k++) {
-            rcol[k] = g->rev_column_indices[k];
-            if (rcol[k] >= V) return fail(GEQ_ERR_INVALID_ARG, "rev column index out of range");
+            uint32_t r = g->rev_column_indices[k];
+            if (r >= V) return fail(GEQ_ERR_INVALID_ARG, "rev column index out of range");
+            rcol[eb + k] = vb + r;
         }
     } else {
-        for (uint32_t k = 0; k < E; k++) rrow[g->column_indices[k] + 1]++;
-        for (uint32_t v = 0; v < V; v++) rrow[v + 1] += rrow[v];
-        std::vector<uint32_t> fill(rrow.begin(), rrow.end() - 1);
+        std::vector<uint32_t> cnt(V + 1, 0);
+        for (uint32_t k = 0; k < E; k++) cnt[g->column_indices[k] + 1]++;
+        for (uint32_t v = 0; v < V; v++) cnt[v + 1] += cnt[v];
+        for (uint32_t v = 0; v < V; v++) rrow[vb + v + 1] = (uint32_t)(eb + cnt[v + 1]);
+        std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
         for (uint32_t v = 0; v < V; v++)
-            for (uint32_t k = g->row_offsets[v]; k < g->row_offsets[v + 1]; k++) rcol[fill[g->column_indices[k]]++] = v;
+            for (uint32_t k = g->row_offsets[v]; k < g->row_offsets[v + 1]; k++) rcol[eb + fill[g->column_indices[k]]++] = vb + v;
+    }
+    return GEQ_OK;
+}
+
+geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs, geq_game** out) {
+    if (!ctx || !graphs || !out || n_games == 0) return fail(GEQ_ERR_INVALID_ARG, "NULL argument or no games");
+    *out = nullptr;
+    ConfInfo ci;
+    geq_status st = check_conf(graphs[0].conf, ci);
+    if (st != GEQ_OK) return st;
+    uint64_t Vt = 0, Et = 0;
+    std::vector<uint32_t> base(n_games + 1, 0);
+    for (uint32_t i = 0; i < n_games; i++) {
+        if (graphs[i].conf.elements != graphs[0].conf.elements || graphs[i].conf.max != graphs[0].conf.max)
+            return fail(GEQ_ERR_INVALID_ARG, "batched games must share one energy configuration");
+        base[i] = (uint32_t)Vt;
+        Vt += graphs[i].n_vertices;
+        Et += graphs[i].n_edges;
+        if (Vt >= 0xffffffffull || Et >= 0xffffffffull) return fail(GEQ_ERR_INVALID_ARG, "more than 2^32 positions or edges");
+    }
+    base[n_games] = (uint32_t)Vt;
+    const uint32_t V = (uint32_t)Vt, E = (uint32_t)Et;
+    std::vector<uint32_t> row(V + 1, 0), rrow(V + 1, 0), rcol(E);
+    std::vector<uint2> edges(E);
+    std::vector<uint8_t> atkv(V);
+    uint64_t eb = 0;
+    for (uint32_t i = 0; i < n_games; i++) {
+        st = append_graph(&graphs[i], ci, base[i], eb, row, edges, rrow, rcol, atkv);
+        if (st != GEQ_OK) {
+            if (n_games > 1) set_error("game " + std::to_string(i) + ": " + geq_last_error());
+            return st;
+        }
+        eb += graphs[i].n_edges;
     }
 
     (void)hipSetDevice(ctx->device);
@@ -186,7 +226,8 @@ geq_status geq_game_upload(geq_ctx* ctx, const geq_graph* g, geq_game** out) {
     gm->conf = ci;
     gm->V = V;
     gm->E = E;
-    gm->h_atk.assign(g->attacker_pos, g->attacker_pos + V);
+    gm->game_base = base;
+    gm->h_atk = std::move(atkv);
     auto oom = [&](hipError_t e, const char* what) -> geq_status {
         delete gm;
         return fail(e == hipErrorOutOfMemory ? GEQ_ERR_OUT_OF_MEMORY : GEQ_ERR_NO_DEVICE,
@@ -207,19 +248,37 @@ geq_status geq_game_upload(geq_ctx* ctx, const geq_graph* g, geq_game** out) {
     if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
     if ((e = gm->stats.alloc(4)) != hipSuccess) return oom(e, "stats");
     gm->arena_cap = std::max<uint32_t>(1u << 20, (uint32_t)std::min<uint64_t>(8ull * E + 1024, 0x7fffffffull));
+    if (const char* env = std::getenv("GEQ_ARENA_ENTRIES")) {
+        // initial energy-arena size (u16 entries); small values force the regrow path
+        long long v = std::atoll(env);
+        if (v >= 64 && v < 0x7fffffffll) gm->arena_cap = (uint32_t)v;
+    }
     if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
         return oom(e, "pinned control block");
     hipStream_t s = ctx->stream;
     auto up = [&](void* dst, const void* src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s); };
-    if (up(gm->row.p, g->row_offsets, 4ull * (V + 1)) != hipSuccess || up(gm->rrow.p, rrow.data(), 4ull * (V + 1)) != hipSuccess ||
+    if (up(gm->row.p, row.data(), 4ull * (V + 1)) != hipSuccess || up(gm->rrow.p, rrow.data(), 4ull * (V + 1)) != hipSuccess ||
         (E && up(gm->rcol.p, rcol.data(), 4ull * E) != hipSuccess) || (E && up(gm->edges.p, edges.data(), 8ull * E) != hipSuccess) ||
-        up(gm->atk.p, g->attacker_pos, V) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        up(gm->atk.p, gm->h_atk.data(), V) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
         delete gm;
         return fail(GEQ_ERR_TRANSFER, "uploading the game graph failed");
     }
     *out = gm;
     return GEQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+geq_status geq_game_upload(geq_ctx* ctx, const geq_graph* g, geq_game** out) {
+    if (!g) return fail(GEQ_ERR_INVALID_ARG, "graph is NULL");
+    return upload_games(ctx, 1, g, out);
+}
+
+geq_status geq_batch_upload(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs, geq_game** out) {
+    return upload_games(ctx, n_games, graphs, out);
 }
 
 void geq_game_free(geq_game* g) {
@@ -260,6 +319,46 @@ geq_status grow_arena(geq_game* g, uint32_t needed_top) {
 
 }  // namespace
 
+namespace {
+
+struct RoundShape {
+    uint32_t eval_blocks, expand_blocks, rounds_per_launch;
+};
+
+// Launch shape of one round.  Kernels use grid-stride loops over the device
+// worklist, so the shape only has to fill the chip: a few blocks per CU.
+RoundShape round_shape(const geq_game* g) {
+    const uint32_t ncu = (uint32_t)std::max(1, g->ctx->n_cu);
+    RoundShape r;
+    r.eval_blocks = std::max(1u, std::min(ncu * g->tune.eval_blocks_per_cu, (g->V + WAVES - 1) / WAVES));
+    r.expand_blocks = std::max(1u, std::min(ncu * g->tune.expand_blocks_per_cu, (g->V + 64u * WAVES - 1) / (64u * WAVES)));
+    r.rounds_per_launch = std::max(1u, g->tune.rounds_per_launch);
+    return r;
+}
+
+geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape) {
+    hipStream_t s = g->ctx->stream;
+    if (g->graph_exec) {
+        (void)hipGraphExecDestroy(g->graph_exec);
+        g->graph_exec = nullptr;
+    }
+    GEQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), GEQ_ERR_NO_DEVICE);
+    for (uint32_t k = 0; k < shape.rounds_per_launch; k++) {
+        k_eval<<<shape.eval_blocks, BLOCK, 0, s>>>(ea);
+        k_expand<<<shape.expand_blocks, BLOCK, 0, s>>>(xa);
+        k_advance<<<1, 64, 0, s>>>(g->ctl.p);
+    }
+    hipGraph_t graph = nullptr;
+    GEQ_HIP(hipStreamEndCapture(s, &graph), GEQ_ERR_NO_DEVICE);
+    hipError_t e = hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(GEQ_ERR_NO_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    g->graph_arena = g->arena.p;
+    return GEQ_OK;
+}
+
+}  // namespace
+
 extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint32_t n_reach, const geq_energies* initial,
                                      int profile_kernels, geq_stats* stats) {
     if (!g) return fail(GEQ_ERR_INVALID_ARG, "game is NULL");
@@ -295,8 +394,8 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
             } else {
                 uint32_t off = (uint32_t)h_arena.size();
                 h_arena.insert(h_arena.end(), vals.begin(), vals.end());
-                h[1] = (uint16_t)(off & 0xffff);
-                h[2] = (uint16_t)(off >> 16);
+                h[2] = (uint16_t)(off & 0xffff);   // second dword of the record = arena offset
+                h[3] = (uint16_t)(off >> 16);
             }
             std::memcpy(&h_cur[v], h, 16);
         }
@@ -328,64 +427,66 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
     std::vector<hipEvent_t> prof;   // 3 events per round: before eval, after eval, after expand
     GEQ_HIP(hipEventRecord(ev_start, s), GEQ_ERR_NO_DEVICE);
 
-    const uint32_t max_blocks = (uint32_t)std::max(1, ctx->n_cu) * 8u;
-    auto expand_grid = [&](uint32_t n) { return std::max(1u, std::min(max_blocks, (n + 64u * WAVES - 1) / (64u * WAVES))); };
-    auto eval_grid = [&](uint32_t n) { return std::max(1u, std::min(max_blocks, (n + WAVES - 1) / WAVES)); };
-    uint32_t* wl[2] = {g->wl0.p, g->wl1.p};
+    const RoundShape shape = round_shape(g);
+    EvalArgs ea{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->cur.p, g->nxt.p, g->arena.p, g->ctl.p,
+                g->arena_cap, g->chg.p, g->conf.max, g->stats.p};
+    ExpandArgs xa{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
 
     // seed: predecessors of the nodes to reach (initialize_visit_lists, :1461-1465)
     if (n_reach) {
-        ExpandArgs x{g->seed.p, n_reach, nullptr, g->cur.p, nullptr, g->rrow.p, g->rcol.p, g->stamp.p, 1u, wl[0], g->ctl.p, 0u};
-        k_expand<<<expand_grid(n_reach), BLOCK, 0, s>>>(x);
+        ExpandArgs sx = xa;
+        sx.seed = g->seed.p;
+        sx.n_seed = n_reach;
+        k_expand<<<std::max(1u, std::min(shape.expand_blocks, (n_reach + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx);
     }
     GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
 
-    uint64_t rounds = 0, launches = 0;
-    uint32_t r = 0;
+    uint64_t launches = 0;
     geq_status result = GEQ_OK;
-    while (true) {
-        const uint32_t n = g->h_ctl->wl_n[r & 1];
-        if (n == 0) break;
-        const uint32_t nx = (r + 1) & 1;
-        GEQ_HIP(hipMemsetAsync(&g->ctl.p->wl_n[nx], 0, 4, s), GEQ_ERR_TRANSFER);
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    while (g->h_ctl->wl_n[g->h_ctl->round & 1u] != 0) {
         if (profile_kernels) {
+            // one round per launch, HIP events around every kernel
+            hipEvent_t e0, e1, e2;
             GEQ_HIP(hipEventCreate(&e0), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventCreate(&e1), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventCreate(&e2), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventRecord(e0, s), GEQ_ERR_NO_DEVICE);
-        }
-        EvalArgs ea{wl[r & 1], n, g->row.p, g->edges.p, g->atk.p, g->cur.p, g->nxt.p, g->arena.p, g->ctl.p,
-                    g->arena_cap, g->chg.p, g->conf.max, g->stats.p};
-        k_eval<<<eval_grid(n), BLOCK, 0, s>>>(ea);
-        launches++;
-        if (profile_kernels) GEQ_HIP(hipEventRecord(e1, s), GEQ_ERR_NO_DEVICE);
-        ExpandArgs x{wl[r & 1], n, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, r + 2u, wl[nx], g->ctl.p, nx};
-        k_expand<<<expand_grid(n), BLOCK, 0, s>>>(x);
-        if (profile_kernels) {
+            k_eval<<<shape.eval_blocks, BLOCK, 0, s>>>(ea);
+            GEQ_HIP(hipEventRecord(e1, s), GEQ_ERR_NO_DEVICE);
+            k_expand<<<shape.expand_blocks, BLOCK, 0, s>>>(xa);
             GEQ_HIP(hipEventRecord(e2, s), GEQ_ERR_NO_DEVICE);
+            k_advance<<<1, 64, 0, s>>>(g->ctl.p);
             prof.push_back(e0);
             prof.push_back(e1);
             prof.push_back(e2);
+            launches++;
+        } else {
+            // rounds_per_launch captured rounds per graph launch
+            if (!g->graph_exec || g->graph_arena != g->arena.p) {
+                result = capture_rounds(g, ea, xa, shape);
+                if (result != GEQ_OK) break;
+            }
+            GEQ_HIP(hipGraphLaunch(g->graph_exec, s), GEQ_ERR_NO_DEVICE);
+            launches += shape.rounds_per_launch;
         }
         GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
         if (g->h_ctl->overflow) {
-            // k_expand did nothing; grow the arena and evaluate the same round again
+            // k_expand / k_advance did nothing: grow the arena, then the same round runs again
             result = grow_arena(g, g->h_ctl->arena_top);
             if (result != GEQ_OK) break;
             Ctl fix = *g->h_ctl;
             fix.overflow = 0;
-            fix.wl_n[nx] = 0;
+            fix.wl_n[(fix.round + 1u) & 1u] = 0;
             GEQ_HIP(hipMemcpyAsync(g->ctl.p, &fix, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
             GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
             *g->h_ctl = fix;
-            continue;
+            ea.arena = g->arena.p;
+            ea.arena_cap = g->arena_cap;
         }
-        rounds++;
-        r++;
     }
+    const uint64_t rounds = g->h_ctl->rounds_done;
     GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
     GEQ_HIP(hipEventSynchronize(ev_end), GEQ_ERR_NO_DEVICE);
     float ms = 0.f;
@@ -475,4 +576,45 @@ extern "C" geq_status geq_solve(geq_ctx* ctx, const geq_graph* graph, const uint
     if (st == GEQ_OK) st = geq_game_energies(g, out);
     geq_game_free(g);
     return st;
+}
+
+extern "C" geq_status geq_batch_solve(geq_game* b, const uint32_t* const* to_reach, const uint32_t* n_reach, int profile_kernels,
+                                      geq_stats* stats) {
+    if (!b || !n_reach) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    const uint32_t n_games = (uint32_t)b->game_base.size() - 1;
+    std::vector<uint32_t> reach;
+    for (uint32_t i = 0; i < n_games; i++) {
+        if (n_reach[i] && (!to_reach || !to_reach[i])) return fail(GEQ_ERR_INVALID_ARG, "to_reach list is NULL");
+        const uint32_t Vi = b->game_base[i + 1] - b->game_base[i];
+        for (uint32_t k = 0; k < n_reach[i]; k++) {
+            if (to_reach[i][k] >= Vi) return fail(GEQ_ERR_INVALID_ARG, "to_reach id out of range in game " + std::to_string(i));
+            reach.push_back(b->game_base[i] + to_reach[i][k]);
+        }
+    }
+    return geq_game_solve(b, reach.data(), (uint32_t)reach.size(), nullptr, profile_kernels, stats);
+}
+
+extern "C" geq_status geq_batch_energies(geq_game* b, geq_energies* outs) {
+    if (!b || !outs) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    geq_energies all{};
+    geq_status st = geq_game_energies(b, &all);
+    if (st != GEQ_OK) return st;
+    const uint32_t n_games = (uint32_t)b->game_base.size() - 1;
+    for (uint32_t i = 0; i < n_games; i++) {
+        const uint32_t v0 = b->game_base[i], v1 = b->game_base[i + 1], n = v1 - v0;
+        const uint32_t e0 = all.offsets[v0], e1 = all.offsets[v1];
+        outs[i].n_nodes = n;
+        outs[i].energy_size = 1;
+        outs[i].offsets = (uint32_t*)std::malloc(4ull * (n + 1));
+        outs[i].data = (uint32_t*)std::malloc(4ull * std::max<uint32_t>(1, e1 - e0));
+        if (!outs[i].offsets || !outs[i].data) {
+            for (uint32_t j = 0; j <= i; j++) geq_energies_free(&outs[j]);
+            geq_energies_free(&all);
+            return fail(GEQ_ERR_OUT_OF_MEMORY, "host allocation of batch results");
+        }
+        for (uint32_t v = 0; v <= n; v++) outs[i].offsets[v] = all.offsets[v0 + v] - e0;
+        if (e1 > e0) std::memcpy(outs[i].data, all.data + e0, 4ull * (e1 - e0));
+    }
+    geq_energies_free(&all);
+    return GEQ_OK;
 }

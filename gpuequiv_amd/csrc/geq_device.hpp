@@ -23,7 +23,8 @@ namespace geq {
 // Per-node energy list descriptor: 16 bytes.
 //   h[0]: bits 0..14 = number of energies, bit 15 = list is not sorted
 //   len <= 7 : h[1..7] hold the energies inline
-//   len  > 7 : h[1] | h[2] << 16 = offset into the energy arena (u16 units)
+//   len  > 7 : the record's second dword (h[2] | h[3] << 16) is the offset
+//              into the energy arena (u16 units)
 // ---------------------------------------------------------------------------
 constexpr uint32_t INLINE_CAP = 7;
 constexpr uint32_t LEN_MASK = 0x7fffu;
@@ -34,7 +35,7 @@ struct __align__(16) NodeList {
 };
 
 __device__ __forceinline__ uint32_t nl_len(const uint4& d) { return d.x & LEN_MASK; }
-__device__ __forceinline__ uint32_t nl_off(const uint4& d) { return d.y; }  // h[1] | h[2] << 16
+__device__ __forceinline__ uint32_t nl_off(const uint4& d) { return d.y; }  // h[2] | h[3] << 16
 // inline energy j (0..6) of a descriptor
 __device__ __forceinline__ uint32_t nl_inline(const uint4& d, uint32_t j) {
     // energies sit at u16 index 1 + j of the 16-byte record

@@ -58,8 +58,11 @@ __global__ __launch_bounds__(BLOCK) void k_eval(EvalArgs a) {
     __syncthreads();
     uint64_t st_evals = 0, st_edges = 0, st_bytes_lane = 0, st_bytes_node = 0, st_changed = 0;
     const uint32_t gstride = gridDim.x * WAVES;
-    for (uint32_t idx = uniform(blockIdx.x * WAVES + wid); idx < a.n; idx += gstride) {
-        const uint32_t u = uniform(a.wl[idx]);
+    const uint32_t slot = uniform(a.ctl->round) & 1u;
+    const uint32_t n = a.ctl->overflow ? 0u : uniform(a.ctl->wl_n[slot]);
+    const uint32_t* __restrict__ wl = a.wl[slot];
+    for (uint32_t idx = uniform(blockIdx.x * WAVES + wid); idx < n; idx += gstride) {
+        const uint32_t u = uniform(wl[idx]);
         const uint32_t r0 = uniform(a.row[u]);
         const uint32_t deg = uniform(a.row[u + 1]) - r0;
         const bool isatk = uniform(a.atk[u]) != 0;
@@ -229,27 +232,34 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = threadIdx.x >> 6;
     uint32_t* stage = s_stage[wid];
-    if (a.ctl->overflow) return;   // the round is re-run after the arena grows
+    if (a.ctl->overflow) return;   // the round is evaluated again after the arena grows
+    const uint32_t r = uniform(a.ctl->round);
+    const bool seeding = a.seed != nullptr;
+    const uint32_t* list = seeding ? a.seed : a.wl[r & 1u];
+    const uint32_t n = seeding ? a.n_seed : uniform(a.ctl->wl_n[r & 1u]);
+    const uint32_t out_slot = seeding ? (r & 1u) : ((r + 1u) & 1u);
+    uint32_t* out = a.wl[out_slot];
+    const uint32_t stamp_val = seeding ? r + 1u : r + 2u;
     uint32_t cnt = 0;
     auto flush = [&]() {
         wave_sync();
         uint32_t base = 0;
-        if (lane == 0 && cnt) base = atomicAdd(&a.ctl->wl_n[a.out_slot], cnt);
+        if (lane == 0 && cnt) base = atomicAdd(&a.ctl->wl_n[out_slot], cnt);
         base = uniform(shfl32(base, 0));
-        for (uint32_t j = lane; j < cnt; j += 64) a.out[base + j] = stage[j];
+        for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
         cnt = 0;
         wave_sync();
     };
     const uint32_t gstride = gridDim.x * WAVES * 64;
-    for (uint32_t b = (blockIdx.x * WAVES + wid) * 64; b < a.n; b += gstride) {
+    for (uint32_t b = (blockIdx.x * WAVES + wid) * 64; b < n; b += gstride) {
         const uint32_t i = b + lane;
         uint32_t u = 0;
         bool f = false;
-        if (i < a.n) {
-            u = a.list[i];
-            f = a.chg ? (a.chg[i] != 0) : true;
+        if (i < n) {
+            u = list[i];
+            f = seeding ? true : (a.chg[i] != 0);
         }
-        if (f && a.nxt) a.cur[u] = a.nxt[u];
+        if (f && !seeding) a.cur[u] = a.nxt[u];
         uint64_t mask = __ballot(f);
         while (mask) {
             const uint32_t s = (uint32_t)__builtin_ctzll(mask);
@@ -262,7 +272,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                 uint32_t w = 0;
                 if (k < r1) {
                     w = a.rcol[k];
-                    win = atomicExch(&a.stamp[w], a.stamp_val) != a.stamp_val;
+                    win = atomicExch(&a.stamp[w], stamp_val) != stamp_val;
                 }
                 const uint64_t bal = __ballot(win);
                 if (win) stage[cnt + mbcnt(bal)] = w;
@@ -272,6 +282,16 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
         }
     }
     flush();
+}
+
+// End of a round: the consumed worklist counter is cleared and the round
+// advances (not after an arena overflow: then the same round runs again).
+__global__ void k_advance(Ctl* ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->overflow) return;
+    const uint32_t r = ctl->round;
+    if (ctl->wl_n[r & 1u]) ctl->rounds_done++;
+    ctl->wl_n[r & 1u] = 0;
+    ctl->round = r + 1;
 }
 
 __global__ void k_list_lens(const uint4* cur, uint32_t n, uint32_t* lens) {

@@ -13,15 +13,19 @@ namespace geq {
 constexpr int WAVES = 4;            // waves per block of the eval/expand kernels
 constexpr int BLOCK = 64 * WAVES;
 
-struct Ctl {                        // small device control block, read back once per round
-    uint32_t wl_n[2];               // worklist sizes (ping-pong)
+struct Ctl {                        // small device control block, read back between round batches
+    uint32_t wl_n[2];               // worklist sizes (ping-pong by round parity)
     uint32_t overflow;              // energy arena exhausted during k_eval
     uint32_t arena_top;             // bump pointer of the energy arena (u16 units)
+    uint32_t round;                 // current round; its worklist is wl[round & 1]
+    uint32_t rounds_done;           // rounds that evaluated at least one node
+    uint32_t pad[2];
 };
 
+// Kernels read the round's worklist and size from Ctl, so a round needs no
+// host-side state and can be captured once in a hipGraph and replayed.
 struct EvalArgs {
-    const uint32_t* wl;
-    uint32_t n;
+    const uint32_t* wl[2];
     const uint32_t* row;
     const uint2* edges;             // (column index, packed update) per edge
     const uint8_t* atk;
@@ -36,22 +40,21 @@ struct EvalArgs {
 };
 
 struct ExpandArgs {
-    const uint32_t* list;           // worklist of the round (or the seed list)
-    uint32_t n;
-    const uint8_t* chg;             // nullptr: every entry counts as changed
+    uint32_t* wl[2];
+    const uint32_t* seed;           // seeding mode: these nodes count as changed, no commit
+    uint32_t n_seed;
+    const uint8_t* chg;
     uint4* cur;
-    const uint4* nxt;               // nullptr: no commit (seeding)
+    const uint4* nxt;
     const uint32_t* rrow;
     const uint32_t* rcol;
     uint32_t* stamp;
-    uint32_t stamp_val;
-    uint32_t* out;
     Ctl* ctl;
-    uint32_t out_slot;              // which ctl->wl_n counter receives the appends
 };
 
 __global__ void k_eval(EvalArgs a);
 __global__ void k_expand(ExpandArgs a);
+__global__ void k_advance(Ctl* ctl);
 __global__ void k_list_lens(const uint4* cur, uint32_t n, uint32_t* lens);
 __global__ void k_gather(const uint4* cur, const uint16_t* arena, uint32_t n, const uint32_t* offs, uint32_t* out);
 

@@ -122,6 +122,20 @@ geq_status geq_game_solve(geq_game* game, const uint32_t* to_reach, uint32_t n_r
 geq_status geq_game_energies(geq_game* game, geq_energies* out);
 void geq_game_free(geq_game* game);
 
+/* ---- batched games (many independent games, one device-resident solve) ----
+ * The games (same EnergyConf) are laid out back to back in HBM; one shared
+ * device worklist holds the positions of every game, so each round evaluates
+ * the frontier of all games together.  Games cannot influence each other and
+ * every round is evaluated per node against the previous round's state, so
+ * each game's result is exactly its standalone result.  There is no
+ * counterpart in the reference, which solves one game per EnergyGame::run. */
+geq_status geq_batch_upload(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs, geq_game** out);
+/* reach lists per game (node ids local to the game) */
+geq_status geq_batch_solve(geq_game* batch, const uint32_t* const* to_reach, const uint32_t* n_reach, int profile_kernels,
+                           geq_stats* stats);
+/* outs: n_games results, each freed with geq_energies_free */
+geq_status geq_batch_energies(geq_game* batch, geq_energies* outs);
+
 /* ---- helpers (no device needed) ---- */
 /* standard_reach: defender positions without successors (:347-352). Writes at
  * most n_vertices ids into out, returns the count through *n_out. */

@@ -388,8 +388,11 @@ static void solve(const Ctx& X, uint32_t V, const uint32_t* row, const uint32_t*
             account(atk_jobs[i], E[atk_jobs[i]], ar[i].changed ? &ar[i].list : nullptr);
             st.atk_evals++;
         }
+        // the new list counts only when it is written (the node changed)
+        std::vector<char> def_changed(def_jobs.size(), 0);
         for (size_t i = 0; i < def_jobs.size(); i++) {
-            account(def_jobs[i].first, E[def_jobs[i].first], dr[i].status >= 0 ? &dr[i].list : nullptr);
+            def_changed[i] = dr[i].status >= 0 && !lists_equal_ref(X, dr[i].list, E[def_jobs[i].first]);
+            account(def_jobs[i].first, E[def_jobs[i].first], def_changed[i] ? &dr[i].list : nullptr);
             st.def_evals++;
         }
         st.rounds++;
@@ -407,7 +410,7 @@ static void solve(const Ctx& X, uint32_t V, const uint32_t* row, const uint32_t*
                 st.def_retries++;
                 continue;
             }
-            if (!lists_equal_ref(X, dr[i].list, E[node])) {
+            if (def_changed[i]) {
                 E[node] = std::move(dr[i].list);
                 changed.push_back(node);
             }

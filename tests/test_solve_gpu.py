@@ -161,3 +161,46 @@ def test_small_confs_match_oracle():
     offs, data, _ = H.oracle_solve_game(g, reach)
     _, out = _run(g)
     assert _canon(out) == H.canon_lists(offs, data)
+
+
+def _solve_raw(graph, profile):
+    import ctypes as C
+    from gpuequiv_amd import _lib
+    lib = _lib.lib()
+    ctx = _lib.Context.get()
+    keep = []
+    gs = graph._c_struct(keep)
+    h = C.c_void_p()
+    _lib.check(lib.geq_game_upload(ctx.handle, C.byref(gs), C.byref(h)))
+    try:
+        reach = np.ascontiguousarray(H.standard_reach(graph), dtype=np.uint32)
+        st = _lib.geq_stats()
+        _lib.check(lib.geq_game_solve(h, _lib.u32p(reach), len(reach), None, profile, C.byref(st)))
+        out = _lib.geq_energies()
+        _lib.check(lib.geq_game_energies(h, C.byref(out)))
+        try:
+            offs, data = _lib.energies_to_numpy(out)
+        finally:
+            lib.geq_energies_free(C.byref(out))
+        return H.canon_lists(offs, data), st.as_dict()
+    finally:
+        lib.geq_game_free(h)
+
+
+def test_profiled_launches_match_graph_replay():
+    g = workloads.random_game(20000, 17)
+    a, sa = _solve_raw(g, 0)
+    b, sb = _solve_raw(g, 1)
+    assert a == b
+    assert sa["rounds"] == sb["rounds"] and sa["edge_updates"] == sb["edge_updates"]
+    assert sb["eval_kernel_ms"] > 0
+
+
+def test_arena_regrow_gives_identical_results(monkeypatch):
+    # a tiny energy arena forces k_eval to overflow and the round to be re-run
+    g = workloads.random_game(3000, 8, mean_degree=8.0, p_ident=0.2, p_dec=0.6)
+    ref, sref = _solve_raw(g, 0)
+    monkeypatch.setenv("GEQ_ARENA_ENTRIES", "64")
+    small, ssmall = _solve_raw(g, 0)
+    assert small == ref
+    assert ssmall["rounds"] == sref["rounds"]
