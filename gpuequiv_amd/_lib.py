@@ -66,6 +66,9 @@ class geq_stats(C.Structure):
         ("eval_kernel_ms", C.c_double),
         ("expand_kernel_ms", C.c_double),
         ("eval_launches", C.c_uint64),
+        ("solve_kernel_ms", C.c_double),
+        ("mode", C.c_uint32),
+        ("pad", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

@@ -246,7 +246,8 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     if ((e = gm->chg.alloc(V)) != hipSuccess) return oom(e, "changed flags");
     if ((e = gm->stamp.alloc(V)) != hipSuccess) return oom(e, "stamps");
     if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
-    if ((e = gm->stats.alloc(4)) != hipSuccess) return oom(e, "stats");
+    if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
+    if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
     gm->arena_cap = std::max<uint32_t>(1u << 20, (uint32_t)std::min<uint64_t>(8ull * E + 1024, 0x7fffffffull));
     if (const char* env = std::getenv("GEQ_ARENA_ENTRIES")) {
         // initial energy-arena size (u16 entries); small values force the regrow path
@@ -322,7 +323,7 @@ geq_status grow_arena(geq_game* g, uint32_t needed_top) {
 namespace {
 
 struct RoundShape {
-    uint32_t eval_blocks, expand_blocks, rounds_per_launch;
+    uint32_t eval_blocks, expand_blocks, rounds_per_launch, coop_blocks;
 };
 
 // Launch shape of one round.  Kernels use grid-stride loops over the device
@@ -333,6 +334,14 @@ RoundShape round_shape(const geq_game* g) {
     r.eval_blocks = std::max(1u, std::min(ncu * g->tune.eval_blocks_per_cu, (g->V + WAVES - 1) / WAVES));
     r.expand_blocks = std::max(1u, std::min(ncu * g->tune.expand_blocks_per_cu, (g->V + 64u * WAVES - 1) / (64u * WAVES)));
     r.rounds_per_launch = std::max(1u, g->tune.rounds_per_launch);
+    r.eval_blocks = std::min(r.eval_blocks, STAT_SLOTS);
+    // cooperative launch: every block must be resident at once
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_solve), BLOCK, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    r.coop_blocks = std::min<uint32_t>(std::min<uint32_t>((uint32_t)per_cu, g->tune.coop_blocks_per_cu) * ncu, STAT_SLOTS);
+    r.coop_blocks = std::max(1u, std::min(r.coop_blocks, (g->V + WAVES - 1) / WAVES));
     return r;
 }
 
@@ -419,12 +428,13 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
     Ctl c0{};
     c0.arena_top = (uint32_t)h_arena.size();
     GEQ_HIP(hipMemcpyAsync(g->ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    GEQ_HIP(hipMemsetAsync(g->stats.p, 0, 4 * sizeof(unsigned long long), s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipMemsetAsync(g->stats.p, 0, (4ull * STAT_SLOTS + 4) * sizeof(unsigned long long), s), GEQ_ERR_TRANSFER);
 
     hipEvent_t ev_start, ev_end;
     GEQ_HIP(hipEventCreate(&ev_start), GEQ_ERR_NO_DEVICE);
     GEQ_HIP(hipEventCreate(&ev_end), GEQ_ERR_NO_DEVICE);
     std::vector<hipEvent_t> prof;   // 3 events per round: before eval, after eval, after expand
+    std::vector<hipEvent_t> coop_events;   // around every cooperative k_solve launch
     GEQ_HIP(hipEventRecord(ev_start, s), GEQ_ERR_NO_DEVICE);
 
     const RoundShape shape = round_shape(g);
@@ -461,6 +471,30 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
             prof.push_back(e1);
             prof.push_back(e2);
             launches++;
+        } else if (g->tune.mode == 0 && !std::getenv("GEQ_ENABLE_COOPERATIVE")) {
+            // The cooperative one-launch solve is switched off: its grid of
+            // min(occupancy API, 8) blocks per CU can exceed the 7 waves/SIMD
+            // that k_solve's 106 SGPRs allow, so its grid barrier can wait for
+            // blocks that are never resident.
+            result = fail(GEQ_ERR_UNSUPPORTED, "the cooperative solve mode (GEQ_SOLVE_MODE=0) is disabled");
+            break;
+        } else if (g->tune.mode == 0) {
+            // the whole fixpoint in one cooperative launch (returns early only
+            // on an arena overflow or after max_rounds rounds)
+            uint32_t max_rounds = 1u << 20;
+            uint32_t spin_limit = 1u << 25;   // ~ seconds of polling before a barrier gives up
+            void* args[] = {&ea, &xa, &max_rounds, &spin_limit};
+            hipEvent_t k0, k1;
+            GEQ_HIP(hipEventCreate(&k0), GEQ_ERR_NO_DEVICE);
+            GEQ_HIP(hipEventCreate(&k1), GEQ_ERR_NO_DEVICE);
+            GEQ_HIP(hipEventRecord(k0, s), GEQ_ERR_NO_DEVICE);
+            GEQ_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve), dim3(shape.coop_blocks), dim3(BLOCK),
+                                               args, 0, s),
+                    GEQ_ERR_NO_DEVICE);
+            GEQ_HIP(hipEventRecord(k1, s), GEQ_ERR_NO_DEVICE);
+            coop_events.push_back(k0);
+            coop_events.push_back(k1);
+            launches++;
         } else {
             // rounds_per_launch captured rounds per graph launch
             if (!g->graph_exec || g->graph_arena != g->arena.p) {
@@ -472,6 +506,10 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
         }
         GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+        if (g->h_ctl->error) {
+            result = fail(GEQ_ERR_INTERNAL, "k_solve: a grid barrier timed out (blocks not co-resident?)");
+            break;
+        }
         if (g->h_ctl->overflow) {
             // k_expand / k_advance did nothing: grow the arena, then the same round runs again
             result = grow_arena(g, g->h_ctl->arena_top);
@@ -499,12 +537,22 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
         eval_ms += a;
         expand_ms += b;
     }
+    double coop_ms = 0;
+    for (size_t i = 0; i + 1 < coop_events.size(); i += 2) {
+        float a = 0;
+        (void)hipEventElapsedTime(&a, coop_events[i], coop_events[i + 1]);
+        coop_ms += a;
+    }
+    for (hipEvent_t ev : coop_events) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : prof) (void)hipEventDestroy(ev);
     (void)hipEventDestroy(ev_start);
     (void)hipEventDestroy(ev_end);
     if (result != GEQ_OK) return result;
     unsigned long long h_stats[4] = {0, 0, 0, 0};
-    GEQ_HIP(hipMemcpy(h_stats, g->stats.p, sizeof(h_stats), hipMemcpyDeviceToHost), GEQ_ERR_TRANSFER);
+    k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, std::max(shape.eval_blocks, shape.coop_blocks),
+                                     g->stats.p + 4ull * STAT_SLOTS);
+    GEQ_HIP(hipMemcpyAsync(h_stats, g->stats.p + 4ull * STAT_SLOTS, sizeof(h_stats), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->rounds = rounds;
@@ -516,6 +564,8 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
         stats->eval_kernel_ms = eval_ms;
         stats->expand_kernel_ms = expand_ms;
         stats->eval_launches = launches;
+        stats->solve_kernel_ms = coop_ms;
+        stats->mode = profile_kernels ? 2u : g->tune.mode;
     }
     return GEQ_OK;
 }

@@ -73,9 +73,11 @@ struct geq_game {
     uint32_t arena_cap = 0;
     std::vector<uint8_t> h_atk;       // kept for standard_reach / reach validation
     struct Tune {                     // launch shape, set by the autotuner or defaults
+        uint32_t mode = 1;            // 0: one cooperative launch per solve, 1: hipGraph of captured rounds
         uint32_t eval_blocks_per_cu = 8;
         uint32_t expand_blocks_per_cu = 4;
         uint32_t rounds_per_launch = 4;
+        uint32_t coop_blocks_per_cu = 8;
     } tune;
     std::vector<uint32_t> game_base;      // batched games: first node of every game (+ total)
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds

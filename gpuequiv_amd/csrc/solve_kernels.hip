@@ -43,8 +43,10 @@ __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint
     return total;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_eval(EvalArgs a) {
-    __shared__ uint32_t s_bm[WAVES][3][128];
+// One round's evaluations: every wave takes worklist entries (grid-stride) and
+// evaluates one node at a time.  Statistics accumulate in `acc`.
+__device__ __forceinline__ void eval_phase(const EvalArgs& a, uint32_t (*s_bm)[3][128], StatAcc& acc, uint32_t r,
+                                           uint32_t n) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = threadIdx.x >> 6;
     uint32_t* A = s_bm[wid][0];   // union (attacker) bitmap
@@ -53,14 +55,9 @@ __global__ __launch_bounds__(BLOCK) void k_eval(EvalArgs a) {
     const uint32_t maxv = a.maxv;
     const uint16_t* __restrict__ arena = a.arena;
 
-    __shared__ unsigned long long s_stats[4];
-    if (threadIdx.x < 4) s_stats[threadIdx.x] = 0ull;
-    __syncthreads();
-    uint64_t st_evals = 0, st_edges = 0, st_bytes_lane = 0, st_bytes_node = 0, st_changed = 0;
+    uint64_t st_bytes_lane = 0;
     const uint32_t gstride = gridDim.x * WAVES;
-    const uint32_t slot = uniform(a.ctl->round) & 1u;
-    const uint32_t n = a.ctl->overflow ? 0u : uniform(a.ctl->wl_n[slot]);
-    const uint32_t* __restrict__ wl = a.wl[slot];
+    const uint32_t* __restrict__ wl = a.wl[r & 1u];
     for (uint32_t idx = uniform(blockIdx.x * WAVES + wid); idx < n; idx += gstride) {
         const uint32_t u = uniform(wl[idx]);
         const uint32_t r0 = uniform(a.row[u]);
@@ -199,57 +196,65 @@ __global__ __launch_bounds__(BLOCK) void k_eval(EvalArgs a) {
                 if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
             }
         }
-        if (new_len == ~0u) changed = false;   // arena overflow: the host re-runs the round
+        if (new_len == ~0u) changed = false;   // arena overflow: the round runs again
         if (lane == 0) {
             a.chg[idx] = changed ? 1 : 0;
             if (evaluated) {
-                st_evals++;
-                st_edges += deg;
-                st_bytes_node += 9u + 4u * own_len + (changed ? 4u * new_len : 0u);
-                st_changed += changed ? 1 : 0;
+                acc.evals++;
+                acc.edges += deg;
+                acc.bytes += 9u + 4u * own_len + (changed ? 4u * new_len : 0u);
+                acc.changed += changed ? 1 : 0;
             }
         }
+        if (evaluated) acc.bytes += st_bytes_lane;
+        st_bytes_lane = 0;
         wave_sync();
     }
-    // statistics: LDS reduction, one global atomic per counter and block
-    if (st_bytes_lane) atomicAdd(&s_stats[2], (unsigned long long)st_bytes_lane);
-    if (lane == 0) {
-        atomicAdd(&s_stats[0], (unsigned long long)st_evals);
-        atomicAdd(&s_stats[1], (unsigned long long)st_edges);
-        atomicAdd(&s_stats[2], (unsigned long long)st_bytes_node);
-        atomicAdd(&s_stats[3], (unsigned long long)st_changed);
+}
+
+// Per-block statistics slot (no cross-block atomics): slot b is only ever
+// written by block b of the fixed-size grid, rounds are stream-ordered.
+__device__ __forceinline__ void flush_stats(const StatAcc& acc, unsigned long long* slots) {
+    __shared__ unsigned long long s_red[4];
+    if (threadIdx.x < 4) s_red[threadIdx.x] = 0ull;
+    __syncthreads();
+    if (acc.bytes) atomicAdd(&s_red[2], (unsigned long long)acc.bytes);
+    if ((threadIdx.x & 63u) == 0) {
+        if (acc.evals) atomicAdd(&s_red[0], (unsigned long long)acc.evals);
+        if (acc.edges) atomicAdd(&s_red[1], (unsigned long long)acc.edges);
+        if (acc.changed) atomicAdd(&s_red[3], (unsigned long long)acc.changed);
     }
     __syncthreads();
-    if (threadIdx.x < 4 && s_stats[threadIdx.x]) atomicAdd(&a.stats[threadIdx.x], s_stats[threadIdx.x]);
+    if (threadIdx.x < 4 && s_red[threadIdx.x]) slots[4ull * blockIdx.x + threadIdx.x] += s_red[threadIdx.x];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_eval(EvalArgs a) {
+    __shared__ uint32_t s_bm[WAVES][3][128];
+    StatAcc acc;
+    const uint32_t r = uniform(a.ctl->round);
+    const uint32_t n = a.ctl->overflow ? 0u : uniform(a.ctl->wl_n[r & 1u]);
+    eval_phase(a, s_bm, acc, r, n);
+    flush_stats(acc, a.stats);
 }
 
 // Commit changed nodes (cur <- nxt) and append their not-yet-queued
 // predecessors to the next worklist (GPURunner::changed_nodes,
-// src/energygame.rs:1467-1479).  Appends are staged per wave in LDS and
-// flushed with one atomic per 192+ entries.
-__global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
-    __shared__ uint32_t s_stage[WAVES][256];
+// src/energygame.rs:1467-1479).  A predecessor is queued once per round: the
+// first wave to swap its stamp to the round's value appends it.  Appends are
+// staged per wave in LDS; full stages are flushed by the wave, the rest once
+// per block with a single atomic.
+// r: the round; n: its worklist size (the seed count when seeding), 0 after an overflow
+__device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_stage)[256], uint32_t* s_cnt, uint32_t r,
+                                             uint32_t n) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = threadIdx.x >> 6;
     uint32_t* stage = s_stage[wid];
-    if (a.ctl->overflow) return;   // the round is evaluated again after the arena grows
-    const uint32_t r = uniform(a.ctl->round);
     const bool seeding = a.seed != nullptr;
     const uint32_t* list = seeding ? a.seed : a.wl[r & 1u];
-    const uint32_t n = seeding ? a.n_seed : uniform(a.ctl->wl_n[r & 1u]);
     const uint32_t out_slot = seeding ? (r & 1u) : ((r + 1u) & 1u);
     uint32_t* out = a.wl[out_slot];
     const uint32_t stamp_val = seeding ? r + 1u : r + 2u;
     uint32_t cnt = 0;
-    auto flush = [&]() {
-        wave_sync();
-        uint32_t base = 0;
-        if (lane == 0 && cnt) base = atomicAdd(&a.ctl->wl_n[out_slot], cnt);
-        base = uniform(shfl32(base, 0));
-        for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
-        cnt = 0;
-        wave_sync();
-    };
     const uint32_t gstride = gridDim.x * WAVES * 64;
     for (uint32_t b = (blockIdx.x * WAVES + wid) * 64; b < n; b += gstride) {
         const uint32_t i = b + lane;
@@ -277,11 +282,39 @@ __global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
                 const uint64_t bal = __ballot(win);
                 if (win) stage[cnt + mbcnt(bal)] = w;
                 cnt += (uint32_t)__popcll(bal);
-                if (cnt > 192) flush();
+                if (cnt > 192) {
+                    wave_sync();
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&a.ctl->wl_n[out_slot], cnt);
+                    base = uniform(shfl32(base, 0));
+                    for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
+                    cnt = 0;
+                    wave_sync();
+                }
             }
         }
     }
-    flush();
+    // block-level flush: one atomic for the whole block
+    if (lane == 0) s_cnt[wid] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < WAVES; w++) tot += s_cnt[w];
+        s_cnt[WAVES] = tot ? atomicAdd(&a.ctl->wl_n[out_slot], tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = s_cnt[WAVES];
+    for (uint32_t w = 0; w < wid; w++) base += s_cnt[w];
+    for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
+    __shared__ uint32_t s_stage[WAVES][256];
+    __shared__ uint32_t s_cnt[WAVES + 1];
+    const uint32_t r = uniform(a.ctl->round);
+    const uint32_t n = a.ctl->overflow ? 0u : (a.seed ? a.n_seed : uniform(a.ctl->wl_n[r & 1u]));
+    expand_phase(a, s_stage, s_cnt, r, n);
 }
 
 // End of a round: the consumed worklist counter is cleared and the round
@@ -292,6 +325,105 @@ __global__ void k_advance(Ctl* ctl) {
     if (ctl->wl_n[r & 1u]) ctl->rounds_done++;
     ctl->wl_n[r & 1u] = 0;
     ctl->round = r + 1;
+}
+
+// Grid-wide barrier of the cooperative k_solve launch (every block is resident).
+// Release / acquire fences at agent scope order all global memory traffic of
+// the blocks across the barrier (per-CU L1s and per-XCD L2s are not coherent
+// for plain accesses).  The wait is bounded: after `spin_limit` polls the
+// block raises ctl->error and returns false; a raised error releases every
+// other waiter too, so the kernel always terminates.
+__device__ __forceinline__ bool grid_barrier(Ctl* ctl, uint32_t nblocks, uint32_t spin_limit) {
+    __shared__ uint32_t s_ok;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1;
+        const uint32_t gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > spin_limit || __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_fetch_or(&ctl->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const bool ok = s_ok != 0;
+    __syncthreads();
+    return ok;
+}
+
+// The whole fixpoint in one cooperative launch: evaluation phase, grid
+// barrier, commit + expansion phase, grid barrier, per round, until the
+// worklist is empty (or max_rounds rounds ran, or the arena overflowed: the
+// host then grows it and launches again, resuming the same round).  The
+// round's control words are read once per block by thread 0 with agent-scope
+// atomic loads and broadcast through LDS, so every wave of every block takes
+// the same branch.
+__global__ __launch_bounds__(BLOCK) void k_solve(EvalArgs ea, ExpandArgs xa, uint32_t max_rounds, uint32_t spin_limit) {
+    __shared__ uint32_t s_bm[WAVES][3][128];
+    __shared__ uint32_t s_stage[WAVES][256];
+    __shared__ uint32_t s_cnt[WAVES + 1];
+    __shared__ uint32_t s_ctl[3];
+    Ctl* ctl = ea.ctl;
+    const uint32_t nblocks = gridDim.x;
+    StatAcc acc;
+    for (uint32_t it = 0; it < max_rounds; it++) {
+        if (threadIdx.x == 0) {
+            const uint32_t r = __hip_atomic_load(&ctl->round, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            s_ctl[0] = r;
+            s_ctl[1] = __hip_atomic_load(&ctl->wl_n[r & 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_ctl[2] = __hip_atomic_load(&ctl->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+                       __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        const uint32_t r = s_ctl[0], n = s_ctl[1], stop = s_ctl[2];
+        __syncthreads();
+        if (stop || n == 0u) break;
+        eval_phase(ea, s_bm, acc, r, n);
+        if (!grid_barrier(ctl, nblocks, spin_limit)) break;
+        if (threadIdx.x == 0) s_ctl[2] = __hip_atomic_load(&ctl->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t ovf = s_ctl[2];
+        __syncthreads();
+        expand_phase(xa, s_stage, s_cnt, r, ovf ? 0u : n);
+        if (blockIdx.x == 0 && threadIdx.x == 0 && !ovf) {
+            // round r's counter is no longer read by anyone: clear it and advance
+            if (n) __hip_atomic_fetch_add(&ctl->rounds_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->wl_n[r & 1u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->round, r + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (!grid_barrier(ctl, nblocks, spin_limit)) break;
+    }
+    flush_stats(acc, ea.stats);
+}
+
+// Sum the per-block statistics slots into out[0..3].
+__global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots, unsigned long long* out) {
+    __shared__ unsigned long long s[4][256];
+    for (int c = 0; c < 4; c++) {
+        unsigned long long v = 0;
+        for (uint32_t b = threadIdx.x; b < nslots; b += blockDim.x) v += slots[4ull * b + c];
+        s[c][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int c = 0; c < 4; c++) s[c][threadIdx.x] += s[c][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 __global__ void k_list_lens(const uint4* cur, uint32_t n, uint32_t* lens) {

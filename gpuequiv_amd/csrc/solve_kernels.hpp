@@ -12,6 +12,7 @@ namespace geq {
 
 constexpr int WAVES = 4;            // waves per block of the eval/expand kernels
 constexpr int BLOCK = 64 * WAVES;
+constexpr uint32_t STAT_SLOTS = 8192;   // max grid size of the statistics-writing kernels
 
 struct Ctl {                        // small device control block, read back between round batches
     uint32_t wl_n[2];               // worklist sizes (ping-pong by round parity)
@@ -19,7 +20,10 @@ struct Ctl {                        // small device control block, read back bet
     uint32_t arena_top;             // bump pointer of the energy arena (u16 units)
     uint32_t round;                 // current round; its worklist is wl[round & 1]
     uint32_t rounds_done;           // rounds that evaluated at least one node
-    uint32_t pad[2];
+    uint32_t error;                 // k_solve: a grid barrier timed out
+    uint32_t pad0;
+    uint32_t bar_count, bar_gen;    // k_solve grid barrier (own 64-byte line)
+    uint32_t pad1[14];
 };
 
 // Kernels read the round's worklist and size from Ctl, so a round needs no
@@ -36,7 +40,11 @@ struct EvalArgs {
     uint32_t arena_cap;
     uint8_t* chg;                   // per worklist entry: node changed
     uint32_t maxv;                  // EnergyConf::max (<= 3)
-    unsigned long long* stats;      // node evals, edge updates, bytes, changed
+    unsigned long long* stats;      // per block: node evals, edge updates, bytes, changed
+};
+
+struct StatAcc {                    // per-lane statistics, flushed once per kernel
+    uint64_t evals = 0, edges = 0, bytes = 0, changed = 0;
 };
 
 struct ExpandArgs {
@@ -55,6 +63,8 @@ struct ExpandArgs {
 __global__ void k_eval(EvalArgs a);
 __global__ void k_expand(ExpandArgs a);
 __global__ void k_advance(Ctl* ctl);
+__global__ void k_solve(EvalArgs ea, ExpandArgs xa, uint32_t max_rounds, uint32_t spin_limit);
+__global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots, unsigned long long* out);
 __global__ void k_list_lens(const uint4* cur, uint32_t n, uint32_t* lens);
 __global__ void k_gather(const uint4* cur, const uint16_t* arena, uint32_t n, const uint32_t* offs, uint32_t* out);
 

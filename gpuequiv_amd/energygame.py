@@ -28,11 +28,21 @@ class GameGraph:
         w = np.ascontiguousarray(weights, dtype=np.uint32)
         self.weights = UpdateArray(w.reshape(-1, conf.update_size()), conf)
         self.attacker_pos = np.ascontiguousarray(attacker_pos, dtype=np.uint8)
-        if rev_row_offsets is None:
+        self._rev_row = None if rev_row_offsets is None else np.ascontiguousarray(rev_row_offsets, dtype=np.uint32)
+        self._rev_col = None if rev_column_indices is None else np.ascontiguousarray(rev_column_indices, dtype=np.uint32)
+
+    # the reverse CSR is derived on first use (the C ABI derives it itself when absent)
+    @property
+    def rev_row_offsets(self) -> np.ndarray:
+        if self._rev_row is None:
             self.make_reverse()
-        else:
-            self.rev_row_offsets = np.ascontiguousarray(rev_row_offsets, dtype=np.uint32)
-            self.rev_column_indices = np.ascontiguousarray(rev_column_indices, dtype=np.uint32)
+        return self._rev_row
+
+    @property
+    def rev_column_indices(self) -> np.ndarray:
+        if self._rev_col is None:
+            self.make_reverse()
+        return self._rev_col
 
     @classmethod
     def new(cls, n_vertices: int, edges: Iterable, attacker_pos: Sequence[bool], conf: EnergyConf) -> "GameGraph":
@@ -87,9 +97,9 @@ class GameGraph:
         V = self.n_vertices()
         src = np.repeat(np.arange(V, dtype=np.uint32), np.diff(self.row_offsets.astype(np.int64)))
         order = np.argsort(self.column_indices, kind="stable")
-        self.rev_column_indices = src[order].astype(np.uint32)
+        self._rev_col = src[order].astype(np.uint32)
         counts = np.bincount(self.column_indices, minlength=V) if V else np.zeros(0, np.int64)
-        self.rev_row_offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+        self._rev_row = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
 
     # serde form (SerdeGameGraph, :34-109) as JSON
     def to_json(self) -> str:
@@ -130,8 +140,9 @@ class GameGraph:
         g.column_indices = _lib.u32p(self.column_indices)
         g.weights = _lib.u32p(w)
         g.attacker_pos = _lib.u8p(self.attacker_pos)
-        g.rev_row_offsets = _lib.u32p(self.rev_row_offsets)
-        g.rev_column_indices = _lib.u32p(self.rev_column_indices)
+        if self._rev_row is not None and self._rev_col is not None:
+            g.rev_row_offsets = _lib.u32p(self._rev_row)
+            g.rev_column_indices = _lib.u32p(self._rev_col)
         return g
 
 

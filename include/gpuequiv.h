@@ -93,6 +93,9 @@ typedef struct {
     double eval_kernel_ms;  /* sum of k_eval durations, when profiling was asked   */
     double expand_kernel_ms;/* sum of k_expand durations, when profiling was asked */
     uint64_t eval_launches;
+    double solve_kernel_ms; /* sum of k_solve (cooperative, whole fixpoint) durations */
+    uint32_t mode;          /* 0 cooperative k_solve, 1 hipGraph rounds, 2 profiled per-kernel launches */
+    uint32_t pad;
 } geq_stats;
 
 /* ---- context (GPUCommon::get_gpu, src/energygame/gpuutils.rs:9-61) ---- */

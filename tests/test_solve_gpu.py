@@ -204,3 +204,14 @@ def test_arena_regrow_gives_identical_results(monkeypatch):
     small, ssmall = _solve_raw(g, 0)
     assert small == ref
     assert ssmall["rounds"] == sref["rounds"]
+
+
+def test_all_launch_modes_agree(monkeypatch):
+    # hipGraph of captured rounds (default) and per-kernel launches
+    g = workloads.random_game(30000, 19)
+    a, sa = _solve_raw(g, 0)
+    assert sa["mode"] == 1
+    c, sc = _solve_raw(g, 1)
+    assert sc["mode"] == 2
+    assert a == c
+    assert sa["rounds"] == sc["rounds"] and sa["node_evals"] == sc["node_evals"] and sa["alg_bytes"] == sc["alg_bytes"]
