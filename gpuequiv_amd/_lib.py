@@ -75,6 +75,51 @@ class geq_stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class geq_lts(C.Structure):
+    _fields_ = [
+        ("n_states", C.c_uint32),
+        ("n_transitions", C.c_uint32),
+        ("row_offsets", C.POINTER(C.c_uint32)),
+        ("targets", C.POINTER(C.c_uint32)),
+        ("labels", C.POINTER(C.c_int32)),
+    ]
+
+
+class geq_build_stats(C.Structure):
+    _fields_ = [
+        ("positions", C.c_uint64),
+        ("edges", C.c_uint64),
+        ("levels", C.c_uint64),
+        ("set_words", C.c_uint64),
+        ("build_ms", C.c_double),
+        ("truncated", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
+class geq_built_game(C.Structure):
+    _fields_ = [
+        ("n_positions", C.c_uint32),
+        ("n_edges", C.c_uint32),
+        ("n_start", C.c_uint32),
+        ("row_offsets", C.POINTER(C.c_uint32)),
+        ("column_indices", C.POINTER(C.c_uint32)),
+        ("weights", C.POINTER(C.c_uint32)),
+        ("attacker_pos", C.POINTER(C.c_uint8)),
+        ("kind", C.POINTER(C.c_uint8)),
+        ("p", C.POINTER(C.c_uint32)),
+        ("q_off", C.POINTER(C.c_uint32)),
+        ("q_len", C.POINTER(C.c_uint32)),
+        ("x_off", C.POINTER(C.c_uint32)),
+        ("x_len", C.POINTER(C.c_uint32)),
+        ("pool", C.POINTER(C.c_uint32)),
+        ("pool_len", C.c_uint64),
+    ]
+
+
 P = C.POINTER
 _SIGS = {
     "geq_last_error": (C.c_char_p, []),
@@ -91,6 +136,16 @@ _SIGS = {
     "geq_batch_upload": (C.c_int, [C.c_void_p, C.c_uint32, P(geq_graph), P(C.c_void_p)]),
     "geq_batch_solve": (C.c_int, [C.c_void_p, P(P(C.c_uint32)), P(C.c_uint32), C.c_int, P(geq_stats)]),
     "geq_batch_energies": (C.c_int, [C.c_void_p, P(geq_energies)]),
+    "geq_gamebuild": (C.c_int, [C.c_void_p, P(geq_lts), P(C.c_uint32), C.c_uint32, C.c_uint64, P(C.c_void_p),
+                                P(geq_build_stats)]),
+    "geq_built_download": (C.c_int, [C.c_void_p, P(geq_built_game)]),
+    "geq_built_game_free": (None, [P(geq_built_game)]),
+    "geq_built_solve": (C.c_int, [C.c_void_p, C.c_int, P(geq_stats)]),
+    "geq_built_energies": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(geq_energies)]),
+    "geq_built_free": (None, [C.c_void_p]),
+    "geq_signature_refinement": (C.c_int, [P(geq_lts), P(C.c_uint32), P(C.c_uint32)]),
+    "geq_bisimilar_minimize": (C.c_int, [P(geq_lts), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
+                                         P(C.c_int32), P(C.c_uint32)]),
 }
 
 _lib = None
@@ -125,6 +180,10 @@ def u32p(a: np.ndarray):
 
 def u8p(a: np.ndarray):
     return a.ctypes.data_as(P(C.c_uint8))
+
+
+def i32p(a: np.ndarray):
+    return a.ctypes.data_as(P(C.c_int32))
 
 
 class Context:

@@ -372,11 +372,20 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
                                      int profile_kernels, geq_stats* stats) {
     if (!g) return fail(GEQ_ERR_INVALID_ARG, "game is NULL");
     if (n_reach && !to_reach) return fail(GEQ_ERR_INVALID_ARG, "to_reach is NULL");
-    const uint32_t V = g->V;
     for (uint32_t i = 0; i < n_reach; i++)
-        if (to_reach[i] >= V) return fail(GEQ_ERR_INVALID_ARG, "to_reach id out of range");
+        if (to_reach[i] >= g->V) return fail(GEQ_ERR_INVALID_ARG, "to_reach id out of range");
+    (void)hipSetDevice(g->ctx->device);
+    if (n_reach) {
+        if (!g->seed.p || g->seed.n < n_reach) GEQ_HIP(g->seed.alloc(n_reach), GEQ_ERR_OUT_OF_MEMORY);
+        GEQ_HIP(hipMemcpyAsync(g->seed.p, to_reach, 4ull * n_reach, hipMemcpyHostToDevice, g->ctx->stream), GEQ_ERR_TRANSFER);
+    }
+    return geq::solve_core(g, n_reach, initial, profile_kernels, stats);
+}
+
+// The fixpoint on a resident game whose n_reach nodes to reach are already in g->seed.
+geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* initial, int profile_kernels, geq_stats* stats) {
+    const uint32_t V = g->V;
     geq_ctx* ctx = g->ctx;
-    (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
 
     // ---- initial state: EnergyGame::with_reach (src/energygame.rs:355-365) or explicit lists
@@ -415,10 +424,6 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
         GEQ_HIP(hipMemcpyAsync(g->cur.p, h_cur.data(), 16ull * V, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
         if (!h_arena.empty())
             GEQ_HIP(hipMemcpyAsync(g->arena.p, h_arena.data(), 2ull * h_arena.size(), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    }
-    if (n_reach) {
-        if (!g->seed.p || g->seed.n < n_reach) GEQ_HIP(g->seed.alloc(n_reach), GEQ_ERR_OUT_OF_MEMORY);
-        GEQ_HIP(hipMemcpyAsync(g->seed.p, to_reach, 4ull * n_reach, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
     if (!initial) {
         GEQ_HIP(hipMemsetAsync(g->cur.p, 0, 16ull * V, s), GEQ_ERR_TRANSFER);
@@ -572,11 +577,16 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
 
 extern "C" geq_status geq_game_energies(geq_game* g, geq_energies* out) {
     if (!g || !out) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    return geq::download_energies(g, 0, g->V, out);
+}
+
+// Energies of the nodes [first, first + V) as packed words, ascending per node.
+geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_energies* out) {
+    if (!g || !out || (uint64_t)first + V > g->V) return fail(GEQ_ERR_INVALID_ARG, "node range out of bounds");
     (void)hipSetDevice(g->ctx->device);
     hipStream_t s = g->ctx->stream;
-    const uint32_t V = g->V;
     if (!g->lens.p || g->lens.n < V) GEQ_HIP(g->lens.alloc(V), GEQ_ERR_OUT_OF_MEMORY);
-    k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->cur.p, V, g->lens.p);
+    k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->cur.p + first, V, g->lens.p);
     std::vector<uint32_t> lens(V), offs(V + 1);
     GEQ_HIP(hipMemcpyAsync(lens.data(), g->lens.p, 4ull * V, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -590,7 +600,7 @@ extern "C" geq_status geq_game_energies(geq_game* g, geq_energies* out) {
     if (!g->offs.p || g->offs.n < V + 1) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
     if (!g->out.p || g->out.n < tot) GEQ_HIP(g->out.alloc(tot), GEQ_ERR_OUT_OF_MEMORY);
     GEQ_HIP(hipMemcpyAsync(g->offs.p, offs.data(), 4ull * (V + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->cur.p, g->arena.p, V, g->offs.p, g->out.p);
+    k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->cur.p + first, g->arena.p, V, g->offs.p, g->out.p);
     uint32_t* data = (uint32_t*)std::malloc(4ull * (tot ? tot : 1));
     uint32_t* ooffs = (uint32_t*)std::malloc(4ull * (V + 1));
     if (!data || !ooffs) {

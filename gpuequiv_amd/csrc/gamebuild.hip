@@ -1,0 +1,402 @@
+// Gamebuild on the device: labelled transition system -> spectroscopy energy
+// game (GameBuild::build_internal, src/gamebuild.rs:229-265, with the position
+// rules of src/gamebuild/gamepos.rs:17-198).
+//
+// The reference grows the game by a FIFO walk over positions in creation order:
+// node idx's successors, in slot order, become new nodes in order of first
+// occurrence.  Here every BFS level (the contiguous id range created by the
+// previous level) is expanded at once:
+//   1. k_bounds   - per frontier position an upper bound of its successor
+//                   count and of the set words they need;
+//   2. scans      - exclusive sums give every parent a private region, so the
+//                   candidate index order is exactly (parent, slot) order;
+//   3. k_generate - each parent writes its successor candidates in the
+//                   reference's slot order (observation moves in transition
+//                   order, then the conjunction challenges);
+//   4. k_insert   - a device hash table with full-equality checks finds known
+//                   positions or claims a pending entry; among duplicates of a
+//                   new position the smallest candidate index wins;
+//   5. k_mark + scan - winners get ranks, i.e. new ids in first-occurrence
+//                   order, which is the reference's numbering;
+//   6. k_finalize / k_edges - node records, hash entries, CSR rows, columns,
+//                   weights.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gamebuild.hpp"
+
+namespace geq {
+namespace gb {
+
+constexpr uint32_t K_ATTACK = 0, K_CLAUSE = 1, K_DEFEND = 2;
+constexpr uint32_t INVALID = 0xffffffffu;
+constexpr uint32_t PENDING = 0x80000000u;
+constexpr unsigned long long EMPTY = EMPTY_ENTRY;
+
+// Updates in the STANDARD packing (4-bit fields): observation [-1], challenge
+// [0,-1], clause [Min(4)] and [Min(5),0,0,0,0,-1], answer [0,0,0,Min(3)],
+// revival [Min(3)] (gamepos.rs:18-24, :139-150, :180-186).
+constexpr uint32_t W_OBS = 0x1, W_CHAL = 0x10, W_CL1 = 0x5, W_CL2 = 0x100006, W_ANS = 0x4000, W_REV = 0x4;
+
+__device__ __forceinline__ uint32_t mix(uint32_t h, uint32_t v) {
+    h ^= v + 0x9e3779b9u + (h << 6) + (h >> 2);
+    return h * 0x85ebca6bu;
+}
+
+__device__ uint32_t pos_hash(uint32_t kind, uint32_t p, const uint32_t* __restrict__ spool, uint32_t qo, uint32_t ql,
+                             uint32_t xo, uint32_t xl) {
+    uint32_t h = mix(0x811c9dc5u, kind);
+    h = mix(h, p);
+    h = mix(h, ql);
+    for (uint32_t k = 0; k < ql; k++) h = mix(h, spool[qo + k]);
+    h = mix(h, xl);
+    for (uint32_t k = 0; k < xl; k++) h = mix(h, spool[xo + k]);
+    h ^= h >> 16;
+    return h == 0xffffffffu ? 0x7fffffffu : h;
+}
+
+__device__ bool sets_equal(const uint32_t* __restrict__ spool, uint32_t ao, uint32_t bo, uint32_t len) {
+    for (uint32_t k = 0; k < len; k++)
+        if (spool[ao + k] != spool[bo + k]) return false;
+    return true;
+}
+
+// label range [lo, hi) of `a` in state s's label-sorted transitions
+__device__ __forceinline__ void label_range(const LtsArgs& L, uint32_t s, int32_t a, uint32_t& lo, uint32_t& hi) {
+    uint32_t b = L.row[s], e = L.row[s + 1];
+    uint32_t l = b, r = e;
+    while (l < r) {
+        uint32_t m = (l + r) >> 1;
+        if (L.lbl[m] < a) l = m + 1; else r = m;
+    }
+    lo = l;
+    r = e;
+    while (l < r) {
+        uint32_t m = (l + r) >> 1;
+        if (L.lbl[m] <= a) l = m + 1; else r = m;
+    }
+    hi = l;
+}
+
+// compare_enabled (src/gamebuild.rs:23-98) on the sorted distinct label lists
+__device__ void compare_enabled(const LtsArgs& L, uint32_t p, uint32_t q, bool& subset, bool& superset) {
+    subset = true;
+    superset = true;
+    if (p == q) return;
+    uint32_t pi = L.en_row[p], pe = L.en_row[p + 1], qi = L.en_row[q], qe = L.en_row[q + 1];
+    while (subset || superset) {
+        if (pi == pe && qi == qe) break;
+        if (pi == pe) { subset = false; break; }
+        if (qi == qe) { superset = false; break; }
+        int32_t pa = L.en_lbl[pi], qa = L.en_lbl[qi];
+        if (pa == qa) { pi++; qi++; }
+        else if (pa < qa) { superset = false; pi++; }
+        else { subset = false; qi++; }
+    }
+}
+
+// in-place ascending sort of a[0..n) (insertion sort for short runs, heap sort otherwise)
+__device__ void sort_u32(uint32_t* a, uint32_t n) {
+    if (n <= 24) {
+        for (uint32_t i = 1; i < n; i++) {
+            uint32_t x = a[i], j = i;
+            while (j > 0 && a[j - 1] > x) { a[j] = a[j - 1]; j--; }
+            a[j] = x;
+        }
+        return;
+    }
+    auto sift = [&](uint32_t root, uint32_t end) {
+        while (2 * root + 1 < end) {
+            uint32_t c = 2 * root + 1;
+            if (c + 1 < end && a[c] < a[c + 1]) c++;
+            if (a[root] >= a[c]) return;
+            uint32_t t = a[root]; a[root] = a[c]; a[c] = t;
+            root = c;
+        }
+    };
+    for (uint32_t i = n / 2; i-- > 0;) sift(i, n);
+    for (uint32_t end = n - 1; end > 0; end--) {
+        uint32_t t = a[0]; a[0] = a[end]; a[end] = t;
+        sift(0, end);
+    }
+}
+
+__device__ __forceinline__ bool contains_sorted(const uint32_t* a, uint32_t n, uint32_t x) {
+    uint32_t l = 0, r = n;
+    while (l < r) {
+        uint32_t m = (l + r) >> 1;
+        if (a[m] < x) l = m + 1; else r = m;
+    }
+    return l < n && a[l] == x;
+}
+
+__device__ __forceinline__ uint32_t deg_of(const LtsArgs& L, uint32_t s) { return L.row[s + 1] - L.row[s]; }
+
+__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint32_t* ub_cand, uint32_t* ub_spool) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n_front) return;
+    if (i == n_front) {        // trailing zero so the exclusive scans yield the totals
+        ub_cand[i] = 0;
+        ub_spool[i] = 0;
+        return;
+    }
+    const uint32_t u = lo + i;
+    const uint32_t kind = N.kind[u], p = N.p[u], ql = N.q_len[u], xl = N.x_len[u];
+    uint32_t c = 0, s = 0;
+    if (kind == K_CLAUSE) {
+        c = 2; s = 1;
+    } else if (kind == K_DEFEND) {
+        c = ql + (xl ? 1u : 0u);
+    } else if (ql == 0) {
+        c = 1;
+    } else {
+        c = deg_of(L, p) + 4;
+        uint32_t sd = 0;
+        for (uint32_t k = 0; k < ql; k++) sd += deg_of(L, N.spool[N.q_off[u] + k]);
+        s = sd + 4 * ql;
+    }
+    ub_cand[i] = c;
+    ub_spool[i] = s;
+}
+
+__global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint32_t* cand_off,
+                           const uint32_t* spool_off, uint64_t spool_base, Cand* cands, uint32_t* nsucc) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_front) return;
+    const uint32_t u = lo + i;
+    const uint32_t kind = N.kind[u], p = N.p[u], qo = N.q_off[u], ql = N.q_len[u], xo = N.x_off[u], xl = N.x_len[u];
+    uint32_t* spool = N.spool;
+    const uint32_t cbase = cand_off[i], cend = cand_off[i + 1];
+    const uint64_t sbase = spool_base + spool_off[i];
+    uint32_t slot = 0;
+    auto emit = [&](uint32_t k, uint32_t pp, uint32_t a_off, uint32_t a_len, uint32_t b_off, uint32_t b_len, uint32_t w) {
+        Cand c;
+        c.kind_p = (k << 30) | pp;
+        c.q_off = a_off; c.q_len = a_len; c.x_off = b_off; c.x_len = b_len;
+        c.weight = w;
+        c.hash = pos_hash(k, pp, spool, a_off, a_len, b_off, b_len);
+        c.res = 0;
+        c.parent = i;
+        c.slot = slot;
+        cands[cbase + slot] = c;
+        slot++;
+    };
+    if (kind == K_CLAUSE) {
+        // Clause(p, q) -> Attack(p, [q]) [Min(4)], Attack(q, [p]) [Min(5),0,0,0,0,-1]
+        const uint32_t q = spool[qo];
+        emit(K_ATTACK, p, qo, 1, 0, 0, W_CL1);
+        spool[sbase] = p;
+        emit(K_ATTACK, q, (uint32_t)sbase, 1, 0, 0, W_CL2);
+    } else if (kind == K_DEFEND) {
+        // Defend(p, Q, Q*) -> Clause(p, q) per q in Q [0,0,0,Min(3)], Attack(p, Q*) [Min(3)]
+        for (uint32_t k = 0; k < ql; k++) emit(K_CLAUSE, p, qo + k, 1, 0, 0, W_ANS);
+        if (xl) emit(K_ATTACK, p, xo, xl, 0, 0, W_REV);
+    } else if (ql == 0) {
+        emit(K_DEFEND, p, 0, 0, 0, 0, W_CHAL);        // Attack(p, {}) -> Defend(p, {}, {})
+    } else {
+        // observation moves, one label group of p's transitions at a time
+        uint64_t sp = sbase;
+        uint32_t sd = 0;
+        for (uint32_t k = 0; k < ql; k++) sd += deg_of(L, spool[qo + k]);
+        const uint32_t t_end = L.row[p + 1];
+        uint32_t t = L.row[p];
+        while (t < t_end) {
+            const int32_t a = L.lbl[t];
+            uint32_t g_end = t;
+            while (g_end < t_end && L.lbl[g_end] == a) g_end++;
+            uint32_t T = 0;
+            for (uint32_t k = 0; k < ql; k++) {
+                uint32_t lo_q, hi_q;
+                label_range(L, spool[qo + k], a, lo_q, hi_q);
+                for (uint32_t x = lo_q; x < hi_q; x++) spool[sp + T++] = L.dst[x];
+            }
+            uint32_t* qa = spool + sp;
+            sort_u32(qa, T);
+            uint32_t Tu = 0;
+            for (uint32_t x = 0; x < T; x++)
+                if (Tu == 0 || qa[Tu - 1] != qa[x]) qa[Tu++] = qa[x];
+            for (uint32_t tt = t; tt < g_end; tt++) {
+                const uint32_t pp = L.dst[tt];
+                if (!contains_sorted(qa, Tu, pp)) emit(K_ATTACK, pp, (uint32_t)sp, Tu, 0, 0, W_OBS);
+            }
+            sp += Tu;
+            t = g_end;
+        }
+        emit(K_DEFEND, p, qo, ql, 0, 0, W_CHAL);     // Q* = {}
+        // conjunction challenges with partitioned Q* (gamepos.rs:329-375)
+        const uint64_t fl = sbase + sd + 3ull * ql;  // per q: bit0 subset, bit1 superset
+        for (uint32_t k = 0; k < ql; k++) {
+            bool sub, sup;
+            compare_enabled(L, p, spool[qo + k], sub, sup);
+            spool[fl + k] = (sub ? 1u : 0u) | (sup ? 2u : 0u);
+        }
+        auto in_x = [&](uint32_t which, uint32_t k) -> bool {
+            uint32_t f = spool[fl + k];
+            return which == 0 ? (f & 1u) : which == 1 ? (f & 2u) : (f == 3u);
+        };
+        uint64_t pb = sbase + sd;
+        uint32_t emitted[3];
+        uint32_t n_emitted = 0;
+        for (uint32_t which = 0; which < 3; which++) {
+            uint32_t cnt = 0;
+            for (uint32_t k = 0; k < ql; k++) cnt += in_x(which, k) ? 1u : 0u;
+            if (cnt == 0 || cnt == ql) continue;
+            bool dup = false;
+            for (uint32_t e = 0; e < n_emitted && !dup; e++) {
+                bool same = true;
+                for (uint32_t k = 0; k < ql && same; k++) same = in_x(which, k) == in_x(emitted[e], k);
+                dup = same;
+            }
+            if (dup) continue;
+            uint32_t a = 0, b = ql - cnt;
+            for (uint32_t k = 0; k < ql; k++) {
+                if (in_x(which, k)) spool[pb + b++] = spool[qo + k];
+                else spool[pb + a++] = spool[qo + k];
+            }
+            emit(K_DEFEND, p, (uint32_t)pb, ql - cnt, (uint32_t)(pb + ql - cnt), cnt, W_CHAL);
+            emitted[n_emitted++] = which;
+            pb += ql;
+        }
+    }
+    nsucc[i] = slot;
+    for (uint32_t j = cbase + slot; j < cend; j++) cands[j].kind_p = INVALID;
+}
+
+__device__ bool cand_equals_cand(const Cand& a, const Cand& b, const uint32_t* spool) {
+    return a.kind_p == b.kind_p && a.q_len == b.q_len && a.x_len == b.x_len && sets_equal(spool, a.q_off, b.q_off, a.q_len) &&
+           sets_equal(spool, a.x_off, b.x_off, a.x_len);
+}
+
+__device__ bool cand_equals_node(const Cand& a, const NodeArgs& N, uint32_t id) {
+    const uint32_t kp = (uint32_t(N.kind[id]) << 30) | N.p[id];
+    return a.kind_p == kp && a.q_len == N.q_len[id] && a.x_len == N.x_len[id] &&
+           sets_equal(N.spool, a.q_off, N.q_off[id], a.q_len) && sets_equal(N.spool, a.x_off, N.x_off[id], a.x_len);
+}
+
+__global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_cand) return;
+    Cand c = cands[j];
+    if (c.kind_p == INVALID) return;
+    const uint32_t h = c.hash;
+    uint32_t s = h & mask;
+    for (uint32_t probe = 0; probe <= mask; probe++) {
+        unsigned long long e = __hip_atomic_load(&table[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e == EMPTY) {
+            const unsigned long long mine = ((unsigned long long)h << 32) | (PENDING | j);
+            unsigned long long old = atomicCAS(&table[s], EMPTY, mine);
+            if (old == EMPTY) {
+                atomicMin(&keymin[s], j);
+                cands[j].res = PENDING | s;
+                return;
+            }
+            e = old;
+        }
+        if ((uint32_t)(e >> 32) == h) {
+            const uint32_t ref = (uint32_t)e;
+            if (ref & PENDING) {
+                if (cand_equals_cand(c, cands[ref & ~PENDING], N.spool)) {
+                    atomicMin(&keymin[s], j);
+                    cands[j].res = PENDING | s;
+                    return;
+                }
+            } else if (cand_equals_node(c, N, ref)) {
+                cands[j].res = ref;
+                return;
+            }
+        }
+        s = (s + 1) & mask;
+    }
+}
+
+__global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > n_cand) return;
+    if (j == n_cand) { flag[j] = 0; return; }
+    const Cand& c = cands[j];
+    flag[j] = (c.kind_p != INVALID && (c.res & PENDING) && keymin[c.res & ~PENDING] == j) ? 1u : 0u;
+}
+
+__global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,
+                           NodeArgs N, unsigned long long* table, uint32_t* keymin) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_cand || !flag[j]) return;
+    const Cand c = cands[j];
+    const uint32_t id = v_base + rank[j];
+    const uint32_t s = c.res & ~PENDING;
+    N.kind[id] = (uint8_t)(c.kind_p >> 30);
+    N.p[id] = c.kind_p & 0x3fffffffu;
+    N.q_off[id] = c.q_off; N.q_len[id] = c.q_len;
+    N.x_off[id] = c.x_off; N.x_len[id] = c.x_len;
+    N.hash[id] = c.hash;
+    table[s] = ((unsigned long long)c.hash << 32) | id;
+    keymin[s] = 0xffffffffu;
+}
+
+__global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
+                        uint32_t* col, uint32_t* wt) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_cand) return;
+    const Cand c = cands[j];
+    if (c.kind_p == INVALID) return;
+    const uint32_t target = (c.res & PENDING) ? (uint32_t)table[c.res & ~PENDING] : c.res;
+    const uint32_t e = e_base + rowl[c.parent] + c.slot;
+    col[e] = target;
+    wt[e] = c.weight;
+}
+
+__global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_front) row[lo + i] = e_base + rowl[i];
+}
+
+// start positions Attack(p, [q]) at ids [0, n): node records + hash entries
+__global__ void k_starts(const uint32_t* pairs, uint32_t n, const uint8_t* insert, NodeArgs N, unsigned long long* table,
+                         uint32_t mask) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    N.kind[i] = K_ATTACK;
+    N.p[i] = pairs[2 * i];
+    N.spool[i] = pairs[2 * i + 1];
+    N.q_off[i] = i; N.q_len[i] = 1;
+    N.x_off[i] = 0; N.x_len[i] = 0;
+    // pairs were written by the host before the launch; spool[i] is this thread's own store
+    const uint32_t h = pos_hash(K_ATTACK, pairs[2 * i], pairs, 2 * i + 1, 1, 0, 0);
+    N.hash[i] = h;
+    if (!insert[i]) return;
+    uint32_t s = h & mask;
+    const unsigned long long mine = ((unsigned long long)h << 32) | i;
+    while (atomicCAS(&table[s], EMPTY, mine) != EMPTY) s = (s + 1) & mask;
+}
+
+// re-insert every node into a larger table
+__global__ void k_rehash(const uint32_t* hash, uint32_t n, unsigned long long* table, uint32_t mask) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = hash[i];
+    uint32_t s = h & mask;
+    const unsigned long long mine = ((unsigned long long)h << 32) | i;
+    while (atomicCAS(&table[s], EMPTY, mine) != EMPTY) s = (s + 1) & mask;
+}
+
+__global__ void k_fill_u64(unsigned long long* a, uint64_t n, unsigned long long v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* a, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atk[i] = kind[i] != K_DEFEND ? 1 : 0;
+}
+
+}  // namespace gb
+}  // namespace geq

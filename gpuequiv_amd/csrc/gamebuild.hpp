@@ -1,0 +1,63 @@
+// Device gamebuild: data layout shared by the kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace geq {
+namespace gb {
+
+constexpr unsigned long long EMPTY_ENTRY = ~0ull;   // free slot of the position table
+
+// Label-sorted LTS in CSR form plus, per state, its sorted distinct labels
+// (the enabledness sets compared by compare_enabled).
+struct LtsArgs {
+    const uint32_t* row;
+    const uint32_t* dst;
+    const int32_t* lbl;
+    const uint32_t* en_row;
+    const int32_t* en_lbl;
+};
+
+// Position records (structure of arrays).  Sets Q / Q* live in the set pool:
+// Attack(p, Q) = (ATTACK, p, Q, -); Clause(p, q) = (CLAUSE, p, [q], -);
+// Defend(p, Q, Q*) = (DEFEND, p, Q, Q*).
+struct NodeArgs {
+    uint8_t* kind;
+    uint32_t* p;
+    uint32_t* q_off;
+    uint32_t* q_len;
+    uint32_t* x_off;
+    uint32_t* x_len;
+    uint32_t* hash;
+    uint32_t* spool;
+};
+
+struct Cand {               // one successor candidate of a frontier position
+    uint32_t kind_p;        // kind << 30 | p, or INVALID for an unused slot
+    uint32_t q_off, q_len, x_off, x_len;
+    uint32_t weight;
+    uint32_t hash;
+    uint32_t res;           // after k_insert: existing node id, or PENDING | table slot
+    uint32_t parent;        // index in the frontier
+    uint32_t slot;          // successor index within the parent
+};
+
+__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint32_t* ub_cand, uint32_t* ub_spool);
+__global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint32_t* cand_off,
+                           const uint32_t* spool_off, uint64_t spool_base, Cand* cands, uint32_t* nsucc);
+__global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask);
+__global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag);
+__global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,
+                           NodeArgs N, unsigned long long* table, uint32_t* keymin);
+__global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
+                        uint32_t* col, uint32_t* wt);
+__global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row);
+__global__ void k_starts(const uint32_t* pairs, uint32_t n, const uint8_t* insert, NodeArgs N, unsigned long long* table,
+                         uint32_t mask);
+__global__ void k_rehash(const uint32_t* hash, uint32_t n, unsigned long long* table, uint32_t mask);
+__global__ void k_fill_u64(unsigned long long* a, uint64_t n, unsigned long long v);
+__global__ void k_fill_u32(uint32_t* a, uint64_t n, uint32_t v);
+__global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk);
+
+}  // namespace gb
+}  // namespace geq

@@ -1,0 +1,446 @@
+// Host driver of the device gamebuild (level loop, capacities) and the C ABI
+// entry points geq_gamebuild / geq_built_* of include/gpuequiv.h.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "gamebuild.hpp"
+#include "runtime.hpp"
+
+using namespace geq;
+using namespace geq::gb;
+
+namespace {
+
+__global__ void k_edges_u2(const uint32_t* col, const uint32_t* wt, uint32_t E, uint2* edges) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) edges[e] = make_uint2(col[e], wt[e]);
+}
+__global__ void k_indeg(const uint32_t* col, uint32_t E, uint32_t* cnt) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) atomicAdd(&cnt[col[e]], 1u);
+}
+__global__ void k_rev_scatter(const uint32_t* row, uint32_t V, const uint32_t* col, uint32_t* fill, uint32_t* rcol) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    for (uint32_t e = row[v]; e < row[v + 1]; e++) rcol[atomicAdd(&fill[col[e]], 1u)] = v;
+}
+// standard_reach on the device: defender positions without successors
+__global__ void k_reach(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t* out, uint32_t* n) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V && !atk[v] && row[v] == row[v + 1]) out[atomicAdd(n, 1u)] = v;
+}
+
+template <class T>
+geq_status ensure(DevBuf<T>& b, size_t need, hipStream_t s, bool keep, const char* what) {
+    if (b.p && b.n >= need) return GEQ_OK;
+    size_t cap = std::max<size_t>(need, b.n ? b.n + b.n / 2 : 1024);
+    T* p = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * cap);
+    if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("gamebuild: allocating ") + what + ": " + hipGetErrorString(e));
+    if (keep && b.p && b.n) {
+        e = hipMemcpyAsync(p, b.p, sizeof(T) * b.n, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return fail(GEQ_ERR_TRANSFER, std::string("gamebuild: growing ") + what);
+        }
+    }
+    b.release();
+    b.p = p;
+    b.n = cap;
+    return GEQ_OK;
+}
+
+template <class T>
+geq_status exclusive_scan(const T* in, T* out, size_t n, DevBuf<uint8_t>& tmp, hipStream_t s) {
+    size_t bytes = 0;
+    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s), GEQ_ERR_INTERNAL);
+    geq_status st = ensure(tmp, bytes + 16, s, false, "scan scratch");
+    if (st != GEQ_OK) return st;
+    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, s), GEQ_ERR_INTERNAL);
+    return GEQ_OK;
+}
+
+uint32_t read_u32(const uint32_t* d, hipStream_t s) {
+    uint32_t v = 0;
+    (void)hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    return v;
+}
+
+}  // namespace
+
+struct geq_built {
+    geq_ctx* ctx = nullptr;
+    uint32_t V = 0, E = 0, n_start = 0;
+    uint64_t spool_top = 0;
+    DevBuf<uint8_t> kind, atk;
+    DevBuf<uint32_t> p, q_off, q_len, x_off, x_len, hash, spool, row, col, wt;
+    geq_game* game = nullptr;      // created on the first solve
+    uint32_t n_reach = 0;
+    ~geq_built() {
+        if (game) geq_game_free(game);
+    }
+};
+
+geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,
+                                 const uint32_t* d_wt, const uint8_t* d_atk, geq_game** out) {
+    hipStream_t s = ctx->stream;
+    geq_game* gm = new geq_game();
+    gm->ctx = ctx;
+    check_conf(geq_conf{6, 3}, gm->conf);
+    gm->V = V;
+    gm->E = E;
+    gm->game_base = {0, V};
+    auto oom = [&](hipError_t e, const char* what) -> geq_status {
+        delete gm;
+        return fail(e == hipErrorOutOfMemory ? GEQ_ERR_OUT_OF_MEMORY : GEQ_ERR_NO_DEVICE,
+                    std::string("allocating ") + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = gm->row.alloc(V + 1)) != hipSuccess) return oom(e, "row_offsets");
+    if ((e = gm->rrow.alloc(V + 1)) != hipSuccess) return oom(e, "rev_row_offsets");
+    if ((e = gm->rcol.alloc(E)) != hipSuccess) return oom(e, "rev_column_indices");
+    if ((e = gm->edges.alloc(E)) != hipSuccess) return oom(e, "edges");
+    if ((e = gm->atk.alloc(V)) != hipSuccess) return oom(e, "attacker_pos");
+    if ((e = gm->cur.alloc(V)) != hipSuccess) return oom(e, "energy lists");
+    if ((e = gm->nxt.alloc(V)) != hipSuccess) return oom(e, "energy lists");
+    if ((e = gm->wl0.alloc(V)) != hipSuccess) return oom(e, "worklist");
+    if ((e = gm->wl1.alloc(V)) != hipSuccess) return oom(e, "worklist");
+    if ((e = gm->chg.alloc(V)) != hipSuccess) return oom(e, "changed flags");
+    if ((e = gm->stamp.alloc(V)) != hipSuccess) return oom(e, "stamps");
+    if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
+    if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
+    gm->arena_cap = std::max<uint32_t>(1u << 20, (uint32_t)std::min<uint64_t>(8ull * E + 1024, 0x7fffffffull));
+    if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
+    if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
+        return oom(e, "pinned control block");
+    if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
+    DevBuf<uint32_t> cnt, fill;
+    DevBuf<uint8_t> tmp;
+    if ((e = cnt.alloc(V + 1)) != hipSuccess || (e = fill.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");
+    const uint32_t tb = 256;
+    GEQ_HIP(hipMemcpyAsync(gm->row.p, d_row, 4ull * (V + 1), hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipMemcpyAsync(gm->atk.p, d_atk, V, hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
+    if (E) k_edges_u2<<<(E + tb - 1) / tb, tb, 0, s>>>(d_col, d_wt, E, gm->edges.p);
+    k_fill_u32<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(cnt.p, V + 1, 0u);
+    if (E) k_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(d_col, E, cnt.p);
+    geq_status st = exclusive_scan<uint32_t>(cnt.p, gm->rrow.p, V + 1, tmp, s);
+    if (st != GEQ_OK) {
+        delete gm;
+        return st;
+    }
+    GEQ_HIP(hipMemcpyAsync(fill.p, gm->rrow.p, 4ull * (V + 1), hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
+    k_rev_scatter<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, V, d_col, fill.p, gm->rcol.p);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    *out = gm;
+    return GEQ_OK;
+}
+
+extern "C" {
+
+geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs, uint64_t max_positions,
+                         geq_built** out, geq_build_stats* stats) {
+    if (!ctx || !lts || !out || (n_pairs && !pairs)) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    const uint32_t n = lts->n_states, m = lts->n_transitions;
+    if (n == 0 || !lts->row_offsets || (m && (!lts->targets || !lts->labels)))
+        return fail(GEQ_ERR_INVALID_ARG, "empty or incomplete LTS");
+    if (n >= (1u << 30)) return fail(GEQ_ERR_INVALID_ARG, "at most 2^30 states");
+    if (lts->row_offsets[0] != 0 || lts->row_offsets[n] != m) return fail(GEQ_ERR_INVALID_ARG, "LTS row_offsets malformed");
+    std::vector<uint32_t> en_row(n + 1, 0);
+    std::vector<int32_t> en_lbl;
+    for (uint32_t s = 0; s < n; s++) {
+        const uint32_t b = lts->row_offsets[s], e = lts->row_offsets[s + 1];
+        if (e < b) return fail(GEQ_ERR_INVALID_ARG, "LTS row_offsets not monotone");
+        for (uint32_t k = b; k < e; k++) {
+            if (lts->targets[k] >= n) return fail(GEQ_ERR_INVALID_ARG, "LTS target out of range");
+            if (k > b && lts->labels[k] < lts->labels[k - 1])
+                return fail(GEQ_ERR_INVALID_ARG, "LTS transitions of a state must be sorted by label");
+            if (k == b || lts->labels[k] != lts->labels[k - 1]) en_lbl.push_back(lts->labels[k]);
+        }
+        en_row[s + 1] = (uint32_t)en_lbl.size();
+    }
+    for (uint32_t i = 0; i < 2 * n_pairs; i++)
+        if (pairs[i] >= n) return fail(GEQ_ERR_INVALID_ARG, "start pair process out of range");
+    // GameBuild::new_node keeps the last id of a repeated start position in its map
+    std::vector<uint8_t> insert(n_pairs, 1);
+    {
+        std::map<std::pair<uint32_t, uint32_t>, uint32_t> last;
+        for (uint32_t i = 0; i < n_pairs; i++) {
+            auto key = std::make_pair(pairs[2 * i], pairs[2 * i + 1]);
+            auto it = last.find(key);
+            if (it != last.end()) insert[it->second] = 0;
+            last[key] = i;
+        }
+    }
+
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    auto t0 = std::chrono::steady_clock::now();
+    geq_built* B = new geq_built();
+    B->ctx = ctx;
+    B->n_start = n_pairs;
+    DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, ub_c, ub_s, c_off, s_off, nsucc, rowl, flag, rank, keymin;
+    DevBuf<int32_t> l_lbl, l_enlbl;
+    DevBuf<uint8_t> d_insert, tmp;
+    DevBuf<Cand> cands;
+    DevBuf<unsigned long long> table;
+    geq_status st = GEQ_OK;
+    auto bail = [&](geq_status e) {
+        delete B;
+        return e;
+    };
+#define GB_TRY(x)                      \
+    do {                               \
+        geq_status _s = (x);           \
+        if (_s != GEQ_OK) return bail(_s); \
+    } while (0)
+#define GB_HIP(x, code)                                                                          \
+    do {                                                                                         \
+        hipError_t _e = (x);                                                                     \
+        if (_e != hipSuccess) return bail(fail((code), std::string(#x) + ": " + hipGetErrorString(_e))); \
+    } while (0)
+    GB_TRY(ensure(l_row, n + 1, s, false, "LTS"));
+    GB_TRY(ensure(l_dst, std::max<uint32_t>(m, 1), s, false, "LTS"));
+    GB_TRY(ensure(l_lbl, std::max<uint32_t>(m, 1), s, false, "LTS"));
+    GB_TRY(ensure(l_enrow, n + 1, s, false, "LTS"));
+    GB_TRY(ensure(l_enlbl, std::max<size_t>(en_lbl.size(), 1), s, false, "LTS"));
+    GB_HIP(hipMemcpyAsync(l_row.p, lts->row_offsets, 4ull * (n + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    if (m) {
+        GB_HIP(hipMemcpyAsync(l_dst.p, lts->targets, 4ull * m, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        GB_HIP(hipMemcpyAsync(l_lbl.p, lts->labels, 4ull * m, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    }
+    GB_HIP(hipMemcpyAsync(l_enrow.p, en_row.data(), 4ull * (n + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    if (!en_lbl.empty())
+        GB_HIP(hipMemcpyAsync(l_enlbl.p, en_lbl.data(), 4ull * en_lbl.size(), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    LtsArgs L{l_row.p, l_dst.p, l_lbl.p, l_enrow.p, l_enlbl.p};
+
+    // node, edge and set-pool capacities grow level by level
+    auto node_cap = [&](size_t need) -> geq_status {
+        geq_status e;
+        if ((e = ensure(B->kind, need, s, true, "positions")) != GEQ_OK) return e;
+        if ((e = ensure(B->p, need, s, true, "positions")) != GEQ_OK) return e;
+        if ((e = ensure(B->q_off, need, s, true, "positions")) != GEQ_OK) return e;
+        if ((e = ensure(B->q_len, need, s, true, "positions")) != GEQ_OK) return e;
+        if ((e = ensure(B->x_off, need, s, true, "positions")) != GEQ_OK) return e;
+        if ((e = ensure(B->x_len, need, s, true, "positions")) != GEQ_OK) return e;
+        if ((e = ensure(B->hash, need, s, true, "positions")) != GEQ_OK) return e;
+        return ensure(B->row, need + 1, s, true, "row offsets");
+    };
+    uint32_t table_mask = 0;
+    uint64_t table_slots = 0;     // power of two in use (the buffers may be larger)
+    auto table_cap = [&](uint64_t entries) -> geq_status {
+        uint64_t cap = 1024;
+        while (cap < 2 * entries) cap <<= 1;
+        if (cap > (1ull << 31)) return fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: position table exceeds 2^31 entries");
+        if (table_slots >= cap) return GEQ_OK;
+        table_slots = cap;
+        geq_status e;
+        if ((e = ensure(table, cap, s, false, "position table")) != GEQ_OK) return e;
+        if ((e = ensure(keymin, cap, s, false, "position table")) != GEQ_OK) return e;
+        table_mask = (uint32_t)(cap - 1);
+        const uint32_t tb = 256;
+        k_fill_u64<<<(uint32_t)((cap + tb - 1) / tb), tb, 0, s>>>(table.p, cap, EMPTY_ENTRY);
+        k_fill_u32<<<(uint32_t)((cap + tb - 1) / tb), tb, 0, s>>>(keymin.p, cap, 0xffffffffu);
+        if (B->V) k_rehash<<<(B->V + tb - 1) / tb, tb, 0, s>>>(B->hash.p, B->V, table.p, table_mask);
+        return GEQ_OK;
+    };
+    auto nodes = [&]() {
+        return NodeArgs{B->kind.p, B->p.p, B->q_off.p, B->q_len.p, B->x_off.p, B->x_len.p, B->hash.p, B->spool.p};
+    };
+
+    const uint32_t tb = 256;
+    GB_TRY(node_cap(std::max<uint32_t>(n_pairs, 1)));
+    GB_TRY(ensure(B->spool, std::max<uint32_t>(n_pairs, 1) + 1024, s, false, "set pool"));
+    GB_TRY(table_cap(std::max<uint32_t>(n_pairs, 1)));
+    if (n_pairs) {
+        GB_TRY(ensure(d_pairs, 2ull * n_pairs, s, false, "start pairs"));
+        GB_TRY(ensure(d_insert, n_pairs, s, false, "start pairs"));
+        GB_HIP(hipMemcpyAsync(d_pairs.p, pairs, 8ull * n_pairs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        GB_HIP(hipMemcpyAsync(d_insert.p, insert.data(), n_pairs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        k_starts<<<(n_pairs + tb - 1) / tb, tb, 0, s>>>(d_pairs.p, n_pairs, d_insert.p, nodes(), table.p, table_mask);
+    }
+    B->V = n_pairs;
+    B->spool_top = n_pairs;
+    uint64_t E = 0, levels = 0;
+    uint32_t lo = 0, hi = n_pairs;
+    bool truncated = false;
+    while (lo < hi) {
+        const uint32_t nf = hi - lo;
+        GB_TRY(ensure(ub_c, nf + 1, s, false, "frontier"));
+        GB_TRY(ensure(ub_s, nf + 1, s, false, "frontier"));
+        GB_TRY(ensure(c_off, nf + 1, s, false, "frontier"));
+        GB_TRY(ensure(s_off, nf + 1, s, false, "frontier"));
+        GB_TRY(ensure(nsucc, nf + 1, s, false, "frontier"));
+        GB_TRY(ensure(rowl, nf + 1, s, false, "frontier"));
+        k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub_c.p, ub_s.p);
+        GB_TRY(exclusive_scan<uint32_t>(ub_c.p, c_off.p, nf + 1, tmp, s));
+        GB_TRY(exclusive_scan<uint32_t>(ub_s.p, s_off.p, nf + 1, tmp, s));
+        const uint32_t n_cand = read_u32(c_off.p + nf, s);
+        const uint32_t n_words = read_u32(s_off.p + nf, s);
+        if ((uint64_t)B->spool_top + n_words >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: set pool exceeds 2^32 words"));
+        GB_TRY(ensure(B->spool, B->spool_top + n_words + 1, s, true, "set pool"));
+        GB_TRY(ensure(cands, std::max<uint32_t>(n_cand, 1), s, false, "candidates"));
+        GB_TRY(ensure(flag, n_cand + 1, s, false, "candidates"));
+        GB_TRY(ensure(rank, n_cand + 1, s, false, "candidates"));
+        GB_TRY(node_cap((size_t)B->V + n_cand + 1));
+        GB_TRY(table_cap((uint64_t)B->V + n_cand));
+        if (E + n_cand >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 edges"));
+        GB_TRY(ensure(B->col, E + n_cand + 1, s, true, "edges"));
+        GB_TRY(ensure(B->wt, E + n_cand + 1, s, true, "edges"));
+        k_generate<<<(nf + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, c_off.p, s_off.p, B->spool_top, cands.p, nsucc.p);
+        GB_HIP(hipMemsetAsync(nsucc.p + nf, 0, 4, s), GEQ_ERR_TRANSFER);
+        GB_TRY(exclusive_scan<uint32_t>(nsucc.p, rowl.p, nf + 1, tmp, s));
+        const uint32_t n_edges = read_u32(rowl.p + nf, s);
+        if (n_cand) {
+            k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
+            k_mark<<<(n_cand + 1 + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, keymin.p, flag.p);
+        } else {
+            GB_HIP(hipMemsetAsync(flag.p, 0, 4, s), GEQ_ERR_TRANSFER);
+        }
+        GB_TRY(exclusive_scan<uint32_t>(flag.p, rank.p, n_cand + 1, tmp, s));
+        const uint32_t n_new = read_u32(rank.p + n_cand, s);
+        if (n_cand) {
+            k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, flag.p, rank.p, B->V, nodes(), table.p, keymin.p);
+            k_edges<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p, B->wt.p);
+        }
+        k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p);
+        E += n_edges;
+        B->spool_top += n_words;
+        B->V += n_new;
+        lo = hi;
+        hi = B->V;
+        levels++;
+        if (max_positions && B->V > max_positions) {
+            truncated = true;
+            break;
+        }
+    }
+    if (truncated) {
+        // the last level's new positions get no successors: close their rows
+        const uint32_t rest = B->V - lo;
+        std::vector<uint32_t> rows(rest + 1, (uint32_t)E);
+        GB_HIP(hipMemcpyAsync(B->row.p + lo, rows.data(), 4ull * (rest + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    } else {
+        const uint32_t e32 = (uint32_t)E;
+        GB_HIP(hipMemcpyAsync(B->row.p + B->V, &e32, 4, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    }
+    B->E = (uint32_t)E;
+    GB_TRY(ensure(B->atk, std::max<uint32_t>(B->V, 1), s, false, "attacker flags"));
+    if (B->V) k_attacker_flags<<<(B->V + tb - 1) / tb, tb, 0, s>>>(B->kind.p, B->V, B->atk.p);
+    GB_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->positions = B->V;
+        stats->edges = E;
+        stats->levels = levels;
+        stats->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->truncated = truncated ? 1u : 0u;
+        stats->set_words = B->spool_top;
+    }
+    (void)st;
+    *out = B;
+    return GEQ_OK;
+#undef GB_TRY
+#undef GB_HIP
+}
+
+geq_status geq_built_download(geq_built* B, geq_built_game* o) {
+    if (!B || !o) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    (void)hipSetDevice(B->ctx->device);
+    hipStream_t s = B->ctx->stream;
+    std::memset(o, 0, sizeof(*o));
+    const uint32_t V = B->V, E = B->E;
+    o->n_positions = V;
+    o->n_edges = E;
+    o->n_start = B->n_start;
+    o->pool_len = B->spool_top;
+    auto a32 = [](size_t n) { return (uint32_t*)std::malloc(4 * std::max<size_t>(n, 1)); };
+    o->row_offsets = a32(V + 1);
+    o->column_indices = a32(E);
+    o->weights = a32(E);
+    o->attacker_pos = (uint8_t*)std::malloc(std::max<size_t>(V, 1));
+    o->kind = (uint8_t*)std::malloc(std::max<size_t>(V, 1));
+    o->p = a32(V);
+    o->q_off = a32(V);
+    o->q_len = a32(V);
+    o->x_off = a32(V);
+    o->x_len = a32(V);
+    o->pool = a32(B->spool_top);
+    if (!o->row_offsets || !o->column_indices || !o->weights || !o->attacker_pos || !o->kind || !o->p || !o->q_off || !o->q_len ||
+        !o->x_off || !o->x_len || !o->pool) {
+        geq_built_game_free(o);
+        return fail(GEQ_ERR_OUT_OF_MEMORY, "host allocation of the built game");
+    }
+    auto dn = [&](void* dst, const void* src, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+    };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = dn(o->row_offsets, B->row.p, 4ull * (V + 1));
+    if (e == hipSuccess) e = dn(o->column_indices, B->col.p, 4ull * E);
+    if (e == hipSuccess) e = dn(o->weights, B->wt.p, 4ull * E);
+    if (e == hipSuccess) e = dn(o->attacker_pos, B->atk.p, V);
+    if (e == hipSuccess) e = dn(o->kind, B->kind.p, V);
+    if (e == hipSuccess) e = dn(o->p, B->p.p, 4ull * V);
+    if (e == hipSuccess) e = dn(o->q_off, B->q_off.p, 4ull * V);
+    if (e == hipSuccess) e = dn(o->q_len, B->q_len.p, 4ull * V);
+    if (e == hipSuccess) e = dn(o->x_off, B->x_off.p, 4ull * V);
+    if (e == hipSuccess) e = dn(o->x_len, B->x_len.p, 4ull * V);
+    if (e == hipSuccess) e = dn(o->pool, B->spool.p, 4ull * B->spool_top);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        geq_built_game_free(o);
+        return fail(GEQ_ERR_TRANSFER, std::string("downloading the built game: ") + hipGetErrorString(e));
+    }
+    return GEQ_OK;
+}
+
+void geq_built_game_free(geq_built_game* o) {
+    if (!o) return;
+    std::free(o->row_offsets); std::free(o->column_indices); std::free(o->weights); std::free(o->attacker_pos);
+    std::free(o->kind); std::free(o->p); std::free(o->q_off); std::free(o->q_len); std::free(o->x_off); std::free(o->x_len);
+    std::free(o->pool);
+    std::memset(o, 0, sizeof(*o));
+}
+
+geq_status geq_built_solve(geq_built* B, int profile_kernels, geq_stats* stats) {
+    if (!B) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    (void)hipSetDevice(B->ctx->device);
+    hipStream_t s = B->ctx->stream;
+    if (!B->game) {
+        geq_status st = game_from_device(B->ctx, B->V, B->E, B->row.p, B->col.p, B->wt.p, B->atk.p, &B->game);
+        if (st != GEQ_OK) return st;
+        geq_game* g = B->game;
+        GEQ_HIP(g->seed.alloc(std::max<uint32_t>(B->V, 1)), GEQ_ERR_OUT_OF_MEMORY);
+        DevBuf<uint32_t> cnt;
+        GEQ_HIP(cnt.alloc(1), GEQ_ERR_OUT_OF_MEMORY);
+        GEQ_HIP(hipMemsetAsync(cnt.p, 0, 4, s), GEQ_ERR_TRANSFER);
+        k_reach<<<(B->V + 255) / 256, 256, 0, s>>>(g->row.p, g->atk.p, B->V, g->seed.p, cnt.p);
+        B->n_reach = read_u32(cnt.p, s);
+    }
+    return solve_core(B->game, B->n_reach, nullptr, profile_kernels, stats);
+}
+
+geq_status geq_built_energies(geq_built* B, uint32_t first, uint32_t count, geq_energies* out) {
+    if (!B || !out) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    if (!B->game) return fail(GEQ_ERR_INVALID_ARG, "the built game was not solved");
+    return download_energies(B->game, first, count, out);
+}
+
+void geq_built_free(geq_built* B) {
+    if (!B) return;
+    (void)hipSetDevice(B->ctx->device);
+    delete B;
+}
+
+}  // extern "C"

@@ -49,8 +49,16 @@ struct DevBuf {
     }
 };
 
+geq_status solve_core(geq_game* g, uint32_t n_reach, const geq_energies* initial, int profile_kernels, geq_stats* stats);
+geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_energies* out);
+// Device-resident game from device arrays (STANDARD conf): rows, columns,
+// packed updates, attacker flags; the reverse CSR is derived on the device.
+geq_status game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,
+                            const uint32_t* d_wt, const uint8_t* d_atk, geq_game** out);
+
 }  // namespace geq
 
+struct geq_game;
 struct geq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;

@@ -1,0 +1,119 @@
+"""Results of comparing many process pairs (src/equivalence.rs)."""
+from __future__ import annotations
+
+from typing import Sequence
+
+from .energy import Energy, EnergyArray
+
+
+class StartInfo:
+    """Which pairs the game was seeded with (src/equivalence.rs:17-38)."""
+
+    def __init__(self, starting_points: Sequence, n_processes: int):
+        self.starting_points = list(starting_points)
+        self.n_processes = int(n_processes)
+
+    def equivalence(self, energies: Sequence[EnergyArray]) -> "Equivalence":
+        return Equivalence(self, energies)
+
+
+class Equivalence:
+    """Preorders / equivalences between processes (src/equivalence.rs:41-175)."""
+
+    def __init__(self, start_info: StartInfo, energies: Sequence[EnergyArray]):
+        if len(energies) < len(start_info.starting_points):
+            raise ValueError("Not enough energies")
+        self.start_info = start_info
+        self.energies = list(energies)[:len(start_info.starting_points)]
+        self.pos_to_idx = {(sp.p, sp.q[0]): i for i, sp in enumerate(start_info.starting_points)}
+        self.minimization = None
+
+    def set_minimization(self, minimization: Sequence[int]) -> None:
+        self.minimization = list(minimization)
+
+    def _mapped(self, p: int) -> int:
+        return self.minimization[p] if self.minimization is not None else p
+
+    def _energies_minimized(self, p: int, q: int):
+        if not (0 <= p < self.start_info.n_processes and 0 <= q < self.start_info.n_processes):
+            raise IndexError("process out of range")
+        idx = self.pos_to_idx.get((p, q))
+        return None if idx is None else self.energies[idx]
+
+    def energies_of(self, p: int, q: int):
+        """Budgets of the pair (p, q), or None if it was not a starting point (:96-98)."""
+        return self._energies_minimized(self._mapped(p), self._mapped(q))
+
+    def preorder(self, p: int, q: int, equivalence: Energy) -> bool:
+        """p <= q for the preorder of `equivalence` (:125-132)."""
+        p, q = self._mapped(p), self._mapped(q)
+        if p == q:
+            return True
+        e = self._energies_minimized(p, q)
+        return e is not None and e.test_equivalence(equivalence)
+
+    def equiv(self, p: int, q: int, equivalence: Energy) -> bool:
+        return self.preorder(p, q, equivalence) and self.preorder(q, p, equivalence)
+
+    def relation(self, equivalence: Energy) -> "EquivalenceRelation":
+        """Classes of processes equivalent under `equivalence` (:154-170)."""
+        uf = EquivalenceRelation(self.start_info.n_processes)
+        for sp, e in zip(self.start_info.starting_points, self.energies):
+            p, q = sp.p, sp.q[0]
+            e2 = self._energies_minimized(q, p)
+            if e2 is None:
+                raise ValueError("Symmetric position not included in starting points for energy game")
+            if e.test_equivalence(equivalence) and e2.test_equivalence(equivalence):
+                uf.union(p, q)
+        if self.minimization is not None:
+            return uf.with_mapping(self.minimization)
+        return uf
+
+
+class EquivalenceRelation:
+    """Union-find over processes (src/equivalence.rs:178-247)."""
+
+    def __init__(self, n: int):
+        self.parent = list(range(n))
+
+    def find(self, x: int) -> int:
+        while self.parent[x] != x:
+            self.parent[x] = self.parent[self.parent[x]]
+            x = self.parent[x]
+        return x
+
+    def union(self, a: int, b: int) -> None:
+        ra, rb = self.find(a), self.find(b)
+        if ra != rb:
+            self.parent[max(ra, rb)] = min(ra, rb)
+
+    def __len__(self) -> int:
+        return len(self.parent)
+
+    def get_classes(self) -> list:
+        idx, classes = {}, []
+        for i in range(len(self)):
+            c = self.find(i)
+            if c not in idx:
+                idx[c] = len(classes)
+                classes.append([])
+            classes[idx[c]].append(i)
+        return classes
+
+    def count_classes(self) -> int:
+        return len({self.find(i) for i in range(len(self))})
+
+    def class_of(self, proc: int):
+        c = self.find(proc)
+        return (i for i in range(len(self)) if self.find(i) == c)
+
+    def with_mapping(self, mapping: Sequence[int]) -> "EquivalenceRelation":
+        out = EquivalenceRelation(len(mapping))
+        rep = {}
+        for proc, m in enumerate(mapping):
+            part = self.find(m)
+            if part in rep:
+                out.union(proc, rep[part])
+            else:
+                rep[part] = proc
+        return out

@@ -139,7 +139,72 @@ geq_status geq_batch_solve(geq_game* batch, const uint32_t* const* to_reach, con
 /* outs: n_games results, each freed with geq_energies_free */
 geq_status geq_batch_energies(geq_game* batch, geq_energies* outs);
 
+/* ---- gamebuild: LTS -> spectroscopy energy game, built on the device ----
+ * Replaces GameBuild::{compare, compare_multiple, compare_all} +
+ * build_internal (src/gamebuild.rs:165-265) and the position successor rules
+ * (src/gamebuild/gamepos.rs:17-198).  The game is seeded with the attack
+ * positions (p, {q}) of `pairs` (ids 0..n_pairs-1, in the given order) and
+ * grown breadth-first; positions get the ids the reference's FIFO walk gives
+ * them.  The result stays in HBM: geq_built_solve runs the fixpoint on it
+ * (EnergyGame::standard_reach + run), geq_built_download copies graph and
+ * positions to the host. */
+typedef struct {
+    uint32_t n_states;
+    uint32_t n_transitions;
+    const uint32_t* row_offsets;   /* n_states + 1; each state's transitions sorted by label (src/lib.rs:489-493) */
+    const uint32_t* targets;       /* n_transitions */
+    const int32_t* labels;         /* n_transitions */
+} geq_lts;
+
+typedef struct {
+    uint64_t positions;
+    uint64_t edges;
+    uint64_t levels;
+    uint64_t set_words;            /* words of Q / Q* set data written */
+    double build_ms;               /* wall time of the device build */
+    uint32_t truncated;            /* stopped after max_positions: the last level has no successors */
+    uint32_t pad;
+} geq_build_stats;
+
+/* Host copy of a built game.  Position i: kind[i] (0 attack, 1 clause,
+ * 2 defend), process p[i], Q = pool[q_off[i] .. +q_len[i]) (the clause's q for
+ * a clause), Q* = pool[x_off[i] .. +x_len[i]) (defend only).  weights use the
+ * STANDARD packing (6 x 4 bits). */
+typedef struct {
+    uint32_t n_positions, n_edges, n_start;
+    uint32_t* row_offsets;
+    uint32_t* column_indices;
+    uint32_t* weights;
+    uint8_t* attacker_pos;
+    uint8_t* kind;
+    uint32_t *p, *q_off, *q_len, *x_off, *x_len;
+    uint32_t* pool;
+    uint64_t pool_len;
+} geq_built_game;
+
+typedef struct geq_built geq_built;
+
+/* max_positions: 0 = unbounded; otherwise the build stops after the level that
+ * crosses it (stats->truncated = 1). */
+geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs, uint64_t max_positions,
+                         geq_built** out, geq_build_stats* stats);
+geq_status geq_built_download(geq_built* built, geq_built_game* out);
+void geq_built_game_free(geq_built_game* g);
+/* standard_reach solve of the built game on the device */
+geq_status geq_built_solve(geq_built* built, int profile_kernels, geq_stats* stats);
+/* energies of positions [first, first + count) after geq_built_solve */
+geq_status geq_built_energies(geq_built* built, uint32_t first, uint32_t count, geq_energies* out);
+void geq_built_free(geq_built* built);
+
 /* ---- helpers (no device needed) ---- */
+/* TransitionSystem::signature_refinement (src/lib.rs:444-478): partition[s] is
+ * the strong-bisimulation class of s, numbered by first occurrence. */
+geq_status geq_signature_refinement(const geq_lts* lts, uint32_t* partition, uint32_t* n_classes);
+/* TransitionSystem::bisimilar_minimize (src/lib.rs:413-432): the quotient LTS
+ * (out_row: n_classes + 1 entries; out_targets / out_labels: at most
+ * n_transitions entries each) and the partition used. */
+geq_status geq_bisimilar_minimize(const geq_lts* lts, uint32_t* partition, uint32_t* n_classes, uint32_t* out_row,
+                                  uint32_t* out_targets, int32_t* out_labels, uint32_t* out_n_transitions);
 /* standard_reach: defender positions without successors (:347-352). Writes at
  * most n_vertices ids into out, returns the count through *n_out. */
 geq_status geq_standard_reach(const geq_graph* graph, uint32_t* out, uint32_t* n_out);

@@ -1,0 +1,144 @@
+"""GPU gamebuild (geq_gamebuild) and the LTS pipeline against the oracle and
+the reference's own tests (tests/complete.rs, src/lib.rs doctests)."""
+import os
+
+import numpy as np
+import pytest
+
+from gpuequiv_amd import (EnergyArray, EnergyConf, GameBuild, Position, TransitionSystem, std_equivalences,
+                          workloads)
+from oracle import oracle as orc
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+STD = EnergyConf.STANDARD
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _fixture_pos(p):
+    if p[0] == "A":
+        return Position.attack(p[1], p[2])
+    if p[0] == "C":
+        return Position.clause(p[1], p[2])
+    return Position.defend(p[1], p[2], p[3])
+
+
+def _oracle_pos(p):
+    return _fixture_pos(p)
+
+
+def _assert_same_game(b: GameBuild, lts, pairs):
+    og = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs)
+    h = b._download()
+    assert b.n_positions() == og["V"]
+    assert b.nodes == [_oracle_pos(p) for p in og["positions"]]
+    assert np.array_equal(h["row"], og["row"])
+    assert np.array_equal(h["col"], og["col"])
+    assert np.array_equal(h["wt"], og["wt"])
+    assert np.array_equal(h["atk"], og["atk"])
+    return og
+
+
+def _assert_same_energies(b: GameBuild, og):
+    reach = [v for v in range(og["V"]) if not og["atk"][v] and og["row"][v] == og["row"][v + 1]]
+    offs, data, st = orc.solve(6, 3, og["row"], og["col"], og["wt"], og["atk"], reach)
+    s = b.solve()
+    o, d = b.energies_raw(0, og["V"])
+    assert H.canon_lists(o, d) == H.canon_lists(offs, data)
+    assert s["rounds"] == st["rounds"] and s["edge_updates"] == st["edge_updates"]
+
+
+# ---------------------------------------------------------------- tests/complete.rs
+def test_bisping_fig4_game_and_energies():
+    d = H.load("complete_lts.json")["bisping_fig4"]
+    lts = TransitionSystem.from_edges(d["transitions"])
+    b = GameBuild.compare(lts, *d["compare"])
+    assert b.nodes == [_fixture_pos(p) for p in d["positions"]]
+    g = b.game
+    for v, adj in enumerate(d["adjacency_sorted"]):
+        assert sorted(int(x) for x in g.adj(v)) == adj
+    b.solve()
+    got = b.energies(0, b.n_positions())
+    for v, exp in enumerate(d["energies"]):
+        assert got[v] == EnergyArray.from_conf(exp, STD), v
+
+
+def test_bisimilar_compare_is_empty():
+    d = H.load("complete_lts.json")["bisimilar"]
+    lts = TransitionSystem.from_edges(d["transitions"])
+    e = lts.compare(*d["compare"])
+    assert e == EnergyArray.empty(STD)
+    assert e.test_equivalence(std_equivalences.bisimulation())
+
+
+def test_failure_trace_verdicts():
+    d = H.load("complete_lts.json")["failure_trace"]
+    lts = TransitionSystem.from_edges(d["transitions"])
+    budgets = lts.compare(*d["compare"])
+    for name in d["holds"]:
+        assert budgets.test_equivalence(getattr(std_equivalences, name)()), name
+    for name in d["fails"]:
+        assert not budgets.test_equivalence(getattr(std_equivalences, name)()), name
+
+
+def test_doctests():
+    d = H.load("complete_lts.json")["doctest"]
+    lts = TransitionSystem.from_edges(d["transitions"])
+    e = lts.compare(0, 4)
+    assert not e.test_equivalence(std_equivalences.simulation())
+    assert e.test_equivalence(std_equivalences.traces())
+    eq = lts.equivalences()
+    for p, q, name in d["preorder_true"]:
+        assert eq.preorder(p, q, getattr(std_equivalences, name)())
+    for p, q, name in d["preorder_false"]:
+        assert not eq.preorder(p, q, getattr(std_equivalences, name)())
+    for p, q, name in d["equiv_true"]:
+        assert eq.equiv(p, q, getattr(std_equivalences, name)())
+    for p, q, name in d["equiv_false"]:
+        assert not eq.equiv(p, q, getattr(std_equivalences, name)())
+    traces = eq.relation(std_equivalences.traces())
+    assert traces.count_classes() == d["traces_count_classes"]
+    assert list(traces.class_of(0)) == d["traces_class_of_0"]
+
+
+# ---------------------------------------------------------------- parity with the oracle gamebuild
+@pytest.mark.parametrize("n,m,labels,seed", [(12, 30, 2, 1), (30, 90, 3, 2), (60, 200, 4, 3), (300, 1200, 8, 1)])
+def test_compare_all_matches_oracle(n, m, labels, seed):
+    print("random lts", n, m, labels, seed, flush=True)
+    n, src, dst, lbl = workloads.random_lts(n, m, labels, seed)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    if n >= 300:
+        lts, _ = lts.bisimilar_minimize()
+    pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q]
+    b = GameBuild(lts, pairs)
+    og = _assert_same_game(b, lts, pairs)
+    _assert_same_energies(b, og)
+
+
+def test_vlts_peterson_matches_oracle():
+    lts = TransitionSystem.from_csv_file(os.path.join(GOLDEN, "peterson_mutex_weak.csv"))
+    lts, _ = lts.bisimilar_minimize()
+    pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q]
+    b = GameBuild(lts, pairs)
+    og = _assert_same_game(b, lts, pairs)
+    _assert_same_energies(b, og)
+
+
+def test_duplicate_start_pairs_keep_the_last_id():
+    # GameBuild::new_node maps a repeated start position to its last id
+    lts = TransitionSystem.from_edges([(0, 1, 0), (1, 0, 0), (0, 2, 1), (2, 2, 0), (1, 2, 1)])
+    pairs = [(0, 1), (1, 0), (0, 1), (2, 1)]
+    b = GameBuild(lts, pairs)
+    _assert_same_game(b, lts, pairs)
+
+
+def test_truncated_build_is_well_formed():
+    n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q][:50]
+    b = GameBuild(lts, pairs, max_positions=2000)
+    assert b.build_stats["truncated"] == 1
+    h = b._download()
+    V = b.n_positions()
+    assert V > 2000 and h["row"][-1] == len(h["col"]) and np.all(np.diff(h["row"].astype(np.int64)) >= 0)
+    assert h["col"].max() < V

@@ -1,0 +1,68 @@
+"""Host-side LTS handling: CSV ingest, label-sorted rows, bisimulation
+minimisation (native code, checked against the oracle).  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+from gpuequiv_amd import TransitionSystem, Transition, workloads
+from oracle import oracle as orc
+from tests import helpers as H
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_from_edges_sorts_rows_stably_by_label():
+    lts = TransitionSystem.from_edges([(0, 3, 2), (0, 1, 1), (0, 2, 1), (1, 0, 0)])
+    assert lts.adj(0) == [Transition(1, 1), Transition(2, 1), Transition(3, 2)]
+    assert lts.adj(1) == [Transition(0, 0)]
+    assert lts.n_vertices() == 4 and lts.adj(3) == []
+
+
+def test_csv_label_numbering():
+    # src/lib.rs:372-406: "i" is label 0, then first appearance order, quotes stripped
+    lts = TransitionSystem.from_csv_lines(['0,1,"a"', "1,2,i", "2,0,b", '0,2,"i"'])
+    assert lts.adj(0) == [Transition(2, 0), Transition(1, 1)]
+    assert lts.adj(1) == [Transition(2, 0)]
+    assert lts.adj(2) == [Transition(0, 2)]
+    with pytest.raises(ValueError):
+        TransitionSystem.from_csv_lines(["0,1"])
+
+
+def test_vlts_csv_matches_oracle_order():
+    lts = TransitionSystem.from_csv_file(os.path.join(GOLDEN, "peterson_mutex_weak.csv"))
+    assert lts.n_vertices() == 20 and lts.n_transitions() == 138
+    for p in range(lts.n_vertices()):
+        labels = [t.label for t in lts.adj(p)]
+        assert labels == sorted(labels)
+
+
+def _lts_random(n, m, labels, seed):
+    n, src, dst, lbl = workloads.random_lts(n, m, labels, seed)
+    return TransitionSystem.from_arrays(n, src, dst, lbl)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_bisimulation_matches_oracle(seed):
+    lts = _lts_random(200, 500, 3, seed)
+    part, cnt = lts.signature_refinement()
+    opart, ocnt = orc.signature_refinement(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels)
+    assert cnt == ocnt and part == [int(x) for x in opart]
+    red, part2 = lts.bisimilar_minimize()
+    op, oc, orow, odst, olbl = orc.bisimilar_minimize(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels)
+    assert part2 == [int(x) for x in op]
+    assert np.array_equal(red.row_offsets, orow) and np.array_equal(red.targets, odst) and np.array_equal(red.labels, olbl)
+
+
+def test_bisimilar_golden():
+    d = H.load("complete_lts.json")["bisimilar"]
+    lts = TransitionSystem.from_edges(d["transitions"])
+    part, _ = lts.signature_refinement()
+    assert part[0] == part[5]
+
+
+def test_bisimulation_on_vlts():
+    lts = TransitionSystem.from_csv_file(os.path.join(GOLDEN, "vasy_0_1.csv"))
+    part, cnt = lts.signature_refinement()
+    opart, ocnt = orc.signature_refinement(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels)
+    assert cnt == ocnt and part == [int(x) for x in opart]
