@@ -120,6 +120,30 @@ class geq_built_game(C.Structure):
     ]
 
 
+class geq_server_opts(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("edges_per_position", C.c_double), ("max_jobs", C.c_uint32),
+                ("rounds_per_step", C.c_uint32), ("compact_entries", C.c_uint64), ("max_positions", C.c_uint64),
+                ("builders", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class geq_job_info(C.Structure):
+    _fields_ = [("state", C.c_uint32), ("status", C.c_uint32), ("positions", C.c_uint32), ("edges", C.c_uint32),
+                ("t_submit_ms", C.c_double), ("t_built_ms", C.c_double), ("t_admit_ms", C.c_double),
+                ("t_done_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class geq_server_stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("steps", "rounds", "compactions", "node_slots", "node_slots_used", "edge_slots",
+                                          "edge_slots_used", "arena_entries", "arena_top", "jobs_resident",
+                                          "jobs_waiting", "jobs_done")]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
 P = C.POINTER
 _SIGS = {
     "geq_last_error": (C.c_char_p, []),
@@ -144,6 +168,14 @@ _SIGS = {
     "geq_built_energies": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(geq_energies)]),
     "geq_built_free": (None, [C.c_void_p]),
     "geq_signature_refinement": (C.c_int, [P(geq_lts), P(C.c_uint32), P(C.c_uint32)]),
+    "geq_server_create": (C.c_int, [C.c_void_p, P(geq_server_opts), P(C.c_void_p)]),
+    "geq_server_destroy": (None, [C.c_void_p]),
+    "geq_server_submit": (C.c_int, [C.c_void_p, P(geq_lts), P(C.c_uint32), C.c_uint32, P(C.c_uint64)]),
+    "geq_server_step": (C.c_int, [C.c_void_p, P(C.c_uint32), P(C.c_uint32)]),
+    "geq_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, P(geq_job_info), P(geq_energies)]),
+    "geq_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "geq_server_get_stats": (C.c_int, [C.c_void_p, P(geq_server_stats)]),
+    "geq_server_finished": (C.c_int, [C.c_void_p, P(C.c_uint64), C.c_uint32, P(C.c_uint32)]),
     "geq_bisimilar_minimize": (C.c_int, [P(geq_lts), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
                                          P(C.c_int32), P(C.c_uint32)]),
 }

@@ -248,12 +248,7 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
     if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
     if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
-    gm->arena_cap = std::max<uint32_t>(1u << 20, (uint32_t)std::min<uint64_t>(8ull * E + 1024, 0x7fffffffull));
-    if (const char* env = std::getenv("GEQ_ARENA_ENTRIES")) {
-        // initial energy-arena size (u16 entries); small values force the regrow path
-        long long v = std::atoll(env);
-        if (v >= 64 && v < 0x7fffffffll) gm->arena_cap = (uint32_t)v;
-    }
+    gm->arena_cap = initial_arena_entries(E, 0);
     if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
         return oom(e, "pinned control block");
@@ -297,8 +292,19 @@ __global__ void k_seed_zero(uint4* cur, const uint32_t* ids, uint32_t n) {
     if (i < n) cur[ids[i]] = make_uint4(1u, 0u, 0u, 0u);   // EnergyArray::zero(1): one all-zero energy
 }
 
+}  // namespace
+
 // Grow the arena keeping its content (lists of the committed state live in [0, top)).
-geq_status grow_arena(geq_game* g, uint32_t needed_top) {
+uint32_t geq::initial_arena_entries(uint64_t E, uint64_t want) {
+    uint64_t n = want ? want : std::max<uint64_t>(1u << 20, 8ull * E + 1024);
+    if (const char* env = std::getenv("GEQ_ARENA_ENTRIES")) {
+        const long long v = std::atoll(env);
+        if (v >= 64) n = (uint64_t)v;
+    }
+    return (uint32_t)std::min<uint64_t>(n, 0x7fffffffull);
+}
+
+geq_status geq::grow_arena(geq_game* g, uint32_t needed_top) {
     uint64_t cap = std::max<uint64_t>(2ull * g->arena_cap, (uint64_t)needed_top + (1u << 20));
     cap = std::min<uint64_t>(cap, 0xffffffffull);
     if (cap <= g->arena_cap) return fail(GEQ_ERR_OUT_OF_MEMORY, "energy arena exceeds 2^32 entries");
@@ -318,13 +324,8 @@ geq_status grow_arena(geq_game* g, uint32_t needed_top) {
     return GEQ_OK;
 }
 
-}  // namespace
 
-namespace {
-
-struct RoundShape {
-    uint32_t eval_blocks, expand_blocks, rounds_per_launch, coop_blocks;
-};
+namespace geq {
 
 // Launch shape of one round.  Kernels use grid-stride loops over the device
 // worklist, so the shape only has to fill the chip: a few blocks per CU.
@@ -366,7 +367,7 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
     return GEQ_OK;
 }
 
-}  // namespace
+}  // namespace geq
 
 extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint32_t n_reach, const geq_energies* initial,
                                      int profile_kernels, geq_stats* stats) {

@@ -78,22 +78,8 @@ uint32_t read_u32(const uint32_t* d, hipStream_t s) {
 
 }  // namespace
 
-struct geq_built {
-    geq_ctx* ctx = nullptr;
-    uint32_t V = 0, E = 0, n_start = 0;
-    uint64_t spool_top = 0;
-    DevBuf<uint8_t> kind, atk;
-    DevBuf<uint32_t> p, q_off, q_len, x_off, x_len, hash, spool, row, col, wt;
-    geq_game* game = nullptr;      // created on the first solve
-    uint32_t n_reach = 0;
-    ~geq_built() {
-        if (game) geq_game_free(game);
-    }
-};
 
-geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,
-                                 const uint32_t* d_wt, const uint8_t* d_atk, geq_game** out) {
-    hipStream_t s = ctx->stream;
+geq_status geq::alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_game** out, uint64_t arena_entries) {
     geq_game* gm = new geq_game();
     gm->ctx = ctx;
     check_conf(geq_conf{6, 3}, gm->conf);
@@ -119,11 +105,27 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
     if ((e = gm->stamp.alloc(V)) != hipSuccess) return oom(e, "stamps");
     if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
     if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
-    gm->arena_cap = std::max<uint32_t>(1u << 20, (uint32_t)std::min<uint64_t>(8ull * E + 1024, 0x7fffffffull));
+    gm->arena_cap = initial_arena_entries(E, arena_entries);
     if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
         return oom(e, "pinned control block");
     if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
+    *out = gm;
+    return GEQ_OK;
+}
+
+geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,
+                                 const uint32_t* d_wt, const uint8_t* d_atk, geq_game** out) {
+    hipStream_t s = ctx->stream;
+    geq_game* gm = nullptr;
+    geq_status st0 = alloc_game_buffers(ctx, V, E, &gm);
+    if (st0 != GEQ_OK) return st0;
+    auto oom = [&](hipError_t e, const char* what) -> geq_status {
+        delete gm;
+        return fail(e == hipErrorOutOfMemory ? GEQ_ERR_OUT_OF_MEMORY : GEQ_ERR_NO_DEVICE,
+                    std::string("allocating ") + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
     DevBuf<uint32_t> cnt, fill;
     DevBuf<uint8_t> tmp;
     if ((e = cnt.alloc(V + 1)) != hipSuccess || (e = fill.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");

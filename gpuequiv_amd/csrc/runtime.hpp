@@ -55,6 +55,19 @@ geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_en
 // packed updates, attacker flags; the reverse CSR is derived on the device.
 geq_status game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,
                             const uint32_t* d_wt, const uint8_t* d_atk, geq_game** out);
+// Solver buffers for V positions / E edges (contents uninitialised).
+// Energy-arena entries (u16) a game starts with: `want`, or 8 per edge when 0;
+// the environment variable GEQ_ARENA_ENTRIES overrides both (small values
+// force the regrow path).
+uint32_t initial_arena_entries(uint64_t E, uint64_t want);
+geq_status alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_game** out, uint64_t arena_entries = 0);
+// Launch shape of one round and the hipGraph of `rounds_per_launch` captured rounds.
+struct RoundShape {
+    uint32_t eval_blocks, expand_blocks, rounds_per_launch, coop_blocks;
+};
+RoundShape round_shape(const geq_game* g);
+geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape);
+geq_status grow_arena(geq_game* g, uint32_t needed_top);
 
 }  // namespace geq
 
@@ -91,4 +104,18 @@ struct geq_game {
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds
     void* graph_arena = nullptr;          // arena pointer baked into graph_exec
     ~geq_game();
+};
+
+// A game built on the device (geq_gamebuild): positions, set pool, CSR.
+struct geq_built {
+    geq_ctx* ctx = nullptr;
+    uint32_t V = 0, E = 0, n_start = 0;
+    uint64_t spool_top = 0;
+    geq::DevBuf<uint8_t> kind, atk;
+    geq::DevBuf<uint32_t> p, q_off, q_len, x_off, x_len, hash, spool, row, col, wt;
+    geq_game* game = nullptr;      // created on the first solve
+    uint32_t n_reach = 0;
+    ~geq_built() {
+        if (game) geq_game_free(game);
+    }
 };

@@ -1,0 +1,643 @@
+// Serving with continuous batching: spectroscopy jobs (LTS + process pairs)
+// arrive at any time; their games are built on a second stream by a builder
+// thread, then placed into shared HBM pools at a round boundary and seeded into
+// the shared device worklist, so every round evaluates the frontier of all
+// resident jobs together.  A job retires as soon as none of its positions is
+// on the worklist; its starting positions' budgets are gathered and its pool
+// ranges are reused.  Jobs are disjoint subgraphs of the pools and every round
+// is evaluated per node, so a job's result is the same alone and in the mix.
+//
+// There is no counterpart in the reference, which solves one game per
+// TransitionSystem::compare / equivalences call (src/lib.rs:275-343).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "gamebuild.hpp"
+#include "runtime.hpp"
+
+using namespace geq;
+
+namespace {
+
+// per-node bytes of the pools (row, rrow, atk, cur, nxt, wl x2, chg, stamp, job_of) and per-edge bytes (edges, rcol)
+constexpr uint64_t NODE_BYTES = 4 + 4 + 1 + 16 + 16 + 4 + 4 + 1 + 4 + 4;
+constexpr uint64_t EDGE_BYTES = 8 + 4;
+
+// Place a built game at node base nb / edge base eb of the pools.
+__global__ void k_place_nodes(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t nb, uint32_t eb, uint32_t job,
+                              uint32_t* prow, uint8_t* patk, uint4* cur, uint32_t* stamp, uint32_t* job_of) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > V) return;
+    prow[nb + v] = eb + row[v];             // v == V: the row end of the job (padding slot)
+    patk[nb + v] = v < V ? atk[v] : (uint8_t)1;
+    cur[nb + v] = make_uint4(0, 0, 0, 0);
+    stamp[nb + v] = 0;
+    job_of[nb + v] = job;
+}
+// a released range holds no lists (the arena compaction walks every slot)
+__global__ void k_clear_nodes(uint4* cur, uint32_t nb, uint32_t n) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) cur[nb + v] = make_uint4(0, 0, 0, 0);
+}
+// arena compaction: arena entries each list needs, then the move to the packed offsets
+__global__ void k_arena_need(const uint4* cur, uint32_t n, uint32_t* need) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > n) return;
+    const uint32_t len = v < n ? nl_len(cur[v]) : 0u;
+    need[v] = len > INLINE_CAP ? len : 0u;
+}
+__global__ void k_arena_move(uint4* cur, const uint16_t* src, uint16_t* dst, const uint32_t* offs, uint32_t n) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint4 d = cur[v];
+    const uint32_t len = nl_len(d);
+    if (len <= INLINE_CAP) return;
+    const uint32_t from = nl_off(d), to = offs[v];
+    for (uint32_t j = 0; j < len; j++) dst[to + j] = src[from + j];
+    d.y = to;
+    cur[v] = d;
+}
+__global__ void k_place_edges(const uint32_t* col, const uint32_t* wt, uint32_t E, uint32_t nb, uint32_t eb, uint2* edges) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) edges[eb + e] = make_uint2(nb + col[e], wt[e]);
+}
+__global__ void k_place_indeg(const uint32_t* col, uint32_t E, uint32_t* cnt) {
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) atomicAdd(&cnt[col[e]], 1u);
+}
+__global__ void k_place_rrow(const uint32_t* excl, uint32_t V, uint32_t eb, uint32_t nb, uint32_t* prrow, uint32_t* fill) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > V) return;
+    prrow[nb + v] = eb + excl[v];
+    if (v < V) fill[v] = excl[v];
+}
+__global__ void k_place_rcol(const uint32_t* row, const uint32_t* col, uint32_t V, uint32_t nb, uint32_t eb, uint32_t* fill,
+                             uint32_t* prcol) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    for (uint32_t e = row[v]; e < row[v + 1]; e++) prcol[eb + atomicAdd(&fill[col[e]], 1u)] = nb + v;
+}
+// nodes to reach (defender dead ends): zero energy, and the seed list
+__global__ void k_place_reach(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t nb, uint4* cur, uint32_t* seed,
+                              uint32_t* n) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V && !atk[v] && row[v] == row[v + 1]) {
+        cur[nb + v] = make_uint4(1u, 0u, 0u, 0u);
+        seed[atomicAdd(n, 1u)] = nb + v;
+    }
+}
+// jobs that still have positions on the worklist of the current round
+__global__ void k_mark_active(const Ctl* ctl, uint32_t* const* wl, const uint32_t* job_of, uint8_t* active) {
+    const uint32_t r = ctl->round;
+    const uint32_t n = ctl->wl_n[r & 1u];
+    const uint32_t* list = wl[r & 1u];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) active[job_of[list[i]]] = 1;
+}
+
+enum JobState : uint32_t { QUEUED = 0, BUILDING = 1, BUILT = 2, ACTIVE = 3, DONE = 4, FAILED = 5 };
+
+struct Job {
+    uint64_t id = 0;
+    uint32_t n_states = 0;
+    std::vector<uint32_t> row, dst, pairs;
+    std::vector<int32_t> lbl;
+    geq_built* built = nullptr;
+    geq_status status = GEQ_OK;
+    std::string error;
+    uint32_t V = 0, E = 0, n_start = 0, nb = 0, eb = 0, slot = 0;
+    uint32_t positions = 0, edges = 0;   // game size once built
+    uint32_t admit_round = 0, done_round = 0;
+    double t_submit = 0, t_built = 0, t_admit = 0, t_done = 0;
+    std::atomic<uint32_t> state{QUEUED};
+    std::vector<uint32_t> offs, data;
+};
+
+#define GEQ_TRY(x)                          \
+    do {                                    \
+        const geq_status st_ = (x);         \
+        if (st_ != GEQ_OK) return st_;      \
+    } while (0)
+
+// Scratch that holds at least n elements.  A buffer that has to grow is
+// replaced only after the stream has drained: queued kernels may still use it.
+template <class T>
+geq_status ensure_buf(DevBuf<T>& b, size_t n, hipStream_t s) {
+    if (b.p && b.n >= n) return GEQ_OK;
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(b.alloc(std::max<size_t>(n + n / 4, 1024)), GEQ_ERR_OUT_OF_MEMORY);
+    return GEQ_OK;
+}
+
+// first-fit allocator over an index range
+struct RangeAlloc {
+    std::map<uint32_t, uint32_t> free_;   // start -> length
+    uint64_t total = 0, used = 0;
+    void init(uint32_t n) {
+        free_.clear();
+        if (n) free_[0] = n;
+        total = n;
+        used = 0;
+    }
+    bool alloc(uint32_t len, uint32_t& start) {
+        if (len == 0) {
+            start = 0;
+            return true;
+        }
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= len) {
+                start = it->first;
+                const uint32_t rest = it->second - len;
+                free_.erase(it);
+                if (rest) free_[start + len] = rest;
+                used += len;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint32_t start, uint32_t len) {
+        if (len == 0) return;
+        used -= len;
+        auto it = free_.emplace(start, len).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+}  // namespace
+
+struct geq_server {
+    geq_ctx* ctx = nullptr;
+    std::vector<geq_ctx*> bctx;        // builders' contexts: a stream each
+    geq_game* pool = nullptr;          // pools of positions / edges / energies
+    DevBuf<uint32_t> job_of, seed_cnt, place_cnt, place_fill, place_excl;
+    DevBuf<uint8_t> active, scan_tmp;
+    DevBuf<uint32_t*> wl_ptrs;
+    std::vector<uint8_t> h_active;
+    uint32_t max_jobs = 0;
+    uint32_t rounds_per_step = 4;
+    RangeAlloc nodes, edges;
+    std::vector<uint32_t> free_slots;
+    std::map<uint64_t, Job*> jobs;
+    std::deque<Job*> build_q, ready_q;
+    std::vector<Job*> resident;
+    std::mutex mu;
+    std::condition_variable cv;
+    bool stop = false;
+    uint32_t max_ready = 32;
+    std::vector<std::thread> builders;
+    std::vector<uint64_t> finished;    // ids of jobs done or failed, not yet taken
+    uint64_t next_id = 1;
+    std::chrono::steady_clock::time_point t0;
+    uint64_t steps = 0, rounds_total = 0, compactions = 0, jobs_done = 0;
+    uint64_t arena_min = 0;            // arena entries the pools start with
+    uint64_t compact_entries = 0;      // compaction threshold (0: 3/4 of the arena)
+    uint64_t max_positions = 0;        // per-job game size limit (0: the node pool)
+    double now_ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+    ~geq_server();
+};
+
+namespace {
+
+void builder_main(geq_server* S, geq_ctx* bctx) {
+    (void)hipSetDevice(bctx->device);
+    while (true) {
+        Job* J = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait(lk, [&] { return S->stop || (!S->build_q.empty() && S->ready_q.size() < S->max_ready); });
+            if (S->stop) return;
+            J = S->build_q.front();
+            S->build_q.pop_front();
+            J->state = BUILDING;
+        }
+        geq_lts l{J->n_states, (uint32_t)J->dst.size(), J->row.data(), J->dst.data(), J->lbl.data()};
+        geq_build_stats bs{};
+        geq_built* B = nullptr;
+        geq_status st = geq_gamebuild(bctx, &l, J->pairs.data(), (uint32_t)(J->pairs.size() / 2), S->max_positions, &B, &bs);
+        if (st == GEQ_OK && bs.truncated) {
+            geq_built_free(B);
+            B = nullptr;
+            st = fail(GEQ_ERR_OUT_OF_MEMORY, "the job's game grows past " + std::to_string(S->max_positions) + " positions");
+        }
+        std::lock_guard<std::mutex> lk(S->mu);
+        J->t_built = S->now_ms();
+        if (st != GEQ_OK) {
+            J->status = st;
+            J->error = geq_last_error();
+            J->state = FAILED;
+            J->t_done = J->t_built;
+            S->finished.push_back(J->id);
+            continue;
+        }
+        J->positions = B->V;
+        J->edges = B->E;
+        J->built = B;
+        J->state = BUILT;
+        S->ready_q.push_back(J);
+    }
+}
+
+// Retired jobs leave their long lists behind in the energy arena, whose top
+// only grows while rounds run.  Between steps (when only the committed lists
+// are live) the lists of every pool slot are packed into a fresh arena.
+geq_status compact_arena(geq_server* S) {
+    geq_game* g = S->pool;
+    hipStream_t s = S->ctx->stream;
+    const uint32_t n = g->V, tb = 256;
+    GEQ_TRY(ensure_buf(S->place_cnt, n + 1, s));
+    GEQ_TRY(ensure_buf(S->place_excl, n + 1, s));
+    k_arena_need<<<(n + 1 + tb - 1) / tb, tb, 0, s>>>(g->cur.p, n, S->place_cnt.p);
+    size_t bytes = 0;
+    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, S->place_cnt.p, S->place_excl.p, n + 1, s), GEQ_ERR_INTERNAL);
+    GEQ_TRY(ensure_buf(S->scan_tmp, bytes + 16, s));
+    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(S->scan_tmp.p, bytes, S->place_cnt.p, S->place_excl.p, n + 1, s), GEQ_ERR_INTERNAL);
+    uint32_t live = 0;
+    GEQ_HIP(hipMemcpyAsync(&live, S->place_excl.p + n, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(S->arena_min, 2ull * live + (1u << 20)), 0xffffffffull);
+    uint16_t* p = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), 2ull * cap);
+    if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("compacting the energy arena: ") + hipGetErrorString(e));
+    k_arena_move<<<(n + tb - 1) / tb, tb, 0, s>>>(g->cur.p, g->arena.p, p, S->place_excl.p, n);
+    e = hipMemcpyAsync(&g->ctl.p->arena_top, &live, 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return fail(GEQ_ERR_TRANSFER, std::string("compacting the energy arena: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(g->arena.p);
+    g->arena.p = p;
+    g->arena.n = cap;
+    g->arena_cap = (uint32_t)cap;
+    g->h_ctl->arena_top = live;
+    S->compactions++;
+    return GEQ_OK;
+}
+
+}  // namespace
+
+geq_server::~geq_server() {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+    }
+    cv.notify_all();
+    for (auto& t : builders)
+        if (t.joinable()) t.join();
+    for (auto& kv : jobs) {
+        if (kv.second->built) geq_built_free(kv.second->built);
+        delete kv.second;
+    }
+    if (pool) geq_game_free(pool);
+    for (geq_ctx* c : bctx) geq_ctx_destroy(c);
+}
+
+extern "C" {
+
+geq_status geq_server_create(geq_ctx* ctx, const geq_server_opts* opts, geq_server** out) {
+    if (!ctx || !out) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    const uint64_t budget = opts && opts->budget_bytes ? opts->budget_bytes : (4ull << 30);
+    const double edge_ratio = opts && opts->edges_per_position > 0 ? opts->edges_per_position : 4.0;
+    // 7/8 of the budget for the position / edge pools, 1/8 for the energy arena
+    const uint64_t pool_bytes = budget - budget / 8;
+    uint64_t V = (uint64_t)(pool_bytes / (NODE_BYTES + edge_ratio * EDGE_BYTES));
+    V = std::min<uint64_t>(V, 0x7fff0000ull);
+    uint64_t E = std::min<uint64_t>((uint64_t)(V * edge_ratio), 0xfff00000ull);
+    if (V < 64) return fail(GEQ_ERR_INVALID_ARG, "serving budget too small");
+    geq_server* S = new geq_server();
+    S->ctx = ctx;
+    S->t0 = std::chrono::steady_clock::now();
+    S->max_jobs = opts && opts->max_jobs ? opts->max_jobs : 4096;
+    S->rounds_per_step = opts && opts->rounds_per_step ? opts->rounds_per_step : 4;
+    S->compact_entries = opts ? opts->compact_entries : 0;
+    S->max_positions = opts && opts->max_positions ? std::min<uint64_t>(opts->max_positions, V) : V;
+    const uint32_t n_builders = opts && opts->builders ? std::min(opts->builders, 16u) : 2u;
+    geq_status st = GEQ_OK;
+    for (uint32_t i = 0; i < n_builders; i++) {
+        geq_ctx* c = nullptr;
+        st = geq_ctx_create(ctx->device, &c);
+        if (st != GEQ_OK) {
+            delete S;
+            return st;
+        }
+        S->bctx.push_back(c);
+    }
+    st = alloc_game_buffers(ctx, (uint32_t)V, (uint32_t)E, &S->pool, std::max<uint64_t>(budget / 16, 4096));
+    if (st != GEQ_OK) {
+        delete S;
+        return st;
+    }
+    S->pool->tune.rounds_per_launch = S->rounds_per_step;
+    S->pool->tune.mode = 1;
+    S->arena_min = S->pool->arena_cap;
+    auto oom = [&](hipError_t e) {
+        delete S;
+        return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("serving pools: ") + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = S->job_of.alloc(V)) != hipSuccess) return oom(e);
+    if ((e = S->active.alloc(S->max_jobs)) != hipSuccess) return oom(e);
+    if ((e = S->seed_cnt.alloc(1)) != hipSuccess) return oom(e);
+    if ((e = S->wl_ptrs.alloc(2)) != hipSuccess) return oom(e);
+    uint32_t* w[2] = {S->pool->wl0.p, S->pool->wl1.p};
+    hipStream_t s = ctx->stream;
+    GEQ_HIP(hipMemcpyAsync(S->wl_ptrs.p, w, sizeof(w), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    Ctl c0{};
+    *S->pool->h_ctl = c0;
+    GEQ_HIP(hipMemcpyAsync(S->pool->ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipMemsetAsync(S->pool->stats.p, 0, (4ull * STAT_SLOTS + 4) * 8, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipMemsetAsync(S->active.p, 0, S->max_jobs, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    S->h_active.assign(S->max_jobs, 0);
+    S->nodes.init((uint32_t)V);
+    S->edges.init((uint32_t)E);
+    for (uint32_t i = S->max_jobs; i-- > 0;) S->free_slots.push_back(i);
+    for (geq_ctx* c : S->bctx) S->builders.emplace_back(builder_main, S, c);
+    *out = S;
+    return GEQ_OK;
+}
+
+void geq_server_destroy(geq_server* S) {
+    if (!S) return;
+    (void)hipSetDevice(S->ctx->device);
+    delete S;
+}
+
+geq_status geq_server_submit(geq_server* S, const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs, uint64_t* job_id) {
+    if (!S || !lts || !job_id || (n_pairs && !pairs) || !lts->row_offsets) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    Job* J = new Job();
+    J->n_states = lts->n_states;
+    J->row.assign(lts->row_offsets, lts->row_offsets + lts->n_states + 1);
+    J->dst.assign(lts->targets, lts->targets + lts->n_transitions);
+    J->lbl.assign(lts->labels, lts->labels + lts->n_transitions);
+    J->pairs.assign(pairs, pairs + 2ull * n_pairs);
+    J->n_start = n_pairs;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        J->id = S->next_id++;
+        J->t_submit = S->now_ms();
+        S->jobs[J->id] = J;
+        S->build_q.push_back(J);
+    }
+    S->cv.notify_all();
+    *job_id = J->id;
+    return GEQ_OK;
+}
+
+// One scheduling step: admit built jobs that fit, run rounds_per_step rounds
+// over all resident jobs, retire the jobs whose worklist share is empty.
+geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_waiting) {
+    if (!S) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    (void)hipSetDevice(S->ctx->device);
+    geq_game* g = S->pool;
+    hipStream_t s = S->ctx->stream;
+    const uint32_t tb = 256;
+    // ---- admission (HBM budget = the pools' free ranges)
+    std::vector<Job*> admit;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        while (!S->ready_q.empty()) {
+            Job* J = S->ready_q.front();
+            const uint32_t V = J->built->V, E = J->built->E;
+            if (V + 1ull > S->nodes.total || E > S->edges.total) {
+                // larger than the whole budget: it could never be admitted
+                J->status = GEQ_ERR_OUT_OF_MEMORY;
+                J->error = "job needs " + std::to_string(V) + " positions / " + std::to_string(E) +
+                           " edges, more than the serving budget holds";
+                J->state = FAILED;
+                J->t_done = S->now_ms();
+                S->finished.push_back(J->id);
+                geq_built_free(J->built);
+                J->built = nullptr;
+                S->ready_q.pop_front();
+                continue;
+            }
+            uint32_t nb = 0, eb = 0;
+            if (S->free_slots.empty() || !S->nodes.alloc(V + 1, nb)) break;
+            if (!S->edges.alloc(E, eb)) {
+                S->nodes.release(nb, V + 1);
+                break;
+            }
+            J->V = V;
+            J->E = E;
+            J->nb = nb;
+            J->eb = eb;
+            J->slot = S->free_slots.back();
+            S->free_slots.pop_back();
+            S->ready_q.pop_front();
+            admit.push_back(J);
+        }
+    }
+    S->cv.notify_all();
+    for (Job* J : admit) {
+        const uint32_t V = J->V, E = J->E;
+        const uint32_t* brow = J->built->row.p;
+        const uint32_t* bcol = J->built->col.p;
+        const uint32_t* bwt = J->built->wt.p;
+        const uint8_t* batk = J->built->atk.p;
+        // the builder's stream finished this game before it was queued as BUILT
+        k_place_nodes<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, J->eb, J->slot, g->row.p, g->atk.p, g->cur.p,
+                                                           g->stamp.p, S->job_of.p);
+        if (E) k_place_edges<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, bwt, E, J->nb, J->eb, g->edges.p);
+        // reverse CSR of the job's range
+        GEQ_TRY(ensure_buf(S->place_cnt, V + 1, s));
+        GEQ_TRY(ensure_buf(S->place_excl, V + 1, s));
+        GEQ_TRY(ensure_buf(S->place_fill, V + 1, s));
+        GEQ_HIP(hipMemsetAsync(S->place_cnt.p, 0, 4ull * (V + 1), s), GEQ_ERR_TRANSFER);
+        if (E) k_place_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, E, S->place_cnt.p);
+        size_t bytes = 0;
+        GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, S->place_cnt.p, S->place_excl.p, V + 1, s), GEQ_ERR_INTERNAL);
+        GEQ_TRY(ensure_buf(S->scan_tmp, bytes + 16, s));
+        GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(S->scan_tmp.p, bytes, S->place_cnt.p, S->place_excl.p, V + 1, s), GEQ_ERR_INTERNAL);
+        k_place_rrow<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(S->place_excl.p, V, J->eb, J->nb, g->rrow.p, S->place_fill.p);
+        k_place_rcol<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, bcol, V, J->nb, J->eb, S->place_fill.p, g->rcol.p);
+        // nodes to reach: zero energy + seeds of the current round
+        GEQ_TRY(ensure_buf(g->seed, V, s));
+        GEQ_HIP(hipMemsetAsync(S->seed_cnt.p, 0, 4, s), GEQ_ERR_TRANSFER);
+        k_place_reach<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, g->cur.p, g->seed.p, S->seed_cnt.p);
+        uint32_t n_seed = 0;
+        GEQ_HIP(hipMemcpyAsync(&n_seed, S->seed_cnt.p, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+        GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+        if (n_seed) {
+            ExpandArgs sx{{g->wl0.p, g->wl1.p}, g->seed.p, n_seed, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
+            k_expand<<<std::max(1u, std::min(1024u, (n_seed + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx);
+        }
+        geq_built_free(J->built);   // the pools hold the game now
+        J->built = nullptr;
+        std::lock_guard<std::mutex> lk(S->mu);
+        J->t_admit = S->now_ms();
+        J->state = ACTIVE;
+        S->resident.push_back(J);
+    }
+    // ---- rounds over every resident job
+    if (!S->resident.empty()) {
+        const RoundShape shape = round_shape(g);
+        EvalArgs ea{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->cur.p, g->nxt.p, g->arena.p, g->ctl.p,
+                    g->arena_cap, g->chg.p, g->conf.max, g->stats.p};
+        ExpandArgs xa{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
+        const uint32_t round0 = g->h_ctl->round;
+        while (true) {
+            if (!g->graph_exec || g->graph_arena != g->arena.p) {
+                geq_status st = capture_rounds(g, ea, xa, shape);
+                if (st != GEQ_OK) return st;
+            }
+            GEQ_HIP(hipGraphLaunch(g->graph_exec, s), GEQ_ERR_NO_DEVICE);
+            GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+            GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+            if (!g->h_ctl->overflow) {
+                S->rounds_total += g->h_ctl->round - round0;
+                break;
+            }
+            geq_status st = grow_arena(g, g->h_ctl->arena_top);
+            if (st != GEQ_OK) return st;
+            Ctl fix = *g->h_ctl;
+            fix.overflow = 0;
+            fix.wl_n[(fix.round + 1u) & 1u] = 0;
+            GEQ_HIP(hipMemcpyAsync(g->ctl.p, &fix, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+            GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+            *g->h_ctl = fix;
+            ea.arena = g->arena.p;
+            ea.arena_cap = g->arena_cap;
+        }
+        S->steps++;
+        // ---- retirement
+        GEQ_HIP(hipMemsetAsync(S->active.p, 0, S->max_jobs, s), GEQ_ERR_TRANSFER);
+        k_mark_active<<<256, 256, 0, s>>>(g->ctl.p, S->wl_ptrs.p, S->job_of.p, S->active.p);
+        GEQ_HIP(hipMemcpyAsync(S->h_active.data(), S->active.p, S->max_jobs, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+        GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+        std::vector<Job*> still;
+        uint32_t retired = 0;
+        for (Job* J : S->resident) {
+            if (S->h_active[J->slot]) {
+                still.push_back(J);
+                continue;
+            }
+            geq_energies res{};
+            geq_status st = download_energies(g, J->nb, J->n_start, &res);
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (st != GEQ_OK) {
+                J->status = st;
+                J->error = geq_last_error();
+                J->state = FAILED;
+            } else {
+                J->offs.assign(res.offsets, res.offsets + J->n_start + 1);
+                J->data.assign(res.data, res.data + res.offsets[J->n_start]);
+                geq_energies_free(&res);
+                J->state = DONE;
+                S->jobs_done++;
+            }
+            J->t_done = S->now_ms();
+            S->finished.push_back(J->id);
+            k_clear_nodes<<<(J->V + 1 + tb - 1) / tb, tb, 0, s>>>(g->cur.p, J->nb, J->V + 1);
+            S->nodes.release(J->nb, J->V + 1);
+            S->edges.release(J->eb, J->E);
+            S->free_slots.push_back(J->slot);
+            retired++;
+        }
+        S->resident.swap(still);
+        const uint64_t top = g->h_ctl->arena_top;
+        if (retired && (S->compact_entries ? top >= S->compact_entries : top * 4 > 3ull * g->arena_cap)) {
+            geq_status st = compact_arena(S);
+            if (st != GEQ_OK) return st;
+        }
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (n_resident) *n_resident = (uint32_t)S->resident.size();
+    if (n_waiting) *n_waiting = (uint32_t)(S->build_q.size() + S->ready_q.size());
+    return GEQ_OK;
+}
+
+geq_status geq_server_poll(geq_server* S, uint64_t job_id, geq_job_info* info, geq_energies* out) {
+    if (!S || !info) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return fail(GEQ_ERR_INVALID_ARG, "unknown job id");
+    Job* J = it->second;
+    std::memset(info, 0, sizeof(*info));
+    info->state = J->state;
+    info->t_submit_ms = J->t_submit;
+    info->t_built_ms = J->t_built;
+    info->t_admit_ms = J->t_admit;
+    info->t_done_ms = J->t_done;
+    info->positions = J->positions;
+    info->edges = J->edges;
+    info->status = J->status;
+    if (J->state == FAILED) {
+        geq::set_error(J->error);
+        return GEQ_OK;
+    }
+    if (J->state == DONE && out) {
+        out->n_nodes = J->n_start;
+        out->energy_size = 1;
+        out->offsets = (uint32_t*)std::malloc(4ull * (J->n_start + 1));
+        out->data = (uint32_t*)std::malloc(4ull * std::max<size_t>(J->data.size(), 1));
+        std::memcpy(out->offsets, J->offs.data(), 4ull * (J->n_start + 1));
+        if (!J->data.empty()) std::memcpy(out->data, J->data.data(), 4ull * J->data.size());
+    }
+    return GEQ_OK;
+}
+
+geq_status geq_server_get_stats(geq_server* S, geq_server_stats* out) {
+    if (!S || !out) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::memset(out, 0, sizeof(*out));
+    out->steps = S->steps;
+    out->rounds = S->rounds_total;
+    out->compactions = S->compactions;
+    out->node_slots = S->nodes.total;
+    out->node_slots_used = S->nodes.used;
+    out->edge_slots = S->edges.total;
+    out->edge_slots_used = S->edges.used;
+    out->arena_entries = S->pool->arena_cap;
+    out->arena_top = S->pool->h_ctl->arena_top;
+    out->jobs_resident = S->resident.size();
+    out->jobs_waiting = S->build_q.size() + S->ready_q.size();
+    out->jobs_done = S->jobs_done;
+    return GEQ_OK;
+}
+
+geq_status geq_server_finished(geq_server* S, uint64_t* ids, uint32_t cap, uint32_t* n) {
+    if (!S || !n || (cap && !ids)) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    const uint32_t k = (uint32_t)std::min<size_t>(cap, S->finished.size());
+    std::copy(S->finished.begin(), S->finished.begin() + k, ids);
+    S->finished.erase(S->finished.begin(), S->finished.begin() + k);
+    *n = k;
+    return GEQ_OK;
+}
+
+geq_status geq_server_release(geq_server* S, uint64_t job_id) {
+    if (!S) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return fail(GEQ_ERR_INVALID_ARG, "unknown job id");
+    if (it->second->state != DONE && it->second->state != FAILED) return fail(GEQ_ERR_INVALID_ARG, "job still running");
+    delete it->second;
+    S->jobs.erase(it);
+    return GEQ_OK;
+}
+
+}  // extern "C"

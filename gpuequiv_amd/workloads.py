@@ -97,3 +97,31 @@ def powerlaw_lts(n_states: int, n_transitions: int, n_labels: int, seed: int, al
     lbl = (rng.choice(n_labels - 1, n_transitions, p=pz) + 1).astype(np.int32)
     lbl[rng.random(n_transitions) < p_tau] = 0
     return n_states, src, dst, lbl
+
+
+def serve_job(rng: np.random.Generator, n_min: int = 100, n_max: int = 20000, per_state: int = 4,
+              labels_min: int = 4, labels_max: int = 64, p_tau: float = 0.05, pairs_max: int = 256):
+    """One spectroscopy job of the serving workload: states log-uniform in
+    [n_min, n_max], per_state transitions per state (uniform sources and
+    targets), the label count uniform in [labels_min, labels_max] with label 0
+    ("i") on a p_tau share of the transitions and the visible labels uniform,
+    and 1..pairs_max distinct pairs (p, q), p != q.
+    Returns (n, src, dst, lbl, pairs)."""
+    n = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+    m = per_state * n
+    n_labels = int(rng.integers(labels_min, labels_max + 1))
+    src = rng.integers(0, n, m, dtype=np.uint32)
+    dst = rng.integers(0, n, m, dtype=np.uint32)
+    lbl = rng.integers(1, n_labels, m).astype(np.int32)
+    lbl[rng.random(m) < p_tau] = 0
+    k = int(rng.integers(1, pairs_max + 1))
+    p = rng.integers(0, n, 4 * k + 8)
+    q = rng.integers(0, n, 4 * k + 8)
+    seen, pairs = set(), []
+    for a, b in zip(p.tolist(), q.tolist()):
+        if a != b and (a, b) not in seen:
+            seen.add((a, b))
+            pairs.append((a, b))
+            if len(pairs) == k:
+                break
+    return n, src, dst, lbl, pairs

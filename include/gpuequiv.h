@@ -196,6 +196,61 @@ geq_status geq_built_solve(geq_built* built, int profile_kernels, geq_stats* sta
 geq_status geq_built_energies(geq_built* built, uint32_t first, uint32_t count, geq_energies* out);
 void geq_built_free(geq_built* built);
 
+/* ---- serving: continuous batching of spectroscopy jobs ----
+ * Jobs (an LTS and the (p, q) pairs to compare; the game of
+ * TransitionSystem::compare_unminimized seeded with every pair) arrive at any
+ * time.  A builder thread builds each job's game on its own stream; at the
+ * next round boundary the game is placed into shared HBM pools (admission:
+ * its positions and edges must fit the pools' free ranges, i.e. the HBM
+ * budget) and its frontier joins the shared device worklist, so resident jobs
+ * are solved together, round by round.  A job retires as soon as none of its
+ * positions is on the worklist; its result is the same as when solved alone.
+ * No counterpart in the reference (one game per compare / equivalences call). */
+typedef struct geq_server geq_server;
+typedef struct {
+    uint64_t budget_bytes;         /* HBM: 7/8 for the position / edge pools, 1/8 for the energy
+                                      arena's initial size (default 4 GiB) */
+    double edges_per_position;     /* pool shape (default 4) */
+    uint32_t max_jobs;             /* resident jobs at most (default 4096) */
+    uint32_t rounds_per_step;      /* rounds per geq_server_step (default 4) */
+    uint64_t compact_entries;      /* after a retirement, pack the energy arena once its top passes this
+                                      many u16 entries (default 0: three quarters of its size) */
+    uint64_t max_positions;        /* a job whose game grows past this many positions fails with
+                                      GEQ_ERR_OUT_OF_MEMORY (0: no limit but the budget) */
+    uint32_t builders;             /* game-building threads, each with its own stream (default 2) */
+    uint32_t pad;
+} geq_server_opts;
+typedef struct {
+    uint32_t state;                /* 0 queued, 1 building, 2 built, 3 resident, 4 done, 5 failed */
+    uint32_t status;               /* geq_status of a failed job */
+    uint32_t positions, edges;
+    double t_submit_ms, t_built_ms, t_admit_ms, t_done_ms;   /* since geq_server_create */
+} geq_job_info;
+geq_status geq_server_create(geq_ctx* ctx, const geq_server_opts* opts, geq_server** out);
+void geq_server_destroy(geq_server* server);
+/* copies the inputs; returns immediately */
+geq_status geq_server_submit(geq_server* server, const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs,
+                             uint64_t* job_id);
+/* admit built jobs that fit, run rounds_per_step rounds, retire finished jobs */
+geq_status geq_server_step(geq_server* server, uint32_t* n_resident, uint32_t* n_waiting);
+/* state of a job; for a finished job, `out` (may be NULL) receives the budgets
+ * of its starting positions (one per pair, in submission order) */
+geq_status geq_server_poll(geq_server* server, uint64_t job_id, geq_job_info* info, geq_energies* out);
+/* ids of jobs that finished (done or failed) since the last call, oldest first;
+ * at most `cap` are taken */
+geq_status geq_server_finished(geq_server* server, uint64_t* ids, uint32_t cap, uint32_t* n);
+/* forget a finished job */
+geq_status geq_server_release(geq_server* server, uint64_t job_id);
+/* counters of the serving loop */
+typedef struct {
+    uint64_t steps, rounds, compactions;      /* steps that ran rounds, rounds, energy-arena compactions */
+    uint64_t node_slots, node_slots_used;     /* position pool: size, slots held by resident jobs */
+    uint64_t edge_slots, edge_slots_used;     /* edge pool: size, slots held by resident jobs */
+    uint64_t arena_entries, arena_top;        /* energy arena: size, entries in use (u16) */
+    uint64_t jobs_resident, jobs_waiting, jobs_done;
+} geq_server_stats;
+geq_status geq_server_get_stats(geq_server* server, geq_server_stats* out);
+
 /* ---- helpers (no device needed) ---- */
 /* TransitionSystem::signature_refinement (src/lib.rs:444-478): partition[s] is
  * the strong-bisimulation class of s, numbered by first occurrence. */

@@ -1,0 +1,133 @@
+"""Serving: every job's budgets are the same alone and in the mix (and equal
+the oracle's), also when the HBM budget forces jobs to wait for others and
+when the energy arena is packed between steps."""
+import numpy as np
+import pytest
+
+from gpuequiv_amd import GameBuild, TransitionSystem, workloads
+from gpuequiv_amd.serve import Server
+from oracle import oracle as orc
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+NODE_BYTES, EDGE_BYTES = 62, 12          # per-slot bytes of the position / edge pools
+
+
+def _budget(positions, edges_per_position):
+    """Bytes for pools of `positions` slots (the pools get 7/8 of the budget)."""
+    return int(positions * (NODE_BYTES + edges_per_position * EDGE_BYTES) * 8 / 7) + 64
+
+
+def _alone(lts, pairs, max_v=0):
+    """Budgets of the starting positions solved on their own (None if the game
+    has more than max_v positions) and the game's size."""
+    b = GameBuild(lts, pairs, max_positions=max_v)
+    if b.build_stats["truncated"]:
+        b.close()
+        return None, None
+    b.solve()
+    o, d = b.energies_raw(0, len(pairs))
+    size = (b.n_positions(), int(b.build_stats["edges"]))
+    b.close()
+    return H.canon_lists(o, d), size
+
+
+def _jobs(n_jobs, seed, n_lo=10, n_hi=50, l_lo=4, l_hi=7, max_v=60000):
+    """Random LTS (3 transitions per state) with 1-11 distinct pairs; jobs
+    whose game exceeds max_v positions are skipped.  Returns (lts, pairs,
+    budgets alone, (positions, edges))."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    k = 0
+    while len(jobs) < n_jobs:
+        k += 1
+        n = int(rng.integers(n_lo, n_hi))
+        n, src, dst, lbl = workloads.random_lts(n, 3 * n, int(rng.integers(l_lo, l_hi)), 1000 * seed + k)
+        lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+        want = int(rng.integers(1, 12))
+        pairs = []
+        for _ in range(100):
+            p, q = (int(x) for x in rng.integers(0, n, 2))
+            if p != q and (p, q) not in pairs:
+                pairs.append((p, q))
+            if len(pairs) == want:
+                break
+        alone, size = _alone(lts, pairs, max_v)
+        if alone is not None:
+            jobs.append((lts, pairs, alone, size))
+    return jobs
+
+
+def _oracle(lts, pairs):
+    g = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs)
+    reach = [v for v in range(g["V"]) if not g["atk"][v] and g["row"][v] == g["row"][v + 1]]
+    offs, data, _ = orc.solve(6, 3, g["row"], g["col"], g["wt"], g["atk"], reach)
+    return H.canon_lists(offs[:len(pairs) + 1], data[:offs[len(pairs)]])
+
+
+def test_jobs_same_alone_and_in_the_mix():
+    print("serve: 24 jobs, one pool", flush=True)
+    jobs = _jobs(24, 1)
+    srv = Server(budget_bytes=256 << 20, rounds_per_step=2)
+    ids = [srv.submit(lts, pairs) for lts, pairs, _, _ in jobs]
+    res = srv.run_until_done(ids)
+    for k, ((lts, pairs, alone, size), j) in enumerate(zip(jobs, ids)):
+        info, r = res[j]
+        assert info["state_name"] == "done", info
+        got = H.canon_lists(*r)
+        assert (info["positions"], info["edges"]) == size
+        assert got == alone, k
+        if size[0] <= 20000:
+            assert got == _oracle(lts, pairs), k
+    st = srv.stats()
+    assert st["jobs_done"] == len(jobs) and st["node_slots_used"] == 0 and st["edge_slots_used"] == 0
+    srv.close()
+
+
+def test_tiny_budget_defers_admission():
+    print("serve: budget for about one game", flush=True)
+    jobs = _jobs(10, 2)
+    big_v = max(size[0] for *_, size in jobs)
+    ratio = max(4.0, 1.1 * max(e / max(v, 1) for *_, (v, e) in jobs))
+    budget = _budget((big_v + 1) * 1.3, ratio)
+    srv = Server(budget_bytes=budget, rounds_per_step=1, edges_per_position=ratio)
+    ids = [srv.submit(lts, pairs) for lts, pairs, _, _ in jobs]
+    res = srv.run_until_done(ids)
+    for k, j in enumerate(ids):
+        assert res[j][0]["state_name"] == "done", res[j][0]
+        assert H.canon_lists(*res[j][1]) == jobs[k][2], k
+    first_done = min(res[j][0]["t_done_ms"] for j in ids)
+    assert any(res[j][0]["t_admit_ms"] >= first_done for j in ids)
+    srv.close()
+
+
+def test_job_larger_than_budget_fails_alone():
+    print("serve: one job larger than the budget", flush=True)
+    jobs = sorted(_jobs(4, 3), key=lambda j: j[3][0])
+    big_v = jobs[-1][3][0]
+    if big_v < 2 * jobs[-2][3][0] + 200:
+        pytest.skip("games too alike in size")
+    ratio = max(4.0, 1.1 * max(e / max(v, 1) for *_, (v, e) in jobs))
+    srv = Server(budget_bytes=_budget(big_v // 2, ratio), edges_per_position=ratio)
+    ids = [srv.submit(lts, pairs) for lts, pairs, _, _ in jobs]
+    res = srv.run_until_done(ids)
+    assert res[ids[-1]][0]["state_name"] == "failed"
+    for k, j in enumerate(ids[:-1]):
+        assert H.canon_lists(*res[j][1]) == jobs[k][2], k
+    srv.close()
+
+
+def test_arena_compaction_keeps_results(monkeypatch):
+    print("serve: compaction after every retirement", flush=True)
+    monkeypatch.setenv("GEQ_ARENA_ENTRIES", "4096")
+    jobs = _jobs(16, 4, 20, 45, 3, 5)
+    srv = Server(budget_bytes=64 << 20, rounds_per_step=1, compact_entries=1)
+    ids = [srv.submit(lts, pairs) for lts, pairs, _, _ in jobs]
+    res = srv.run_until_done(ids)
+    st = srv.stats()
+    for k, j in enumerate(ids):
+        assert H.canon_lists(*res[j][1]) == jobs[k][2], k
+    print("serve stats", st, flush=True)
+    assert st["compactions"] > 0
+    srv.close()
