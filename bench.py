@@ -152,6 +152,128 @@ def bench_batch(args) -> dict:
     }
 
 
+def bench_spectroscopy(args) -> dict:
+    """Full spectroscopy of a random LTS (300 states, 1,200 transitions with
+    uniform sources and targets, 8 labels): bisimulation minimisation, the
+    game of every pair in the same enabledness class built on the GPU, the
+    solve, and the 13 verdicts of every ordered state pair."""
+    from gpuequiv_amd import GameBuild, TransitionSystem, workloads
+    from gpuequiv_amd.energy import std_equivalences
+    from gpuequiv_amd.equivalence import Equivalence
+
+    n, src, dst, lbl = workloads.random_lts(args.states, 4 * args.states, 8, args.seed)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    eqs = [std_equivalences.get(nm) for nm in std_equivalences.NAMES]
+
+    def pipeline():
+        t0 = time.perf_counter()
+        reduced, part = lts.bisimilar_minimize()
+        t1 = time.perf_counter()
+        b, info = GameBuild.compare_all(reduced)
+        t2 = time.perf_counter()
+        st = b.solve()
+        t3 = time.perf_counter()
+        eq = Equivalence(info, b.energies(0, b.n_starting_points))
+        eq.set_minimization(part)
+        t4 = time.perf_counter()
+        verdicts = 0
+        for (p, q) in eq.pos_to_idx:
+            for e in eqs:
+                verdicts += eq.preorder(p, q, e)
+        t5 = time.perf_counter()
+        sizes = (b.n_positions(), int(b.build_stats["edges"]), len(info.starting_points))
+        b.close()
+        return [t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4], st, sizes
+
+    for _ in range(args.warmup):
+        pipeline()
+    runs = [pipeline() for _ in range(args.steps)]
+    phases = np.median(np.array([r[0] for r in runs]), axis=0)
+    st, (V, E, n_pairs) = runs[-1][1], runs[-1][2]
+    total = float(phases.sum())
+    line = {
+        "metric": "edge updates/sec + games/sec (batched vs plain), p50/p99 solve latency, 1 GPU",
+        "value": n * n / total, "unit": "ordered state pairs decided/s (13 verdicts each)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": total * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+        "config": {"workload": f"full spectroscopy (13 equivalences, all state pairs) of a random LTS: {n} states, "
+                               f"{len(src)} transitions (uniform sources/targets), 8 labels, seed {args.seed}",
+                   "positions": V, "edges": E, "start_pairs": n_pairs, "rounds": int(st["rounds"]),
+                   "edge_updates": int(st["edge_updates"])},
+        "phases_ms": {"minimise_host": phases[0] * 1e3, "gamebuild_gpu": phases[1] * 1e3,
+                      "solve_gpu": phases[2] * 1e3, "download": phases[3] * 1e3, "verdicts_host": phases[4] * 1e3},
+        "solve_edge_updates_per_s": int(st["edge_updates"]) / phases[2],
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        t0 = time.perf_counter()
+        part, cnt, orow, odst, olbl = orc.bisimilar_minimize(n, lts.row_offsets, lts.targets, lts.labels)
+        red = TransitionSystem(orow, odst, olbl)
+        pairs = [(p, q) for grp in red.enabledness_partitions() for p in grp for q in grp if p != q]
+        t1 = time.perf_counter()
+        g = orc.gamebuild(red.n_vertices(), red.row_offsets, red.targets, red.labels, pairs)
+        t2 = time.perf_counter()
+        reach = [v for v in range(g["V"]) if not g["atk"][v] and g["row"][v] == g["row"][v + 1]]
+        orc.solve(6, 3, g["row"], g["col"], g["wt"], g["atk"], reach)
+        t3 = time.perf_counter()
+        cpu_total = t3 - t0
+        line["cpu_baseline"] = {"value": n * n / cpu_total, "unit": line["unit"], "cores": 1, "kind": "port",
+                                "sample": f"the same pipeline on the oracle once: minimise {1e3*(t1-t0):.1f} ms, "
+                                          f"gamebuild {1e3*(t2-t1):.0f} ms, solve {1e3*(t3-t2):.0f} ms "
+                                          f"(verdicts not timed), single thread, {cpu_model()}"}
+        line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+    return line
+
+
+def bench_gamebuild(args) -> dict:
+    """A power-law LTS (50,000 states, 250,000 transitions, out-degree
+    alpha 2.1 capped at 200, 60 labels with Zipf(1.2) frequencies, 10% tau)
+    and 4,096 uniformly sampled ordered pairs p != q: the game is built on the
+    GPU, then solved.  The build stops after the level that passes
+    --max-positions (the game of this workload can outgrow any budget)."""
+    from gpuequiv_amd import GameBuild, TransitionSystem, workloads
+
+    n, src, dst, lbl = workloads.powerlaw_lts(args.states, 5 * args.states, 60, args.seed)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    rng = np.random.default_rng(args.seed)
+    pairs = []
+    seen = set()
+    while len(pairs) < args.pairs:
+        p, q = (int(x) for x in rng.integers(0, n, 2))
+        if p != q and (p, q) not in seen:
+            seen.add((p, q))
+            pairs.append((p, q))
+    for _ in range(args.warmup):
+        GameBuild(lts, pairs[:64], max_positions=args.max_positions).close()
+    builds, solves = [], []
+    for _ in range(args.steps):
+        b = GameBuild(lts, pairs, max_positions=args.max_positions)
+        bs = b.build_stats
+        builds.append(bs["build_ms"])
+        if not bs["truncated"]:
+            st = b.solve()
+            solves.append(st)
+        b.close()
+    V, E = int(bs["positions"]), int(bs["edges"])
+    bms = float(np.median(builds))
+    line = {
+        "metric": "edge updates/sec + games/sec (batched vs plain), p50/p99 solve latency, 1 GPU",
+        "value": V / (bms / 1e3), "unit": "positions built/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": bms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"gamebuild + solve: power-law LTS {n} states / {len(src)} transitions, 60 labels "
+                               f"Zipf(1.2), 10% tau, {len(pairs)} sampled pairs, seed {args.seed}, "
+                               f"build cap {args.max_positions} positions",
+                   "positions": V, "edges": E, "levels": int(bs["levels"]), "truncated": bool(bs["truncated"])},
+        "gamebuild_edges_per_s": E / (bms / 1e3),
+    }
+    if solves:
+        s = solves[-1]
+        line["solve"] = {"rounds": int(s["rounds"]), "edge_updates": int(s["edge_updates"]), "solve_ms": s["solve_ms"],
+                         "edge_updates_per_s": int(s["edge_updates"]) / (s["solve_ms"] / 1e3)}
+    return line
+
+
 def _serve_jobs(n_jobs: int, seed: int):
     from gpuequiv_amd import TransitionSystem, workloads
     rng = np.random.default_rng(seed)
@@ -291,10 +413,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["plain", "batch", "serve"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "batch", "serve", "spectroscopy", "gamebuild"], default="plain")
+    ap.add_argument("--states", type=int, default=None, help="spectroscopy / gamebuild: LTS states")
+    ap.add_argument("--pairs", type=int, default=4096, help="gamebuild: sampled pairs")
     ap.add_argument("--jobs", type=int, default=400, help="serve: jobs per arrival rate")
     ap.add_argument("--rates", default="50,200,1000,2000", help="serve: arrival rates (jobs/s)")
-    ap.add_argument("--max-positions", type=int, default=2000000, help="serve: per-job game size limit")
+    ap.add_argument("--max-positions", type=int, default=None,
+                    help="serve: per-job game size limit (2M); gamebuild: build cap (200M)")
     ap.add_argument("--rounds-per-step", type=int, default=4, help="serve: rounds per server step")
     ap.add_argument("--positions", type=int, default=100000)
     ap.add_argument("--games", type=int, default=2048)
@@ -305,9 +430,21 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design note, not a launched path", file=sys.stderr)
         return 2
+    if args.workload in ("spectroscopy", "gamebuild"):
+        if args.seed is None:
+            args.seed = 1
+        if args.workload == "spectroscopy":
+            args.states = args.states or 300
+            print(json.dumps(bench_spectroscopy(args)), flush=True)
+        else:
+            args.states = args.states or 50000
+            args.max_positions = args.max_positions or 200000000
+            print(json.dumps(bench_gamebuild(args)), flush=True)
+        return 0
     if args.workload == "serve":
         if args.seed is None:
             args.seed = 5
+        args.max_positions = args.max_positions or 2000000
         print(json.dumps(bench_serve(args)), flush=True)
         return 0
     if args.workload == "batch":

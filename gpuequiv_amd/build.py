@@ -39,5 +39,26 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+def build_cpp_test(verbose: bool = False) -> str:
+    """The C++ host-API test program (tests/cpp/test_api.cpp), linked against
+    the library built above."""
+    root = os.path.dirname(HERE)
+    out_dir = os.path.join(root, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "test_api")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"),
+           os.path.join(root, "tests", "cpp", "test_api.cpp"), "-L", HERE, "-lgpuequiv_hip",
+           "-Wl,-rpath,$ORIGIN/../../../gpuequiv_amd", "-Wl,--allow-shlib-undefined", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the C++ API test")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cpp_test(verbose=True))

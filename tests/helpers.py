@@ -116,3 +116,37 @@ def energies_le(e: int, eq: list) -> bool:
 def holds_equivalence(energies: list, eq: list) -> bool:
     """EnergyArray::test_equivalence on packed STANDARD words (src/energy.rs:307-318)."""
     return not any(energies_le(e, eq) for e in energies)
+
+
+EQ_NAMES = ("enabledness", "traces", "failures", "failure_traces", "readiness", "readiness_traces", "revivals",
+            "impossible_futures", "possible_futures", "simulation", "ready_simulation", "nested_2simulation",
+            "bisimulation")
+
+
+def oracle_verdicts(n_states, lrow, ldst, llbl):
+    """The full spectroscopy of an LTS on the oracle: minimise, build the game
+    of every pair in the same enabledness class, solve, and decide the 13
+    preorders for every ordered pair (src/lib.rs:329-343,
+    src/equivalence.rs:125-132).  Returns a bool array [n, n, 13]."""
+    from gpuequiv_amd.lts import TransitionSystem
+    from gpuequiv_amd.energy import std_equivalences
+    part, cnt, orow, odst, olbl = orc.bisimilar_minimize(n_states, lrow, ldst, llbl)
+    red = TransitionSystem(orow, odst, olbl)
+    pairs = [(p, q) for grp in red.enabledness_partitions() for p in grp for q in grp if p != q]
+    g = orc.gamebuild(red.n_vertices(), red.row_offsets, red.targets, red.labels, pairs)
+    reach = [v for v in range(g["V"]) if not g["atk"][v] and g["row"][v] == g["row"][v + 1]]
+    offs, data, _ = orc.solve(6, 3, g["row"], g["col"], g["wt"], g["atk"], reach)
+    eqs = np.array([std_equivalences.get(nm).to_vec() for nm in EQ_NAMES], dtype=np.int64)   # [13, 6]
+    red_v = np.zeros((cnt, cnt, 13), dtype=bool)
+    for i, (p, q) in enumerate(pairs):
+        es = np.array([orc.unpack_energy(6, 3, [int(w)]) for w in data[offs[i]:offs[i + 1]]], dtype=np.int64)
+        if es.size == 0:
+            red_v[p, q] = True
+        else:
+            below = np.all(es[:, None, :] <= eqs[None, :, :], axis=2)      # [k, 13]
+            red_v[p, q] = ~below.any(axis=0)
+    part = np.asarray(part, dtype=np.int64)
+    out = red_v[part[:, None], part[None, :]]
+    same = part[:, None] == part[None, :]
+    out[same] = True
+    return out

@@ -142,3 +142,23 @@ def test_truncated_build_is_well_formed():
     V = b.n_positions()
     assert V > 2000 and h["row"][-1] == len(h["col"]) and np.all(np.diff(h["row"].astype(np.int64)) >= 0)
     assert h["col"].max() < V
+
+
+def test_full_spectroscopy_verdict_matrix():
+    """300 states, 1,200 transitions, 8 labels: the 300 x 300 x 13 preorder
+    matrix of equivalences() equals the oracle's."""
+    print("spectroscopy 300/1200/8 seed 1", flush=True)
+    n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    want = H.oracle_verdicts(n, lts.row_offsets, lts.targets, lts.labels)
+    eq = lts.equivalences()
+    eqs = [getattr(std_equivalences, nm)() for nm in H.EQ_NAMES]
+    got = np.zeros_like(want)
+    part = eq.minimization
+    seeded = set(eq.pos_to_idx)
+    for p in range(n):
+        for q in range(n):
+            if part[p] == part[q] or (part[p], part[q]) in seeded:
+                got[p, q] = [eq.preorder(p, q, e) for e in eqs]
+    assert np.array_equal(got, want)
+    assert want.sum() > n * 13          # verdicts beyond the diagonal
