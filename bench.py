@@ -305,7 +305,7 @@ def bench_serve(args) -> dict:
 
     jobs = _serve_jobs(args.jobs, args.seed)
     free, _total = torch.cuda.mem_get_info()
-    budget = int(min(200e9, 0.6 * free))
+    budget = int(min(args.budget_gb * 1e9, 0.6 * free))
     max_pos = args.max_positions
     print(f"bench: {len(jobs)} jobs, budget {budget/1e9:.0f} GB, max {max_pos} positions per job", file=sys.stderr,
           flush=True)
@@ -336,12 +336,16 @@ def bench_serve(args) -> dict:
     print(f"bench: plain path {plain_s:.2f} s for {len(jobs)} jobs ({sum(ok)} within the size limit)", file=sys.stderr,
           flush=True)
 
+    # one long-lived server for every rate (creating and freeing a pool of
+    # this size per rate would time the driver's memory recycling instead)
+    srv = Server(budget_bytes=budget, rounds_per_step=args.rounds_per_step, max_positions=max_pos)
+    srv.run_until_done([srv.submit(lts, pairs) for lts, pairs in jobs[:16]])     # warm-up
     rates = [float(r) for r in args.rates.split(",")]
     sweep = []
     for rate in rates:
         rng = np.random.default_rng(args.seed + int(rate))
         arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
-        srv = Server(budget_bytes=budget, rounds_per_step=args.rounds_per_step, max_positions=max_pos)
+        st0 = srv.stats()
         ids, submit_at, delay = [], {}, {}
         out = {}
         nxt = 0
@@ -362,8 +366,11 @@ def bench_serve(args) -> dict:
             elif resident == 0:                              # games still being built
                 time.sleep(1e-4)
         wall = time.perf_counter() - t0
-        st = srv.stats()
-        srv.close()
+        st1 = srv.stats()
+        st = {k: st1[k] - st0[k] for k in ("steps", "rounds", "compactions", "builds", "jobs_done")}
+        st.update({k: st1[k] for k in ("node_slots", "edge_slots", "arena_entries")})
+        for j in ids:
+            srv.release(j)
         lat, mism, failed = [], 0, 0
         for k, j in enumerate(ids):
             info, res = out[j]
@@ -388,6 +395,7 @@ def bench_serve(args) -> dict:
         print(f"bench: rate {rate:g}/s: {len(jobs)/span:.1f} jobs/s, p50 {sweep[-1]['latency_ms_p50']:.1f} ms, "
               f"p99 {sweep[-1]['latency_ms_p99']:.1f} ms, mismatches {mism}", file=sys.stderr, flush=True)
 
+    srv.close()
     best = max(sweep, key=lambda r: r["served_jobs_per_s"])
     V = sum(v for (v, _), k in zip(sizes, ok) if k)
     E = sum(e for (_, e), k in zip(sizes, ok) if k)
@@ -421,6 +429,7 @@ def main() -> int:
     ap.add_argument("--max-positions", type=int, default=None,
                     help="serve: per-job game size limit (2M); gamebuild: build cap (200M)")
     ap.add_argument("--rounds-per-step", type=int, default=4, help="serve: rounds per server step")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM admission budget (GB)")
     ap.add_argument("--positions", type=int, default=100000)
     ap.add_argument("--games", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=None)

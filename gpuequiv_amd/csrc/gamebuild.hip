@@ -148,7 +148,9 @@ __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, u
     const uint32_t u = lo + i;
     const uint32_t kind = N.kind[u], p = N.p[u], ql = N.q_len[u], xl = N.x_len[u];
     uint32_t c = 0, s = 0;
-    if (kind == K_CLAUSE) {
+    if (L.job_dead && L.job_dead[L.job_of_state[p]]) {
+        // a job past its position cap: no successors
+    } else if (kind == K_CLAUSE) {
         c = 2; s = 1;
     } else if (kind == K_DEFEND) {
         c = ql + (xl ? 1u : 0u);
@@ -174,6 +176,10 @@ __global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L,
     const uint32_t cbase = cand_off[i], cend = cand_off[i + 1];
     const uint64_t sbase = spool_base + spool_off[i];
     uint32_t slot = 0;
+    if (L.job_dead && L.job_dead[L.job_of_state[p]]) {   // k_bounds gave it no room
+        nsucc[i] = 0;
+        return;
+    }
     auto emit = [&](uint32_t k, uint32_t pp, uint32_t a_off, uint32_t a_len, uint32_t b_off, uint32_t b_len, uint32_t w) {
         Cand c;
         c.kind_p = (k << 30) | pp;

@@ -3,6 +3,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
+#include "runtime.hpp"
+
 namespace geq {
 namespace gb {
 
@@ -16,6 +20,10 @@ struct LtsArgs {
     const int32_t* lbl;
     const uint32_t* en_row;
     const int32_t* en_lbl;
+    // merged builds of several jobs: the job of each state and the jobs that
+    // stopped growing (NULL for a single game)
+    const uint32_t* job_of_state;
+    const uint8_t* job_dead;
 };
 
 // Position records (structure of arrays).  Sets Q / Q* live in the set pool:
@@ -59,5 +67,37 @@ __global__ void k_fill_u64(unsigned long long* a, uint64_t n, unsigned long long
 __global__ void k_fill_u32(uint32_t* a, uint64_t n, uint32_t v);
 __global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk);
 
+// Device buffers of one build (level loop and job extraction), kept by a
+// caller that builds repeatedly so that steady-state builds allocate nothing.
+struct BuildScratch {
+    DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, ub_c, ub_s, c_off, s_off, nsucc, rowl, flag, rank, keymin, d_jcnt;
+    DevBuf<int32_t> l_lbl, l_enlbl;
+    DevBuf<uint8_t> d_insert, tmp, d_dead;
+    DevBuf<Cand> cands;
+    DevBuf<unsigned long long> table;
+    DevBuf<uint32_t> job_of_state, key, key_s, val, order, dsort, escan, local, d_ps, d_es;
+};
+
 }  // namespace gb
+
+// Games of several jobs built together (gamebuild_jobs): job j's positions are
+// the contiguous range [pos_start[j], pos_start[j] + n_pos[j]) with local ids
+// in the order its own build gives them; its rows are
+// row[pos_start[j] + j .. + n_pos[j]] (n_pos[j] + 1 entries, relative to
+// edge_start[j]), its columns (local ids) and updates col / wt[edge_start[j] ..).
+struct BuiltJobs {
+    DevBuf<uint32_t> row, col, wt;
+    DevBuf<uint8_t> atk;
+    std::vector<uint32_t> pos_start, n_pos, edge_start, n_edges;
+    std::vector<uint8_t> dead;       // grew past the position cap
+    uint32_t levels = 0;
+    double build_ms = 0;
+};
+// Builds the games of jobs (validated, each with at least one pair) in one
+// level loop over the disjoint union of their LTSs.  `ws` and `store` (the
+// merged positions) are reused across calls; `out`'s buffers only grow.
+geq_status gamebuild_jobs(geq_ctx* ctx, const std::vector<geq_lts>& ltss, const std::vector<const uint32_t*>& pairs,
+                          const std::vector<uint32_t>& n_pairs, uint64_t cap, gb::BuildScratch& ws, geq_built& store,
+                          BuiltJobs* out);
+
 }  // namespace geq

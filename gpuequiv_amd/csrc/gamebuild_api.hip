@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -147,32 +148,67 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
     return GEQ_OK;
 }
 
-extern "C" {
-
-geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs, uint64_t max_positions,
-                         geq_built** out, geq_build_stats* stats) {
-    if (!ctx || !lts || !out || (n_pairs && !pairs)) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
-    *out = nullptr;
+// Checks of an LTS + start pairs: what geq_gamebuild rejects with INVALID_ARG.
+geq_status geq::validate_job(const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs) {
+    if (!lts || (n_pairs && !pairs)) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
     const uint32_t n = lts->n_states, m = lts->n_transitions;
     if (n == 0 || !lts->row_offsets || (m && (!lts->targets || !lts->labels)))
         return fail(GEQ_ERR_INVALID_ARG, "empty or incomplete LTS");
     if (n >= (1u << 30)) return fail(GEQ_ERR_INVALID_ARG, "at most 2^30 states");
     if (lts->row_offsets[0] != 0 || lts->row_offsets[n] != m) return fail(GEQ_ERR_INVALID_ARG, "LTS row_offsets malformed");
-    std::vector<uint32_t> en_row(n + 1, 0);
-    std::vector<int32_t> en_lbl;
-    for (uint32_t s = 0; s < n; s++) {
-        const uint32_t b = lts->row_offsets[s], e = lts->row_offsets[s + 1];
+    for (uint32_t st = 0; st < n; st++) {
+        const uint32_t b = lts->row_offsets[st], e = lts->row_offsets[st + 1];
         if (e < b) return fail(GEQ_ERR_INVALID_ARG, "LTS row_offsets not monotone");
         for (uint32_t k = b; k < e; k++) {
             if (lts->targets[k] >= n) return fail(GEQ_ERR_INVALID_ARG, "LTS target out of range");
             if (k > b && lts->labels[k] < lts->labels[k - 1])
                 return fail(GEQ_ERR_INVALID_ARG, "LTS transitions of a state must be sorted by label");
-            if (k == b || lts->labels[k] != lts->labels[k - 1]) en_lbl.push_back(lts->labels[k]);
         }
-        en_row[s + 1] = (uint32_t)en_lbl.size();
     }
     for (uint32_t i = 0; i < 2 * n_pairs; i++)
         if (pairs[i] >= n) return fail(GEQ_ERR_INVALID_ARG, "start pair process out of range");
+    return GEQ_OK;
+}
+
+namespace {
+
+// Per-job bookkeeping of a merged build: the job of every state, each job's
+// position count so far, its cap and whether it went past it.
+struct JobTrack {
+    std::vector<uint32_t> job_of_state;    // uploaded to BuildScratch::job_of_state by build_core
+    std::vector<uint64_t> count;
+    std::vector<uint8_t> dead;
+    uint64_t cap = 0;
+};
+
+// Positions per job among the ids [lo, hi) (LDS histogram per block).
+__global__ void k_job_count(const uint32_t* p, uint32_t lo, uint32_t hi, const uint32_t* job_of_state, uint32_t n_jobs,
+                            uint32_t* cnt) {
+    __shared__ uint32_t h[1024];
+    for (uint32_t j = threadIdx.x; j < n_jobs; j += blockDim.x) h[j] = 0;
+    __syncthreads();
+    for (uint32_t v = lo + blockIdx.x * blockDim.x + threadIdx.x; v < hi; v += gridDim.x * blockDim.x)
+        atomicAdd(&h[job_of_state[p[v]]], 1u);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < n_jobs; j += blockDim.x)
+        if (h[j]) atomicAdd(&cnt[j], h[j]);
+}
+
+// The level-synchronous build of the game seeded with `pairs` over a
+// label-sorted LTS (already validated).  With `jt`, the LTS is a disjoint
+// union of jobs: a job whose position count passes jt->cap is marked dead and
+// its positions get no further successors; `max_positions` is then unused.
+geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs, uint64_t max_positions,
+                      JobTrack* jt, geq_built* B, BuildScratch& ws, geq_build_stats* stats) {
+    const uint32_t n = lts->n_states, m = lts->n_transitions;
+    std::vector<uint32_t> en_row(n + 1, 0);
+    std::vector<int32_t> en_lbl;
+    for (uint32_t st = 0; st < n; st++) {
+        const uint32_t b = lts->row_offsets[st], e = lts->row_offsets[st + 1];
+        for (uint32_t k = b; k < e; k++)
+            if (k == b || lts->labels[k] != lts->labels[k - 1]) en_lbl.push_back(lts->labels[k]);
+        en_row[st + 1] = (uint32_t)en_lbl.size();
+    }
     // GameBuild::new_node keeps the last id of a repeated start position in its map
     std::vector<uint8_t> insert(n_pairs, 1);
     {
@@ -188,19 +224,17 @@ geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
-    geq_built* B = new geq_built();
     B->ctx = ctx;
     B->n_start = n_pairs;
-    DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, ub_c, ub_s, c_off, s_off, nsucc, rowl, flag, rank, keymin;
-    DevBuf<int32_t> l_lbl, l_enlbl;
-    DevBuf<uint8_t> d_insert, tmp;
-    DevBuf<Cand> cands;
-    DevBuf<unsigned long long> table;
+    B->V = B->E = 0;
+    B->spool_top = 0;
+    auto& l_row = ws.l_row; auto& l_dst = ws.l_dst; auto& l_enrow = ws.l_enrow; auto& d_pairs = ws.d_pairs;
+    auto& ub_c = ws.ub_c; auto& ub_s = ws.ub_s; auto& c_off = ws.c_off; auto& s_off = ws.s_off; auto& nsucc = ws.nsucc;
+    auto& rowl = ws.rowl; auto& flag = ws.flag; auto& rank = ws.rank; auto& keymin = ws.keymin;
+    auto& l_lbl = ws.l_lbl; auto& l_enlbl = ws.l_enlbl; auto& d_insert = ws.d_insert; auto& tmp = ws.tmp;
+    auto& cands = ws.cands; auto& table = ws.table;
     geq_status st = GEQ_OK;
-    auto bail = [&](geq_status e) {
-        delete B;
-        return e;
-    };
+    auto bail = [&](geq_status e) { return e; };
 #define GB_TRY(x)                      \
     do {                               \
         geq_status _s = (x);           \
@@ -224,7 +258,20 @@ geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs
     GB_HIP(hipMemcpyAsync(l_enrow.p, en_row.data(), 4ull * (n + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     if (!en_lbl.empty())
         GB_HIP(hipMemcpyAsync(l_enlbl.p, en_lbl.data(), 4ull * en_lbl.size(), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    LtsArgs L{l_row.p, l_dst.p, l_lbl.p, l_enrow.p, l_enlbl.p};
+    auto& d_jcnt = ws.d_jcnt;
+    auto& d_dead = ws.d_dead;
+    auto& d_job_of_state = ws.job_of_state;
+    const uint32_t n_jobs = jt ? (uint32_t)jt->count.size() : 0u;
+    if (jt) {
+        GB_TRY(ensure(d_job_of_state, n, s, false, "job map"));
+        GB_TRY(ensure(d_jcnt, n_jobs, s, false, "job counters"));
+        GB_TRY(ensure(d_dead, n_jobs, s, false, "job flags"));
+        GB_HIP(hipMemcpyAsync(d_job_of_state.p, jt->job_of_state.data(), 4ull * n, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        GB_HIP(hipMemsetAsync(d_jcnt.p, 0, 4ull * n_jobs, s), GEQ_ERR_TRANSFER);
+        GB_HIP(hipMemcpyAsync(d_dead.p, jt->dead.data(), n_jobs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    }
+    LtsArgs L{l_row.p, l_dst.p, l_lbl.p, l_enrow.p, l_enlbl.p, jt ? d_job_of_state.p : nullptr, jt ? d_dead.p : nullptr};
+    std::vector<uint32_t> h_jcnt(n_jobs);
 
     // node, edge and set-pool capacities grow level by level
     auto node_cap = [&](size_t need) -> geq_status {
@@ -273,6 +320,10 @@ geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs
     }
     B->V = n_pairs;
     B->spool_top = n_pairs;
+    if (jt && n_pairs) {
+        const uint32_t nb = std::min<uint32_t>(1024, (n_pairs + tb - 1) / tb);
+        k_job_count<<<nb, tb, 0, s>>>(B->p.p, 0, n_pairs, d_job_of_state.p, n_jobs, d_jcnt.p);
+    }
     uint64_t E = 0, levels = 0;
     uint32_t lo = 0, hi = n_pairs;
     bool truncated = false;
@@ -322,7 +373,25 @@ geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs
         lo = hi;
         hi = B->V;
         levels++;
-        if (max_positions && B->V > max_positions) {
+        if (jt) {
+            // positions of every job so far; jobs past the cap stop growing
+            if (hi > lo) {
+                const uint32_t nb = std::min<uint32_t>(1024, (hi - lo + tb - 1) / tb);
+                k_job_count<<<nb, tb, 0, s>>>(B->p.p, lo, hi, d_job_of_state.p, n_jobs, d_jcnt.p);
+            }
+            GB_HIP(hipMemcpyAsync(h_jcnt.data(), d_jcnt.p, 4ull * n_jobs, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+            GB_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+            bool newly_dead = false;
+            for (uint32_t j = 0; j < n_jobs; j++) {
+                jt->count[j] = h_jcnt[j];
+                if (!jt->dead[j] && jt->cap && h_jcnt[j] > jt->cap) {
+                    jt->dead[j] = 1;
+                    newly_dead = true;
+                }
+            }
+            if (newly_dead)
+                GB_HIP(hipMemcpyAsync(d_dead.p, jt->dead.data(), n_jobs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        } else if (max_positions && B->V > max_positions) {
             truncated = true;
             break;
         }
@@ -351,10 +420,165 @@ geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs
         stats->set_words = B->spool_top;
     }
     (void)st;
-    *out = B;
     return GEQ_OK;
 #undef GB_TRY
 #undef GB_HIP
+}
+
+__global__ void k_pos_job(const uint32_t* p, uint32_t V, const uint32_t* job_of_state, uint32_t* key, uint32_t* val) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) {
+        key[v] = job_of_state[p[v]];
+        val[v] = v;
+    }
+}
+__global__ void k_sorted_deg(const uint32_t* order, const uint32_t* row, uint32_t V, uint32_t* dsort) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) dsort[i] = row[order[i] + 1] - row[order[i]];
+    else if (i == V) dsort[i] = 0;
+}
+__global__ void k_at(const uint32_t* a, const uint32_t* idx, uint32_t n, uint32_t* out) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = a[idx[j]];
+}
+__global__ void k_local_ids(const uint32_t* order, const uint32_t* key, const uint32_t* pos_start, uint32_t V, uint32_t* local) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) local[order[i]] = i - pos_start[key[i]];
+}
+// job-major rows: job j's row entries sit at row_out[pos_start[j] + j ...], relative to its first edge
+__global__ void k_job_rows(const uint32_t* key, const uint32_t* escan, const uint32_t* pos_start, const uint32_t* edge_start,
+                           uint32_t V, uint32_t* row_out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const uint32_t j = key[i];
+    row_out[i + j] = escan[i] - edge_start[j];
+    if (i + 1 == pos_start[j + 1]) row_out[i + j + 1] = escan[i + 1] - edge_start[j];
+}
+__global__ void k_job_edges(const uint32_t* order, const uint32_t* row, const uint32_t* col, const uint32_t* wt,
+                            const uint8_t* atk, const uint32_t* local, const uint32_t* escan, uint32_t V, uint32_t* col_out,
+                            uint32_t* wt_out, uint8_t* atk_out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const uint32_t v = order[i], r0 = row[v], deg = row[v + 1] - r0, b = escan[i];
+    for (uint32_t k = 0; k < deg; k++) {
+        col_out[b + k] = local[col[r0 + k]];
+        wt_out[b + k] = wt[r0 + k];
+    }
+    atk_out[i] = atk[v];
+}
+
+}  // namespace
+
+geq_status geq::gamebuild_jobs(geq_ctx* ctx, const std::vector<geq_lts>& ltss, const std::vector<const uint32_t*>& pairs,
+                               const std::vector<uint32_t>& n_pairs, uint64_t cap, BuildScratch& ws, geq_built& store,
+                               BuiltJobs* out) {
+    const uint32_t J = (uint32_t)ltss.size();
+    if (J == 0 || J > 1024) return fail(GEQ_ERR_INVALID_ARG, "gamebuild_jobs: 1..1024 jobs");
+    // disjoint union of the jobs' LTSs; pairs shifted to the merged state ids
+    uint64_t n = 0, m = 0, np = 0;
+    for (uint32_t j = 0; j < J; j++) {
+        n += ltss[j].n_states;
+        m += ltss[j].n_transitions;
+        np += n_pairs[j];
+    }
+    if (n >= (1u << 30) || m >= 0xffffffffull || np >= (1u << 30)) return fail(GEQ_ERR_INVALID_ARG, "gamebuild_jobs: batch too large");
+    std::vector<uint32_t> row(n + 1), dst(std::max<uint64_t>(m, 1)), mp(std::max<uint64_t>(2 * np, 2));
+    std::vector<int32_t> lbl(std::max<uint64_t>(m, 1));
+    JobTrack jt;
+    jt.job_of_state.resize(n);
+    jt.count.assign(J, 0);
+    jt.dead.assign(J, 0);
+    jt.cap = cap;
+    uint32_t sb = 0, tb_ = 0, pb = 0;
+    row[0] = 0;
+    for (uint32_t j = 0; j < J; j++) {
+        const geq_lts& l = ltss[j];
+        for (uint32_t st = 0; st < l.n_states; st++) {
+            row[sb + st + 1] = tb_ + l.row_offsets[st + 1];
+            jt.job_of_state[sb + st] = j;
+        }
+        for (uint32_t k = 0; k < l.n_transitions; k++) {
+            dst[tb_ + k] = sb + l.targets[k];
+            lbl[tb_ + k] = l.labels[k];
+        }
+        for (uint32_t i = 0; i < 2 * n_pairs[j]; i++) mp[2 * pb + i] = sb + pairs[j][i];
+        sb += l.n_states;
+        tb_ += l.n_transitions;
+        pb += n_pairs[j];
+    }
+    geq_lts merged{(uint32_t)n, (uint32_t)m, row.data(), dst.data(), lbl.data()};
+    geq_build_stats bs{};
+    geq_built* B = &store;
+    geq_status st = build_core(ctx, &merged, mp.data(), (uint32_t)np, 0, &jt, B, ws, &bs);
+    if (st != GEQ_OK) return st;
+    const uint32_t V = B->V, E = B->E, tb = 256;
+    uint64_t total = 0;
+    for (uint64_t c : jt.count) total += c;
+    if (total != V) return fail(GEQ_ERR_INTERNAL, "gamebuild_jobs: per-job position counts do not add up");
+    hipStream_t s = ctx->stream;
+    // stable sort of the positions by job: order[i] = merged id of the i-th position in job-major order
+    auto& key = ws.key; auto& key_s = ws.key_s; auto& val = ws.val; auto& order = ws.order; auto& dsort = ws.dsort;
+    auto& escan = ws.escan; auto& local = ws.local; auto& d_ps = ws.d_ps; auto& d_es = ws.d_es; auto& tmp = ws.tmp;
+    for (auto* b : {&key, &key_s, &val, &order, &local})
+        if ((st = ensure(*b, V, s, false, "job extraction")) != GEQ_OK) return st;
+    for (auto* b : {&dsort, &escan})
+        if ((st = ensure(*b, V + 1, s, false, "job extraction")) != GEQ_OK) return st;
+    for (auto* b : {&d_ps, &d_es})
+        if ((st = ensure(*b, J + 1, s, false, "job extraction")) != GEQ_OK) return st;
+    k_pos_job<<<(V + tb - 1) / tb, tb, 0, s>>>(B->p.p, V, ws.job_of_state.p, key.p, val.p);
+    int end_bit = 1;
+    while ((1u << end_bit) < J) end_bit++;
+    size_t bytes = 0;
+    GEQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, key_s.p, val.p, order.p, V, 0, end_bit, s), GEQ_ERR_INTERNAL);
+    if ((st = ensure(tmp, bytes + 16, s, false, "sort scratch")) != GEQ_OK) return st;
+    GEQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, key_s.p, val.p, order.p, V, 0, end_bit, s), GEQ_ERR_INTERNAL);
+    k_sorted_deg<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(order.p, B->row.p, V, dsort.p);
+    st = exclusive_scan<uint32_t>(dsort.p, escan.p, V + 1, tmp, s);
+    if (st != GEQ_OK) return st;
+    out->pos_start.assign(J + 1, 0);
+    out->n_pos.assign(J, 0);
+    for (uint32_t j = 0; j < J; j++) {
+        out->n_pos[j] = (uint32_t)jt.count[j];
+        out->pos_start[j + 1] = out->pos_start[j] + out->n_pos[j];
+    }
+    GEQ_HIP(hipMemcpyAsync(d_ps.p, out->pos_start.data(), 4ull * (J + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    k_at<<<(J + 1 + tb - 1) / tb, tb, 0, s>>>(escan.p, d_ps.p, J + 1, d_es.p);
+    out->edge_start.assign(J + 1, 0);
+    GEQ_HIP(hipMemcpyAsync(out->edge_start.data(), d_es.p, 4ull * (J + 1), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+    k_local_ids<<<(V + tb - 1) / tb, tb, 0, s>>>(order.p, key_s.p, d_ps.p, V, local.p);
+    if ((st = ensure(out->row, (size_t)V + J, s, false, "job games")) != GEQ_OK) return st;
+    if ((st = ensure(out->col, std::max<uint32_t>(E, 1), s, false, "job games")) != GEQ_OK) return st;
+    if ((st = ensure(out->wt, std::max<uint32_t>(E, 1), s, false, "job games")) != GEQ_OK) return st;
+    if ((st = ensure(out->atk, std::max<uint32_t>(V, 1), s, false, "job games")) != GEQ_OK) return st;
+    k_job_rows<<<(V + tb - 1) / tb, tb, 0, s>>>(key_s.p, escan.p, d_ps.p, d_es.p, V, out->row.p);
+    k_job_edges<<<(V + tb - 1) / tb, tb, 0, s>>>(order.p, B->row.p, B->col.p, B->wt.p, B->atk.p, local.p, escan.p, V, out->col.p,
+                                                 out->wt.p, out->atk.p);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    out->n_edges.assign(J, 0);
+    for (uint32_t j = 0; j < J; j++) out->n_edges[j] = out->edge_start[j + 1] - out->edge_start[j];
+    out->dead = jt.dead;
+    out->levels = (uint32_t)bs.levels;
+    out->build_ms = bs.build_ms;
+    return GEQ_OK;
+}
+
+extern "C" {
+
+geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs, uint64_t max_positions,
+                         geq_built** out, geq_build_stats* stats) {
+    if (!ctx || !lts || !out || (n_pairs && !pairs)) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    geq_status st = validate_job(lts, pairs, n_pairs);
+    if (st != GEQ_OK) return st;
+    BuildScratch ws;
+    geq_built* B = new geq_built();
+    st = build_core(ctx, lts, pairs, n_pairs, max_positions, nullptr, B, ws, stats);
+    if (st != GEQ_OK) {
+        delete B;
+        return st;
+    }
+    *out = B;
+    return GEQ_OK;
 }
 
 geq_status geq_built_download(geq_built* B, geq_built_game* o) {

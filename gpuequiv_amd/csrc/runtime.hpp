@@ -55,11 +55,13 @@ geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_en
 // packed updates, attacker flags; the reverse CSR is derived on the device.
 geq_status game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,
                             const uint32_t* d_wt, const uint8_t* d_atk, geq_game** out);
-// Solver buffers for V positions / E edges (contents uninitialised).
 // Energy-arena entries (u16) a game starts with: `want`, or 8 per edge when 0;
 // the environment variable GEQ_ARENA_ENTRIES overrides both (small values
 // force the regrow path).
 uint32_t initial_arena_entries(uint64_t E, uint64_t want);
+// INVALID_ARG checks of geq_gamebuild's inputs (LTS shape, label order, pairs)
+geq_status validate_job(const geq_lts* lts, const uint32_t* pairs, uint32_t n_pairs);
+// Solver buffers for V positions / E edges (contents uninitialised).
 geq_status alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_game** out, uint64_t arena_entries = 0);
 // Launch shape of one round and the hipGraph of `rounds_per_launch` captured rounds.
 struct RoundShape {
@@ -105,6 +107,7 @@ struct geq_game {
     void* graph_arena = nullptr;          // arena pointer baked into graph_exec
     ~geq_game();
 };
+
 
 // A game built on the device (geq_gamebuild): positions, set pool, CSR.
 struct geq_built {

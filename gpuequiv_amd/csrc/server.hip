@@ -1,11 +1,14 @@
 // Serving with continuous batching: spectroscopy jobs (LTS + process pairs)
-// arrive at any time; their games are built on a second stream by a builder
-// thread, then placed into shared HBM pools at a round boundary and seeded into
-// the shared device worklist, so every round evaluates the frontier of all
-// resident jobs together.  A job retires as soon as none of its positions is
-// on the worklist; its starting positions' budgets are gathered and its pool
-// ranges are reused.  Jobs are disjoint subgraphs of the pools and every round
-// is evaluated per node, so a job's result is the same alone and in the mix.
+// arrive at any time.  Builder threads (each with its own stream) take every
+// queued job and build their games together, in one level loop over the
+// disjoint union of the jobs' LTSs; each job's game comes out as its own
+// contiguous, locally numbered range.  At a round boundary built jobs are
+// placed into shared HBM pools and seeded into the shared device worklist, so
+// every round evaluates the frontier of all resident jobs together.  A job
+// retires as soon as none of its positions is on the worklist; its starting
+// positions' budgets are gathered and its pool ranges are reused.  Jobs are
+// disjoint subgraphs and every round is evaluated per node, so a job's result
+// is the same alone and in the mix.
 //
 // There is no counterpart in the reference, which solves one game per
 // TransitionSystem::compare / equivalences call (src/lib.rs:275-343).
@@ -15,9 +18,12 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -112,7 +118,8 @@ struct Job {
     uint32_t n_states = 0;
     std::vector<uint32_t> row, dst, pairs;
     std::vector<int32_t> lbl;
-    geq_built* built = nullptr;
+    std::shared_ptr<BuiltJobs> batch;   // the build this job's game sits in, until placed
+    uint32_t bj = 0;                    // index of the job in `batch`
     geq_status status = GEQ_OK;
     std::string error;
     uint32_t V = 0, E = 0, n_start = 0, nb = 0, eb = 0, slot = 0;
@@ -205,12 +212,16 @@ struct geq_server {
     std::mutex mu;
     std::condition_variable cv;
     bool stop = false;
-    uint32_t max_ready = 32;
+    uint32_t max_ready = 256;          // built jobs waiting for admission at most
+    uint32_t max_batch = 64;           // jobs per merged build
+    bool trace = false;                // GEQ_SERVER_TRACE: one stderr line per build and step
     std::vector<std::thread> builders;
     std::vector<uint64_t> finished;    // ids of jobs done or failed, not yet taken
+    std::mutex spare_mu;
+    std::vector<BuiltJobs*> spare_outs;  // job-game buffers of placed builds, for reuse
     uint64_t next_id = 1;
     std::chrono::steady_clock::time_point t0;
-    uint64_t steps = 0, rounds_total = 0, compactions = 0, jobs_done = 0;
+    uint64_t steps = 0, rounds_total = 0, compactions = 0, jobs_done = 0, builds = 0;
     uint64_t arena_min = 0;            // arena entries the pools start with
     uint64_t compact_entries = 0;      // compaction threshold (0: 3/4 of the arena)
     uint64_t max_positions = 0;        // per-job game size limit (0: the node pool)
@@ -220,42 +231,100 @@ struct geq_server {
 
 namespace {
 
+// Builder thread: takes every queued job (up to max_batch) and builds their
+// games in one level loop over the union of their LTSs (gamebuild_jobs).
 void builder_main(geq_server* S, geq_ctx* bctx) {
     (void)hipSetDevice(bctx->device);
+    gb::BuildScratch ws;   // reused by every build of this thread
+    geq_built store;
     while (true) {
-        Job* J = nullptr;
+        std::vector<Job*> take;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv.wait(lk, [&] { return S->stop || (!S->build_q.empty() && S->ready_q.size() < S->max_ready); });
             if (S->stop) return;
-            J = S->build_q.front();
-            S->build_q.pop_front();
-            J->state = BUILDING;
+            while (!S->build_q.empty() && take.size() < S->max_batch) {
+                take.push_back(S->build_q.front());
+                S->build_q.pop_front();
+                take.back()->state = BUILDING;
+            }
         }
-        geq_lts l{J->n_states, (uint32_t)J->dst.size(), J->row.data(), J->dst.data(), J->lbl.data()};
-        geq_build_stats bs{};
-        geq_built* B = nullptr;
-        geq_status st = geq_gamebuild(bctx, &l, J->pairs.data(), (uint32_t)(J->pairs.size() / 2), S->max_positions, &B, &bs);
-        if (st == GEQ_OK && bs.truncated) {
-            geq_built_free(B);
-            B = nullptr;
-            st = fail(GEQ_ERR_OUT_OF_MEMORY, "the job's game grows past " + std::to_string(S->max_positions) + " positions");
+        // invalid jobs fail alone; jobs without pairs have nothing to solve
+        std::vector<Job*> build;
+        std::vector<geq_lts> ltss;
+        std::vector<const uint32_t*> prs;
+        std::vector<uint32_t> nps;
+        for (Job* J : take) {
+            geq_lts l{J->n_states, (uint32_t)J->dst.size(), J->row.data(), J->dst.data(), J->lbl.data()};
+            const geq_status st = validate_job(&l, J->pairs.data(), J->n_start);
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (st != GEQ_OK || J->n_start == 0) {
+                J->t_built = J->t_done = S->now_ms();
+                J->status = st;
+                if (st != GEQ_OK) {
+                    J->error = geq_last_error();
+                    J->state = FAILED;
+                } else {
+                    J->offs.assign(1, 0);
+                    J->state = DONE;
+                    S->jobs_done++;
+                }
+                S->finished.push_back(J->id);
+                continue;
+            }
+            build.push_back(J);
+            ltss.push_back(l);
+            prs.push_back(J->pairs.data());
+            nps.push_back(J->n_start);
+        }
+        if (build.empty()) continue;
+        BuiltJobs* raw = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(S->spare_mu);
+            if (!S->spare_outs.empty()) {
+                raw = S->spare_outs.back();
+                S->spare_outs.pop_back();
+            }
+        }
+        if (!raw) raw = new BuiltJobs();
+        // returned for reuse once its last job has been placed
+        std::shared_ptr<BuiltJobs> B(raw, [S](BuiltJobs* b) {
+            std::lock_guard<std::mutex> lk(S->spare_mu);
+            S->spare_outs.push_back(b);
+        });
+        const double tb0 = S->now_ms();
+        const geq_status st = gamebuild_jobs(bctx, ltss, prs, nps, S->max_positions, ws, store, B.get());
+        const std::string err = st != GEQ_OK ? std::string(geq_last_error()) : std::string();
+        if (S->trace) {
+            uint64_t pos = 0, dead = 0;
+            for (uint32_t k = 0; st == GEQ_OK && k < build.size(); k++) {
+                pos += B->n_pos[k];
+                dead += B->dead[k];
+            }
+            std::fprintf(stderr, "[geq server] t=%.1f ms build of %zu jobs: %llu positions, %u levels, %llu over the cap, %.1f ms (status %d)\n",
+                         tb0, build.size(), (unsigned long long)pos, B->levels, (unsigned long long)dead, S->now_ms() - tb0, (int)st);
         }
         std::lock_guard<std::mutex> lk(S->mu);
-        J->t_built = S->now_ms();
-        if (st != GEQ_OK) {
-            J->status = st;
-            J->error = geq_last_error();
-            J->state = FAILED;
-            J->t_done = J->t_built;
-            S->finished.push_back(J->id);
-            continue;
+        const double now = S->now_ms();
+        S->builds++;
+        for (uint32_t k = 0; k < build.size(); k++) {
+            Job* J = build[k];
+            J->t_built = now;
+            if (st != GEQ_OK || B->dead[k]) {
+                J->status = st != GEQ_OK ? st : GEQ_ERR_OUT_OF_MEMORY;
+                J->error = st != GEQ_OK ? err : "the job's game grows past " + std::to_string(S->max_positions) + " positions";
+                J->state = FAILED;
+                J->t_done = now;
+                S->finished.push_back(J->id);
+                continue;
+            }
+            J->batch = B;
+            J->bj = k;
+            J->positions = B->n_pos[k];
+            J->edges = B->n_edges[k];
+            J->state = BUILT;
+            S->ready_q.push_back(J);
         }
-        J->positions = B->V;
-        J->edges = B->E;
-        J->built = B;
-        J->state = BUILT;
-        S->ready_q.push_back(J);
     }
 }
 
@@ -306,10 +375,8 @@ geq_server::~geq_server() {
     cv.notify_all();
     for (auto& t : builders)
         if (t.joinable()) t.join();
-    for (auto& kv : jobs) {
-        if (kv.second->built) geq_built_free(kv.second->built);
-        delete kv.second;
-    }
+    for (auto& kv : jobs) delete kv.second;
+    for (BuiltJobs* b : spare_outs) delete b;
     if (pool) geq_game_free(pool);
     for (geq_ctx* c : bctx) geq_ctx_destroy(c);
 }
@@ -333,6 +400,7 @@ geq_status geq_server_create(geq_ctx* ctx, const geq_server_opts* opts, geq_serv
     S->max_jobs = opts && opts->max_jobs ? opts->max_jobs : 4096;
     S->rounds_per_step = opts && opts->rounds_per_step ? opts->rounds_per_step : 4;
     S->compact_entries = opts ? opts->compact_entries : 0;
+    S->trace = std::getenv("GEQ_SERVER_TRACE") != nullptr;
     S->max_positions = opts && opts->max_positions ? std::min<uint64_t>(opts->max_positions, V) : V;
     const uint32_t n_builders = opts && opts->builders ? std::min(opts->builders, 16u) : 2u;
     geq_status st = GEQ_OK;
@@ -421,7 +489,7 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         std::lock_guard<std::mutex> lk(S->mu);
         while (!S->ready_q.empty()) {
             Job* J = S->ready_q.front();
-            const uint32_t V = J->built->V, E = J->built->E;
+            const uint32_t V = J->positions, E = J->edges;
             if (V + 1ull > S->nodes.total || E > S->edges.total) {
                 // larger than the whole budget: it could never be admitted
                 J->status = GEQ_ERR_OUT_OF_MEMORY;
@@ -430,8 +498,7 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
                 J->state = FAILED;
                 J->t_done = S->now_ms();
                 S->finished.push_back(J->id);
-                geq_built_free(J->built);
-                J->built = nullptr;
+                J->batch.reset();
                 S->ready_q.pop_front();
                 continue;
             }
@@ -454,10 +521,11 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
     S->cv.notify_all();
     for (Job* J : admit) {
         const uint32_t V = J->V, E = J->E;
-        const uint32_t* brow = J->built->row.p;
-        const uint32_t* bcol = J->built->col.p;
-        const uint32_t* bwt = J->built->wt.p;
-        const uint8_t* batk = J->built->atk.p;
+        const BuiltJobs& Bj = *J->batch;
+        const uint32_t* brow = Bj.row.p + Bj.pos_start[J->bj] + J->bj;
+        const uint32_t* bcol = Bj.col.p + Bj.edge_start[J->bj];
+        const uint32_t* bwt = Bj.wt.p + Bj.edge_start[J->bj];
+        const uint8_t* batk = Bj.atk.p + Bj.pos_start[J->bj];
         // the builder's stream finished this game before it was queued as BUILT
         k_place_nodes<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, J->eb, J->slot, g->row.p, g->atk.p, g->cur.p,
                                                            g->stamp.p, S->job_of.p);
@@ -485,8 +553,7 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
             ExpandArgs sx{{g->wl0.p, g->wl1.p}, g->seed.p, n_seed, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
             k_expand<<<std::max(1u, std::min(1024u, (n_seed + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx);
         }
-        geq_built_free(J->built);   // the pools hold the game now
-        J->built = nullptr;
+        J->batch.reset();   // the pools hold the game now (the build is freed with its last job)
         std::lock_guard<std::mutex> lk(S->mu);
         J->t_admit = S->now_ms();
         J->state = ACTIVE;
@@ -523,6 +590,10 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
             ea.arena_cap = g->arena_cap;
         }
         S->steps++;
+        if (S->trace)
+            std::fprintf(stderr, "[geq server] t=%.1f ms step %llu: %zu resident, %zu admitted, round %u, worklist %u\n", S->now_ms(),
+                         (unsigned long long)S->steps, S->resident.size(), admit.size(), g->h_ctl->round,
+                         g->h_ctl->wl_n[g->h_ctl->round & 1u]);
         // ---- retirement
         GEQ_HIP(hipMemsetAsync(S->active.p, 0, S->max_jobs, s), GEQ_ERR_TRANSFER);
         k_mark_active<<<256, 256, 0, s>>>(g->ctl.p, S->wl_ptrs.p, S->job_of.p, S->active.p);
@@ -616,6 +687,7 @@ geq_status geq_server_get_stats(geq_server* S, geq_server_stats* out) {
     out->jobs_resident = S->resident.size();
     out->jobs_waiting = S->build_q.size() + S->ready_q.size();
     out->jobs_done = S->jobs_done;
+    out->builds = S->builds;
     return GEQ_OK;
 }
 

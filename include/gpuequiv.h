@@ -199,7 +199,8 @@ void geq_built_free(geq_built* built);
 /* ---- serving: continuous batching of spectroscopy jobs ----
  * Jobs (an LTS and the (p, q) pairs to compare; the game of
  * TransitionSystem::compare_unminimized seeded with every pair) arrive at any
- * time.  A builder thread builds each job's game on its own stream; at the
+ * time.  Builder threads build the queued jobs' games together (one level
+ * loop over the union of their LTSs, on the builder's own stream); at the
  * next round boundary the game is placed into shared HBM pools (admission:
  * its positions and edges must fit the pools' free ranges, i.e. the HBM
  * budget) and its frontier joins the shared device worklist, so resident jobs
@@ -248,6 +249,7 @@ typedef struct {
     uint64_t edge_slots, edge_slots_used;     /* edge pool: size, slots held by resident jobs */
     uint64_t arena_entries, arena_top;        /* energy arena: size, entries in use (u16) */
     uint64_t jobs_resident, jobs_waiting, jobs_done;
+    uint64_t builds;                          /* merged builds (jobs built together) so far */
 } geq_server_stats;
 geq_status geq_server_get_stats(geq_server* server, geq_server_stats* out);
 

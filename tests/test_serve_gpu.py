@@ -131,3 +131,19 @@ def test_arena_compaction_keeps_results(monkeypatch):
     print("serve stats", st, flush=True)
     assert st["compactions"] > 0
     srv.close()
+
+
+def test_invalid_and_empty_jobs_finish_alone():
+    print("serve: an invalid job, a job without pairs, and valid jobs in one batch", flush=True)
+    jobs = _jobs(3, 5)
+    bad = TransitionSystem(np.array([0, 2, 1], np.uint32), np.array([0, 1], np.uint32), np.array([0, 0], np.int32))
+    srv = Server(budget_bytes=64 << 20)
+    ids = [srv.submit(jobs[0][0], jobs[0][1]), srv.submit(bad, [(0, 1)]), srv.submit(jobs[1][0], []),
+           srv.submit(jobs[2][0], jobs[2][1])]
+    res = srv.run_until_done(ids)
+    assert res[ids[1]][0]["state_name"] == "failed" and res[ids[1]][0]["status"] == 4     # INVALID_ARG
+    assert res[ids[2]][0]["state_name"] == "done" and res[ids[2]][1][0].tolist() == [0]
+    assert H.canon_lists(*res[ids[0]][1]) == jobs[0][2]
+    assert H.canon_lists(*res[ids[3]][1]) == jobs[2][2]
+    assert srv.stats()["builds"] >= 1
+    srv.close()
