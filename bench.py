@@ -74,6 +74,12 @@ def bench_batch(args) -> dict:
     b = GameBatch(graphs)
     for _ in range(args.warmup):
         b.solve()
+    tune = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tune = b.autotune(repeats=1)
+        tune["tuning_s"] = time.perf_counter() - t0
+        print(f"bench: autotune {tune}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s = b.solve()
@@ -81,6 +87,11 @@ def bench_batch(args) -> dict:
     prof = b.solve(profile=True)
     print(f"bench: batched solve {batched_s*1e3:.2f} ms", file=sys.stderr, flush=True)
 
+    if args.no_plain:          # profiling runs: the batched solve alone
+        kms = prof["eval_kernel_ms"]
+        return {"metric": "batched solve only (profiling run)", "ms_per_step": batched_s * 1e3,
+                "edge_updates_per_step": int(s["edge_updates"]), "rounds": int(s["rounds"]),
+                "k_eval_ms": kms, "alg_bytes": int(prof["alg_bytes"])}
     lib = _lib.lib()
     ctx = _lib.Context.get()
     handles, keep, reaches = [], [], []
@@ -144,6 +155,7 @@ def bench_batch(args) -> dict:
                        "latency_ms_p50": float(np.percentile(lat_ms, 50)),
                        "latency_ms_p99": float(np.percentile(lat_ms, 99))},
         "parity_sample": {"games_compared": len(sample), "mismatches": mismatches},
+        "autotune": tune,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_eval",
                      "launches": int(prof["eval_launches"]),
@@ -434,7 +446,19 @@ def main() -> int:
     ap.add_argument("--games", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="batch: keep the default launch shape")
+    ap.add_argument("--no-plain", action="store_true", help="batch: skip the one-game-per-call comparison")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB copied once by a plain device copy first (a known-size kernel to scale PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"bench: copy probe of {args.copy_probe} MiB done", file=sys.stderr, flush=True)
+        del x, y
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design note, not a launched path", file=sys.stderr)

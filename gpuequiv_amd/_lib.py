@@ -144,6 +144,23 @@ class geq_server_stats(C.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
+class geq_tune_params(C.Structure):
+    _fields_ = [("waves_per_block", C.c_uint32), ("eval_blocks_per_cu", C.c_uint32), ("expand_blocks_per_cu", C.c_uint32),
+                ("rounds_per_launch", C.c_uint32), ("arena_entries", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+class geq_tune_report(C.Structure):
+    _fields_ = [("best", geq_tune_params), ("default_ms", C.c_double), ("best_ms", C.c_double),
+                ("candidates", C.c_uint32), ("repeats", C.c_uint32), ("arena_peak", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {"best": self.best.as_dict(), "default_ms": self.default_ms, "best_ms": self.best_ms,
+                "candidates": int(self.candidates), "repeats": int(self.repeats), "arena_peak": int(self.arena_peak)}
+
+
 P = C.POINTER
 _SIGS = {
     "geq_last_error": (C.c_char_p, []),
@@ -175,6 +192,10 @@ _SIGS = {
     "geq_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, P(geq_job_info), P(geq_energies)]),
     "geq_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
     "geq_server_get_stats": (C.c_int, [C.c_void_p, P(geq_server_stats)]),
+    "geq_game_get_tune": (C.c_int, [C.c_void_p, P(geq_tune_params)]),
+    "geq_game_set_tune": (C.c_int, [C.c_void_p, P(geq_tune_params)]),
+    "geq_game_autotune": (C.c_int, [C.c_void_p, P(C.c_uint32), C.c_uint32, C.c_uint32, P(geq_tune_report)]),
+    "geq_batch_autotune": (C.c_int, [C.c_void_p, P(P(C.c_uint32)), P(C.c_uint32), C.c_uint32, P(geq_tune_report)]),
     "geq_server_finished": (C.c_int, [C.c_void_p, P(C.c_uint64), C.c_uint32, P(C.c_uint32)]),
     "geq_bisimilar_minimize": (C.c_int, [P(geq_lts), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
                                          P(C.c_int32), P(C.c_uint32)]),

@@ -47,6 +47,26 @@ class GameBatch:
         self.last_stats = st.as_dict()
         return self.last_stats
 
+    def autotune(self, repeats: int = 3) -> dict:
+        """Pick workgroup size, grid sizes, rounds per graph replay and arena
+        capacity by timing candidates on this batch (geq_batch_autotune)."""
+        rep = _lib.geq_tune_report()
+        _lib.check(_lib.lib().geq_batch_autotune(self.handle, self._reach_ptrs, _lib.u32p(self._reach_n), int(repeats),
+                                                 C.byref(rep)))
+        return rep.as_dict()
+
+    def tune(self) -> dict:
+        p = _lib.geq_tune_params()
+        _lib.check(_lib.lib().geq_game_get_tune(self.handle, C.byref(p)))
+        return p.as_dict()
+
+    def set_tune(self, **kw) -> None:
+        p = _lib.geq_tune_params()
+        _lib.check(_lib.lib().geq_game_get_tune(self.handle, C.byref(p)))
+        for k, v in kw.items():
+            setattr(p, k, int(v))
+        _lib.check(_lib.lib().geq_game_set_tune(self.handle, C.byref(p)))
+
     def energies(self) -> list:
         n = len(self.graphs)
         outs = (_lib.geq_energies * n)()

@@ -296,7 +296,8 @@ __global__ void k_seed_zero(uint4* cur, const uint32_t* ids, uint32_t n) {
 
 // Grow the arena keeping its content (lists of the committed state live in [0, top)).
 uint32_t geq::initial_arena_entries(uint64_t E, uint64_t want) {
-    uint64_t n = want ? want : std::max<uint64_t>(1u << 20, 8ull * E + 1024);
+    // most antichains fit the 7 inline slots of a record; the arena grows on demand
+    uint64_t n = want ? want : std::max<uint64_t>(1u << 20, E / 2);
     if (const char* env = std::getenv("GEQ_ARENA_ENTRIES")) {
         const long long v = std::atoll(env);
         if (v >= 64) n = (uint64_t)v;
@@ -332,13 +333,16 @@ namespace geq {
 RoundShape round_shape(const geq_game* g) {
     const uint32_t ncu = (uint32_t)std::max(1, g->ctx->n_cu);
     RoundShape r;
-    r.eval_blocks = std::max(1u, std::min(ncu * g->tune.eval_blocks_per_cu, (g->V + WAVES - 1) / WAVES));
-    r.expand_blocks = std::max(1u, std::min(ncu * g->tune.expand_blocks_per_cu, (g->V + 64u * WAVES - 1) / (64u * WAVES)));
+    const uint32_t w = std::max(1u, std::min<uint32_t>(g->tune.waves_per_block, MAX_WAVES));
+    r.threads = 64u * w;
+    r.eval_blocks = std::max(1u, std::min(ncu * g->tune.eval_blocks_per_cu, (g->V + w - 1) / w));
+    r.expand_blocks = std::max(1u, std::min(ncu * g->tune.expand_blocks_per_cu, (g->V + 64u * w - 1) / (64u * w)));
     r.rounds_per_launch = std::max(1u, g->tune.rounds_per_launch);
     r.eval_blocks = std::min(r.eval_blocks, STAT_SLOTS);
     // cooperative launch: every block must be resident at once
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_solve), BLOCK, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_solve), BLOCK,
+                                                     WAVES * EVAL_LDS_PER_WAVE) != hipSuccess ||
         per_cu <= 0)
         per_cu = 1;
     r.coop_blocks = std::min<uint32_t>(std::min<uint32_t>((uint32_t)per_cu, g->tune.coop_blocks_per_cu) * ncu, STAT_SLOTS);
@@ -354,8 +358,8 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
     }
     GEQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), GEQ_ERR_NO_DEVICE);
     for (uint32_t k = 0; k < shape.rounds_per_launch; k++) {
-        k_eval<<<shape.eval_blocks, BLOCK, 0, s>>>(ea);
-        k_expand<<<shape.expand_blocks, BLOCK, 0, s>>>(xa);
+        k_eval<<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea);
+        k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa);
         k_advance<<<1, 64, 0, s>>>(g->ctl.p);
     }
     hipGraph_t graph = nullptr;
@@ -468,9 +472,9 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             GEQ_HIP(hipEventCreate(&e1), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventCreate(&e2), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventRecord(e0, s), GEQ_ERR_NO_DEVICE);
-            k_eval<<<shape.eval_blocks, BLOCK, 0, s>>>(ea);
+            k_eval<<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea);
             GEQ_HIP(hipEventRecord(e1, s), GEQ_ERR_NO_DEVICE);
-            k_expand<<<shape.expand_blocks, BLOCK, 0, s>>>(xa);
+            k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa);
             GEQ_HIP(hipEventRecord(e2, s), GEQ_ERR_NO_DEVICE);
             k_advance<<<1, 64, 0, s>>>(g->ctl.p);
             prof.push_back(e0);
@@ -495,7 +499,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             GEQ_HIP(hipEventCreate(&k1), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventRecord(k0, s), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve), dim3(shape.coop_blocks), dim3(BLOCK),
-                                               args, 0, s),
+                                               args, WAVES * EVAL_LDS_PER_WAVE, s),
                     GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventRecord(k1, s), GEQ_ERR_NO_DEVICE);
             coop_events.push_back(k0);

@@ -55,7 +55,8 @@ geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_en
 // packed updates, attacker flags; the reverse CSR is derived on the device.
 geq_status game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,
                             const uint32_t* d_wt, const uint8_t* d_atk, geq_game** out);
-// Energy-arena entries (u16) a game starts with: `want`, or 8 per edge when 0;
+// Energy-arena entries (u16) a game starts with: `want`, or one per two edges
+// (at least 2^20) when 0;
 // the environment variable GEQ_ARENA_ENTRIES overrides both (small values
 // force the regrow path).
 uint32_t initial_arena_entries(uint64_t E, uint64_t want);
@@ -66,6 +67,7 @@ geq_status alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_game** o
 // Launch shape of one round and the hipGraph of `rounds_per_launch` captured rounds.
 struct RoundShape {
     uint32_t eval_blocks, expand_blocks, rounds_per_launch, coop_blocks;
+    uint32_t threads;                 // block size of k_eval / k_expand
 };
 RoundShape round_shape(const geq_game* g);
 geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape);
@@ -101,6 +103,7 @@ struct geq_game {
         uint32_t expand_blocks_per_cu = 4;
         uint32_t rounds_per_launch = 4;
         uint32_t coop_blocks_per_cu = 8;
+        uint32_t waves_per_block = 4;     // k_eval / k_expand workgroup: 1..8 waves
     } tune;
     std::vector<uint32_t> game_base;      // batched games: first node of every game (+ total)
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds

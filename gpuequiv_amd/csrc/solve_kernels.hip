@@ -43,172 +43,288 @@ __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint
     return total;
 }
 
-// One round's evaluations: every wave takes worklist entries (grid-stride) and
-// evaluates one node at a time.  Statistics accumulate in `acc`.
-__device__ __forceinline__ void eval_phase(const EvalArgs& a, uint32_t (*s_bm)[3][128], StatAcc& acc, uint32_t r,
-                                           uint32_t n) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wid = threadIdx.x >> 6;
-    uint32_t* A = s_bm[wid][0];   // union (attacker) bitmap
-    uint32_t* B = s_bm[wid][1];   // one successor's images (defender)
-    uint32_t* O = s_bm[wid][2];   // old list of the node
+// Successors of the node being evaluated, read straight from HBM (nodes with
+// more successors than the wave's staging area) ...
+struct SuccGlobal {
+    const uint2* __restrict__ edges;
+    const uint4* __restrict__ cur;
+    uint32_t r0;
+    __device__ __forceinline__ void get(uint32_t j, uint4& d, uint32_t& w) const {
+        const uint2 ed = edges[r0 + j];
+        w = ed.y;
+        d = cur[ed.x];
+    }
+};
+// ... or from the wave's LDS staging area, gathered for a run of worklist nodes at once.
+struct SuccStaged {
+    const uint4* rec;
+    const uint32_t* w;
+    uint32_t off;
+    __device__ __forceinline__ void get(uint32_t j, uint4& d, uint32_t& ww) const {
+        d = rec[off + j];
+        ww = w[off + j];
+    }
+};
+
+// Evaluate node u (worklist slot idx): the attacker merge or the defender
+// combination, writing nxt[u] and chg[idx].  Successor j's record and update
+// come from `src`; everything else is the same for both sources.
+template <class Src>
+__device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t* A, uint32_t* B, uint32_t* O, uint32_t lane, uint32_t idx,
+                                          uint32_t u, uint32_t deg, bool isatk, const uint4 own, const Src& src, StatAcc& acc,
+                                          uint64_t& st_bytes_lane) {
     const uint32_t maxv = a.maxv;
     const uint16_t* __restrict__ arena = a.arena;
+    const uint32_t own_len = uniform(nl_len(own));
 
-    uint64_t st_bytes_lane = 0;
-    const uint32_t gstride = gridDim.x * WAVES;
-    const uint32_t* __restrict__ wl = a.wl[r & 1u];
-    for (uint32_t idx = uniform(blockIdx.x * WAVES + wid); idx < n; idx += gstride) {
-        const uint32_t u = uniform(wl[idx]);
-        const uint32_t r0 = uniform(a.row[u]);
-        const uint32_t deg = uniform(a.row[u + 1]) - r0;
-        const bool isatk = uniform(a.atk[u]) != 0;
-        const uint4 own = a.cur[u];
-        const uint32_t own_len = uniform(nl_len(own));
+    A[lane] = 0u; A[lane + 64] = 0u;
+    O[lane] = 0u; O[lane + 64] = 0u;
+    wave_sync();
+    for (uint32_t j = lane; j < own_len; j += 64) {
+        uint32_t e = nl_get(own, arena, j);
+        bm_set(O, e);
+        if (isatk) bm_set(A, inv_update(e, 0u, maxv));
+    }
 
-        A[lane] = 0u; A[lane + 64] = 0u;
-        O[lane] = 0u; O[lane + 64] = 0u;
-        wave_sync();
-        for (uint32_t j = lane; j < own_len; j += 64) {
-            uint32_t e = nl_get(own, arena, j);
-            bm_set(O, e);
-            if (isatk) bm_set(A, inv_update(e, 0u, maxv));
+    bool changed = false;
+    bool evaluated = true;
+    uint32_t new_len = 0;
+    if (isatk) {
+        // ---- attacker: OR of the inverse-updated successor lists + own list
+        for (uint32_t base = 0; base < deg; base += 64) {
+            const uint32_t k = base + lane;
+            uint32_t len = 0, w = 0;
+            uint4 d = make_uint4(0, 0, 0, 0);
+            if (k < deg) {
+                src.get(k, d, w);
+                len = nl_len(d);
+                st_bytes_lane += 8u + 4u * len;
+            }
+            if (len <= INLINE_CAP) {
+                for (uint32_t j = 0; j < len; j++) bm_set(A, inv_update(nl_inline(d, j), w, maxv));
+            }
+            uint64_t longm = __ballot(len > INLINE_CAP);
+            while (longm) {
+                const uint32_t s = (uint32_t)__builtin_ctzll(longm);
+                longm &= longm - 1;
+                const uint32_t lw = uniform(shfl32(w, s));
+                const uint32_t loff = uniform(shfl32(d.y, s));
+                const uint32_t llen = uniform(shfl32(len, s));
+                for (uint32_t j = lane; j < llen; j += 64) bm_set(A, inv_update(arena[loff + j], lw, maxv));
+            }
         }
-
-        bool changed = false;
-        bool evaluated = true;
-        uint32_t new_len = 0;
-        if (isatk) {
-            // ---- attacker: OR of the inverse-updated successor lists + own list
+        wave_sync();
+        const uint64_t W = up_close(bm_word(A, lane), lane);
+        const uint64_t M = minimal_of(W, lane);
+        const uint64_t Ow = bm_word(O, lane);
+        // changed iff a minimal budget is not already one of the node's own
+        // energies (attack_minimize.wgsl:208-214: a survivor from a successor slice)
+        changed = __ballot((M & ~Ow) != 0ull) != 0ull;
+        if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
+    } else {
+        // ---- defender: skipped while any successor has no energies (src/energygame.rs:789-792)
+        bool any_empty = false;
+        for (uint32_t base = 0; base < deg; base += 64) {
+            const uint32_t k = base + lane;
+            uint32_t len = 1;
+            if (k < deg) {
+                uint4 d;
+                uint32_t w;
+                src.get(k, d, w);
+                len = nl_len(d);
+            }
+            any_empty |= (__ballot(len == 0) != 0ull);
+        }
+        if (any_empty || deg == 0) {
+            evaluated = false;
+        } else if (deg == 1) {
+            // A single successor is copied, not minimised
+            // (defend_iterative.wgsl:301-313 and the empty fold loop).
+            uint4 d;
+            uint32_t w;
+            src.get(0, d, w);
+            w = uniform(w);
+            const uint32_t len = uniform(nl_len(d));
+            st_bytes_lane += (lane == 0) ? 8u + 4u * len : 0u;
+            bool all_in = true;
+            for (uint32_t j = lane; j < len; j += 64) all_in &= bm_test(O, inv_update(nl_get(d, arena, j), w, maxv));
+            const bool same = (len == own_len) && (__ballot(!all_in) == 0ull);
+            changed = !same;
+            if (changed) {
+                uint4* rec = &a.nxt[u];
+                if (len <= INLINE_CAP) {
+                    uint16_t* h = reinterpret_cast<uint16_t*>(rec);
+                    if (lane < len) h[1 + lane] = (uint16_t)inv_update(nl_inline(d, lane), w, maxv);
+                    if (lane == 0) h[0] = (uint16_t)(len | (len > 1 ? UNSORTED : 0u));
+                    new_len = len;
+                } else {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&a.ctl->arena_top, len);
+                    base = uniform(shfl32(base, 0));
+                    if ((uint64_t)base + len > a.arena_cap) {
+                        if (lane == 0) atomicOr(&a.ctl->overflow, 1u);
+                        new_len = ~0u;
+                    } else {
+                        for (uint32_t j = lane; j < len; j += 64)
+                            a.arena[base + j] = (uint16_t)inv_update(arena[nl_off(d) + j], w, maxv);
+                        if (lane == 0) *rec = make_uint4(len | UNSORTED, base, 0u, 0u);
+                        new_len = len;
+                    }
+                }
+            }
+        } else {
+            // AND of the up-closures of every successor's inverse-updated list
+            uint64_t acc_bm = ~0ull;
             for (uint32_t base = 0; base < deg; base += 64) {
                 const uint32_t k = base + lane;
-                uint32_t len = 0, w = 0;
+                uint32_t w = 0;
                 uint4 d = make_uint4(0, 0, 0, 0);
                 if (k < deg) {
-                    uint2 ed = a.edges[r0 + k];
-                    w = ed.y;
-                    d = a.cur[ed.x];
-                    len = nl_len(d);
-                    st_bytes_lane += 8u + 4u * len;
+                    src.get(k, d, w);
+                    st_bytes_lane += 8u + 4u * nl_len(d);
                 }
-                if (len <= INLINE_CAP) {
-                    for (uint32_t j = 0; j < len; j++) bm_set(A, inv_update(nl_inline(d, j), w, maxv));
+                const uint32_t nk = min(64u, deg - base);
+                for (uint32_t s = 0; s < nk; s++) {
+                    const uint32_t ws = uniform(shfl32(w, s));
+                    uint4 ds;
+                    ds.x = uniform(shfl32(d.x, s));
+                    ds.y = uniform(shfl32(d.y, s));
+                    ds.z = uniform(shfl32(d.z, s));
+                    ds.w = uniform(shfl32(d.w, s));
+                    const uint32_t ls = nl_len(ds);
+                    B[lane] = 0u; B[lane + 64] = 0u;
+                    wave_sync();
+                    for (uint32_t j = lane; j < ls; j += 64) bm_set(B, inv_update(nl_get(ds, arena, j), ws, maxv));
+                    wave_sync();
+                    acc_bm &= up_close(bm_word(B, lane), lane);
+                    wave_sync();
                 }
-                uint64_t longm = __ballot(len > INLINE_CAP);
-                while (longm) {
-                    const uint32_t s = (uint32_t)__builtin_ctzll(longm);
-                    longm &= longm - 1;
-                    const uint32_t lw = uniform(shfl32(w, s));
-                    const uint32_t loff = uniform(shfl32(d.y, s));
-                    const uint32_t llen = uniform(shfl32(len, s));
-                    for (uint32_t j = lane; j < llen; j += 64) bm_set(A, inv_update(arena[loff + j], lw, maxv));
-                }
+            }
+            const uint64_t M = minimal_of(acc_bm, lane);
+            const uint64_t Ow = bm_word(O, lane);
+            const uint32_t total = uniform(wave_sum((uint32_t)__popcll(M)));
+            // equality test of src/energygame.rs:929 (same size, new within old)
+            const bool same = (total == own_len) && (__ballot((M & ~Ow) != 0ull) == 0ull);
+            changed = !same;
+            if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
+        }
+    }
+    if (new_len == ~0u) changed = false;   // arena overflow: the round runs again
+    if (lane == 0) {
+        a.chg[idx] = changed ? 1 : 0;
+        if (evaluated) {
+            acc.evals++;
+            acc.edges += deg;
+            acc.bytes += 9u + 4u * own_len + (changed ? 4u * new_len : 0u);
+            acc.changed += changed ? 1 : 0;
+        }
+    }
+    if (evaluated) acc.bytes += st_bytes_lane;
+    st_bytes_lane = 0;
+    wave_sync();
+}
+
+// One round's evaluations.  A wave takes 64 worklist entries at a time: one
+// lane per node loads the node's row, flag and record, then the successor
+// records of a run of these nodes are gathered into the wave's LDS staging
+// area with all lanes' loads in flight together, and the nodes are evaluated
+// one after the other from LDS.  A node with more successors than the staging
+// area holds reads them from HBM directly.
+// r: the round; n: its worklist size (0 after an arena overflow).
+// lds: the block's dynamic LDS, EVAL_LDS_PER_WAVE bytes per wave.
+__device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds, StatAcc& acc, uint32_t r, uint32_t n) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wid = threadIdx.x >> 6;
+    unsigned char* mine = lds + (size_t)wid * EVAL_LDS_PER_WAVE;
+    uint4* rec = reinterpret_cast<uint4*>(mine);                          // staged successor records
+    uint32_t* wts = reinterpret_cast<uint32_t*>(mine + STAGE * 16);        // staged successor updates
+    uint32_t* ex = wts + STAGE;                                           // first staging slot of each node of the run
+    uint32_t* rs = ex + 64;                                               // first edge of each node of the run
+    uint32_t* A = rs + 64;        // union (attacker) bitmap
+    uint32_t* B = A + 128;        // one successor's images (defender)
+    uint32_t* O = B + 128;        // old list of the node
+
+    uint64_t st_bytes_lane = 0;
+    const uint32_t nw = blockDim.x >> 6;
+    const uint32_t* __restrict__ wl = a.wl[r & 1u];
+    // entries per wave visit: up to 64, fewer when the worklist is too short
+    // to give every wave of the grid a full set
+    const uint32_t waves = gridDim.x * nw;
+    const uint32_t chunk = max(1u, min(64u, (n + waves - 1) / max(waves, 1u)));
+    for (uint32_t base = uniform((blockIdx.x * nw + wid) * chunk); base < n; base += waves * chunk) {
+        const uint32_t cnt = min(chunk, n - base);
+        // lane k: worklist entry base + k
+        uint32_t u = 0, r0 = 0, deg = 0;
+        bool isatk = false;
+        uint4 own = make_uint4(0, 0, 0, 0);
+        if (lane < cnt) {
+            u = wl[base + lane];
+            r0 = a.row[u];
+            deg = a.row[u + 1] - r0;
+            isatk = a.atk[u] != 0;
+            own = a.cur[u];
+        }
+        uint32_t k0 = 0;
+        while (k0 < cnt) {
+            // the run [k0, k1): consecutive nodes whose successors fit the staging area
+            const uint32_t d = (lane >= k0 && lane < cnt) ? deg : 0u;
+            const uint32_t incl = wave_incl_scan(d, lane);
+            const uint64_t over = __ballot(lane >= k0 && lane < cnt && incl > STAGE);
+            const uint32_t k1 = over ? (uint32_t)__builtin_ctzll(over) : cnt;
+            if (k1 == k0) {
+                // node k0 alone has more successors than the staging area
+                const uint32_t v = uniform(shfl32(u, k0));
+                const uint32_t vr0 = uniform(shfl32(r0, k0));
+                const uint32_t vdeg = uniform(shfl32(deg, k0));
+                const bool vatk = uniform(shfl32(isatk ? 1u : 0u, k0)) != 0u;
+                uint4 vown;
+                vown.x = shfl32(own.x, k0); vown.y = shfl32(own.y, k0); vown.z = shfl32(own.z, k0); vown.w = shfl32(own.w, k0);
+                eval_node(a, A, B, O, lane, base + k0, v, vdeg, vatk, vown, SuccGlobal{a.edges, a.cur, vr0}, acc, st_bytes_lane);
+                k0++;
+                continue;
+            }
+            const uint32_t total = uniform(shfl32(incl, k1 - 1));
+            if (lane >= k0 && lane < k1) {
+                ex[lane] = incl - d;
+                rs[lane] = r0;
             }
             wave_sync();
-            const uint64_t W = up_close(bm_word(A, lane), lane);
-            const uint64_t M = minimal_of(W, lane);
-            const uint64_t Ow = bm_word(O, lane);
-            // changed iff a minimal budget is not already one of the node's own
-            // energies (attack_minimize.wgsl:208-214: a survivor from a successor slice)
-            changed = __ballot((M & ~Ow) != 0ull) != 0ull;
-            if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
-        } else {
-            // ---- defender: skipped while any successor has no energies (src/energygame.rs:789-792)
-            bool any_empty = false;
-            for (uint32_t base = 0; base < deg; base += 64) {
-                const uint32_t k = base + lane;
-                uint32_t len = 1;
-                if (k < deg) len = nl_len(a.cur[a.edges[r0 + k].x]);
-                any_empty |= (__ballot(len == 0) != 0ull);
-            }
-            if (any_empty || deg == 0) {
-                evaluated = false;
-            } else if (deg == 1) {
-                // A single successor is copied, not minimised
-                // (defend_iterative.wgsl:301-313 and the empty fold loop).
-                const uint2 ed = a.edges[r0];
-                const uint32_t w = uniform(ed.y);
-                const uint4 d = a.cur[ed.x];
-                const uint32_t len = uniform(nl_len(d));
-                st_bytes_lane += (lane == 0) ? 8u + 4u * len : 0u;
-                bool all_in = true;
-                for (uint32_t j = lane; j < len; j += 64) all_in &= bm_test(O, inv_update(nl_get(d, arena, j), w, maxv));
-                const bool same = (len == own_len) && (__ballot(!all_in) == 0ull);
-                changed = !same;
-                if (changed) {
-                    uint4* rec = &a.nxt[u];
-                    if (len <= INLINE_CAP) {
-                        uint16_t* h = reinterpret_cast<uint16_t*>(rec);
-                        if (lane < len) h[1 + lane] = (uint16_t)inv_update(nl_inline(d, lane), w, maxv);
-                        if (lane == 0) h[0] = (uint16_t)(len | (len > 1 ? UNSORTED : 0u));
-                        new_len = len;
-                    } else {
-                        uint32_t base = 0;
-                        if (lane == 0) base = atomicAdd(&a.ctl->arena_top, len);
-                        base = uniform(shfl32(base, 0));
-                        if ((uint64_t)base + len > a.arena_cap) {
-                            if (lane == 0) atomicOr(&a.ctl->overflow, 1u);
-                            new_len = ~0u;
-                        } else {
-                            for (uint32_t j = lane; j < len; j += 64)
-                                a.arena[base + j] = (uint16_t)inv_update(arena[nl_off(d) + j], w, maxv);
-                            if (lane == 0) *rec = make_uint4(len | UNSORTED, base, 0u, 0u);
-                            new_len = len;
-                        }
-                    }
+            // gather: slot s belongs to the last node of the run whose first slot is <= s;
+            // both slots of a lane issue their loads before either is waited for
+            static_assert(STAGE == 128, "the gather below handles two slots per lane");
+            auto edge_of = [&](uint32_t s) -> uint2 {
+                uint32_t lo = k0, hi = k1 - 1;   // ex[lo] <= s: find the largest such index
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (ex[mid] <= s) lo = mid; else hi = mid - 1;
                 }
-            } else {
-                // AND of the up-closures of every successor's inverse-updated list
-                uint64_t acc = ~0ull;
-                for (uint32_t base = 0; base < deg; base += 64) {
-                    const uint32_t k = base + lane;
-                    uint32_t w = 0;
-                    uint4 d = make_uint4(0, 0, 0, 0);
-                    if (k < deg) {
-                        uint2 ed = a.edges[r0 + k];
-                        w = ed.y;
-                        d = a.cur[ed.x];
-                        st_bytes_lane += 8u + 4u * nl_len(d);
-                    }
-                    const uint32_t nk = min(64u, deg - base);
-                    for (uint32_t s = 0; s < nk; s++) {
-                        const uint32_t ws = uniform(shfl32(w, s));
-                        uint4 ds;
-                        ds.x = uniform(shfl32(d.x, s));
-                        ds.y = uniform(shfl32(d.y, s));
-                        ds.z = uniform(shfl32(d.z, s));
-                        ds.w = uniform(shfl32(d.w, s));
-                        const uint32_t ls = nl_len(ds);
-                        B[lane] = 0u; B[lane + 64] = 0u;
-                        wave_sync();
-                        for (uint32_t j = lane; j < ls; j += 64) bm_set(B, inv_update(nl_get(ds, arena, j), ws, maxv));
-                        wave_sync();
-                        acc &= up_close(bm_word(B, lane), lane);
-                        wave_sync();
-                    }
-                }
-                const uint64_t M = minimal_of(acc, lane);
-                const uint64_t Ow = bm_word(O, lane);
-                const uint32_t total = uniform(wave_sum((uint32_t)__popcll(M)));
-                // equality test of src/energygame.rs:929 (same size, new within old)
-                const bool same = (total == own_len) && (__ballot((M & ~Ow) != 0ull) == 0ull);
-                changed = !same;
-                if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
+                return a.edges[rs[lo] + (s - ex[lo])];   // (LDS, not a lane shuffle: lane lo may be inactive here)
+            };
+            const uint32_t s0 = lane, s1 = 64u + lane;
+            const uint2 ed0 = s0 < total ? edge_of(s0) : make_uint2(0u, 0u);
+            const uint2 ed1 = s1 < total ? edge_of(s1) : make_uint2(0u, 0u);
+            const uint4 rc0 = s0 < total ? a.cur[ed0.x] : make_uint4(0, 0, 0, 0);
+            const uint4 rc1 = s1 < total ? a.cur[ed1.x] : make_uint4(0, 0, 0, 0);
+            if (s0 < total) {
+                rec[s0] = rc0;
+                wts[s0] = ed0.y;
             }
-        }
-        if (new_len == ~0u) changed = false;   // arena overflow: the round runs again
-        if (lane == 0) {
-            a.chg[idx] = changed ? 1 : 0;
-            if (evaluated) {
-                acc.evals++;
-                acc.edges += deg;
-                acc.bytes += 9u + 4u * own_len + (changed ? 4u * new_len : 0u);
-                acc.changed += changed ? 1 : 0;
+            if (s1 < total) {
+                rec[s1] = rc1;
+                wts[s1] = ed1.y;
             }
+            wave_sync();
+            for (uint32_t k = k0; k < k1; k++) {
+                const uint32_t v = uniform(shfl32(u, k));
+                const uint32_t vdeg = uniform(shfl32(deg, k));
+                const bool vatk = uniform(shfl32(isatk ? 1u : 0u, k)) != 0u;
+                uint4 vown;
+                vown.x = shfl32(own.x, k); vown.y = shfl32(own.y, k); vown.z = shfl32(own.z, k); vown.w = shfl32(own.w, k);
+                const uint32_t off = uniform(ex[k]);
+                eval_node(a, A, B, O, lane, base + k, v, vdeg, vatk, vown, SuccStaged{rec, wts, off}, acc, st_bytes_lane);
+            }
+            k0 = k1;
         }
-        if (evaluated) acc.bytes += st_bytes_lane;
-        st_bytes_lane = 0;
-        wave_sync();
     }
 }
 
@@ -228,12 +344,12 @@ __device__ __forceinline__ void flush_stats(const StatAcc& acc, unsigned long lo
     if (threadIdx.x < 4 && s_red[threadIdx.x]) slots[4ull * blockIdx.x + threadIdx.x] += s_red[threadIdx.x];
 }
 
-__global__ __launch_bounds__(BLOCK) void k_eval(EvalArgs a) {
-    __shared__ uint32_t s_bm[WAVES][3][128];
+__global__ __launch_bounds__(MAX_BLOCK) void k_eval(EvalArgs a) {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
     StatAcc acc;
     const uint32_t r = uniform(a.ctl->round);
     const uint32_t n = a.ctl->overflow ? 0u : uniform(a.ctl->wl_n[r & 1u]);
-    eval_phase(a, s_bm, acc, r, n);
+    eval_phase(a, dyn_lds, acc, r, n);
     flush_stats(acc, a.stats);
 }
 
@@ -255,8 +371,9 @@ __device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_s
     uint32_t* out = a.wl[out_slot];
     const uint32_t stamp_val = seeding ? r + 1u : r + 2u;
     uint32_t cnt = 0;
-    const uint32_t gstride = gridDim.x * WAVES * 64;
-    for (uint32_t b = (blockIdx.x * WAVES + wid) * 64; b < n; b += gstride) {
+    const uint32_t nw = blockDim.x >> 6;
+    const uint32_t gstride = gridDim.x * nw * 64;
+    for (uint32_t b = (blockIdx.x * nw + wid) * 64; b < n; b += gstride) {
         const uint32_t i = b + lane;
         uint32_t u = 0;
         bool f = false;
@@ -299,19 +416,19 @@ __device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_s
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (int w = 0; w < WAVES; w++) tot += s_cnt[w];
-        s_cnt[WAVES] = tot ? atomicAdd(&a.ctl->wl_n[out_slot], tot) : 0u;
+        for (uint32_t w = 0; w < nw; w++) tot += s_cnt[w];
+        s_cnt[MAX_WAVES] = tot ? atomicAdd(&a.ctl->wl_n[out_slot], tot) : 0u;
     }
     __syncthreads();
-    uint32_t base = s_cnt[WAVES];
+    uint32_t base = s_cnt[MAX_WAVES];
     for (uint32_t w = 0; w < wid; w++) base += s_cnt[w];
     for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
     __syncthreads();
 }
 
-__global__ __launch_bounds__(BLOCK) void k_expand(ExpandArgs a) {
-    __shared__ uint32_t s_stage[WAVES][256];
-    __shared__ uint32_t s_cnt[WAVES + 1];
+__global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a) {
+    __shared__ uint32_t s_stage[MAX_WAVES][256];
+    __shared__ uint32_t s_cnt[MAX_WAVES + 1];
     const uint32_t r = uniform(a.ctl->round);
     const uint32_t n = a.ctl->overflow ? 0u : (a.seed ? a.n_seed : uniform(a.ctl->wl_n[r & 1u]));
     expand_phase(a, s_stage, s_cnt, r, n);
@@ -371,10 +488,10 @@ __device__ __forceinline__ bool grid_barrier(Ctl* ctl, uint32_t nblocks, uint32_
 // round's control words are read once per block by thread 0 with agent-scope
 // atomic loads and broadcast through LDS, so every wave of every block takes
 // the same branch.
-__global__ __launch_bounds__(BLOCK) void k_solve(EvalArgs ea, ExpandArgs xa, uint32_t max_rounds, uint32_t spin_limit) {
-    __shared__ uint32_t s_bm[WAVES][3][128];
-    __shared__ uint32_t s_stage[WAVES][256];
-    __shared__ uint32_t s_cnt[WAVES + 1];
+__global__ __launch_bounds__(MAX_BLOCK) void k_solve(EvalArgs ea, ExpandArgs xa, uint32_t max_rounds, uint32_t spin_limit) {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    __shared__ uint32_t s_stage[MAX_WAVES][256];
+    __shared__ uint32_t s_cnt[MAX_WAVES + 1];
     __shared__ uint32_t s_ctl[3];
     Ctl* ctl = ea.ctl;
     const uint32_t nblocks = gridDim.x;
@@ -391,7 +508,7 @@ __global__ __launch_bounds__(BLOCK) void k_solve(EvalArgs ea, ExpandArgs xa, uin
         const uint32_t r = s_ctl[0], n = s_ctl[1], stop = s_ctl[2];
         __syncthreads();
         if (stop || n == 0u) break;
-        eval_phase(ea, s_bm, acc, r, n);
+        eval_phase(ea, dyn_lds, acc, r, n);
         if (!grid_barrier(ctl, nblocks, spin_limit)) break;
         if (threadIdx.x == 0) s_ctl[2] = __hip_atomic_load(&ctl->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();

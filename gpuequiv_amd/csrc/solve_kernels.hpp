@@ -10,9 +10,14 @@
 
 namespace geq {
 
-constexpr int WAVES = 4;            // waves per block of the eval/expand kernels
+constexpr int WAVES = 4;            // default waves per block of the eval/expand kernels
 constexpr int BLOCK = 64 * WAVES;
+constexpr int MAX_WAVES = 8;        // k_eval / k_expand take any block of 1..MAX_WAVES waves
+constexpr int MAX_BLOCK = 64 * MAX_WAVES;
 constexpr uint32_t STAT_SLOTS = 8192;   // max grid size of the statistics-writing kernels
+constexpr uint32_t STAGE = 128;         // successor records staged per wave in k_eval
+// k_eval's dynamic LDS per wave: staged records + updates, run offsets and first edges, three 4096-bit sets
+constexpr uint32_t EVAL_LDS_PER_WAVE = STAGE * 16 + STAGE * 4 + 2 * 64 * 4 + 3 * 128 * 4;
 
 struct Ctl {                        // small device control block, read back between round batches
     uint32_t wl_n[2];               // worklist sizes (ping-pong by round parity)

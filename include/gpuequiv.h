@@ -139,6 +139,35 @@ geq_status geq_batch_solve(geq_game* batch, const uint32_t* const* to_reach, con
 /* outs: n_games results, each freed with geq_energies_free */
 geq_status geq_batch_energies(geq_game* batch, geq_energies* outs);
 
+/* ---- launch parameters and the autotuner ----
+ * The fixpoint's launch shape: workgroup size of k_eval / k_expand (waves per
+ * block), their grid sizes (blocks per CU), the rounds captured per replayed
+ * hipGraph, and the energy-arena capacity (u16 entries; it only grows).
+ * geq_*_autotune times candidate shapes on the game itself (median device time
+ * of `repeats` solves each, coordinate search from the current shape), keeps
+ * the fastest and sizes the arena for the largest solve seen.  Every shape
+ * computes the same result.  No counterpart in the reference (fixed WGSL
+ * workgroup sizes). */
+typedef struct {
+    uint32_t waves_per_block;       /* 1..8 (default 4) */
+    uint32_t eval_blocks_per_cu;    /* 1..64 (default 8) */
+    uint32_t expand_blocks_per_cu;  /* 1..64 (default 4) */
+    uint32_t rounds_per_launch;     /* 1..256 (default 4) */
+    uint64_t arena_entries;
+} geq_tune_params;
+typedef struct {
+    geq_tune_params best;
+    double default_ms, best_ms;     /* median device time per solve: starting shape, chosen shape */
+    uint32_t candidates, repeats;
+    uint64_t arena_peak;            /* arena entries the solves used */
+} geq_tune_report;
+geq_status geq_game_get_tune(geq_game* game, geq_tune_params* out);
+geq_status geq_game_set_tune(geq_game* game, const geq_tune_params* params);
+geq_status geq_game_autotune(geq_game* game, const uint32_t* to_reach, uint32_t n_reach, uint32_t repeats,
+                             geq_tune_report* out);
+geq_status geq_batch_autotune(geq_game* batch, const uint32_t* const* to_reach, const uint32_t* n_reach, uint32_t repeats,
+                              geq_tune_report* out);
+
 /* ---- gamebuild: LTS -> spectroscopy energy game, built on the device ----
  * Replaces GameBuild::{compare, compare_multiple, compare_all} +
  * build_internal (src/gamebuild.rs:165-265) and the position successor rules

@@ -40,3 +40,47 @@ def test_batch_equals_plain_path():
     b.solve()
     assert [H.canon_lists(o, d) for o, d in b.raw_energies()] == [H.canon_lists(o, d) for o, d in batched]
     b.close()
+
+
+@pytest.mark.parametrize("waves,eval_bpc,expand_bpc,rounds", [(1, 1, 1, 1), (2, 32, 16, 3), (8, 8, 4, 16), (8, 1, 64, 32)])
+def test_launch_shapes_give_identical_results(waves, eval_bpc, expand_bpc, rounds):
+    print("launch shape", waves, eval_bpc, expand_bpc, rounds, flush=True)
+    graphs = [workloads.random_game(v, 300 + v) for v in (4000, 50, 12000, 700)]
+    b = GameBatch(graphs)
+    st0 = b.solve()
+    want = b.raw_energies()
+    b.set_tune(waves_per_block=waves, eval_blocks_per_cu=eval_bpc, expand_blocks_per_cu=expand_bpc,
+               rounds_per_launch=rounds)
+    assert b.tune()["waves_per_block"] == waves and b.tune()["rounds_per_launch"] == rounds
+    st1 = b.solve()
+    got = b.raw_energies()
+    for (o0, d0), (o1, d1) in zip(want, got):
+        assert np.array_equal(o0, o1) and np.array_equal(d0, d1)
+    # Jacobi rounds: the schedule does not depend on the launch shape
+    assert (st0["rounds"], st0["node_evals"], st0["edge_updates"]) == (st1["rounds"], st1["node_evals"], st1["edge_updates"])
+    b.close()
+
+
+def test_autotune_keeps_results_and_reports():
+    graphs = [workloads.random_game(v, 900 + v) for v in (20000, 3000, 500)]
+    b = GameBatch(graphs)
+    b.solve()
+    want = b.raw_energies()
+    rep = b.autotune(repeats=2)
+    print("autotune", rep, flush=True)
+    assert rep["candidates"] > 5 and rep["best_ms"] <= rep["default_ms"]
+    assert rep["best"] == b.tune() and rep["best"]["arena_entries"] >= rep["arena_peak"]
+    b.solve()
+    for (o0, d0), (o1, d1) in zip(want, b.raw_energies()):
+        assert np.array_equal(o0, o1) and np.array_equal(d0, d1)
+    b.close()
+
+
+def test_set_tune_rejects_out_of_range():
+    from gpuequiv_amd._lib import Error
+    b = GameBatch([workloads.random_game(100, 1)])
+    with pytest.raises(Error):
+        b.set_tune(waves_per_block=9)
+    with pytest.raises(Error):
+        b.set_tune(rounds_per_launch=0)
+    b.close()
