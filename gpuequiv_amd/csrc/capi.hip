@@ -358,10 +358,10 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
     }
     GEQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), GEQ_ERR_NO_DEVICE);
     for (uint32_t k = 0; k < shape.rounds_per_launch; k++) {
-        k_eval<<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea);
-        k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa);
-        k_advance<<<1, 64, 0, s>>>(g->ctl.p);
+        k_eval<<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea, k);
+        k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa, k);
     }
+    k_advance<<<1, 64, 0, s>>>(g->ctl.p, shape.rounds_per_launch);
     hipGraph_t graph = nullptr;
     GEQ_HIP(hipStreamEndCapture(s, &graph), GEQ_ERR_NO_DEVICE);
     hipError_t e = hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0);
@@ -457,7 +457,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         ExpandArgs sx = xa;
         sx.seed = g->seed.p;
         sx.n_seed = n_reach;
-        k_expand<<<std::max(1u, std::min(shape.expand_blocks, (n_reach + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx);
+        k_expand<<<std::max(1u, std::min(shape.expand_blocks, (n_reach + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx, 0u);
     }
     GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -472,11 +472,11 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             GEQ_HIP(hipEventCreate(&e1), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventCreate(&e2), GEQ_ERR_NO_DEVICE);
             GEQ_HIP(hipEventRecord(e0, s), GEQ_ERR_NO_DEVICE);
-            k_eval<<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea);
+            k_eval<<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea, 0u);
             GEQ_HIP(hipEventRecord(e1, s), GEQ_ERR_NO_DEVICE);
-            k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa);
+            k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa, 0u);
             GEQ_HIP(hipEventRecord(e2, s), GEQ_ERR_NO_DEVICE);
-            k_advance<<<1, 64, 0, s>>>(g->ctl.p);
+            k_advance<<<1, 64, 0, s>>>(g->ctl.p, 1u);
             prof.push_back(e0);
             prof.push_back(e1);
             prof.push_back(e2);

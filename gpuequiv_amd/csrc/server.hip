@@ -551,7 +551,7 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
         if (n_seed) {
             ExpandArgs sx{{g->wl0.p, g->wl1.p}, g->seed.p, n_seed, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
-            k_expand<<<std::max(1u, std::min(1024u, (n_seed + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx);
+            k_expand<<<std::max(1u, std::min(1024u, (n_seed + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx, 0u);
         }
         J->batch.reset();   // the pools hold the game now (the build is freed with its last job)
         std::lock_guard<std::mutex> lk(S->mu);

@@ -6,9 +6,9 @@ namespace geq {
 // Write the members of a 4096-point bitmap (one u64 word per lane) as an
 // ascending list: inline into the node's record when it fits, else into the
 // arena.  Returns the list length (uniform).  On arena exhaustion sets
-// ctl->overflow and returns ~0u.
+// ctl->overflow to the round + 1 and returns ~0u.
 __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint4* rec, uint16_t* arena, Ctl* ctl,
-                                                 uint32_t arena_cap) {
+                                                 uint32_t arena_cap, uint32_t r) {
     const uint32_t c = (uint32_t)__popcll(M);
     const uint32_t incl = wave_incl_scan(c, lane);
     const uint32_t total = uniform(shfl32(incl, 63));
@@ -29,7 +29,7 @@ __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint
     if (lane == 0) base = atomicAdd(&ctl->arena_top, total);
     base = uniform(shfl32(base, 0));
     if ((uint64_t)base + total > arena_cap) {
-        if (lane == 0) atomicOr(&ctl->overflow, 1u);
+        if (lane == 0) atomicCAS(&ctl->overflow, 0u, r + 1u);
         return ~0u;
     }
     uint64_t m = M;
@@ -70,7 +70,7 @@ struct SuccStaged {
 // combination, writing nxt[u] and chg[idx].  Successor j's record and update
 // come from `src`; everything else is the same for both sources.
 template <class Src>
-__device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t* A, uint32_t* B, uint32_t* O, uint32_t lane, uint32_t idx,
+__device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_t* A, uint32_t* B, uint32_t* O, uint32_t lane, uint32_t idx,
                                           uint32_t u, uint32_t deg, bool isatk, const uint4 own, const Src& src, StatAcc& acc,
                                           uint64_t& st_bytes_lane) {
     const uint32_t maxv = a.maxv;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t* A, uint32
         // changed iff a minimal budget is not already one of the node's own
         // energies (attack_minimize.wgsl:208-214: a survivor from a successor slice)
         changed = __ballot((M & ~Ow) != 0ull) != 0ull;
-        if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
+        if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap, r);
     } else {
         // ---- defender: skipped while any successor has no energies (src/energygame.rs:789-792)
         bool any_empty = false;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t* A, uint32
                     if (lane == 0) base = atomicAdd(&a.ctl->arena_top, len);
                     base = uniform(shfl32(base, 0));
                     if ((uint64_t)base + len > a.arena_cap) {
-                        if (lane == 0) atomicOr(&a.ctl->overflow, 1u);
+                        if (lane == 0) atomicCAS(&a.ctl->overflow, 0u, r + 1u);
                         new_len = ~0u;
                     } else {
                         for (uint32_t j = lane; j < len; j += 64)
@@ -206,7 +206,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t* A, uint32
             // equality test of src/energygame.rs:929 (same size, new within old)
             const bool same = (total == own_len) && (__ballot((M & ~Ow) != 0ull) == 0ull);
             changed = !same;
-            if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap);
+            if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap, r);
         }
     }
     if (new_len == ~0u) changed = false;   // arena overflow: the round runs again
@@ -279,7 +279,7 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
                 const bool vatk = uniform(shfl32(isatk ? 1u : 0u, k0)) != 0u;
                 uint4 vown;
                 vown.x = shfl32(own.x, k0); vown.y = shfl32(own.y, k0); vown.z = shfl32(own.z, k0); vown.w = shfl32(own.w, k0);
-                eval_node(a, A, B, O, lane, base + k0, v, vdeg, vatk, vown, SuccGlobal{a.edges, a.cur, vr0}, acc, st_bytes_lane);
+                eval_node(a, r, A, B, O, lane, base + k0, v, vdeg, vatk, vown, SuccGlobal{a.edges, a.cur, vr0}, acc, st_bytes_lane);
                 k0++;
                 continue;
             }
@@ -321,7 +321,7 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
                 uint4 vown;
                 vown.x = shfl32(own.x, k); vown.y = shfl32(own.y, k); vown.z = shfl32(own.z, k); vown.w = shfl32(own.w, k);
                 const uint32_t off = uniform(ex[k]);
-                eval_node(a, A, B, O, lane, base + k, v, vdeg, vatk, vown, SuccStaged{rec, wts, off}, acc, st_bytes_lane);
+                eval_node(a, r, A, B, O, lane, base + k, v, vdeg, vatk, vown, SuccStaged{rec, wts, off}, acc, st_bytes_lane);
             }
             k0 = k1;
         }
@@ -344,27 +344,36 @@ __device__ __forceinline__ void flush_stats(const StatAcc& acc, unsigned long lo
     if (threadIdx.x < 4 && s_red[threadIdx.x]) slots[4ull * blockIdx.x + threadIdx.x] += s_red[threadIdx.x];
 }
 
-__global__ __launch_bounds__(MAX_BLOCK) void k_eval(EvalArgs a) {
+// k: the kernel's round within the current batch of rounds (round = ctl->round + k).
+__global__ __launch_bounds__(MAX_BLOCK) void k_eval(EvalArgs a, uint32_t k) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     StatAcc acc;
-    const uint32_t r = uniform(a.ctl->round);
-    const uint32_t n = a.ctl->overflow ? 0u : uniform(a.ctl->wl_n[r & 1u]);
+    const uint32_t r = uniform(a.ctl->round) + k;
+    const bool ovf = a.ctl->overflow != 0u;
+    const uint32_t n = ovf ? 0u : uniform(a.ctl->wl_n[r & 1u]);
+    // the previous round's list count has been consumed by its k_expand:
+    // clear it for this round's appends
+    if (!ovf && blockIdx.x == 0 && threadIdx.x == 0) a.ctl->wl_n[(r + 1u) & 1u] = 0u;
     eval_phase(a, dyn_lds, acc, r, n);
     flush_stats(acc, a.stats);
 }
 
 // Commit changed nodes (cur <- nxt) and append their not-yet-queued
 // predecessors to the next worklist (GPURunner::changed_nodes,
-// src/energygame.rs:1467-1479).  A predecessor is queued once per round: the
-// first wave to swap its stamp to the round's value appends it.  Appends are
-// staged per wave in LDS; full stages are flushed by the wave, the rest once
-// per block with a single atomic.
+// src/energygame.rs:1467-1479).  A wave takes 64 worklist entries: the changed
+// ones load their reverse-CSR ranges together, then the predecessor slots of
+// all of them are walked 64 at a time (one lane per slot).  A predecessor is
+// queued once per round: the first lane to swap its stamp to the round's
+// value appends it.  Appends are staged per wave in LDS; full stages are
+// flushed by the wave, the rest once per block with a single atomic.
 // r: the round; n: its worklist size (the seed count when seeding), 0 after an overflow
-__device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_stage)[256], uint32_t* s_cnt, uint32_t r,
-                                             uint32_t n) {
+__device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_stage)[256], uint32_t (*s_off)[64],
+                                             uint32_t (*s_rs)[64], uint32_t* s_cnt, uint32_t r, uint32_t n) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = threadIdx.x >> 6;
     uint32_t* stage = s_stage[wid];
+    uint32_t* off = s_off[wid];   // first predecessor slot of each lane's node
+    uint32_t* rs = s_rs[wid];     // first reverse edge of each lane's node
     const bool seeding = a.seed != nullptr;
     const uint32_t* list = seeding ? a.seed : a.wl[r & 1u];
     const uint32_t out_slot = seeding ? (r & 1u) : ((r + 1u) & 1u);
@@ -382,34 +391,44 @@ __device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_s
             f = seeding ? true : (a.chg[i] != 0);
         }
         if (f && !seeding) a.cur[u] = a.nxt[u];
-        uint64_t mask = __ballot(f);
-        while (mask) {
-            const uint32_t s = (uint32_t)__builtin_ctzll(mask);
-            mask &= mask - 1;
-            const uint32_t v = uniform(shfl32(u, s));
-            const uint32_t r0 = uniform(a.rrow[v]), r1 = uniform(a.rrow[v + 1]);
-            for (uint32_t k0 = r0; k0 < r1; k0 += 64) {
-                const uint32_t k = k0 + lane;
-                bool win = false;
-                uint32_t w = 0;
-                if (k < r1) {
-                    w = a.rcol[k];
-                    win = atomicExch(&a.stamp[w], stamp_val) != stamp_val;
+        uint32_t p0 = 0, pc = 0;
+        if (f) {
+            p0 = a.rrow[u];
+            pc = a.rrow[u + 1] - p0;
+        }
+        const uint32_t incl = wave_incl_scan(pc, lane);
+        const uint32_t total = uniform(shfl32(incl, 63));
+        if (total == 0) continue;
+        off[lane] = incl - pc;
+        rs[lane] = p0;
+        wave_sync();
+        for (uint32_t s0 = 0; s0 < total; s0 += 64) {
+            const uint32_t s = s0 + lane;
+            bool win = false;
+            uint32_t w = 0;
+            if (s < total) {
+                uint32_t lo = 0, hi = 63;   // off[0] = 0 <= s: the largest lane with off <= s owns slot s
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (off[mid] <= s) lo = mid; else hi = mid - 1;
                 }
-                const uint64_t bal = __ballot(win);
-                if (win) stage[cnt + mbcnt(bal)] = w;
-                cnt += (uint32_t)__popcll(bal);
-                if (cnt > 192) {
-                    wave_sync();
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&a.ctl->wl_n[out_slot], cnt);
-                    base = uniform(shfl32(base, 0));
-                    for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
-                    cnt = 0;
-                    wave_sync();
-                }
+                w = a.rcol[rs[lo] + (s - off[lo])];
+                win = atomicExch(&a.stamp[w], stamp_val) != stamp_val;
+            }
+            const uint64_t bal = __ballot(win);
+            if (win) stage[cnt + mbcnt(bal)] = w;
+            cnt += (uint32_t)__popcll(bal);
+            if (cnt > 192) {
+                wave_sync();
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&a.ctl->wl_n[out_slot], cnt);
+                base = uniform(shfl32(base, 0));
+                for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
+                cnt = 0;
+                wave_sync();
             }
         }
+        wave_sync();   // off / rs are rewritten for the next entries
     }
     // block-level flush: one atomic for the whole block
     if (lane == 0) s_cnt[wid] = cnt;
@@ -426,22 +445,24 @@ __device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_s
     __syncthreads();
 }
 
-__global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a) {
+// k: the kernel's round within the current batch of rounds (round = ctl->round + k).
+__global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a, uint32_t k) {
     __shared__ uint32_t s_stage[MAX_WAVES][256];
+    __shared__ uint32_t s_off[MAX_WAVES][64], s_rs[MAX_WAVES][64];
     __shared__ uint32_t s_cnt[MAX_WAVES + 1];
-    const uint32_t r = uniform(a.ctl->round);
+    const uint32_t r = uniform(a.ctl->round) + k;
     const uint32_t n = a.ctl->overflow ? 0u : (a.seed ? a.n_seed : uniform(a.ctl->wl_n[r & 1u]));
-    expand_phase(a, s_stage, s_cnt, r, n);
+    // a round counts once its evaluation completed without an arena overflow
+    if (!a.seed && n && blockIdx.x == 0 && threadIdx.x == 0) a.ctl->rounds_done++;
+    expand_phase(a, s_stage, s_off, s_rs, s_cnt, r, n);
 }
 
-// End of a round: the consumed worklist counter is cleared and the round
-// advances (not after an arena overflow: then the same round runs again).
-__global__ void k_advance(Ctl* ctl) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->overflow) return;
-    const uint32_t r = ctl->round;
-    if (ctl->wl_n[r & 1u]) ctl->rounds_done++;
-    ctl->wl_n[r & 1u] = 0;
-    ctl->round = r + 1;
+// End of a batch of `rounds` rounds: the round advances past them, or, after
+// an arena overflow, stops at the round that overflowed (it runs again).
+__global__ void k_advance(Ctl* ctl, uint32_t rounds) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t o = ctl->overflow;
+    ctl->round = o ? o - 1u : ctl->round + rounds;
 }
 
 // Grid-wide barrier of the cooperative k_solve launch (every block is resident).
@@ -491,6 +512,7 @@ __device__ __forceinline__ bool grid_barrier(Ctl* ctl, uint32_t nblocks, uint32_
 __global__ __launch_bounds__(MAX_BLOCK) void k_solve(EvalArgs ea, ExpandArgs xa, uint32_t max_rounds, uint32_t spin_limit) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     __shared__ uint32_t s_stage[MAX_WAVES][256];
+    __shared__ uint32_t s_off[MAX_WAVES][64], s_rs[MAX_WAVES][64];
     __shared__ uint32_t s_cnt[MAX_WAVES + 1];
     __shared__ uint32_t s_ctl[3];
     Ctl* ctl = ea.ctl;
@@ -514,7 +536,7 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_solve(EvalArgs ea, ExpandArgs xa,
         __syncthreads();
         const uint32_t ovf = s_ctl[2];
         __syncthreads();
-        expand_phase(xa, s_stage, s_cnt, r, ovf ? 0u : n);
+        expand_phase(xa, s_stage, s_off, s_rs, s_cnt, r, ovf ? 0u : n);
         if (blockIdx.x == 0 && threadIdx.x == 0 && !ovf) {
             // round r's counter is no longer read by anyone: clear it and advance
             if (n) __hip_atomic_fetch_add(&ctl->rounds_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

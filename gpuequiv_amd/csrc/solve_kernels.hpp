@@ -21,7 +21,7 @@ constexpr uint32_t EVAL_LDS_PER_WAVE = STAGE * 16 + STAGE * 4 + 2 * 64 * 4 + 3 *
 
 struct Ctl {                        // small device control block, read back between round batches
     uint32_t wl_n[2];               // worklist sizes (ping-pong by round parity)
-    uint32_t overflow;              // energy arena exhausted during k_eval
+    uint32_t overflow;              // energy arena exhausted during k_eval: that round + 1 (0: none)
     uint32_t arena_top;             // bump pointer of the energy arena (u16 units)
     uint32_t round;                 // current round; its worklist is wl[round & 1]
     uint32_t rounds_done;           // rounds that evaluated at least one node
@@ -65,9 +65,9 @@ struct ExpandArgs {
     Ctl* ctl;
 };
 
-__global__ void k_eval(EvalArgs a);
-__global__ void k_expand(ExpandArgs a);
-__global__ void k_advance(Ctl* ctl);
+__global__ void k_eval(EvalArgs a, uint32_t k);
+__global__ void k_expand(ExpandArgs a, uint32_t k);
+__global__ void k_advance(Ctl* ctl, uint32_t rounds);
 __global__ void k_solve(EvalArgs ea, ExpandArgs xa, uint32_t max_rounds, uint32_t spin_limit);
 __global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots, unsigned long long* out);
 __global__ void k_list_lens(const uint4* cur, uint32_t n, uint32_t* lens);
