@@ -75,6 +75,41 @@ __device__ __forceinline__ uint32_t inv_update(uint32_t e, uint32_t u, uint32_t 
     return e;
 }
 
+// The same map with the per-update work done once per edge: prep_update
+// derives the fields to increment (bit 2i of `dec` for a Decrement in field i)
+// and the Min fields (bit 4i of `mins`); inv_apply then needs a saturating
+// add and one max per Min field, in field order.
+struct InvUpd {
+    uint32_t dec;
+    uint32_t mins;
+    uint32_t u;
+};
+__device__ __forceinline__ InvUpd prep_update(uint32_t u) {
+    const uint32_t t = u ^ 0x111111u;                                  // nibble i == 1  <=>  t's nibble i == 0
+    const uint32_t nonzero = (((t & 0x777777u) + 0x777777u) | t) & 0x888888u;
+    const uint32_t one = ~nonzero & 0x888888u;                          // bit 4i+3: nibble i is Decrement
+    uint32_t dec = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) dec |= ((one >> (4 * i + 3)) & 1u) << (2 * i);
+    return InvUpd{dec, ((u >> 1) | (u >> 2) | (u >> 3)) & 0x111111u, u};   // bit 4i: nibble i >= 2 (Min)
+}
+// maxrep: maxv in every 2-bit field (maxv * 0x555); every field of e is <= maxv
+__device__ __forceinline__ uint32_t inv_apply(uint32_t e, const InvUpd& p, uint32_t maxrep) {
+    const uint32_t x = e ^ maxrep;
+    const uint32_t at_max = ~(x | (x >> 1)) & 0x555u;                   // bit 2i: field i == maxv
+    e += p.dec & ~at_max;                                               // no carries: those fields are < maxv
+    uint32_t mm = p.mins;
+    while (mm) {
+        const uint32_t b = (uint32_t)__builtin_ctz(mm);                 // 4i
+        mm &= mm - 1u;
+        const uint32_t i2 = b >> 1;
+        const uint32_t j2 = 2u * (((p.u >> b) & 0xfu) - 2u);             // Min(j + 1) is stored as j + 2
+        const uint32_t ei = (e >> i2) & 3u, ej = (e >> j2) & 3u;
+        if (ei > ej) e += (ei - ej) << j2;
+    }
+    return e;
+}
+
 // ---------------------------------------------------------------------------
 // Wave-level helpers
 // ---------------------------------------------------------------------------

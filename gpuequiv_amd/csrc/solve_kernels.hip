@@ -73,7 +73,7 @@ template <class Src>
 __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_t* A, uint32_t* B, uint32_t* O, uint32_t lane, uint32_t idx,
                                           uint32_t u, uint32_t deg, bool isatk, const uint4 own, const Src& src, StatAcc& acc,
                                           uint64_t& st_bytes_lane) {
-    const uint32_t maxv = a.maxv;
+    const uint32_t maxrep = a.maxv * 0x555u;
     const uint16_t* __restrict__ arena = a.arena;
     const uint32_t own_len = uniform(nl_len(own));
 
@@ -83,7 +83,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
     for (uint32_t j = lane; j < own_len; j += 64) {
         uint32_t e = nl_get(own, arena, j);
         bm_set(O, e);
-        if (isatk) bm_set(A, inv_update(e, 0u, maxv));
+        if (isatk) bm_set(A, e);   // own slice: the identity update
     }
 
     bool changed = false;
@@ -100,17 +100,18 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
                 len = nl_len(d);
                 st_bytes_lane += 8u + 4u * len;
             }
+            const InvUpd pu = prep_update(w);
             if (len <= INLINE_CAP) {
-                for (uint32_t j = 0; j < len; j++) bm_set(A, inv_update(nl_inline(d, j), w, maxv));
+                for (uint32_t j = 0; j < len; j++) bm_set(A, inv_apply(nl_inline(d, j), pu, maxrep));
             }
             uint64_t longm = __ballot(len > INLINE_CAP);
             while (longm) {
                 const uint32_t s = (uint32_t)__builtin_ctzll(longm);
                 longm &= longm - 1;
-                const uint32_t lw = uniform(shfl32(w, s));
+                const InvUpd lpu = prep_update(uniform(shfl32(w, s)));
                 const uint32_t loff = uniform(shfl32(d.y, s));
                 const uint32_t llen = uniform(shfl32(len, s));
-                for (uint32_t j = lane; j < llen; j += 64) bm_set(A, inv_update(arena[loff + j], lw, maxv));
+                for (uint32_t j = lane; j < llen; j += 64) bm_set(A, inv_apply(arena[loff + j], lpu, maxrep));
             }
         }
         wave_sync();
@@ -143,18 +144,18 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
             uint4 d;
             uint32_t w;
             src.get(0, d, w);
-            w = uniform(w);
+            const InvUpd pu = prep_update(uniform(w));
             const uint32_t len = uniform(nl_len(d));
             st_bytes_lane += (lane == 0) ? 8u + 4u * len : 0u;
             bool all_in = true;
-            for (uint32_t j = lane; j < len; j += 64) all_in &= bm_test(O, inv_update(nl_get(d, arena, j), w, maxv));
+            for (uint32_t j = lane; j < len; j += 64) all_in &= bm_test(O, inv_apply(nl_get(d, arena, j), pu, maxrep));
             const bool same = (len == own_len) && (__ballot(!all_in) == 0ull);
             changed = !same;
             if (changed) {
                 uint4* rec = &a.nxt[u];
                 if (len <= INLINE_CAP) {
                     uint16_t* h = reinterpret_cast<uint16_t*>(rec);
-                    if (lane < len) h[1 + lane] = (uint16_t)inv_update(nl_inline(d, lane), w, maxv);
+                    if (lane < len) h[1 + lane] = (uint16_t)inv_apply(nl_inline(d, lane), pu, maxrep);
                     if (lane == 0) h[0] = (uint16_t)(len | (len > 1 ? UNSORTED : 0u));
                     new_len = len;
                 } else {
@@ -166,7 +167,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
                         new_len = ~0u;
                     } else {
                         for (uint32_t j = lane; j < len; j += 64)
-                            a.arena[base + j] = (uint16_t)inv_update(arena[nl_off(d) + j], w, maxv);
+                            a.arena[base + j] = (uint16_t)inv_apply(arena[nl_off(d) + j], pu, maxrep);
                         if (lane == 0) *rec = make_uint4(len | UNSORTED, base, 0u, 0u);
                         new_len = len;
                     }
@@ -185,7 +186,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
                 }
                 const uint32_t nk = min(64u, deg - base);
                 for (uint32_t s = 0; s < nk; s++) {
-                    const uint32_t ws = uniform(shfl32(w, s));
+                    const InvUpd spu = prep_update(uniform(shfl32(w, s)));
                     uint4 ds;
                     ds.x = uniform(shfl32(d.x, s));
                     ds.y = uniform(shfl32(d.y, s));
@@ -194,7 +195,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
                     const uint32_t ls = nl_len(ds);
                     B[lane] = 0u; B[lane + 64] = 0u;
                     wave_sync();
-                    for (uint32_t j = lane; j < ls; j += 64) bm_set(B, inv_update(nl_get(ds, arena, j), ws, maxv));
+                    for (uint32_t j = lane; j < ls; j += 64) bm_set(B, inv_apply(nl_get(ds, arena, j), spu, maxrep));
                     wave_sync();
                     acc_bm &= up_close(bm_word(B, lane), lane);
                     wave_sync();

@@ -33,11 +33,20 @@ def random_update_words(rng: np.random.Generator, n: int, p_ident: float = 0.5, 
     return w
 
 
+def any_update_words(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Packed STANDARD updates with every field drawn independently from
+    Zero, Decrement and Min(1..6): several Decrements and Mins per update,
+    chained Mins and self-Mins included."""
+    nib = rng.integers(0, 8, (n, 6)).astype(np.uint32)
+    return (nib << (4 * np.arange(6, dtype=np.uint32))).sum(axis=1).astype(np.uint32)
+
+
 def random_game(n_vertices: int, seed: int, p_attacker: float = 0.5, mean_degree: float = 4.0,
-                p_ident: float = 0.5, p_dec: float = 0.35) -> GameGraph:
+                p_ident: float = 0.5, p_dec: float = 0.35, any_updates: bool = False) -> GameGraph:
     """Standalone game: out-degree Geometric on {0,1,...} with the given mean,
     targets uniform, self-loops and multi-edges allowed (GameGraph::new allows
-    both, src/energygame.rs:141-194)."""
+    both, src/energygame.rs:141-194).  any_updates: every update field random
+    (any_update_words) instead of the identity / one Decrement / one Min mix."""
     rng = np.random.default_rng(seed)
     V = int(n_vertices)
     atk = (rng.random(V) < p_attacker).astype(np.uint8)
@@ -47,7 +56,7 @@ def random_game(n_vertices: int, seed: int, p_attacker: float = 0.5, mean_degree
     np.cumsum(deg, out=row[1:])
     E = int(row[-1])
     col = rng.integers(0, V, E, dtype=np.uint32)
-    w = random_update_words(rng, E, p_ident, p_dec)
+    w = any_update_words(rng, E) if any_updates else random_update_words(rng, E, p_ident, p_dec)
     return GameGraph(EnergyConf.STANDARD, row.astype(np.uint32), col, w.reshape(E, 1), atk)
 
 

@@ -131,6 +131,19 @@ def test_random_games_varied_shapes(seed):
     assert _canon(out) == H.canon_lists(offs, data)
 
 
+@pytest.mark.parametrize("V,seed", [(300, 11), (3000, 12), (20000, 13)])
+def test_arbitrary_updates_match_oracle(V, seed):
+    # every field of every update random: several Decrements, chained and
+    # self-referencing Mins in one update (the inverse update's field order matters)
+    g = workloads.random_game(V, seed, any_updates=True)
+    print("any-update game", V, seed, flush=True)
+    reach = H.standard_reach(g)
+    offs, data, st = H.oracle_solve_game(g, reach)
+    game, out = _run(g)
+    assert _canon(out) == H.canon_lists(offs, data)
+    assert game.last_stats["rounds"] == st["rounds"] and game.last_stats["edge_updates"] == st["edge_updates"]
+
+
 def test_initial_energies_and_empty_reach():
     d = next(g for g in H.load("energygame_games.json")["games"] if g["name"] == "combinations")
     graph = H.fixture_game(d)
