@@ -141,13 +141,29 @@ __device__ __forceinline__ uint32_t shfl32(uint32_t v, uint32_t src) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
 }
 
-// inclusive wave prefix sum
+// Value of lane (lane - n) within the lane's row of 16 (DPP row_shr:n, no
+// LDS traffic); 0 for the first n lanes of a row.  n in {1, 2, 4, 8}.
+template <int N>
+__device__ __forceinline__ uint32_t row_shr(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 | N, 0xF, 0xF, true);
+}
+template <int N>
+__device__ __forceinline__ uint64_t row_shr64(uint64_t v) {
+    return ((uint64_t)row_shr<N>((uint32_t)(v >> 32)) << 32) | row_shr<N>((uint32_t)v);
+}
+
+// inclusive wave prefix sum: within rows of 16 by DPP, then across rows
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-        uint32_t t = shfl32(v, (lane - d) & 63u);
-        if (lane >= d) v += t;
-    }
+    v += row_shr<1>(v);
+    v += row_shr<2>(v);
+    v += row_shr<4>(v);
+    v += row_shr<8>(v);
+    // row sums: lane 15 of each row holds it
+    const uint32_t r0 = shfl32(v, 15), r1 = shfl32(v, 31), r2 = shfl32(v, 47);
+    const uint32_t row = lane >> 4;
+    if (row >= 1) v += r0;
+    if (row >= 2) v += r1;
+    if (row >= 3) v += r2;
     return v;
 }
 
@@ -173,12 +189,13 @@ __device__ __forceinline__ uint64_t up_close(uint64_t w, uint32_t lane) {
     w |= (w & D1_LT2) << 8;
     w |= (w & D2_LT3) << 16;
     w |= (w & D2_LT2) << 32;
+    // dims 3 and 4 live within a row of 16 lanes (DPP shifts), dim 5 across rows
     const uint32_t e3 = lane & 3u, e4 = (lane >> 2) & 3u, e5 = lane >> 4;
     uint64_t t;
-    t = shfl_up64(w, 1, lane);  if (e3 >= 1) w |= t;
-    t = shfl_up64(w, 2, lane);  if (e3 >= 2) w |= t;
-    t = shfl_up64(w, 4, lane);  if (e4 >= 1) w |= t;
-    t = shfl_up64(w, 8, lane);  if (e4 >= 2) w |= t;
+    t = row_shr64<1>(w);        if (e3 >= 1) w |= t;
+    t = row_shr64<2>(w);        if (e3 >= 2) w |= t;
+    t = row_shr64<4>(w);        if (e4 >= 1) w |= t;
+    t = row_shr64<8>(w);        if (e4 >= 2) w |= t;
     t = shfl_up64(w, 16, lane); if (e5 >= 1) w |= t;
     t = shfl_up64(w, 32, lane); if (e5 >= 2) w |= t;
     return w;
@@ -189,8 +206,8 @@ __device__ __forceinline__ uint64_t minimal_of(uint64_t w, uint32_t lane) {
     uint64_t nm = ((w & D0_LT3) << 1) | ((w & D1_LT3) << 4) | ((w & D2_LT3) << 16);
     const uint32_t e3 = lane & 3u, e4 = (lane >> 2) & 3u, e5 = lane >> 4;
     uint64_t t;
-    t = shfl_up64(w, 1, lane);  if (e3 >= 1) nm |= t;
-    t = shfl_up64(w, 4, lane);  if (e4 >= 1) nm |= t;
+    t = row_shr64<1>(w);        if (e3 >= 1) nm |= t;
+    t = row_shr64<4>(w);        if (e4 >= 1) nm |= t;
     t = shfl_up64(w, 16, lane); if (e5 >= 1) nm |= t;
     return w & ~nm;
 }
