@@ -41,6 +41,20 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def measured_traffic(profile: str) -> float | None:
+    """HBM bytes per launch of a kernel, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary committed under profiles/ (null when absent)."""
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(graph, reach, edge_updates: int, budget_s: float = 12.0) -> dict:
     from oracle import oracle as orc
     times = []
@@ -553,8 +567,9 @@ def main() -> int:
                    "positions": graph.n_vertices(), "edges": graph.n_edges(), "rounds": int(s["rounds"]),
                    "edge_updates_per_solve": edge_updates, "node_evals_per_solve": int(s["node_evals"])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_eval",
-                     "launches": launches, "avg_launch_us": eval_ms * 1e3 / max(1, launches),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("plain_100k_k_eval_pmc.md"),
+                     "traffic_source": "profiles/plain_100k_k_eval_pmc.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
+                     "kernel": "k_eval", "launches": launches, "avg_launch_us": eval_ms * 1e3 / max(1, launches),
                      "alg_bytes_per_launch": alg / max(1, launches)},
         "per_kernel_profile": phase,
         "device_solve_ms": s["solve_ms"],

@@ -1,5 +1,7 @@
 // C ABI of the solver (include/gpuequiv.h): context, game upload, the
 // device-resident fixpoint loop and result download.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -466,20 +468,17 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     geq_status result = GEQ_OK;
     while (g->h_ctl->wl_n[g->h_ctl->round & 1u] != 0) {
         if (profile_kernels) {
-            // one round per launch, HIP events around every kernel
-            hipEvent_t e0, e1, e2;
-            GEQ_HIP(hipEventCreate(&e0), GEQ_ERR_NO_DEVICE);
-            GEQ_HIP(hipEventCreate(&e1), GEQ_ERR_NO_DEVICE);
-            GEQ_HIP(hipEventCreate(&e2), GEQ_ERR_NO_DEVICE);
-            GEQ_HIP(hipEventRecord(e0, s), GEQ_ERR_NO_DEVICE);
-            k_eval<<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea, 0u);
-            GEQ_HIP(hipEventRecord(e1, s), GEQ_ERR_NO_DEVICE);
-            k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa, 0u);
-            GEQ_HIP(hipEventRecord(e2, s), GEQ_ERR_NO_DEVICE);
+            // one round per launch; each kernel's dispatch carries its own
+            // start / end events (hipExtLaunchKernelGGL), so the times are the
+            // kernels' own, without the gaps between dispatches
+            hipEvent_t ev[4];
+            for (hipEvent_t& e : ev) GEQ_HIP(hipEventCreate(&e), GEQ_ERR_NO_DEVICE);
+            hipExtLaunchKernelGGL(k_eval, dim3(shape.eval_blocks), dim3(shape.threads), (shape.threads / 64) * EVAL_LDS_PER_WAVE,
+                                  s, ev[0], ev[1], 0, ea, 0u);
+            hipExtLaunchKernelGGL(k_expand, dim3(shape.expand_blocks), dim3(shape.threads), 0, s, ev[2], ev[3], 0, xa, 0u);
+            GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
             k_advance<<<1, 64, 0, s>>>(g->ctl.p, 1u);
-            prof.push_back(e0);
-            prof.push_back(e1);
-            prof.push_back(e2);
+            for (hipEvent_t e : ev) prof.push_back(e);
             launches++;
         } else if (g->tune.mode == 0 && !std::getenv("GEQ_ENABLE_COOPERATIVE")) {
             // The cooperative one-launch solve is switched off: its grid of
@@ -540,10 +539,10 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev_start, ev_end);
     double eval_ms = 0, expand_ms = 0;
-    for (size_t i = 0; i + 2 < prof.size(); i += 3) {
+    for (size_t i = 0; i + 3 < prof.size(); i += 4) {
         float a = 0, b = 0;
         (void)hipEventElapsedTime(&a, prof[i], prof[i + 1]);
-        (void)hipEventElapsedTime(&b, prof[i + 1], prof[i + 2]);
+        (void)hipEventElapsedTime(&b, prof[i + 2], prof[i + 3]);
         eval_ms += a;
         expand_ms += b;
     }
