@@ -7,9 +7,18 @@ identity 0.5 / one Decrement 0.35 / one Min 0.15), seed 2, solved by the plain
 one-game path with the graph already resident in HBM.  A step is one full
 solve (energies reset on the device, fixpoint to convergence).
 
-Prints one JSON line: edge updates/s as `value`, the k_eval roofline measured
-with HIP events on the solver's stream, and the oracle's single-core time on
-the same game as `cpu_baseline`.
+The launch shape is picked on the game by the autotuner before the timed
+steps (--no-tune keeps the defaults).  Prints one JSON line: edge updates/s as
+`value`, the k_eval roofline (kernel time from start/stop events carried by
+its dispatches on the solver's stream; traffic from the rocprofv3 summary under
+profiles/) and the oracle's single-core time on the same game as
+`cpu_baseline`.
+
+Other workloads (--workload): batch (2,048 games in one device-resident solve
+against one call per game), serve (Poisson arrivals into the continuous-batching
+server against the plain path), spectroscopy (the full 13-equivalence pipeline
+of a 300-state LTS, GPU against the oracle), gamebuild (a 50,000-state
+power-law LTS with 4,096 pairs).
 """
 from __future__ import annotations
 
@@ -363,7 +372,7 @@ def bench_serve(args) -> dict:
           flush=True)
 
     # one long-lived server for every rate (creating and freeing a pool of
-    # this size per rate would time the driver's memory recycling instead)
+    # this size per rate would time the release and re-clearing of device memory)
     srv = Server(budget_bytes=budget, rounds_per_step=args.rounds_per_step, max_positions=max_pos)
     srv.run_until_done([srv.submit(lts, pairs) for lts, pairs in jobs[:16]])     # warm-up
     rates = [float(r) for r in args.rates.split(",")]
@@ -460,7 +469,7 @@ def main() -> int:
     ap.add_argument("--games", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
-    ap.add_argument("--no-tune", action="store_true", help="batch: keep the default launch shape")
+    ap.add_argument("--no-tune", action="store_true", help="plain / batch: keep the default launch shape")
     ap.add_argument("--no-plain", action="store_true", help="batch: skip the one-game-per-call comparison")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied once by a plain device copy first (a known-size kernel to scale PMC byte counters)")
@@ -521,6 +530,15 @@ def main() -> int:
 
     for _ in range(args.warmup):
         solve()
+    tune = None
+    if not args.no_tune:
+        # launch shape picked on this game by the autotuner, outside the timed region
+        t0 = time.perf_counter()
+        rep = _lib.geq_tune_report()
+        _lib.check(lib.geq_game_autotune(handle, _lib.u32p(reach), len(reach), 5, C.byref(rep)))
+        tune = rep.as_dict()
+        tune["tuning_s"] = time.perf_counter() - t0
+        print(f"bench: autotune {tune}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s = solve()
@@ -573,6 +591,7 @@ def main() -> int:
                      "alg_bytes_per_launch": alg / max(1, launches)},
         "per_kernel_profile": phase,
         "device_solve_ms": s["solve_ms"],
+        "autotune": tune,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(graph, reach, edge_updates)
