@@ -211,10 +211,7 @@ def bench_spectroscopy(args) -> dict:
         eq = Equivalence(info, b.energies(0, b.n_starting_points))
         eq.set_minimization(part)
         t4 = time.perf_counter()
-        verdicts = 0
-        for (p, q) in eq.pos_to_idx:
-            for e in eqs:
-                verdicts += eq.preorder(p, q, e)
+        eq.preorder_matrix(eqs)        # all 13 preorders of all ordered state pairs
         t5 = time.perf_counter()
         sizes = (b.n_positions(), int(b.build_stats["edges"]), len(info.starting_points))
         b.close()
@@ -235,7 +232,7 @@ def bench_spectroscopy(args) -> dict:
                                f"{len(src)} transitions (uniform sources/targets), 8 labels, seed {args.seed}",
                    "positions": V, "edges": E, "start_pairs": n_pairs, "rounds": int(st["rounds"]),
                    "edge_updates": int(st["edge_updates"])},
-        "phases_ms": {"minimise_host": phases[0] * 1e3, "gamebuild_gpu": phases[1] * 1e3,
+        "phases_ms": {"minimise_gpu": phases[0] * 1e3, "gamebuild_gpu": phases[1] * 1e3,
                       "solve_gpu": phases[2] * 1e3, "download": phases[3] * 1e3, "verdicts_host": phases[4] * 1e3},
         "solve_edge_updates_per_s": int(st["edge_updates"]) / phases[2],
     }
@@ -373,7 +370,8 @@ def bench_serve(args) -> dict:
 
     # one long-lived server for every rate (creating and freeing a pool of
     # this size per rate would time the release and re-clearing of device memory)
-    srv = Server(budget_bytes=budget, rounds_per_step=args.rounds_per_step, max_positions=max_pos)
+    srv = Server(budget_bytes=budget, rounds_per_step=args.rounds_per_step, max_positions=max_pos,
+                 builders=args.builders)
     srv.run_until_done([srv.submit(lts, pairs) for lts, pairs in jobs[:16]])     # warm-up
     rates = [float(r) for r in args.rates.split(",")]
     sweep = []
@@ -465,6 +463,7 @@ def main() -> int:
                     help="serve: per-job game size limit (2M); gamebuild: build cap (200M)")
     ap.add_argument("--rounds-per-step", type=int, default=4, help="serve: rounds per server step")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM admission budget (GB)")
+    ap.add_argument("--builders", type=int, default=2, help="serve: game-building threads")
     ap.add_argument("--positions", type=int, default=100000)
     ap.add_argument("--games", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=None)

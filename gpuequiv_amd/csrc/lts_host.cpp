@@ -16,6 +16,9 @@
 
 namespace geq {
 geq_status fail(geq_status s, const std::string& msg);
+// quotient of an LTS by a partition with classes numbered by first occurrence
+void quotient_lts(const geq_lts* lts, const uint32_t* partition, uint32_t c, uint32_t* out_row, uint32_t* out_targets,
+                  int32_t* out_labels, uint32_t* out_n_transitions);
 }
 
 namespace {
@@ -100,13 +103,13 @@ geq_status geq_signature_refinement(const geq_lts* lts, uint32_t* partition, uin
     return GEQ_OK;
 }
 
+}  // extern "C"
+
 // The minimised LTS: class c takes the transitions of its first process, with
 // targets mapped to classes, then each row is stably sorted by label.
-geq_status geq_bisimilar_minimize(const geq_lts* lts, uint32_t* partition, uint32_t* n_classes, uint32_t* out_row,
-                                  uint32_t* out_targets, int32_t* out_labels, uint32_t* out_n_transitions) {
-    geq_status st = geq_signature_refinement(lts, partition, n_classes);
-    if (st != GEQ_OK) return st;
-    const uint32_t n = lts->n_states, c = *n_classes;
+void geq::quotient_lts(const geq_lts* lts, const uint32_t* partition, uint32_t c, uint32_t* out_row, uint32_t* out_targets,
+                       int32_t* out_labels, uint32_t* out_n_transitions) {
+    const uint32_t n = lts->n_states;
     std::vector<int64_t> first(c, -1);
     for (uint32_t s = 0; s < n; s++)
         if (first[partition[s]] < 0) first[partition[s]] = s;
@@ -127,7 +130,12 @@ geq_status geq_bisimilar_minimize(const geq_lts* lts, uint32_t* partition, uint3
     }
     out_row[c] = k;
     *out_n_transitions = k;
-    return GEQ_OK;
 }
 
-}  // extern "C"
+extern "C" geq_status geq_bisimilar_minimize(const geq_lts* lts, uint32_t* partition, uint32_t* n_classes, uint32_t* out_row,
+                                             uint32_t* out_targets, int32_t* out_labels, uint32_t* out_n_transitions) {
+    geq_status st = geq_signature_refinement(lts, partition, n_classes);
+    if (st != GEQ_OK) return st;
+    geq::quotient_lts(lts, partition, *n_classes, out_row, out_targets, out_labels, out_n_transitions);
+    return GEQ_OK;
+}

@@ -55,6 +55,36 @@ class Equivalence:
     def equiv(self, p: int, q: int, equivalence: Energy) -> bool:
         return self.preorder(p, q, equivalence) and self.preorder(q, p, equivalence)
 
+    def preorder_matrix(self, equivalences: Sequence[Energy]):
+        """preorder(p, q, e) for every ordered pair of processes and every e
+        in `equivalences`, as a bool array [n, n, len(equivalences)]:
+        vectorised over the budgets of all seeded pairs."""
+        import numpy as np
+        n, k = self.start_info.n_processes, len(equivalences)
+        conf = equivalences[0].conf if k else None
+        eqs = np.array([e.to_vec() for e in equivalences], dtype=np.int64).reshape(k, -1)
+        sp = self.start_info.starting_points
+        m = (max(self.minimization) + 1) if self.minimization is not None else n
+        red = np.zeros((max(m, 1), max(m, 1), k), dtype=bool)
+        rows, owner = [], []
+        for i, (pt, e) in enumerate(zip(sp, self.energies)):
+            if e.conf != conf:
+                raise ValueError("Incompatible Energy configurations")
+            for v in e:
+                rows.append(v.to_vec())
+                owner.append(i)
+        below_any = np.zeros((len(sp), k), dtype=bool)
+        if rows:
+            vals = np.array(rows, dtype=np.int64)                       # [r, elements]
+            below = np.all(vals[:, None, :] <= eqs[None, :, :], axis=2)  # [r, k]
+            np.logical_or.at(below_any, np.array(owner), below)
+        for i, pt in enumerate(sp):
+            red[pt.p, pt.q[0]] = ~below_any[i]
+        idx = np.arange(m)
+        red[idx, idx] = True
+        part = np.asarray(self.minimization if self.minimization is not None else range(n), dtype=np.int64)
+        return red[part[:, None], part[None, :]]
+
     def relation(self, equivalence: Energy) -> "EquivalenceRelation":
         """Classes of processes equivalent under `equivalence` (:154-170)."""
         uf = EquivalenceRelation(self.start_info.n_processes)

@@ -125,17 +125,24 @@ class TransitionSystem:
         return s
 
     # -------------------------------------------------------------- bisimulation
-    def signature_refinement(self):
-        """(partition, class count) of strong bisimilarity (src/lib.rs:444-478)."""
+    def signature_refinement(self, on_device: bool = True):
+        """(partition, class count) of strong bisimilarity (src/lib.rs:444-478),
+        refined on the GPU (geq_signature_refinement_gpu) or, with
+        on_device=False, by the library's host routine."""
         n = self.n_vertices()
         part = np.zeros(max(n, 1), np.uint32)
         cnt = C.c_uint32()
         s = self._c_struct()
-        _lib.check(_lib.lib().geq_signature_refinement(C.byref(s), _lib.u32p(part), C.byref(cnt)))
+        if on_device:
+            _lib.check(_lib.lib().geq_signature_refinement_gpu(_lib.Context.get().handle, C.byref(s), _lib.u32p(part),
+                                                               C.byref(cnt), None))
+        else:
+            _lib.check(_lib.lib().geq_signature_refinement(C.byref(s), _lib.u32p(part), C.byref(cnt)))
         return part[:n].tolist(), int(cnt.value)
 
-    def bisimilar_minimize(self):
-        """(quotient LTS, partition) (src/lib.rs:413-432)."""
+    def bisimilar_minimize(self, on_device: bool = True):
+        """(quotient LTS, partition) (src/lib.rs:413-432); the partition is
+        refined on the GPU unless on_device=False."""
         n, m = self.n_vertices(), self.n_transitions()
         part = np.zeros(max(n, 1), np.uint32)
         cnt = C.c_uint32()
@@ -144,8 +151,13 @@ class TransitionSystem:
         olbl = np.zeros(max(m, 1), np.int32)
         om = C.c_uint32()
         s = self._c_struct()
-        _lib.check(_lib.lib().geq_bisimilar_minimize(C.byref(s), _lib.u32p(part), C.byref(cnt), _lib.u32p(orow),
-                                                     _lib.u32p(odst), _lib.i32p(olbl), C.byref(om)))
+        if on_device:
+            _lib.check(_lib.lib().geq_bisimilar_minimize_gpu(_lib.Context.get().handle, C.byref(s), _lib.u32p(part),
+                                                             C.byref(cnt), _lib.u32p(orow), _lib.u32p(odst),
+                                                             _lib.i32p(olbl), C.byref(om)))
+        else:
+            _lib.check(_lib.lib().geq_bisimilar_minimize(C.byref(s), _lib.u32p(part), C.byref(cnt), _lib.u32p(orow),
+                                                         _lib.u32p(odst), _lib.i32p(olbl), C.byref(om)))
         c, k = int(cnt.value), int(om.value)
         return TransitionSystem(orow[:c + 1], odst[:k], olbl[:k]), part[:n].tolist()
 

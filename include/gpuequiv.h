@@ -282,6 +282,19 @@ typedef struct {
 } geq_server_stats;
 geq_status geq_server_get_stats(geq_server* server, geq_server_stats* out);
 
+/* ---- strong-bisimulation minimisation on the device ----
+ * The same results as geq_signature_refinement / geq_bisimilar_minimize below
+ * (TransitionSystem::{signature_refinement, bisimilar_minimize},
+ * src/lib.rs:413-478), computed on the device: per refinement round a
+ * segmented sort of (label, class of target) keys, a hash table of the
+ * signatures and a scan numbering the classes by first occurrence.  rounds
+ * (may be NULL) receives the number of refinement rounds. */
+geq_status geq_signature_refinement_gpu(geq_ctx* ctx, const geq_lts* lts, uint32_t* partition, uint32_t* n_classes,
+                                        uint32_t* rounds);
+geq_status geq_bisimilar_minimize_gpu(geq_ctx* ctx, const geq_lts* lts, uint32_t* partition, uint32_t* n_classes,
+                                      uint32_t* out_row, uint32_t* out_targets, int32_t* out_labels,
+                                      uint32_t* out_n_transitions);
+
 /* ---- helpers (no device needed) ---- */
 /* TransitionSystem::signature_refinement (src/lib.rs:444-478): partition[s] is
  * the strong-bisimulation class of s, numbered by first occurrence. */

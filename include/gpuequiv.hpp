@@ -667,22 +667,27 @@ public:
                        lbl_.empty() ? nullptr : lbl_.data()};
     }
 
-    // (partition, class count) of strong bisimilarity (:444-478)
-    std::pair<std::vector<size_t>, size_t> signature_refinement() const {
+    // (partition, class count) of strong bisimilarity (:444-478), refined on the device
+    // (on_device = false: the library's host routine, same result)
+    std::pair<std::vector<size_t>, size_t> signature_refinement(bool on_device = true) const {
         std::vector<uint32_t> part(std::max<uint32_t>(n_vertices(), 1));
         uint32_t cnt = 0;
         const geq_lts l = c_lts();
-        detail::check(geq_signature_refinement(&l, part.data(), &cnt));
+        if (on_device) detail::check(geq_signature_refinement_gpu(detail::context(), &l, part.data(), &cnt, nullptr));
+        else detail::check(geq_signature_refinement(&l, part.data(), &cnt));
         return {std::vector<size_t>(part.begin(), part.begin() + n_vertices()), cnt};
     }
     // (quotient LTS, partition) (:413-432)
-    std::pair<TransitionSystem, std::vector<size_t>> bisimilar_minimize() const {
+    std::pair<TransitionSystem, std::vector<size_t>> bisimilar_minimize(bool on_device = true) const {
         const uint32_t n = n_vertices(), m = (uint32_t)dst_.size();
         std::vector<uint32_t> part(std::max(n, 1u)), orow(n + 1), odst(std::max(m, 1u));
         std::vector<int32_t> olbl(std::max(m, 1u));
         uint32_t cnt = 0, om = 0;
         const geq_lts l = c_lts();
-        detail::check(geq_bisimilar_minimize(&l, part.data(), &cnt, orow.data(), odst.data(), olbl.data(), &om));
+        if (on_device)
+            detail::check(geq_bisimilar_minimize_gpu(detail::context(), &l, part.data(), &cnt, orow.data(), odst.data(), olbl.data(), &om));
+        else
+            detail::check(geq_bisimilar_minimize(&l, part.data(), &cnt, orow.data(), odst.data(), olbl.data(), &om));
         TransitionSystem t;
         t.row_.assign(orow.begin(), orow.begin() + cnt + 1);
         t.dst_.assign(odst.begin(), odst.begin() + om);

@@ -163,7 +163,7 @@ int main(int argc, char** argv) {
                     }
                     expect(got == want, name + " transitions");
                     // strong bisimulation classes are numbered by first occurrence
-                    const auto [part, cnt] = lts.signature_refinement();
+                    const auto [part, cnt] = lts.signature_refinement(!cpu_only);
                     expect(part.size() == n && (n == 0 || part[0] == 0), name + " signature_refinement");
                     for (size_t v : part) expect(v < cnt, name + " partition id range");
                 }

@@ -161,4 +161,5 @@ def test_full_spectroscopy_verdict_matrix():
             if part[p] == part[q] or (part[p], part[q]) in seeded:
                 got[p, q] = [eq.preorder(p, q, e) for e in eqs]
     assert np.array_equal(got, want)
+    assert np.array_equal(eq.preorder_matrix(eqs), want)
     assert want.sum() > n * 13          # verdicts beyond the diagonal

@@ -45,10 +45,10 @@ def _lts_random(n, m, labels, seed):
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_bisimulation_matches_oracle(seed):
     lts = _lts_random(200, 500, 3, seed)
-    part, cnt = lts.signature_refinement()
+    part, cnt = lts.signature_refinement(on_device=False)
     opart, ocnt = orc.signature_refinement(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels)
     assert cnt == ocnt and part == [int(x) for x in opart]
-    red, part2 = lts.bisimilar_minimize()
+    red, part2 = lts.bisimilar_minimize(on_device=False)
     op, oc, orow, odst, olbl = orc.bisimilar_minimize(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels)
     assert part2 == [int(x) for x in op]
     assert np.array_equal(red.row_offsets, orow) and np.array_equal(red.targets, odst) and np.array_equal(red.labels, olbl)
@@ -57,12 +57,31 @@ def test_bisimulation_matches_oracle(seed):
 def test_bisimilar_golden():
     d = H.load("complete_lts.json")["bisimilar"]
     lts = TransitionSystem.from_edges(d["transitions"])
-    part, _ = lts.signature_refinement()
+    part, _ = lts.signature_refinement(on_device=False)
     assert part[0] == part[5]
 
 
 def test_bisimulation_on_vlts():
     lts = TransitionSystem.from_csv_file(os.path.join(GOLDEN, "vasy_0_1.csv"))
-    part, cnt = lts.signature_refinement()
+    part, cnt = lts.signature_refinement(on_device=False)
     opart, ocnt = orc.signature_refinement(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels)
     assert cnt == ocnt and part == [int(x) for x in opart]
+
+
+def test_preorder_matrix_matches_preorder_calls():
+    # host-side only: an Equivalence over hand-made budgets
+    from gpuequiv_amd import EnergyArray, EnergyConf, std_equivalences
+    from gpuequiv_amd.equivalence import Equivalence, StartInfo
+    from gpuequiv_amd.gamebuild import AttackPosition
+    STD = EnergyConf.STANDARD
+    pairs = [(0, 1), (1, 0), (0, 2), (2, 0), (1, 2), (2, 1)]
+    budgets = [[[1, 1]], [[3, 1]], [], [[2, 2, 0, 0, 1, 1], [0, 0, 0, 0, 3]], [[3, 3, 3, 3, 3, 3]], [[1]]]
+    info = StartInfo([AttackPosition(p, (q,)) for p, q in pairs], 4)
+    eq = Equivalence(info, [EnergyArray.from_conf(b, STD) for b in budgets])
+    eq.set_minimization([0, 1, 2, 2])
+    eqs = [getattr(std_equivalences, nm)() for nm in std_equivalences.NAMES]
+    mat = eq.preorder_matrix(eqs)
+    for p in range(4):
+        for q in range(4):
+            for k, e in enumerate(eqs):
+                assert mat[p, q, k] == eq.preorder(p, q, e), (p, q, k)
