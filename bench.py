@@ -276,7 +276,7 @@ def bench_gamebuild(args) -> dict:
             seen.add((p, q))
             pairs.append((p, q))
     for _ in range(args.warmup):
-        GameBuild(lts, pairs[:64], max_positions=args.max_positions).close()
+        GameBuild(lts, pairs, max_positions=args.max_positions).close()
     builds, solves = [], []
     for _ in range(args.steps):
         b = GameBuild(lts, pairs, max_positions=args.max_positions)
@@ -493,7 +493,7 @@ def main() -> int:
             print(json.dumps(bench_spectroscopy(args)), flush=True)
         else:
             args.states = args.states or 50000
-            args.max_positions = args.max_positions or 200000000
+            args.max_positions = args.max_positions or 20000000
             print(json.dumps(bench_gamebuild(args)), flush=True)
         return 0
     if args.workload == "serve":

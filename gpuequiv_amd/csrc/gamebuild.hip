@@ -169,6 +169,7 @@ __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, u
 __global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint32_t* cand_off,
                            const uint32_t* spool_off, uint64_t spool_base, Cand* cands, uint32_t* nsucc) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == n_front) nsucc[i] = 0;   // trailing zero: the exclusive scan yields the edge total
     if (i >= n_front) return;
     const uint32_t u = lo + i;
     const uint32_t kind = N.kind[u], p = N.p[u], qo = N.q_off[u], ql = N.q_len[u], xo = N.x_off[u], xl = N.x_len[u];

@@ -70,6 +70,12 @@ geq_status exclusive_scan(const T* in, T* out, size_t n, DevBuf<uint8_t>& tmp, h
     return GEQ_OK;
 }
 
+// several device words with one stream synchronisation
+void read_u32s(std::initializer_list<std::pair<const uint32_t*, uint32_t*>> words, hipStream_t s) {
+    for (const auto& w : words) (void)hipMemcpyAsync(w.second, w.first, 4, hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+}
+
 uint32_t read_u32(const uint32_t* d, hipStream_t s) {
     uint32_t v = 0;
     (void)hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, s);
@@ -180,6 +186,20 @@ struct JobTrack {
     std::vector<uint8_t> dead;
     uint64_t cap = 0;
 };
+
+// New positions per job among a level's candidates: the winners (flag set)
+// counted by the job of their process (LDS histogram per block).
+__global__ void k_job_count_cands(const Cand* cands, const uint32_t* flag, uint32_t n_cand, const uint32_t* job_of_state,
+                                  uint32_t n_jobs, uint32_t* cnt) {
+    __shared__ uint32_t h[1024];
+    for (uint32_t j = threadIdx.x; j < n_jobs; j += blockDim.x) h[j] = 0;
+    __syncthreads();
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cand; c += gridDim.x * blockDim.x)
+        if (flag[c]) atomicAdd(&h[job_of_state[cands[c].kind_p & 0x3fffffffu]], 1u);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < n_jobs; j += blockDim.x)
+        if (h[j]) atomicAdd(&cnt[j], h[j]);
+}
 
 // Positions per job among the ids [lo, hi) (LDS histogram per block).
 __global__ void k_job_count(const uint32_t* p, uint32_t lo, uint32_t hi, const uint32_t* job_of_state, uint32_t n_jobs,
@@ -338,8 +358,8 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub_c.p, ub_s.p);
         GB_TRY(exclusive_scan<uint32_t>(ub_c.p, c_off.p, nf + 1, tmp, s));
         GB_TRY(exclusive_scan<uint32_t>(ub_s.p, s_off.p, nf + 1, tmp, s));
-        const uint32_t n_cand = read_u32(c_off.p + nf, s);
-        const uint32_t n_words = read_u32(s_off.p + nf, s);
+        uint32_t n_cand = 0, n_words = 0;
+        read_u32s({{c_off.p + nf, &n_cand}, {s_off.p + nf, &n_words}}, s);
         if ((uint64_t)B->spool_top + n_words >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: set pool exceeds 2^32 words"));
         GB_TRY(ensure(B->spool, B->spool_top + n_words + 1, s, true, "set pool"));
         GB_TRY(ensure(cands, std::max<uint32_t>(n_cand, 1), s, false, "candidates"));
@@ -350,10 +370,8 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (E + n_cand >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 edges"));
         GB_TRY(ensure(B->col, E + n_cand + 1, s, true, "edges"));
         GB_TRY(ensure(B->wt, E + n_cand + 1, s, true, "edges"));
-        k_generate<<<(nf + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, c_off.p, s_off.p, B->spool_top, cands.p, nsucc.p);
-        GB_HIP(hipMemsetAsync(nsucc.p + nf, 0, 4, s), GEQ_ERR_TRANSFER);
+        k_generate<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, c_off.p, s_off.p, B->spool_top, cands.p, nsucc.p);
         GB_TRY(exclusive_scan<uint32_t>(nsucc.p, rowl.p, nf + 1, tmp, s));
-        const uint32_t n_edges = read_u32(rowl.p + nf, s);
         if (n_cand) {
             k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
             k_mark<<<(n_cand + 1 + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, keymin.p, flag.p);
@@ -361,12 +379,20 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             GB_HIP(hipMemsetAsync(flag.p, 0, 4, s), GEQ_ERR_TRANSFER);
         }
         GB_TRY(exclusive_scan<uint32_t>(flag.p, rank.p, n_cand + 1, tmp, s));
-        const uint32_t n_new = read_u32(rank.p + n_cand, s);
         if (n_cand) {
             k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, flag.p, rank.p, B->V, nodes(), table.p, keymin.p);
             k_edges<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p, B->wt.p);
+            if (jt) {
+                // this level's new positions per job (the winners among the candidates)
+                const uint32_t nb = std::min<uint32_t>(1024, (n_cand + tb - 1) / tb);
+                k_job_count_cands<<<nb, tb, 0, s>>>(cands.p, flag.p, n_cand, d_job_of_state.p, n_jobs, d_jcnt.p);
+            }
         }
         k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p);
+        // one synchronisation per level for the edge and position counts (and the per-job counts)
+        uint32_t n_edges = 0, n_new = 0;
+        if (jt) GB_HIP(hipMemcpyAsync(h_jcnt.data(), d_jcnt.p, 4ull * n_jobs, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+        read_u32s({{rowl.p + nf, &n_edges}, {rank.p + n_cand, &n_new}}, s);
         E += n_edges;
         B->spool_top += n_words;
         B->V += n_new;
@@ -374,13 +400,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         hi = B->V;
         levels++;
         if (jt) {
-            // positions of every job so far; jobs past the cap stop growing
-            if (hi > lo) {
-                const uint32_t nb = std::min<uint32_t>(1024, (hi - lo + tb - 1) / tb);
-                k_job_count<<<nb, tb, 0, s>>>(B->p.p, lo, hi, d_job_of_state.p, n_jobs, d_jcnt.p);
-            }
-            GB_HIP(hipMemcpyAsync(h_jcnt.data(), d_jcnt.p, 4ull * n_jobs, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
-            GB_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+            // positions of every job so far (read with the counts above); jobs past the cap stop growing
             bool newly_dead = false;
             for (uint32_t j = 0; j < n_jobs; j++) {
                 jt->count[j] = h_jcnt[j];
