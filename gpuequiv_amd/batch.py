@@ -34,15 +34,23 @@ class GameBatch:
         h = C.c_void_p()
         _lib.check(_lib.lib().geq_batch_upload(self.ctx.handle, len(self.graphs), arr, C.byref(h)))
         self.handle = h
-        self.reach = [np.ascontiguousarray(r, dtype=np.uint32) for r in
-                      (reach if reach is not None else [standard_reach(g) for g in self.graphs])]
-        self._reach_ptrs = (C.POINTER(C.c_uint32) * len(self.reach))(*[_lib.u32p(r) for r in self.reach])
-        self._reach_n = np.array([len(r) for r in self.reach], dtype=np.uint32)
+        if reach is None:
+            # every game's standard_reach, found on the device (NULL lists)
+            self.reach = None
+            self._reach_ptrs = None
+            self._reach_n = None
+        else:
+            self.reach = [np.ascontiguousarray(r, dtype=np.uint32) for r in reach]
+            self._reach_ptrs = (C.POINTER(C.c_uint32) * len(self.reach))(*[_lib.u32p(r) for r in self.reach])
+            self._reach_n = np.array([len(r) for r in self.reach], dtype=np.uint32)
         self.last_stats: dict | None = None
+
+    def _n_ptr(self):
+        return None if self._reach_n is None else _lib.u32p(self._reach_n)
 
     def solve(self, profile: bool = False) -> dict:
         st = _lib.geq_stats()
-        _lib.check(_lib.lib().geq_batch_solve(self.handle, self._reach_ptrs, _lib.u32p(self._reach_n),
+        _lib.check(_lib.lib().geq_batch_solve(self.handle, self._reach_ptrs, self._n_ptr(),
                                               1 if profile else 0, C.byref(st)))
         self.last_stats = st.as_dict()
         return self.last_stats
@@ -51,7 +59,7 @@ class GameBatch:
         """Pick workgroup size, grid sizes, rounds per graph replay and arena
         capacity by timing candidates on this batch (geq_batch_autotune)."""
         rep = _lib.geq_tune_report()
-        _lib.check(_lib.lib().geq_batch_autotune(self.handle, self._reach_ptrs, _lib.u32p(self._reach_n), int(repeats),
+        _lib.check(_lib.lib().geq_batch_autotune(self.handle, self._reach_ptrs, self._n_ptr(), int(repeats),
                                                  C.byref(rep)))
         return rep.as_dict()
 

@@ -1,6 +1,7 @@
 // Autotuner of the solver's launch parameters: workgroup size (waves per
-// block), grid sizes of k_eval / k_expand (blocks per CU), rounds per replayed
-// hipGraph, and the energy-arena capacity.  Candidates are timed on the game
+// block), grid sizes of k_eval / k_expand (blocks per CU; no k_expand grid:
+// the expansion runs inside k_eval), rounds per replayed hipGraph, and the
+// energy-arena capacity.  Candidates are timed on the game
 // itself (median device time of repeated solves), searched coordinate by
 // coordinate from the current parameters.  Every candidate computes the same
 // fixpoint; only the launch shape changes.  No counterpart in the reference,
@@ -33,12 +34,14 @@ geq_tune_params params_of(const geq_game* g) {
     return p;
 }
 
-// median device time (ms) of `repeats` solves after one warm-up solve
-geq_status measure(geq_game* g, const std::vector<uint32_t>& reach, uint32_t repeats, double* ms, uint64_t* arena_peak) {
+// median device time (ms) of `repeats` solves after one warm-up solve;
+// reach NULL: the game's standard_reach, kept on the device
+geq_status measure(geq_game* g, const std::vector<uint32_t>* reach, uint32_t repeats, double* ms, uint64_t* arena_peak) {
     std::vector<double> t;
     for (uint32_t r = 0; r <= repeats; r++) {
         geq_stats st{};
-        geq_status s = geq_game_solve(g, reach.data(), (uint32_t)reach.size(), nullptr, 0, &st);
+        geq_status s = reach ? geq_game_solve(g, reach->data(), (uint32_t)reach->size(), nullptr, 0, &st)
+                             : solve_core(g, g->std_n, nullptr, 0, &st, g->std_seed.p);
         if (s != GEQ_OK) return s;
         *arena_peak = std::max<uint64_t>(*arena_peak, g->h_ctl->arena_top);
         if (r) t.push_back(st.solve_ms);
@@ -48,8 +51,12 @@ geq_status measure(geq_game* g, const std::vector<uint32_t>& reach, uint32_t rep
     return GEQ_OK;
 }
 
-geq_status autotune(geq_game* g, const std::vector<uint32_t>& reach, uint32_t repeats, geq_tune_report* rep) {
+geq_status autotune(geq_game* g, const std::vector<uint32_t>* reach, uint32_t repeats, geq_tune_report* rep) {
     repeats = std::max(1u, std::min(repeats, 50u));
+    if (!reach) {
+        geq_status s = standard_seed(g);
+        if (s != GEQ_OK) return s;
+    }
     using Tune = geq_game::Tune;
     struct Dim {
         uint32_t Tune::*field;
@@ -58,42 +65,60 @@ geq_status autotune(geq_game* g, const std::vector<uint32_t>& reach, uint32_t re
     const std::vector<Dim> dims = {
         {&Tune::waves_per_block, {1, 2, 4, 8}},
         {&Tune::eval_blocks_per_cu, {1, 2, 4, 8, 16, 32}},
-        {&Tune::expand_blocks_per_cu, {1, 2, 4, 8, 16}},
+        {&Tune::expand_blocks_per_cu, {0, 1, 2, 4, 8, 16}},   // 0: expansion inside k_eval
         {&Tune::rounds_per_launch, {1, 2, 4, 8, 16, 32}},
     };
-    Tune best = g->tune;
-    best.mode = 1;
-    apply(g, best);
     uint64_t peak = 0;
-    double best_ms = 0;
-    geq_status s = measure(g, reach, repeats, &best_ms, &peak);
-    if (s != GEQ_OK) return s;
-    const double default_ms = best_ms;
-    uint32_t candidates = 1;
-    for (int pass = 0; pass < 2; pass++) {
-        bool improved = false;
-        for (const Dim& d : dims) {
-            for (uint32_t v : d.values) {
-                if (best.*d.field == v) continue;
-                Tune c = best;
-                c.*d.field = v;
-                apply(g, c);
-                double ms = 0;
-                s = measure(g, reach, repeats, &ms, &peak);
-                if (s != GEQ_OK) {
-                    apply(g, best);
-                    return s;
-                }
-                candidates++;
-                if (ms < 0.97 * best_ms) {   // keep a change only for a clear gain
-                    best = c;
-                    best_ms = ms;
-                    improved = true;
+    uint32_t candidates = 0;
+    // coordinate search from `start`; returns the best shape and its time
+    auto search = [&](Tune start, Tune& best, double& best_ms, double& start_ms) -> geq_status {
+        best = start;
+        best.mode = 1;
+        apply(g, best);
+        geq_status s = measure(g, reach, repeats, &best_ms, &peak);
+        if (s != GEQ_OK) return s;
+        start_ms = best_ms;
+        candidates++;
+        for (int pass = 0; pass < 2; pass++) {
+            bool improved = false;
+            for (const Dim& d : dims) {
+                for (uint32_t v : d.values) {
+                    if (best.*d.field == v) continue;
+                    Tune c = best;
+                    c.*d.field = v;
+                    apply(g, c);
+                    double ms = 0;
+                    s = measure(g, reach, repeats, &ms, &peak);
+                    if (s != GEQ_OK) return s;
+                    candidates++;
+                    if (ms < 0.97 * best_ms) {   // keep a change only for a clear gain
+                        best = c;
+                        best_ms = ms;
+                        improved = true;
+                    }
                 }
             }
+            if (!improved) break;
         }
-        if (!improved) break;
+        return GEQ_OK;
+    };
+    // two starts: the current shape with a separate k_expand, and the same
+    // shape with the expansion inside k_eval (whose best grid tends to differ)
+    const Tune start = g->tune;
+    Tune sep = start, fus = start;
+    if (sep.expand_blocks_per_cu == 0) sep.expand_blocks_per_cu = 4;
+    fus.expand_blocks_per_cu = 0;
+    Tune best_a{}, best_b{};
+    double ms_a = 0, ms_b = 0, start_a = 0, start_b = 0;
+    geq_status s = search(sep, best_a, ms_a, start_a);
+    if (s == GEQ_OK) s = search(fus, best_b, ms_b, start_b);
+    const double default_ms = start.expand_blocks_per_cu == 0 ? start_b : start_a;
+    if (s != GEQ_OK) {
+        apply(g, start);
+        return s;
     }
+    Tune best = ms_b < ms_a ? best_b : best_a;
+    const double best_ms = std::min(ms_a, ms_b);
     apply(g, best);
     // antichain capacity: the arena holds the largest solve seen without regrowing
     const uint64_t want = peak + peak / 4 + 1024;
@@ -125,9 +150,9 @@ geq_status geq_game_get_tune(geq_game* g, geq_tune_params* out) {
 geq_status geq_game_set_tune(geq_game* g, const geq_tune_params* p) {
     if (!g || !p) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
     if (p->waves_per_block < 1 || p->waves_per_block > (uint32_t)MAX_WAVES || p->eval_blocks_per_cu < 1 ||
-        p->eval_blocks_per_cu > 64 || p->expand_blocks_per_cu < 1 || p->expand_blocks_per_cu > 64 || p->rounds_per_launch < 1 ||
-        p->rounds_per_launch > 256)
-        return fail(GEQ_ERR_INVALID_ARG, "launch parameters out of range (waves 1..8, blocks per CU 1..64, rounds 1..256)");
+        p->eval_blocks_per_cu > 64 || p->expand_blocks_per_cu > 64 || p->rounds_per_launch < 1 || p->rounds_per_launch > 256)
+        return fail(GEQ_ERR_INVALID_ARG,
+                    "launch parameters out of range (waves 1..8, eval blocks per CU 1..64, expand blocks per CU 0..64, rounds 1..256)");
     (void)hipSetDevice(g->ctx->device);
     geq_game::Tune t = g->tune;
     t.waves_per_block = p->waves_per_block;
@@ -146,11 +171,16 @@ geq_status geq_game_autotune(geq_game* g, const uint32_t* to_reach, uint32_t n_r
     if (!g) return fail(GEQ_ERR_INVALID_ARG, "game is NULL");
     if (n_reach && !to_reach) return fail(GEQ_ERR_INVALID_ARG, "to_reach is NULL");
     (void)hipSetDevice(g->ctx->device);
-    return autotune(g, std::vector<uint32_t>(to_reach, to_reach + n_reach), repeats, out);
+    const std::vector<uint32_t> reach(to_reach, to_reach + n_reach);
+    return autotune(g, &reach, repeats, out);
 }
 
 geq_status geq_batch_autotune(geq_game* b, const uint32_t* const* to_reach, const uint32_t* n_reach, uint32_t repeats,
                               geq_tune_report* out) {
+    if (b && !to_reach && !n_reach) {   // every game's standard_reach
+        (void)hipSetDevice(b->ctx->device);
+        return autotune(b, nullptr, repeats, out);
+    }
     if (!b || !n_reach) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
     const uint32_t n_games = (uint32_t)b->game_base.size() - 1;
     std::vector<uint32_t> reach;
@@ -163,7 +193,7 @@ geq_status geq_batch_autotune(geq_game* b, const uint32_t* const* to_reach, cons
         }
     }
     (void)hipSetDevice(b->ctx->device);
-    return autotune(b, reach, repeats, out);
+    return autotune(b, &reach, repeats, out);
 }
 
 }  // extern "C"

@@ -241,8 +241,7 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     if ((e = gm->rcol.alloc(E)) != hipSuccess) return oom(e, "rev_column_indices");
     if ((e = gm->edges.alloc(E)) != hipSuccess) return oom(e, "edges");
     if ((e = gm->atk.alloc(V)) != hipSuccess) return oom(e, "attacker_pos");
-    if ((e = gm->cur.alloc(V)) != hipSuccess) return oom(e, "energy lists");
-    if ((e = gm->nxt.alloc(V)) != hipSuccess) return oom(e, "energy lists");
+    if ((e = gm->st.alloc(2ull * V)) != hipSuccess) return oom(e, "energy lists");
     if ((e = gm->wl0.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->wl1.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->chg.alloc(V)) != hipSuccess) return oom(e, "changed flags");
@@ -289,16 +288,17 @@ void geq_game_free(geq_game* g) {
 
 namespace {
 
-__global__ void k_seed_zero(uint4* cur, const uint32_t* ids, uint32_t n) {
+// slot 0 (tag 0, current while both slots are untouched) of every node to reach
+__global__ void k_seed_zero(uint4* st, const uint32_t* ids, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) cur[ids[i]] = make_uint4(1u, 0u, 0u, 0u);   // EnergyArray::zero(1): one all-zero energy
+    if (i < n) st[2ull * ids[i]] = make_uint4(1u, 0u, 0u, 0u);   // EnergyArray::zero(1): one all-zero energy
 }
 
 }  // namespace
 
 // Grow the arena keeping its content (lists of the committed state live in [0, top)).
 uint32_t geq::initial_arena_entries(uint64_t E, uint64_t want) {
-    // most antichains fit the 7 inline slots of a record; the arena grows on demand
+    // most antichains fit the 6 inline entries of a record; the arena grows on demand
     uint64_t n = want ? want : std::max<uint64_t>(1u << 20, E / 2);
     if (const char* env = std::getenv("GEQ_ARENA_ENTRIES")) {
         const long long v = std::atoll(env);
@@ -337,19 +337,39 @@ RoundShape round_shape(const geq_game* g) {
     RoundShape r;
     const uint32_t w = std::max(1u, std::min<uint32_t>(g->tune.waves_per_block, MAX_WAVES));
     r.threads = 64u * w;
+    r.fused = g->tune.expand_blocks_per_cu == 0;
     r.eval_blocks = std::max(1u, std::min(ncu * g->tune.eval_blocks_per_cu, (g->V + w - 1) / w));
-    r.expand_blocks = std::max(1u, std::min(ncu * g->tune.expand_blocks_per_cu, (g->V + 64u * w - 1) / (64u * w)));
+    // (the seeding launch uses this grid in the fused shape too)
+    const uint32_t xpc = r.fused ? 4u : g->tune.expand_blocks_per_cu;
+    r.expand_blocks = std::max(1u, std::min(ncu * xpc, (g->V + 64u * w - 1) / (64u * w)));
     r.rounds_per_launch = std::max(1u, g->tune.rounds_per_launch);
+    // k_eval's grid-stride loop covers any worklist: a grid larger than what
+    // is resident at once only adds a second, mostly idle wave of blocks
+    r.eval_blocks = std::min(r.eval_blocks, ncu * eval_blocks_resident(r.fused, w));
     r.eval_blocks = std::min(r.eval_blocks, STAT_SLOTS);
-    // cooperative launch: every block must be resident at once
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_solve), BLOCK,
-                                                     WAVES * EVAL_LDS_PER_WAVE) != hipSuccess ||
-        per_cu <= 0)
-        per_cu = 1;
-    r.coop_blocks = std::min<uint32_t>(std::min<uint32_t>((uint32_t)per_cu, g->tune.coop_blocks_per_cu) * ncu, STAT_SLOTS);
-    r.coop_blocks = std::max(1u, std::min(r.coop_blocks, (g->V + WAVES - 1) / WAVES));
     return r;
+}
+
+// Blocks of k_eval (w waves each, fused or not) that one CU holds at once.
+uint32_t eval_blocks_resident(bool fused, uint32_t w) {
+    static int cache[2][MAX_WAVES + 1] = {};
+    int& c = cache[fused ? 1 : 0][w];
+    if (c == 0) {
+        int per_cu = 0;
+        const void* fn = fused ? reinterpret_cast<const void*>(&k_eval<true>) : reinterpret_cast<const void*>(&k_eval<false>);
+        const size_t lds = (size_t)w * (fused ? EVAL_FUSED_LDS_PER_WAVE : EVAL_LDS_PER_WAVE);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)(64 * w), lds) != hipSuccess || per_cu <= 0) per_cu = 64;
+        c = per_cu;
+    }
+    return (uint32_t)c;
+}
+
+EvalArgs eval_args(geq_game* g) {
+    return EvalArgs{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->st.p, g->arena.p, g->ctl.p, g->arena_cap,
+                    g->chg.p, g->conf.max, g->stats.p, g->rrow.p, g->rcol.p, g->stamp.p};
+}
+ExpandArgs expand_args(geq_game* g) {
+    return ExpandArgs{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
 }
 
 geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape) {
@@ -360,8 +380,12 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
     }
     GEQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), GEQ_ERR_NO_DEVICE);
     for (uint32_t k = 0; k < shape.rounds_per_launch; k++) {
-        k_eval<<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea, k);
-        k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa, k);
+        if (shape.fused) {
+            k_eval<true><<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_FUSED_LDS_PER_WAVE, s>>>(ea, k);
+        } else {
+            k_eval<false><<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea, k);
+            k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa, k);
+        }
     }
     k_advance<<<1, 64, 0, s>>>(g->ctl.p, shape.rounds_per_launch);
     hipGraph_t graph = nullptr;
@@ -371,6 +395,46 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
     if (e != hipSuccess) return fail(GEQ_ERR_NO_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     g->graph_arena = g->arena.p;
     return GEQ_OK;
+}
+
+// After a batch of rounds stopped at an arena overflow (ctl->round is the
+// round that overflowed, nothing in flight): grow the arena and reset what
+// the interrupted round left behind, so that it runs again from its start.
+// Its half-written slots carry its own tag and are ignored by the rerun; its
+// appends to the next worklist are dropped, and the stamps are cleared so the
+// rerun can queue the same predecessors again.
+geq_status recover_overflow(geq_game* g) {
+    hipStream_t s = g->ctx->stream;
+    geq_status st = grow_arena(g, g->h_ctl->arena_top);
+    if (st != GEQ_OK) return st;
+    Ctl fix = *g->h_ctl;
+    fix.overflow = 0;
+    fix.wl_n[(fix.round + 1u) % 3u] = 0;
+    GEQ_HIP(hipMemsetAsync(g->stamp.p, 0, 4ull * g->V, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipMemcpyAsync(g->ctl.p, &fix, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    *g->h_ctl = fix;
+    return GEQ_OK;
+}
+
+// Between batches of rounds: re-tag every node's slots once RETAG_ROUNDS
+// rounds have passed since the last re-tag, so slot ages stay below 2^16.
+geq_status maybe_retag(geq_game* g) {
+    const uint32_t r = g->h_ctl->round;
+    if (r - g->retag_round < retag_interval()) return GEQ_OK;
+    if (g->V) k_retag<<<(g->V + 255) / 256, 256, 0, g->ctx->stream>>>(g->st.p, g->V, round_tag(r));
+    GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
+    g->retag_round = r;
+    return GEQ_OK;
+}
+
+uint32_t retag_interval() {
+    // GEQ_RETAG_ROUNDS (1 .. RETAG_ROUNDS) re-tags more often, for tests
+    if (const char* env = std::getenv("GEQ_RETAG_ROUNDS")) {
+        const long x = std::atol(env);
+        if (x >= 1 && x <= (long)RETAG_ROUNDS) return (uint32_t)x;
+    }
+    return RETAG_ROUNDS;
 }
 
 }  // namespace geq
@@ -390,7 +454,30 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
 }
 
 // The fixpoint on a resident game whose n_reach nodes to reach are already in g->seed.
-geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* initial, int profile_kernels, geq_stats* stats) {
+namespace {
+__global__ void k_standard_reach(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t* out, uint32_t* n) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V && !atk[v] && row[v] == row[v + 1]) out[atomicAdd(n, 1u)] = v;
+}
+}  // namespace
+
+geq_status geq::standard_seed(geq_game* g) {
+    if (g->std_ready) return GEQ_OK;
+    hipStream_t s = g->ctx->stream;
+    const uint32_t V = g->V;
+    GEQ_HIP(g->std_seed.alloc(std::max<uint32_t>(V, 1) + 1), GEQ_ERR_OUT_OF_MEMORY);
+    uint32_t* cnt = g->std_seed.p + std::max<uint32_t>(V, 1);   // the count sits after the list
+    GEQ_HIP(hipMemsetAsync(cnt, 0, 4, s), GEQ_ERR_TRANSFER);
+    if (V) k_standard_reach<<<(V + 255) / 256, 256, 0, s>>>(g->row.p, g->atk.p, V, g->std_seed.p, cnt);
+    GEQ_HIP(hipMemcpyAsync(&g->std_n, cnt, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    g->std_ready = true;
+    return GEQ_OK;
+}
+
+geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* initial, int profile_kernels, geq_stats* stats,
+                           const uint32_t* seed) {
+    if (!seed) seed = g->seed.p;
     const uint32_t V = g->V;
     geq_ctx* ctx = g->ctx;
     hipStream_t s = ctx->stream;
@@ -401,7 +488,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         if (initial->n_nodes != V || !initial->offsets || (initial->offsets[V] && !initial->data))
             return fail(GEQ_ERR_INVALID_ARG, "initial energies must cover every node");
         if (initial->energy_size != 1) return fail(GEQ_ERR_INVALID_ARG, "initial energies: energy_size must be 1 for this conf");
-        std::vector<uint4> h_cur(V);
+        std::vector<uint4> h_st(2ull * V, make_uint4(0, 0, 0, 0));   // slot 0 current (tags 0), slot 1 empty
         for (uint32_t v = 0; v < V; v++) {
             uint32_t b = initial->offsets[v], e = initial->offsets[v + 1];
             if (e < b || e - b > LEN_MASK) return fail(GEQ_ERR_INVALID_ARG, "initial energies: bad offsets / list too long");
@@ -422,42 +509,41 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
                 h[2] = (uint16_t)(off & 0xffff);   // second dword of the record = arena offset
                 h[3] = (uint16_t)(off >> 16);
             }
-            std::memcpy(&h_cur[v], h, 16);
+            std::memcpy(&h_st[2ull * v], h, 16);
         }
         if (h_arena.size() + (1u << 20) > g->arena_cap) {
             geq_status st = grow_arena(g, (uint32_t)h_arena.size());
             if (st != GEQ_OK) return st;
         }
-        GEQ_HIP(hipMemcpyAsync(g->cur.p, h_cur.data(), 16ull * V, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        GEQ_HIP(hipMemcpyAsync(g->st.p, h_st.data(), 32ull * V, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
         if (!h_arena.empty())
             GEQ_HIP(hipMemcpyAsync(g->arena.p, h_arena.data(), 2ull * h_arena.size(), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
     if (!initial) {
-        GEQ_HIP(hipMemsetAsync(g->cur.p, 0, 16ull * V, s), GEQ_ERR_TRANSFER);
-        if (n_reach) k_seed_zero<<<(n_reach + 255) / 256, 256, 0, s>>>(g->cur.p, g->seed.p, n_reach);
+        GEQ_HIP(hipMemsetAsync(g->st.p, 0, 32ull * V, s), GEQ_ERR_TRANSFER);
+        if (n_reach) k_seed_zero<<<(n_reach + 255) / 256, 256, 0, s>>>(g->st.p, seed, n_reach);
     }
     GEQ_HIP(hipMemsetAsync(g->stamp.p, 0, 4ull * V, s), GEQ_ERR_TRANSFER);
     Ctl c0{};
     c0.arena_top = (uint32_t)h_arena.size();
+    g->retag_round = 0;
     GEQ_HIP(hipMemcpyAsync(g->ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemsetAsync(g->stats.p, 0, (4ull * STAT_SLOTS + 4) * sizeof(unsigned long long), s), GEQ_ERR_TRANSFER);
 
     hipEvent_t ev_start, ev_end;
     GEQ_HIP(hipEventCreate(&ev_start), GEQ_ERR_NO_DEVICE);
     GEQ_HIP(hipEventCreate(&ev_end), GEQ_ERR_NO_DEVICE);
-    std::vector<hipEvent_t> prof;   // 3 events per round: before eval, after eval, after expand
-    std::vector<hipEvent_t> coop_events;   // around every cooperative k_solve launch
+    std::vector<hipEvent_t> prof;   // 4 events per round: around k_eval, around k_expand
     GEQ_HIP(hipEventRecord(ev_start, s), GEQ_ERR_NO_DEVICE);
 
     const RoundShape shape = round_shape(g);
-    EvalArgs ea{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->cur.p, g->nxt.p, g->arena.p, g->ctl.p,
-                g->arena_cap, g->chg.p, g->conf.max, g->stats.p};
-    ExpandArgs xa{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
+    EvalArgs ea = eval_args(g);
+    ExpandArgs xa = expand_args(g);
 
     // seed: predecessors of the nodes to reach (initialize_visit_lists, :1461-1465)
     if (n_reach) {
         ExpandArgs sx = xa;
-        sx.seed = g->seed.p;
+        sx.seed = seed;
         sx.n_seed = n_reach;
         k_expand<<<std::max(1u, std::min(shape.expand_blocks, (n_reach + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx, 0u);
     }
@@ -466,44 +552,34 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
 
     uint64_t launches = 0;
     geq_status result = GEQ_OK;
-    while (g->h_ctl->wl_n[g->h_ctl->round & 1u] != 0) {
+    while (g->h_ctl->wl_n[g->h_ctl->round % 3u] != 0) {
         if (profile_kernels) {
             // one round per launch; each kernel's dispatch carries its own
             // start / end events (hipExtLaunchKernelGGL), so the times are the
             // kernels' own, without the gaps between dispatches
             hipEvent_t ev[4];
             for (hipEvent_t& e : ev) GEQ_HIP(hipEventCreate(&e), GEQ_ERR_NO_DEVICE);
-            hipExtLaunchKernelGGL(k_eval, dim3(shape.eval_blocks), dim3(shape.threads), (shape.threads / 64) * EVAL_LDS_PER_WAVE,
-                                  s, ev[0], ev[1], 0, ea, 0u);
-            hipExtLaunchKernelGGL(k_expand, dim3(shape.expand_blocks), dim3(shape.threads), 0, s, ev[2], ev[3], 0, xa, 0u);
+            if (shape.fused) {
+                hipExtLaunchKernelGGL(k_eval<true>, dim3(shape.eval_blocks), dim3(shape.threads),
+                                      (shape.threads / 64) * EVAL_FUSED_LDS_PER_WAVE, s, ev[0], ev[1], 0, ea, 0u);
+                GEQ_HIP(hipEventRecord(ev[2], s), GEQ_ERR_NO_DEVICE);
+                GEQ_HIP(hipEventRecord(ev[3], s), GEQ_ERR_NO_DEVICE);
+            } else {
+                hipExtLaunchKernelGGL(k_eval<false>, dim3(shape.eval_blocks), dim3(shape.threads),
+                                      (shape.threads / 64) * EVAL_LDS_PER_WAVE, s, ev[0], ev[1], 0, ea, 0u);
+                hipExtLaunchKernelGGL(k_expand, dim3(shape.expand_blocks), dim3(shape.threads), 0, s, ev[2], ev[3], 0, xa, 0u);
+            }
             GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
             k_advance<<<1, 64, 0, s>>>(g->ctl.p, 1u);
             for (hipEvent_t e : ev) prof.push_back(e);
             launches++;
-        } else if (g->tune.mode == 0 && !std::getenv("GEQ_ENABLE_COOPERATIVE")) {
+        } else if (g->tune.mode == 0) {
             // The cooperative one-launch solve is switched off: its grid of
-            // min(occupancy API, 8) blocks per CU can exceed the 7 waves/SIMD
-            // that k_solve's 106 SGPRs allow, so its grid barrier can wait for
+            // min(occupancy API, 8) blocks per CU could exceed the waves per
+            // SIMD its register use allows, so its grid barrier could wait for
             // blocks that are never resident.
             result = fail(GEQ_ERR_UNSUPPORTED, "the cooperative solve mode (GEQ_SOLVE_MODE=0) is disabled");
             break;
-        } else if (g->tune.mode == 0) {
-            // the whole fixpoint in one cooperative launch (returns early only
-            // on an arena overflow or after max_rounds rounds)
-            uint32_t max_rounds = 1u << 20;
-            uint32_t spin_limit = 1u << 25;   // ~ seconds of polling before a barrier gives up
-            void* args[] = {&ea, &xa, &max_rounds, &spin_limit};
-            hipEvent_t k0, k1;
-            GEQ_HIP(hipEventCreate(&k0), GEQ_ERR_NO_DEVICE);
-            GEQ_HIP(hipEventCreate(&k1), GEQ_ERR_NO_DEVICE);
-            GEQ_HIP(hipEventRecord(k0, s), GEQ_ERR_NO_DEVICE);
-            GEQ_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve), dim3(shape.coop_blocks), dim3(BLOCK),
-                                               args, WAVES * EVAL_LDS_PER_WAVE, s),
-                    GEQ_ERR_NO_DEVICE);
-            GEQ_HIP(hipEventRecord(k1, s), GEQ_ERR_NO_DEVICE);
-            coop_events.push_back(k0);
-            coop_events.push_back(k1);
-            launches++;
         } else {
             // rounds_per_launch captured rounds per graph launch
             if (!g->graph_exec || g->graph_arena != g->arena.p) {
@@ -515,25 +591,16 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         }
         GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
-        if (g->h_ctl->error) {
-            result = fail(GEQ_ERR_INTERNAL, "k_solve: a grid barrier timed out (blocks not co-resident?)");
-            break;
-        }
         if (g->h_ctl->overflow) {
-            // k_expand / k_advance did nothing: grow the arena, then the same round runs again
-            result = grow_arena(g, g->h_ctl->arena_top);
+            result = recover_overflow(g);
             if (result != GEQ_OK) break;
-            Ctl fix = *g->h_ctl;
-            fix.overflow = 0;
-            fix.wl_n[(fix.round + 1u) & 1u] = 0;
-            GEQ_HIP(hipMemcpyAsync(g->ctl.p, &fix, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-            GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
-            *g->h_ctl = fix;
             ea.arena = g->arena.p;
             ea.arena_cap = g->arena_cap;
         }
+        result = maybe_retag(g);
+        if (result != GEQ_OK) break;
     }
-    const uint64_t rounds = g->h_ctl->rounds_done;
+    const uint64_t rounds = g->h_ctl->last_nonempty;
     GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
     GEQ_HIP(hipEventSynchronize(ev_end), GEQ_ERR_NO_DEVICE);
     float ms = 0.f;
@@ -546,19 +613,12 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         eval_ms += a;
         expand_ms += b;
     }
-    double coop_ms = 0;
-    for (size_t i = 0; i + 1 < coop_events.size(); i += 2) {
-        float a = 0;
-        (void)hipEventElapsedTime(&a, coop_events[i], coop_events[i + 1]);
-        coop_ms += a;
-    }
-    for (hipEvent_t ev : coop_events) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : prof) (void)hipEventDestroy(ev);
     (void)hipEventDestroy(ev_start);
     (void)hipEventDestroy(ev_end);
     if (result != GEQ_OK) return result;
     unsigned long long h_stats[4] = {0, 0, 0, 0};
-    k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, std::max(shape.eval_blocks, shape.coop_blocks),
+    k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, shape.eval_blocks,
                                      g->stats.p + 4ull * STAT_SLOTS);
     GEQ_HIP(hipMemcpyAsync(h_stats, g->stats.p + 4ull * STAT_SLOTS, sizeof(h_stats), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -573,7 +633,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         stats->eval_kernel_ms = eval_ms;
         stats->expand_kernel_ms = expand_ms;
         stats->eval_launches = launches;
-        stats->solve_kernel_ms = coop_ms;
+        stats->solve_kernel_ms = 0;
         stats->mode = profile_kernels ? 2u : g->tune.mode;
     }
     return GEQ_OK;
@@ -590,7 +650,8 @@ geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_e
     (void)hipSetDevice(g->ctx->device);
     hipStream_t s = g->ctx->stream;
     if (!g->lens.p || g->lens.n < V) GEQ_HIP(g->lens.alloc(V), GEQ_ERR_OUT_OF_MEMORY);
-    k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->cur.p + first, V, g->lens.p);
+    const uint32_t t = round_tag(g->h_ctl->round);   // the lists after the last round that ran
+    k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + 2ull * first, V, t, g->lens.p);
     std::vector<uint32_t> lens(V), offs(V + 1);
     GEQ_HIP(hipMemcpyAsync(lens.data(), g->lens.p, 4ull * V, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -604,7 +665,7 @@ geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_e
     if (!g->offs.p || g->offs.n < V + 1) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
     if (!g->out.p || g->out.n < tot) GEQ_HIP(g->out.alloc(tot), GEQ_ERR_OUT_OF_MEMORY);
     GEQ_HIP(hipMemcpyAsync(g->offs.p, offs.data(), 4ull * (V + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->cur.p + first, g->arena.p, V, g->offs.p, g->out.p);
+    k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + 2ull * first, g->arena.p, V, t, g->offs.p, g->out.p);
     uint32_t* data = (uint32_t*)std::malloc(4ull * (tot ? tot : 1));
     uint32_t* ooffs = (uint32_t*)std::malloc(4ull * (V + 1));
     if (!data || !ooffs) {
@@ -644,6 +705,13 @@ extern "C" geq_status geq_solve(geq_ctx* ctx, const geq_graph* graph, const uint
 
 extern "C" geq_status geq_batch_solve(geq_game* b, const uint32_t* const* to_reach, const uint32_t* n_reach, int profile_kernels,
                                       geq_stats* stats) {
+    if (b && !to_reach && !n_reach) {
+        // every game's standard_reach, kept on the device
+        (void)hipSetDevice(b->ctx->device);
+        geq_status st = geq::standard_seed(b);
+        if (st != GEQ_OK) return st;
+        return geq::solve_core(b, b->std_n, nullptr, profile_kernels, stats, b->std_seed.p);
+    }
     if (!b || !n_reach) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
     const uint32_t n_games = (uint32_t)b->game_base.size() - 1;
     std::vector<uint32_t> reach;

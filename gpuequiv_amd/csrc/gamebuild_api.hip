@@ -33,11 +33,7 @@ __global__ void k_rev_scatter(const uint32_t* row, uint32_t V, const uint32_t* c
     if (v >= V) return;
     for (uint32_t e = row[v]; e < row[v + 1]; e++) rcol[atomicAdd(&fill[col[e]], 1u)] = v;
 }
-// standard_reach on the device: defender positions without successors
-__global__ void k_reach(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t* out, uint32_t* n) {
-    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V && !atk[v] && row[v] == row[v + 1]) out[atomicAdd(n, 1u)] = v;
-}
+
 
 template <class T>
 geq_status ensure(DevBuf<T>& b, size_t need, hipStream_t s, bool keep, const char* what) {
@@ -104,8 +100,7 @@ geq_status geq::alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_gam
     if ((e = gm->rcol.alloc(E)) != hipSuccess) return oom(e, "rev_column_indices");
     if ((e = gm->edges.alloc(E)) != hipSuccess) return oom(e, "edges");
     if ((e = gm->atk.alloc(V)) != hipSuccess) return oom(e, "attacker_pos");
-    if ((e = gm->cur.alloc(V)) != hipSuccess) return oom(e, "energy lists");
-    if ((e = gm->nxt.alloc(V)) != hipSuccess) return oom(e, "energy lists");
+    if ((e = gm->st.alloc(2ull * V)) != hipSuccess) return oom(e, "energy lists");
     if ((e = gm->wl0.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->wl1.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->chg.alloc(V)) != hipSuccess) return oom(e, "changed flags");
@@ -662,19 +657,14 @@ void geq_built_game_free(geq_built_game* o) {
 geq_status geq_built_solve(geq_built* B, int profile_kernels, geq_stats* stats) {
     if (!B) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
     (void)hipSetDevice(B->ctx->device);
-    hipStream_t s = B->ctx->stream;
     if (!B->game) {
         geq_status st = game_from_device(B->ctx, B->V, B->E, B->row.p, B->col.p, B->wt.p, B->atk.p, &B->game);
         if (st != GEQ_OK) return st;
-        geq_game* g = B->game;
-        GEQ_HIP(g->seed.alloc(std::max<uint32_t>(B->V, 1)), GEQ_ERR_OUT_OF_MEMORY);
-        DevBuf<uint32_t> cnt;
-        GEQ_HIP(cnt.alloc(1), GEQ_ERR_OUT_OF_MEMORY);
-        GEQ_HIP(hipMemsetAsync(cnt.p, 0, 4, s), GEQ_ERR_TRANSFER);
-        k_reach<<<(B->V + 255) / 256, 256, 0, s>>>(g->row.p, g->atk.p, B->V, g->seed.p, cnt.p);
-        B->n_reach = read_u32(cnt.p, s);
+        st = standard_seed(B->game);
+        if (st != GEQ_OK) return st;
+        B->n_reach = B->game->std_n;
     }
-    return solve_core(B->game, B->n_reach, nullptr, profile_kernels, stats);
+    return solve_core(B->game, B->n_reach, nullptr, profile_kernels, stats, B->game->std_seed.p);
 }
 
 geq_status geq_built_energies(geq_built* B, uint32_t first, uint32_t count, geq_energies* out) {

@@ -20,23 +20,57 @@
 namespace geq {
 
 // ---------------------------------------------------------------------------
-// Per-node energy list descriptor: 16 bytes.
+// Per-node energy list descriptor ("slot"): 16 bytes.
 //   h[0]: bits 0..14 = number of energies, bit 15 = list is not sorted
-//   len <= 7 : h[1..7] hold the energies inline
-//   len  > 7 : the record's second dword (h[2] | h[3] << 16) is the offset
+//   len <= 6 : h[1..6] hold the energies inline
+//   len  > 6 : the record's second dword (h[2] | h[3] << 16) is the offset
 //              into the energy arena (u16 units)
+//   h[7]: tag of the round that wrote the slot (round + 1, mod 2^16)
+//
+// Every node owns two slots (32 bytes, st[2v] and st[2v + 1]).  A round
+// writes a node's new list into the slot that is not current, so the other
+// evaluations of the same round keep reading the list as of the round's
+// start (Jacobi) and no commit pass is needed.  At round r (tag t):
+//   * a slot tagged t was written during round r itself: not current yet;
+//   * otherwise the younger slot (smaller age t - tag, mod 2^16) is current,
+//     slot 0 on a tie (both slots untouched since the initialisation).
+// Ages stay far below 2^16: the host re-tags every node (k_retag) at least
+// every RETAG_ROUNDS rounds.
 // ---------------------------------------------------------------------------
-constexpr uint32_t INLINE_CAP = 7;
+constexpr uint32_t INLINE_CAP = 6;
 constexpr uint32_t LEN_MASK = 0x7fffu;
 constexpr uint32_t UNSORTED = 0x8000u;
+constexpr uint32_t RETAG_ROUNDS = 16384;
 
 struct __align__(16) NodeList {
     uint16_t h[8];
 };
 
+__host__ __device__ __forceinline__ uint32_t round_tag(uint32_t r) { return (r + 1u) & 0xffffu; }
 __device__ __forceinline__ uint32_t nl_len(const uint4& d) { return d.x & LEN_MASK; }
 __device__ __forceinline__ uint32_t nl_off(const uint4& d) { return d.y; }  // h[2] | h[3] << 16
-// inline energy j (0..6) of a descriptor
+__device__ __forceinline__ uint32_t nl_tag(const uint4& d) { return d.w >> 16; }
+// the record with its tag field replaced
+__device__ __forceinline__ uint4 nl_with_tag(uint4 d, uint32_t tag) {
+    d.w = (d.w & 0xffffu) | ((tag & 0xffffu) << 16);
+    return d;
+}
+// which of a node's two slots is current at round tag t (see above)
+__device__ __forceinline__ uint32_t cur_slot(const uint4& a, const uint4& b, uint32_t t) {
+    const uint32_t age_a = (t - nl_tag(a)) & 0xffffu, age_b = (t - nl_tag(b)) & 0xffffu;
+    return (age_a == 0u || (age_b != 0u && age_b < age_a)) ? 1u : 0u;
+}
+// node v's current record at round tag t (both slots are loaded together)
+__device__ __forceinline__ uint4 load_cur(const uint4* __restrict__ st, uint32_t v, uint32_t t) {
+    const uint4 a = st[2ull * v], b = st[2ull * v + 1];
+    return cur_slot(a, b, t) ? b : a;
+}
+__device__ __forceinline__ uint4 load_cur(const uint4* __restrict__ st, uint32_t v, uint32_t t, uint32_t& slot) {
+    const uint4 a = st[2ull * v], b = st[2ull * v + 1];
+    slot = cur_slot(a, b, t);
+    return slot ? b : a;
+}
+// inline energy j (0..5) of a descriptor
 __device__ __forceinline__ uint32_t nl_inline(const uint4& d, uint32_t j) {
     // energies sit at u16 index 1 + j of the 16-byte record
     uint32_t idx = 1 + j;

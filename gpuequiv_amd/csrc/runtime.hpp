@@ -49,7 +49,14 @@ struct DevBuf {
     }
 };
 
-geq_status solve_core(geq_game* g, uint32_t n_reach, const geq_energies* initial, int profile_kernels, geq_stats* stats);
+// The fixpoint on a resident game seeded with the n_reach device node ids
+// `seed` (g->seed when NULL).
+geq_status solve_core(geq_game* g, uint32_t n_reach, const geq_energies* initial, int profile_kernels, geq_stats* stats,
+                      const uint32_t* seed = nullptr);
+// EnergyGame::standard_reach (src/energygame.rs:347-352) of a resident game:
+// the defender positions without successors, found on the device once and
+// kept in g->std_seed / g->std_n.
+geq_status standard_seed(geq_game* g);
 geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_energies* out);
 // Device-resident game from device arrays (STANDARD conf): rows, columns,
 // packed updates, attacker flags; the reverse CSR is derived on the device.
@@ -66,12 +73,21 @@ geq_status validate_job(const geq_lts* lts, const uint32_t* pairs, uint32_t n_pa
 geq_status alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_game** out, uint64_t arena_entries = 0);
 // Launch shape of one round and the hipGraph of `rounds_per_launch` captured rounds.
 struct RoundShape {
-    uint32_t eval_blocks, expand_blocks, rounds_per_launch, coop_blocks;
+    uint32_t eval_blocks, expand_blocks, rounds_per_launch;
     uint32_t threads;                 // block size of k_eval / k_expand
+    bool fused;                       // k_eval expands too: one kernel per round
 };
 RoundShape round_shape(const geq_game* g);
+uint32_t eval_blocks_resident(bool fused, uint32_t waves_per_block);
+EvalArgs eval_args(geq_game* g);
+ExpandArgs expand_args(geq_game* g);
 geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape);
 geq_status grow_arena(geq_game* g, uint32_t needed_top);
+// ctl->overflow was raised: grow the arena and make the round run again
+geq_status recover_overflow(geq_game* g);
+// between batches of rounds: k_retag when due (slot tags stay unambiguous)
+geq_status maybe_retag(geq_game* g);
+uint32_t retag_interval();
 
 }  // namespace geq
 
@@ -88,22 +104,25 @@ struct geq_game {
     geq::ConfInfo conf{};
     uint32_t V = 0, E = 0;
     geq::DevBuf<uint32_t> row, rrow, rcol, wl0, wl1, stamp, seed, lens, offs, out;
+    geq::DevBuf<uint32_t> std_seed;   // standard_reach, once computed
+    uint32_t std_n = 0;
+    bool std_ready = false;
     geq::DevBuf<uint2> edges;
     geq::DevBuf<uint8_t> atk, chg;
-    geq::DevBuf<uint4> cur, nxt;
+    geq::DevBuf<uint4> st;            // two list slots per node
     geq::DevBuf<uint16_t> arena;
     geq::DevBuf<geq::Ctl> ctl;
     geq::DevBuf<unsigned long long> stats;
     geq::Ctl* h_ctl = nullptr;        // pinned
     uint32_t arena_cap = 0;
+    uint32_t retag_round = 0;         // round of the last k_retag (or of the initialisation)
     std::vector<uint8_t> h_atk;       // kept for standard_reach / reach validation
     struct Tune {                     // launch shape, set by the autotuner or defaults
-        uint32_t mode = 1;            // 0: one cooperative launch per solve, 1: hipGraph of captured rounds
-        uint32_t eval_blocks_per_cu = 8;
-        uint32_t expand_blocks_per_cu = 4;
-        uint32_t rounds_per_launch = 4;
-        uint32_t coop_blocks_per_cu = 8;
-        uint32_t waves_per_block = 4;     // k_eval / k_expand workgroup: 1..8 waves
+        uint32_t mode = 1;            // 1: hipGraph of captured rounds (0, the cooperative launch, is switched off)
+        uint32_t eval_blocks_per_cu = 8;     // capped at the blocks a CU holds at once
+        uint32_t expand_blocks_per_cu = 0;   // 0: the expansion runs inside k_eval
+        uint32_t rounds_per_launch = 8;
+        uint32_t waves_per_block = 8;     // k_eval / k_expand workgroup: 1..8 waves
     } tune;
     std::vector<uint32_t> game_base;      // batched games: first node of every game (+ total)
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds

@@ -36,43 +36,55 @@ using namespace geq;
 
 namespace {
 
-// per-node bytes of the pools (row, rrow, atk, cur, nxt, wl x2, chg, stamp, job_of) and per-edge bytes (edges, rcol)
-constexpr uint64_t NODE_BYTES = 4 + 4 + 1 + 16 + 16 + 4 + 4 + 1 + 4 + 4;
+// per-node bytes of the pools (row, rrow, atk, two list slots, wl x2, chg, stamp, job_of) and per-edge bytes (edges, rcol)
+constexpr uint64_t NODE_BYTES = 4 + 4 + 1 + 32 + 4 + 4 + 1 + 4 + 4;
 constexpr uint64_t EDGE_BYTES = 8 + 4;
 
-// Place a built game at node base nb / edge base eb of the pools.
+// Place a built game at node base nb / edge base eb of the pools.  Its
+// slots start empty, tagged so that slot 0 is current from round tag t0 on.
 __global__ void k_place_nodes(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t nb, uint32_t eb, uint32_t job,
-                              uint32_t* prow, uint8_t* patk, uint4* cur, uint32_t* stamp, uint32_t* job_of) {
+                              uint32_t t0, uint32_t* prow, uint8_t* patk, uint4* st, uint32_t* stamp, uint32_t* job_of) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v > V) return;
     prow[nb + v] = eb + row[v];             // v == V: the row end of the job (padding slot)
     patk[nb + v] = v < V ? atk[v] : (uint8_t)1;
-    cur[nb + v] = make_uint4(0, 0, 0, 0);
+    st[2ull * (nb + v)] = make_uint4(0, 0, 0, ((t0 - 1u) & 0xffffu) << 16);
+    st[2ull * (nb + v) + 1] = make_uint4(0, 0, 0, ((t0 - 2u) & 0xffffu) << 16);
     stamp[nb + v] = 0;
     job_of[nb + v] = job;
 }
 // a released range holds no lists (the arena compaction walks every slot)
-__global__ void k_clear_nodes(uint4* cur, uint32_t nb, uint32_t n) {
+__global__ void k_clear_nodes(uint4* st, uint32_t nb, uint32_t n) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) cur[nb + v] = make_uint4(0, 0, 0, 0);
+    if (v < n) {
+        st[2ull * (nb + v)] = make_uint4(0, 0, 0, 0);
+        st[2ull * (nb + v) + 1] = make_uint4(0, 0, 0, 0);
+    }
 }
-// arena compaction: arena entries each list needs, then the move to the packed offsets
-__global__ void k_arena_need(const uint4* cur, uint32_t n, uint32_t* need) {
+// arena compaction (t: tag of the next round): arena entries each current
+// list needs, then the move to the packed offsets.  The other slot of a node
+// is emptied: it is never current again before it is rewritten, and holds no
+// offsets into the old arena.
+__global__ void k_arena_need(const uint4* st, uint32_t n, uint32_t t, uint32_t* need) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v > n) return;
-    const uint32_t len = v < n ? nl_len(cur[v]) : 0u;
+    const uint32_t len = v < n ? nl_len(load_cur(st, v, t)) : 0u;
     need[v] = len > INLINE_CAP ? len : 0u;
 }
-__global__ void k_arena_move(uint4* cur, const uint16_t* src, uint16_t* dst, const uint32_t* offs, uint32_t n) {
+__global__ void k_arena_move(uint4* st, const uint16_t* src, uint16_t* dst, const uint32_t* offs, uint32_t n, uint32_t t) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    uint4 d = cur[v];
+    uint32_t c = 0;
+    uint4 d = load_cur(st, v, t, c);
+    uint4 o = st[2ull * v + (c ^ 1u)];
+    o.x &= 0xffff0000u;                     // length 0
+    st[2ull * v + (c ^ 1u)] = o;
     const uint32_t len = nl_len(d);
     if (len <= INLINE_CAP) return;
     const uint32_t from = nl_off(d), to = offs[v];
     for (uint32_t j = 0; j < len; j++) dst[to + j] = src[from + j];
     d.y = to;
-    cur[v] = d;
+    st[2ull * v + c] = d;
 }
 __global__ void k_place_edges(const uint32_t* col, const uint32_t* wt, uint32_t E, uint32_t nb, uint32_t eb, uint2* edges) {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,18 +107,18 @@ __global__ void k_place_rcol(const uint32_t* row, const uint32_t* col, uint32_t 
     for (uint32_t e = row[v]; e < row[v + 1]; e++) prcol[eb + atomicAdd(&fill[col[e]], 1u)] = nb + v;
 }
 // nodes to reach (defender dead ends): zero energy, and the seed list
-__global__ void k_place_reach(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t nb, uint4* cur, uint32_t* seed,
-                              uint32_t* n) {
+__global__ void k_place_reach(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t nb, uint32_t t0, uint4* st,
+                              uint32_t* seed, uint32_t* n) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < V && !atk[v] && row[v] == row[v + 1]) {
-        cur[nb + v] = make_uint4(1u, 0u, 0u, 0u);
+        st[2ull * (nb + v)] = make_uint4(1u, 0u, 0u, ((t0 - 1u) & 0xffffu) << 16);   // slot 0 stays current
         seed[atomicAdd(n, 1u)] = nb + v;
     }
 }
 // jobs that still have positions on the worklist of the current round
 __global__ void k_mark_active(const Ctl* ctl, uint32_t* const* wl, const uint32_t* job_of, uint8_t* active) {
     const uint32_t r = ctl->round;
-    const uint32_t n = ctl->wl_n[r & 1u];
+    const uint32_t n = ctl->wl_n[r % 3u];
     const uint32_t* list = wl[r & 1u];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) active[job_of[list[i]]] = 1;
 }
@@ -337,7 +349,8 @@ geq_status compact_arena(geq_server* S) {
     const uint32_t n = g->V, tb = 256;
     GEQ_TRY(ensure_buf(S->place_cnt, n + 1, s));
     GEQ_TRY(ensure_buf(S->place_excl, n + 1, s));
-    k_arena_need<<<(n + 1 + tb - 1) / tb, tb, 0, s>>>(g->cur.p, n, S->place_cnt.p);
+    const uint32_t t = round_tag(g->h_ctl->round);
+    k_arena_need<<<(n + 1 + tb - 1) / tb, tb, 0, s>>>(g->st.p, n, t, S->place_cnt.p);
     size_t bytes = 0;
     GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, S->place_cnt.p, S->place_excl.p, n + 1, s), GEQ_ERR_INTERNAL);
     GEQ_TRY(ensure_buf(S->scan_tmp, bytes + 16, s));
@@ -349,7 +362,7 @@ geq_status compact_arena(geq_server* S) {
     uint16_t* p = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), 2ull * cap);
     if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("compacting the energy arena: ") + hipGetErrorString(e));
-    k_arena_move<<<(n + tb - 1) / tb, tb, 0, s>>>(g->cur.p, g->arena.p, p, S->place_excl.p, n);
+    k_arena_move<<<(n + tb - 1) / tb, tb, 0, s>>>(g->st.p, g->arena.p, p, S->place_excl.p, n, t);
     e = hipMemcpyAsync(&g->ctl.p->arena_top, &live, 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
@@ -527,7 +540,8 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         const uint32_t* bwt = Bj.wt.p + Bj.edge_start[J->bj];
         const uint8_t* batk = Bj.atk.p + Bj.pos_start[J->bj];
         // the builder's stream finished this game before it was queued as BUILT
-        k_place_nodes<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, J->eb, J->slot, g->row.p, g->atk.p, g->cur.p,
+        const uint32_t t0 = round_tag(g->h_ctl->round);   // the next round to run
+        k_place_nodes<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, J->eb, J->slot, t0, g->row.p, g->atk.p, g->st.p,
                                                            g->stamp.p, S->job_of.p);
         if (E) k_place_edges<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, bwt, E, J->nb, J->eb, g->edges.p);
         // reverse CSR of the job's range
@@ -545,12 +559,14 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         // nodes to reach: zero energy + seeds of the current round
         GEQ_TRY(ensure_buf(g->seed, V, s));
         GEQ_HIP(hipMemsetAsync(S->seed_cnt.p, 0, 4, s), GEQ_ERR_TRANSFER);
-        k_place_reach<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, g->cur.p, g->seed.p, S->seed_cnt.p);
+        k_place_reach<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, t0, g->st.p, g->seed.p, S->seed_cnt.p);
         uint32_t n_seed = 0;
         GEQ_HIP(hipMemcpyAsync(&n_seed, S->seed_cnt.p, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
         if (n_seed) {
-            ExpandArgs sx{{g->wl0.p, g->wl1.p}, g->seed.p, n_seed, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
+            ExpandArgs sx = expand_args(g);
+            sx.seed = g->seed.p;
+            sx.n_seed = n_seed;
             k_expand<<<std::max(1u, std::min(1024u, (n_seed + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx, 0u);
         }
         J->batch.reset();   // the pools hold the game now (the build is freed with its last job)
@@ -562,9 +578,8 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
     // ---- rounds over every resident job
     if (!S->resident.empty()) {
         const RoundShape shape = round_shape(g);
-        EvalArgs ea{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->cur.p, g->nxt.p, g->arena.p, g->ctl.p,
-                    g->arena_cap, g->chg.p, g->conf.max, g->stats.p};
-        ExpandArgs xa{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->cur.p, g->nxt.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
+        EvalArgs ea = eval_args(g);
+        ExpandArgs xa = expand_args(g);
         const uint32_t round0 = g->h_ctl->round;
         while (true) {
             if (!g->graph_exec || g->graph_arena != g->arena.p) {
@@ -576,16 +591,10 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
             GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
             if (!g->h_ctl->overflow) {
                 S->rounds_total += g->h_ctl->round - round0;
+                GEQ_TRY(maybe_retag(g));
                 break;
             }
-            geq_status st = grow_arena(g, g->h_ctl->arena_top);
-            if (st != GEQ_OK) return st;
-            Ctl fix = *g->h_ctl;
-            fix.overflow = 0;
-            fix.wl_n[(fix.round + 1u) & 1u] = 0;
-            GEQ_HIP(hipMemcpyAsync(g->ctl.p, &fix, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-            GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
-            *g->h_ctl = fix;
+            GEQ_TRY(recover_overflow(g));
             ea.arena = g->arena.p;
             ea.arena_cap = g->arena_cap;
         }
@@ -593,7 +602,7 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         if (S->trace)
             std::fprintf(stderr, "[geq server] t=%.1f ms step %llu: %zu resident, %zu admitted, round %u, worklist %u\n", S->now_ms(),
                          (unsigned long long)S->steps, S->resident.size(), admit.size(), g->h_ctl->round,
-                         g->h_ctl->wl_n[g->h_ctl->round & 1u]);
+                         g->h_ctl->wl_n[g->h_ctl->round % 3u]);
         // ---- retirement
         GEQ_HIP(hipMemsetAsync(S->active.p, 0, S->max_jobs, s), GEQ_ERR_TRANSFER);
         k_mark_active<<<256, 256, 0, s>>>(g->ctl.p, S->wl_ptrs.p, S->job_of.p, S->active.p);
@@ -622,7 +631,7 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
             }
             J->t_done = S->now_ms();
             S->finished.push_back(J->id);
-            k_clear_nodes<<<(J->V + 1 + tb - 1) / tb, tb, 0, s>>>(g->cur.p, J->nb, J->V + 1);
+            k_clear_nodes<<<(J->V + 1 + tb - 1) / tb, tb, 0, s>>>(g->st.p, J->nb, J->V + 1);
             S->nodes.release(J->nb, J->V + 1);
             S->edges.release(J->eb, J->E);
             S->free_slots.push_back(J->slot);

@@ -4,9 +4,10 @@
 namespace geq {
 
 // Write the members of a 4096-point bitmap (one u64 word per lane) as an
-// ascending list: inline into the node's record when it fits, else into the
-// arena.  Returns the list length (uniform).  On arena exhaustion sets
-// ctl->overflow to the round + 1 and returns ~0u.
+// ascending list: inline into the node's record (slot) when it fits, else
+// into the arena; the record is tagged with round r.  Returns the list length
+// (uniform).  On arena exhaustion sets ctl->overflow to the round + 1 and
+// returns ~0u.
 __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint4* rec, uint16_t* arena, Ctl* ctl,
                                                  uint32_t arena_cap, uint32_t r) {
     const uint32_t c = (uint32_t)__popcll(M);
@@ -22,7 +23,10 @@ __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint
             h[1 + p] = (uint16_t)(64u * lane + b);
             p++;
         }
-        if (lane == 0) h[0] = (uint16_t)total;
+        if (lane == 0) {
+            h[0] = (uint16_t)total;
+            h[7] = (uint16_t)round_tag(r);
+        }
         return total;
     }
     uint32_t base = 0;
@@ -39,7 +43,7 @@ __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint
         arena[base + p] = (uint16_t)(64u * lane + b);
         p++;
     }
-    if (lane == 0) *rec = make_uint4(total, base, 0u, 0u);
+    if (lane == 0) *rec = make_uint4(total, base, 0u, round_tag(r) << 16);
     return total;
 }
 
@@ -47,12 +51,13 @@ __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint
 // more successors than the wave's staging area) ...
 struct SuccGlobal {
     const uint2* __restrict__ edges;
-    const uint4* __restrict__ cur;
+    const uint4* __restrict__ st;
     uint32_t r0;
+    uint32_t t;                     // tag of the round
     __device__ __forceinline__ void get(uint32_t j, uint4& d, uint32_t& w) const {
         const uint2 ed = edges[r0 + j];
         w = ed.y;
-        d = cur[ed.x];
+        d = load_cur(st, ed.x, t);
     }
 };
 // ... or from the wave's LDS staging area, gathered for a run of worklist nodes at once.
@@ -67,11 +72,13 @@ struct SuccStaged {
 };
 
 // Evaluate node u (worklist slot idx): the attacker merge or the defender
-// combination, writing nxt[u] and chg[idx].  Successor j's record and update
-// come from `src`; everything else is the same for both sources.
-template <class Src>
-__device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_t* A, uint32_t* B, uint32_t* O, uint32_t lane, uint32_t idx,
-                                          uint32_t u, uint32_t deg, bool isatk, const uint4 own, const Src& src, StatAcc& acc,
+// combination, writing a changed list into `out` (u's non-current slot) and,
+// unless FUSED, the changed flag chg[idx].  Returns whether u changed
+// (uniform).  Successor j's record and update come from `src`; everything
+// else is the same for both sources.
+template <bool FUSED, class Src>
+__device__ __forceinline__ bool eval_node(const EvalArgs& a, uint32_t r, uint32_t* A, uint32_t* B, uint32_t* O, uint32_t lane, uint32_t idx,
+                                          uint32_t deg, bool isatk, const uint4 own, uint4* out, const Src& src, StatAcc& acc,
                                           uint64_t& st_bytes_lane) {
     const uint32_t maxrep = a.maxv * 0x555u;
     const uint16_t* __restrict__ arena = a.arena;
@@ -121,7 +128,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
         // changed iff a minimal budget is not already one of the node's own
         // energies (attack_minimize.wgsl:208-214: a survivor from a successor slice)
         changed = __ballot((M & ~Ow) != 0ull) != 0ull;
-        if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap, r);
+        if (changed) new_len = write_sorted(M, lane, out, a.arena, a.ctl, a.arena_cap, r);
     } else {
         // ---- defender: skipped while any successor has no energies (src/energygame.rs:789-792)
         bool any_empty = false;
@@ -152,11 +159,14 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
             const bool same = (len == own_len) && (__ballot(!all_in) == 0ull);
             changed = !same;
             if (changed) {
-                uint4* rec = &a.nxt[u];
+                uint4* rec = out;
                 if (len <= INLINE_CAP) {
                     uint16_t* h = reinterpret_cast<uint16_t*>(rec);
                     if (lane < len) h[1 + lane] = (uint16_t)inv_apply(nl_inline(d, lane), pu, maxrep);
-                    if (lane == 0) h[0] = (uint16_t)(len | (len > 1 ? UNSORTED : 0u));
+                    if (lane == 0) {
+                        h[0] = (uint16_t)(len | (len > 1 ? UNSORTED : 0u));
+                        h[7] = (uint16_t)round_tag(r);
+                    }
                     new_len = len;
                 } else {
                     uint32_t base = 0;
@@ -168,7 +178,7 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
                     } else {
                         for (uint32_t j = lane; j < len; j += 64)
                             a.arena[base + j] = (uint16_t)inv_apply(arena[nl_off(d) + j], pu, maxrep);
-                        if (lane == 0) *rec = make_uint4(len | UNSORTED, base, 0u, 0u);
+                        if (lane == 0) *rec = make_uint4(len | UNSORTED, base, 0u, round_tag(r) << 16);
                         new_len = len;
                     }
                 }
@@ -207,12 +217,12 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
             // equality test of src/energygame.rs:929 (same size, new within old)
             const bool same = (total == own_len) && (__ballot((M & ~Ow) != 0ull) == 0ull);
             changed = !same;
-            if (changed) new_len = write_sorted(M, lane, &a.nxt[u], a.arena, a.ctl, a.arena_cap, r);
+            if (changed) new_len = write_sorted(M, lane, out, a.arena, a.ctl, a.arena_cap, r);
         }
     }
     if (new_len == ~0u) changed = false;   // arena overflow: the round runs again
     if (lane == 0) {
-        a.chg[idx] = changed ? 1 : 0;
+        if (!FUSED) a.chg[idx] = changed ? 1 : 0;
         if (evaluated) {
             acc.evals++;
             acc.edges += deg;
@@ -223,20 +233,95 @@ __device__ __forceinline__ void eval_node(const EvalArgs& a, uint32_t r, uint32_
     if (evaluated) acc.bytes += st_bytes_lane;
     st_bytes_lane = 0;
     wave_sync();
+    return changed;
 }
 
-// One round's evaluations.  A wave takes 64 worklist entries at a time: one
-// lane per node loads the node's row, flag and record, then the successor
-// records of a run of these nodes are gathered into the wave's LDS staging
-// area with all lanes' loads in flight together, and the nodes are evaluated
-// one after the other from LDS.  A node with more successors than the staging
-// area holds reads them from HBM directly.
-// r: the round; n: its worklist size (0 after an arena overflow).
-// lds: the block's dynamic LDS, EVAL_LDS_PER_WAVE bytes per wave.
-__device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds, StatAcc& acc, uint32_t r, uint32_t n) {
+// Append the not-yet-queued predecessors of the lanes' nodes u with f set
+// (GPURunner::changed_nodes, src/energygame.rs:1467-1479).  The changed nodes'
+// reverse-CSR ranges are loaded together, then their predecessor slots are
+// walked 64 at a time (one lane per slot).  A predecessor is queued once per
+// target list: the first lane to swap its stamp to `stamp_val` appends it.
+// Appends go to the wave's LDS stage; a stage past 192 entries is flushed by
+// the wave with one atomic on *out_n, the rest waits for flush_appends.
+// off / rs: 64-entry LDS scratch of the wave.
+__device__ __forceinline__ void expand_nodes(const uint32_t* __restrict__ rrow, const uint32_t* __restrict__ rcol, uint32_t* stamp,
+                                             uint32_t stamp_val, uint32_t u, bool f, uint32_t lane, uint32_t* off, uint32_t* rs,
+                                             uint32_t* stage, uint32_t& cnt, uint32_t* out, uint32_t* out_n) {
+    uint32_t p0 = 0, pc = 0;
+    if (f) {
+        p0 = rrow[u];
+        pc = rrow[u + 1] - p0;
+    }
+    const uint32_t incl = wave_incl_scan(pc, lane);
+    const uint32_t total = uniform(shfl32(incl, 63));
+    if (total == 0) return;
+    off[lane] = incl - pc;
+    rs[lane] = p0;
+    wave_sync();
+    for (uint32_t s0 = 0; s0 < total; s0 += 64) {
+        const uint32_t s = s0 + lane;
+        bool win = false;
+        uint32_t w = 0;
+        if (s < total) {
+            uint32_t lo = 0, hi = 63;   // off[0] = 0 <= s: the largest lane with off <= s owns slot s
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= s) lo = mid; else hi = mid - 1;
+            }
+            w = rcol[rs[lo] + (s - off[lo])];
+            win = atomicExch(&stamp[w], stamp_val) != stamp_val;
+        }
+        const uint64_t bal = __ballot(win);
+        if (win) stage[cnt + mbcnt(bal)] = w;
+        cnt += (uint32_t)__popcll(bal);
+        if (cnt > APPEND_STAGE - 64) {
+            wave_sync();
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(out_n, cnt);
+            base = uniform(shfl32(base, 0));
+            for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
+            cnt = 0;
+            wave_sync();
+        }
+    }
+    wave_sync();   // off / rs are rewritten by the next use
+}
+
+// Block-level flush of every wave's remaining appends: one atomic per block.
+// Every thread of the block calls it.
+__device__ __forceinline__ void flush_appends(const uint32_t* stage, uint32_t cnt, uint32_t* s_cnt, uint32_t* out, uint32_t* out_n) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = threadIdx.x >> 6;
-    unsigned char* mine = lds + (size_t)wid * EVAL_LDS_PER_WAVE;
+    const uint32_t nw = blockDim.x >> 6;
+    if (lane == 0) s_cnt[wid] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < nw; w++) tot += s_cnt[w];
+        s_cnt[MAX_WAVES] = tot ? atomicAdd(out_n, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = s_cnt[MAX_WAVES];
+    for (uint32_t w = 0; w < wid; w++) base += s_cnt[w];
+    for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
+    __syncthreads();
+}
+
+// One round's evaluations.  A wave takes up to 64 worklist entries at a time:
+// one lane per node loads the node's row, flag and current record, then the
+// successor records of a run of these nodes are gathered into the wave's LDS
+// staging area with all lanes' loads in flight together, and the nodes are
+// evaluated one after the other from LDS.  A node with more successors than
+// the staging area holds reads them from HBM directly.  FUSED: the wave then
+// appends the predecessors of its changed nodes to the next worklist.
+// r: the round; n: its worklist size (0 after an arena overflow).
+// lds: the block's dynamic LDS, EVAL_(FUSED_)LDS_PER_WAVE bytes per wave.
+template <bool FUSED>
+__device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds, uint32_t* s_cnt, StatAcc& acc, uint32_t r,
+                                           uint32_t n) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wid = threadIdx.x >> 6;
+    unsigned char* mine = lds + (size_t)wid * (FUSED ? EVAL_FUSED_LDS_PER_WAVE : EVAL_LDS_PER_WAVE);
     uint4* rec = reinterpret_cast<uint4*>(mine);                          // staged successor records
     uint32_t* wts = reinterpret_cast<uint32_t*>(mine + STAGE * 16);        // staged successor updates
     uint32_t* ex = wts + STAGE;                                           // first staging slot of each node of the run
@@ -244,7 +329,12 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
     uint32_t* A = rs + 64;        // union (attacker) bitmap
     uint32_t* B = A + 128;        // one successor's images (defender)
     uint32_t* O = B + 128;        // old list of the node
+    uint32_t* stage = O + 128;    // FUSED: staged appends (APPEND_STAGE entries)
 
+    const uint32_t t = round_tag(r);
+    uint32_t* out = a.wl[(r + 1u) & 1u];
+    uint32_t* out_n = &a.ctl->wl_n[(r + 1u) % 3u];
+    uint32_t app = 0;             // FUSED: appends staged by this wave
     uint64_t st_bytes_lane = 0;
     const uint32_t nw = blockDim.x >> 6;
     const uint32_t* __restrict__ wl = a.wl[r & 1u];
@@ -255,15 +345,15 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
     for (uint32_t base = uniform((blockIdx.x * nw + wid) * chunk); base < n; base += waves * chunk) {
         const uint32_t cnt = min(chunk, n - base);
         // lane k: worklist entry base + k
-        uint32_t u = 0, r0 = 0, deg = 0;
-        bool isatk = false;
+        uint32_t u = 0, r0 = 0, deg = 0, slot = 0;
+        bool isatk = false, chg = false;
         uint4 own = make_uint4(0, 0, 0, 0);
         if (lane < cnt) {
             u = wl[base + lane];
             r0 = a.row[u];
             deg = a.row[u + 1] - r0;
             isatk = a.atk[u] != 0;
-            own = a.cur[u];
+            own = load_cur(a.st, u, t, slot);
         }
         uint32_t k0 = 0;
         while (k0 < cnt) {
@@ -274,13 +364,16 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
             const uint32_t k1 = over ? (uint32_t)__builtin_ctzll(over) : cnt;
             if (k1 == k0) {
                 // node k0 alone has more successors than the staging area
-                const uint32_t v = uniform(shfl32(u, k0));
                 const uint32_t vr0 = uniform(shfl32(r0, k0));
                 const uint32_t vdeg = uniform(shfl32(deg, k0));
                 const bool vatk = uniform(shfl32(isatk ? 1u : 0u, k0)) != 0u;
                 uint4 vown;
                 vown.x = shfl32(own.x, k0); vown.y = shfl32(own.y, k0); vown.z = shfl32(own.z, k0); vown.w = shfl32(own.w, k0);
-                eval_node(a, r, A, B, O, lane, base + k0, v, vdeg, vatk, vown, SuccGlobal{a.edges, a.cur, vr0}, acc, st_bytes_lane);
+                // the node's new list goes to its non-current slot
+                uint4* vout = a.st + 2ull * uniform(shfl32(u, k0)) + (uniform(shfl32(slot, k0)) ^ 1u);
+                const bool c = eval_node<FUSED>(a, r, A, B, O, lane, base + k0, vdeg, vatk, vown, vout,
+                                                SuccGlobal{a.edges, a.st, vr0, t}, acc, st_bytes_lane);
+                chg |= c && lane == k0;
                 k0++;
                 continue;
             }
@@ -304,8 +397,8 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
             const uint32_t s0 = lane, s1 = 64u + lane;
             const uint2 ed0 = s0 < total ? edge_of(s0) : make_uint2(0u, 0u);
             const uint2 ed1 = s1 < total ? edge_of(s1) : make_uint2(0u, 0u);
-            const uint4 rc0 = s0 < total ? a.cur[ed0.x] : make_uint4(0, 0, 0, 0);
-            const uint4 rc1 = s1 < total ? a.cur[ed1.x] : make_uint4(0, 0, 0, 0);
+            const uint4 rc0 = s0 < total ? load_cur(a.st, ed0.x, t) : make_uint4(0, 0, 0, 0);
+            const uint4 rc1 = s1 < total ? load_cur(a.st, ed1.x, t) : make_uint4(0, 0, 0, 0);
             if (s0 < total) {
                 rec[s0] = rc0;
                 wts[s0] = ed0.y;
@@ -316,17 +409,24 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
             }
             wave_sync();
             for (uint32_t k = k0; k < k1; k++) {
-                const uint32_t v = uniform(shfl32(u, k));
                 const uint32_t vdeg = uniform(shfl32(deg, k));
                 const bool vatk = uniform(shfl32(isatk ? 1u : 0u, k)) != 0u;
                 uint4 vown;
                 vown.x = shfl32(own.x, k); vown.y = shfl32(own.y, k); vown.z = shfl32(own.z, k); vown.w = shfl32(own.w, k);
+                uint4* vout = a.st + 2ull * uniform(shfl32(u, k)) + (uniform(shfl32(slot, k)) ^ 1u);
                 const uint32_t off = uniform(ex[k]);
-                eval_node(a, r, A, B, O, lane, base + k, v, vdeg, vatk, vown, SuccStaged{rec, wts, off}, acc, st_bytes_lane);
+                const bool c = eval_node<FUSED>(a, r, A, B, O, lane, base + k, vdeg, vatk, vown, vout, SuccStaged{rec, wts, off},
+                                                acc, st_bytes_lane);
+                chg |= c && lane == k;
             }
             k0 = k1;
         }
+        if (FUSED) {
+            wave_sync();   // ex / rs of the last run are reused as expansion scratch
+            expand_nodes(a.rrow, a.rcol, a.stamp, r + 2u, u, chg, lane, ex, rs, stage, app, out, out_n);
+        }
     }
+    if (FUSED) flush_appends(stage, app, s_cnt, out, out_n);
 }
 
 // Per-block statistics slot (no cross-block atomics): slot b is only ever
@@ -346,40 +446,44 @@ __device__ __forceinline__ void flush_stats(const StatAcc& acc, unsigned long lo
 }
 
 // k: the kernel's round within the current batch of rounds (round = ctl->round + k).
+template <bool FUSED>
 __global__ __launch_bounds__(MAX_BLOCK) void k_eval(EvalArgs a, uint32_t k) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
+    __shared__ uint32_t s_cnt[MAX_WAVES + 1];
     StatAcc acc;
     const uint32_t r = uniform(a.ctl->round) + k;
     const bool ovf = a.ctl->overflow != 0u;
-    const uint32_t n = ovf ? 0u : uniform(a.ctl->wl_n[r & 1u]);
-    // the previous round's list count has been consumed by its k_expand:
-    // clear it for this round's appends
-    if (!ovf && blockIdx.x == 0 && threadIdx.x == 0) a.ctl->wl_n[(r + 1u) & 1u] = 0u;
-    eval_phase(a, dyn_lds, acc, r, n);
+    const uint32_t n = ovf ? 0u : uniform(a.ctl->wl_n[r % 3u]);
+    if (!ovf && blockIdx.x == 0 && threadIdx.x == 0) {
+        // round r - 1's list count is consumed (its k_eval and k_expand are
+        // done): it becomes the count of round r + 1's appends
+        a.ctl->wl_n[(r + 2u) % 3u] = 0u;
+        if (n) a.ctl->last_nonempty = r + 1u;
+    }
+    eval_phase<FUSED>(a, dyn_lds, s_cnt, acc, r, n);
     flush_stats(acc, a.stats);
 }
+template __global__ void k_eval<false>(EvalArgs, uint32_t);
+template __global__ void k_eval<true>(EvalArgs, uint32_t);
 
-// Commit changed nodes (cur <- nxt) and append their not-yet-queued
-// predecessors to the next worklist (GPURunner::changed_nodes,
-// src/energygame.rs:1467-1479).  A wave takes 64 worklist entries: the changed
-// ones load their reverse-CSR ranges together, then the predecessor slots of
-// all of them are walked 64 at a time (one lane per slot).  A predecessor is
-// queued once per round: the first lane to swap its stamp to the round's
-// value appends it.  Appends are staged per wave in LDS; full stages are
-// flushed by the wave, the rest once per block with a single atomic.
-// r: the round; n: its worklist size (the seed count when seeding), 0 after an overflow
-__device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_stage)[256], uint32_t (*s_off)[64],
-                                             uint32_t (*s_rs)[64], uint32_t* s_cnt, uint32_t r, uint32_t n) {
+// The predecessors of round r's changed nodes form round r + 1's worklist
+// (stamp r + 2); in seeding mode the seeds' predecessors join round r's own
+// worklist (stamp r + 1, the value its entries already carry).  A wave takes
+// 64 worklist entries at a time.
+// k: the kernel's round within the current batch of rounds (round = ctl->round + k).
+__global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a, uint32_t k) {
+    __shared__ uint32_t s_stage[MAX_WAVES][APPEND_STAGE];
+    __shared__ uint32_t s_off[MAX_WAVES][64], s_rs[MAX_WAVES][64];
+    __shared__ uint32_t s_cnt[MAX_WAVES + 1];
+    const uint32_t r = uniform(a.ctl->round) + k;
+    const bool seeding = a.seed != nullptr;
+    const uint32_t n = a.ctl->overflow ? 0u : (seeding ? a.n_seed : uniform(a.ctl->wl_n[r % 3u]));
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = threadIdx.x >> 6;
-    uint32_t* stage = s_stage[wid];
-    uint32_t* off = s_off[wid];   // first predecessor slot of each lane's node
-    uint32_t* rs = s_rs[wid];     // first reverse edge of each lane's node
-    const bool seeding = a.seed != nullptr;
     const uint32_t* list = seeding ? a.seed : a.wl[r & 1u];
-    const uint32_t out_slot = seeding ? (r & 1u) : ((r + 1u) & 1u);
-    uint32_t* out = a.wl[out_slot];
-    const uint32_t stamp_val = seeding ? r + 1u : r + 2u;
+    const uint32_t tr = seeding ? r : r + 1u;   // the round whose worklist receives the appends
+    uint32_t* out = a.wl[tr & 1u];
+    uint32_t* out_n = &a.ctl->wl_n[tr % 3u];
     uint32_t cnt = 0;
     const uint32_t nw = blockDim.x >> 6;
     const uint32_t gstride = gridDim.x * nw * 64;
@@ -391,71 +495,9 @@ __device__ __forceinline__ void expand_phase(const ExpandArgs& a, uint32_t (*s_s
             u = list[i];
             f = seeding ? true : (a.chg[i] != 0);
         }
-        if (f && !seeding) a.cur[u] = a.nxt[u];
-        uint32_t p0 = 0, pc = 0;
-        if (f) {
-            p0 = a.rrow[u];
-            pc = a.rrow[u + 1] - p0;
-        }
-        const uint32_t incl = wave_incl_scan(pc, lane);
-        const uint32_t total = uniform(shfl32(incl, 63));
-        if (total == 0) continue;
-        off[lane] = incl - pc;
-        rs[lane] = p0;
-        wave_sync();
-        for (uint32_t s0 = 0; s0 < total; s0 += 64) {
-            const uint32_t s = s0 + lane;
-            bool win = false;
-            uint32_t w = 0;
-            if (s < total) {
-                uint32_t lo = 0, hi = 63;   // off[0] = 0 <= s: the largest lane with off <= s owns slot s
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi + 1) >> 1;
-                    if (off[mid] <= s) lo = mid; else hi = mid - 1;
-                }
-                w = a.rcol[rs[lo] + (s - off[lo])];
-                win = atomicExch(&a.stamp[w], stamp_val) != stamp_val;
-            }
-            const uint64_t bal = __ballot(win);
-            if (win) stage[cnt + mbcnt(bal)] = w;
-            cnt += (uint32_t)__popcll(bal);
-            if (cnt > 192) {
-                wave_sync();
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&a.ctl->wl_n[out_slot], cnt);
-                base = uniform(shfl32(base, 0));
-                for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
-                cnt = 0;
-                wave_sync();
-            }
-        }
-        wave_sync();   // off / rs are rewritten for the next entries
+        expand_nodes(a.rrow, a.rcol, a.stamp, tr + 1u, u, f, lane, s_off[wid], s_rs[wid], s_stage[wid], cnt, out, out_n);
     }
-    // block-level flush: one atomic for the whole block
-    if (lane == 0) s_cnt[wid] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (uint32_t w = 0; w < nw; w++) tot += s_cnt[w];
-        s_cnt[MAX_WAVES] = tot ? atomicAdd(&a.ctl->wl_n[out_slot], tot) : 0u;
-    }
-    __syncthreads();
-    uint32_t base = s_cnt[MAX_WAVES];
-    for (uint32_t w = 0; w < wid; w++) base += s_cnt[w];
-    for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
-    __syncthreads();
-}
-
-// k: the kernel's round within the current batch of rounds (round = ctl->round + k).
-__global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a, uint32_t k) {
-    __shared__ uint32_t s_stage[MAX_WAVES][256];
-    __shared__ uint32_t s_off[MAX_WAVES][64], s_rs[MAX_WAVES][64];
-    __shared__ uint32_t s_cnt[MAX_WAVES + 1];
-    const uint32_t r = uniform(a.ctl->round) + k;
-    const uint32_t n = a.ctl->overflow ? 0u : (a.seed ? a.n_seed : uniform(a.ctl->wl_n[r & 1u]));
-    // a round counts once its evaluation completed without an arena overflow
-    if (!a.seed && n && blockIdx.x == 0 && threadIdx.x == 0) a.ctl->rounds_done++;
-    expand_phase(a, s_stage, s_off, s_rs, s_cnt, r, n);
+    flush_appends(s_stage[wid], cnt, s_cnt, out, out_n);
 }
 
 // End of a batch of `rounds` rounds: the round advances past them, or, after
@@ -466,87 +508,15 @@ __global__ void k_advance(Ctl* ctl, uint32_t rounds) {
     ctl->round = o ? o - 1u : ctl->round + rounds;
 }
 
-// Grid-wide barrier of the cooperative k_solve launch (every block is resident).
-// Release / acquire fences at agent scope order all global memory traffic of
-// the blocks across the barrier (per-CU L1s and per-XCD L2s are not coherent
-// for plain accesses).  The wait is bounded: after `spin_limit` polls the
-// block raises ctl->error and returns false; a raised error releases every
-// other waiter too, so the kernel always terminates.
-__device__ __forceinline__ bool grid_barrier(Ctl* ctl, uint32_t nblocks, uint32_t spin_limit) {
-    __shared__ uint32_t s_ok;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t ok = 1;
-        const uint32_t gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t arrived = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > spin_limit || __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    __hip_atomic_fetch_or(&ctl->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
-                    break;
-                }
-            }
-        }
-        s_ok = ok;
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const bool ok = s_ok != 0;
-    __syncthreads();
-    return ok;
-}
-
-// The whole fixpoint in one cooperative launch: evaluation phase, grid
-// barrier, commit + expansion phase, grid barrier, per round, until the
-// worklist is empty (or max_rounds rounds ran, or the arena overflowed: the
-// host then grows it and launches again, resuming the same round).  The
-// round's control words are read once per block by thread 0 with agent-scope
-// atomic loads and broadcast through LDS, so every wave of every block takes
-// the same branch.
-__global__ __launch_bounds__(MAX_BLOCK) void k_solve(EvalArgs ea, ExpandArgs xa, uint32_t max_rounds, uint32_t spin_limit) {
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
-    __shared__ uint32_t s_stage[MAX_WAVES][256];
-    __shared__ uint32_t s_off[MAX_WAVES][64], s_rs[MAX_WAVES][64];
-    __shared__ uint32_t s_cnt[MAX_WAVES + 1];
-    __shared__ uint32_t s_ctl[3];
-    Ctl* ctl = ea.ctl;
-    const uint32_t nblocks = gridDim.x;
-    StatAcc acc;
-    for (uint32_t it = 0; it < max_rounds; it++) {
-        if (threadIdx.x == 0) {
-            const uint32_t r = __hip_atomic_load(&ctl->round, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            s_ctl[0] = r;
-            s_ctl[1] = __hip_atomic_load(&ctl->wl_n[r & 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_ctl[2] = __hip_atomic_load(&ctl->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
-                       __hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __syncthreads();
-        const uint32_t r = s_ctl[0], n = s_ctl[1], stop = s_ctl[2];
-        __syncthreads();
-        if (stop || n == 0u) break;
-        eval_phase(ea, dyn_lds, acc, r, n);
-        if (!grid_barrier(ctl, nblocks, spin_limit)) break;
-        if (threadIdx.x == 0) s_ctl[2] = __hip_atomic_load(&ctl->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const uint32_t ovf = s_ctl[2];
-        __syncthreads();
-        expand_phase(xa, s_stage, s_off, s_rs, s_cnt, r, ovf ? 0u : n);
-        if (blockIdx.x == 0 && threadIdx.x == 0 && !ovf) {
-            // round r's counter is no longer read by anyone: clear it and advance
-            if (n) __hip_atomic_fetch_add(&ctl->rounds_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl->wl_n[r & 1u], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl->round, r + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (!grid_barrier(ctl, nblocks, spin_limit)) break;
-    }
-    flush_stats(acc, ea.stats);
+// Between rounds (nothing in flight): give every node's current slot age 1
+// and its other slot age 2 at round tag t, keeping which one is current.
+__global__ void k_retag(uint4* st, uint32_t n, uint32_t t) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint4 a = st[2ull * v], b = st[2ull * v + 1];
+    const uint32_t c = cur_slot(a, b, t);
+    st[2ull * v + c] = nl_with_tag(c ? b : a, t - 1u);
+    st[2ull * v + (c ^ 1u)] = nl_with_tag(c ? a : b, t - 2u);
 }
 
 // Sum the per-block statistics slots into out[0..3].
@@ -566,18 +536,19 @@ __global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots,
     if (threadIdx.x < 4) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void k_list_lens(const uint4* cur, uint32_t n, uint32_t* lens) {
+// t: tag of the round after the last one that ran
+__global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) lens[v] = nl_len(cur[v]);
+    if (v < n) lens[v] = nl_len(load_cur(st, v, t));
 }
 
 // Copy every node's list into the packed output, ascending.  Lists produced
 // by the single-successor defender copy keep the successor's order and are
 // sorted here (insertion sort; such lists are short antichain images).
-__global__ void k_gather(const uint4* cur, const uint16_t* arena, uint32_t n, const uint32_t* offs, uint32_t* out) {
+__global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const uint4 d = cur[v];
+    const uint4 d = load_cur(st, v, t);
     const uint32_t len = nl_len(d);
     uint32_t* o = out + offs[v];
     for (uint32_t j = 0; j < len; j++) o[j] = nl_get(d, arena, j);

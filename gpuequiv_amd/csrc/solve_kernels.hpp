@@ -1,4 +1,5 @@
-// Kernels of the energy-game fixpoint (one round = k_eval + k_expand).
+// Kernels of the energy-game fixpoint (one round = k_eval + k_expand, or
+// k_eval alone with the expansion folded in).
 //
 // Round structure follows GPURunner::execute_gpu (src/energygame.rs:1481-1523):
 // every node on the round's worklist is evaluated against the state at the
@@ -16,36 +17,38 @@ constexpr int MAX_WAVES = 8;        // k_eval / k_expand take any block of 1..MA
 constexpr int MAX_BLOCK = 64 * MAX_WAVES;
 constexpr uint32_t STAT_SLOTS = 8192;   // max grid size of the statistics-writing kernels
 constexpr uint32_t STAGE = 128;         // successor records staged per wave in k_eval
+constexpr uint32_t APPEND_STAGE = 256;  // worklist appends staged per wave before a flush
 // k_eval's dynamic LDS per wave: staged records + updates, run offsets and first edges, three 4096-bit sets
 constexpr uint32_t EVAL_LDS_PER_WAVE = STAGE * 16 + STAGE * 4 + 2 * 64 * 4 + 3 * 128 * 4;
+// ... plus the staged worklist appends when k_eval also expands
+constexpr uint32_t EVAL_FUSED_LDS_PER_WAVE = EVAL_LDS_PER_WAVE + APPEND_STAGE * 4;
 
 struct Ctl {                        // small device control block, read back between round batches
-    uint32_t wl_n[2];               // worklist sizes (ping-pong by round parity)
+    uint32_t wl_n[3];               // worklist sizes: round r reads wl_n[r % 3] and appends to wl_n[(r + 1) % 3]
     uint32_t overflow;              // energy arena exhausted during k_eval: that round + 1 (0: none)
     uint32_t arena_top;             // bump pointer of the energy arena (u16 units)
     uint32_t round;                 // current round; its worklist is wl[round & 1]
-    uint32_t rounds_done;           // rounds that evaluated at least one node
-    uint32_t error;                 // k_solve: a grid barrier timed out
-    uint32_t pad0;
-    uint32_t bar_count, bar_gen;    // k_solve grid barrier (own 64-byte line)
-    uint32_t pad1[14];
+    uint32_t last_nonempty;         // 1 + the last round whose worklist was not empty
+    uint32_t pad[9];
 };
 
 // Kernels read the round's worklist and size from Ctl, so a round needs no
 // host-side state and can be captured once in a hipGraph and replayed.
 struct EvalArgs {
-    const uint32_t* wl[2];
+    uint32_t* wl[2];
     const uint32_t* row;
     const uint2* edges;             // (column index, packed update) per edge
     const uint8_t* atk;
-    const uint4* cur;               // NodeList per node: state at the start of the round
-    uint4* nxt;                     // NodeList per node: new state of changed nodes
+    uint4* st;                      // two list slots per node
     uint16_t* arena;
     Ctl* ctl;
     uint32_t arena_cap;
-    uint8_t* chg;                   // per worklist entry: node changed
+    uint8_t* chg;                   // per worklist entry: node changed (read by a separate k_expand)
     uint32_t maxv;                  // EnergyConf::max (<= 3)
     unsigned long long* stats;      // per block: node evals, edge updates, bytes, changed
+    const uint32_t* rrow;           // reverse CSR and stamps: the expansion folded into k_eval
+    const uint32_t* rcol;
+    uint32_t* stamp;
 };
 
 struct StatAcc {                    // per-lane statistics, flushed once per kernel
@@ -54,23 +57,23 @@ struct StatAcc {                    // per-lane statistics, flushed once per ker
 
 struct ExpandArgs {
     uint32_t* wl[2];
-    const uint32_t* seed;           // seeding mode: these nodes count as changed, no commit
+    const uint32_t* seed;           // seeding mode: these nodes count as changed
     uint32_t n_seed;
     const uint8_t* chg;
-    uint4* cur;
-    const uint4* nxt;
     const uint32_t* rrow;
     const uint32_t* rcol;
     uint32_t* stamp;
     Ctl* ctl;
 };
 
+// FUSED: k_eval also appends the predecessors of its changed nodes (no k_expand).
+template <bool FUSED>
 __global__ void k_eval(EvalArgs a, uint32_t k);
 __global__ void k_expand(ExpandArgs a, uint32_t k);
 __global__ void k_advance(Ctl* ctl, uint32_t rounds);
-__global__ void k_solve(EvalArgs ea, ExpandArgs xa, uint32_t max_rounds, uint32_t spin_limit);
+__global__ void k_retag(uint4* st, uint32_t n, uint32_t t);
 __global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots, unsigned long long* out);
-__global__ void k_list_lens(const uint4* cur, uint32_t n, uint32_t* lens);
-__global__ void k_gather(const uint4* cur, const uint16_t* arena, uint32_t n, const uint32_t* offs, uint32_t* out);
+__global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens);
+__global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out);
 
 }  // namespace geq

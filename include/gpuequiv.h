@@ -93,8 +93,8 @@ typedef struct {
     double eval_kernel_ms;  /* sum of k_eval durations, when profiling was asked   */
     double expand_kernel_ms;/* sum of k_expand durations, when profiling was asked */
     uint64_t eval_launches;
-    double solve_kernel_ms; /* sum of k_solve (cooperative, whole fixpoint) durations */
-    uint32_t mode;          /* 0 cooperative k_solve, 1 hipGraph rounds, 2 profiled per-kernel launches */
+    double solve_kernel_ms; /* reserved (0) */
+    uint32_t mode;          /* 1 hipGraph rounds, 2 profiled per-kernel launches */
     uint32_t pad;
 } geq_stats;
 
@@ -133,7 +133,10 @@ void geq_game_free(geq_game* game);
  * each game's result is exactly its standalone result.  There is no
  * counterpart in the reference, which solves one game per EnergyGame::run. */
 geq_status geq_batch_upload(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs, geq_game** out);
-/* reach lists per game (node ids local to the game) */
+/* reach lists per game (node ids local to the game); to_reach and n_reach both
+ * NULL: every game's standard_reach (defender positions without successors,
+ * EnergyGame::standard_reach src/energygame.rs:347-352), found on the device
+ * once and kept with the batch (also for geq_batch_autotune) */
 geq_status geq_batch_solve(geq_game* batch, const uint32_t* const* to_reach, const uint32_t* n_reach, int profile_kernels,
                            geq_stats* stats);
 /* outs: n_games results, each freed with geq_energies_free */
@@ -144,15 +147,16 @@ geq_status geq_batch_energies(geq_game* batch, geq_energies* outs);
  * block), their grid sizes (blocks per CU), the rounds captured per replayed
  * hipGraph, and the energy-arena capacity (u16 entries; it only grows).
  * geq_*_autotune times candidate shapes on the game itself (median device time
- * of `repeats` solves each, coordinate search from the current shape), keeps
- * the fastest and sizes the arena for the largest solve seen.  Every shape
- * computes the same result.  No counterpart in the reference (fixed WGSL
- * workgroup sizes). */
+ * of `repeats` solves each, coordinate search from the current shape with a
+ * separate k_expand and from the same shape with the expansion inside k_eval),
+ * keeps the fastest and sizes the arena for the largest solve seen.  Every
+ * shape computes the same result.  No counterpart in the reference (fixed
+ * WGSL workgroup sizes). */
 typedef struct {
-    uint32_t waves_per_block;       /* 1..8 (default 4) */
-    uint32_t eval_blocks_per_cu;    /* 1..64 (default 8) */
-    uint32_t expand_blocks_per_cu;  /* 1..64 (default 4) */
-    uint32_t rounds_per_launch;     /* 1..256 (default 4) */
+    uint32_t waves_per_block;       /* 1..8 (default 8) */
+    uint32_t eval_blocks_per_cu;    /* 1..64 (default 8; capped at the blocks a CU holds at once) */
+    uint32_t expand_blocks_per_cu;  /* 0..64 (default 0: no k_expand, k_eval queues the predecessors itself) */
+    uint32_t rounds_per_launch;     /* 1..256 (default 8) */
     uint64_t arena_entries;
 } geq_tune_params;
 typedef struct {

@@ -27,6 +27,22 @@ def test_batch_matches_oracle_per_game():
     b.close()
 
 
+def test_explicit_reach_lists_equal_device_standard_reach():
+    # GameBatch(graphs) seeds every game with its standard_reach found on the
+    # device; the same lists passed explicitly give the same solve
+    from gpuequiv_amd.batch import standard_reach
+    graphs = [workloads.random_game(v, 40 + v) for v in (5000, 10, 800, 15000)]
+    a = GameBatch(graphs)
+    sa = a.solve()
+    b = GameBatch(graphs, reach=[standard_reach(g) for g in graphs])
+    sb = b.solve()
+    for (o0, d0), (o1, d1) in zip(a.raw_energies(), b.raw_energies()):
+        assert np.array_equal(o0, o1) and np.array_equal(d0, d1)
+    assert (sa["rounds"], sa["node_evals"], sa["edge_updates"]) == (sb["rounds"], sb["node_evals"], sb["edge_updates"])
+    a.close()
+    b.close()
+
+
 def test_batch_equals_plain_path():
     graphs = [workloads.random_game(v, 77 + v) for v in (3000, 100, 20000, 1, 700)]
     b = GameBatch(graphs)
@@ -42,7 +58,8 @@ def test_batch_equals_plain_path():
     b.close()
 
 
-@pytest.mark.parametrize("waves,eval_bpc,expand_bpc,rounds", [(1, 1, 1, 1), (2, 32, 16, 3), (8, 8, 4, 16), (8, 1, 64, 32)])
+@pytest.mark.parametrize("waves,eval_bpc,expand_bpc,rounds", [(1, 1, 1, 1), (2, 32, 16, 3), (8, 8, 4, 16), (8, 1, 64, 32),
+                                                             (1, 1, 0, 1), (4, 8, 0, 8), (8, 2, 0, 32)])
 def test_launch_shapes_give_identical_results(waves, eval_bpc, expand_bpc, rounds):
     print("launch shape", waves, eval_bpc, expand_bpc, rounds, flush=True)
     graphs = [workloads.random_game(v, 300 + v) for v in (4000, 50, 12000, 700)]
@@ -83,4 +100,6 @@ def test_set_tune_rejects_out_of_range():
         b.set_tune(waves_per_block=9)
     with pytest.raises(Error):
         b.set_tune(rounds_per_launch=0)
+    with pytest.raises(Error):
+        b.set_tune(expand_blocks_per_cu=65)
     b.close()

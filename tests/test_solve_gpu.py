@@ -176,7 +176,7 @@ def test_small_confs_match_oracle():
     assert _canon(out) == H.canon_lists(offs, data)
 
 
-def _solve_raw(graph, profile):
+def _solve_raw(graph, profile, **tune):
     import ctypes as C
     from gpuequiv_amd import _lib
     lib = _lib.lib()
@@ -186,6 +186,12 @@ def _solve_raw(graph, profile):
     h = C.c_void_p()
     _lib.check(lib.geq_game_upload(ctx.handle, C.byref(gs), C.byref(h)))
     try:
+        if tune:
+            p = _lib.geq_tune_params()
+            _lib.check(lib.geq_game_get_tune(h, C.byref(p)))
+            for k, v in tune.items():
+                setattr(p, k, int(v))
+            _lib.check(lib.geq_game_set_tune(h, C.byref(p)))
         reach = np.ascontiguousarray(H.standard_reach(graph), dtype=np.uint32)
         st = _lib.geq_stats()
         _lib.check(lib.geq_game_solve(h, _lib.u32p(reach), len(reach), None, profile, C.byref(st)))
@@ -228,3 +234,42 @@ def test_all_launch_modes_agree(monkeypatch):
     assert sc["mode"] == 2
     assert a == c
     assert sa["rounds"] == sc["rounds"] and sa["node_evals"] == sc["node_evals"] and sa["alg_bytes"] == sc["alg_bytes"]
+
+
+@pytest.mark.parametrize("rounds", [1, 5])
+def test_fused_expansion_matches_separate_kernel(rounds):
+    # expand_blocks_per_cu = 0: k_eval queues the predecessors itself
+    g = workloads.random_game(20000, 23)
+    a, sa = _solve_raw(g, 0, expand_blocks_per_cu=4, rounds_per_launch=rounds)
+    b, sb = _solve_raw(g, 0, expand_blocks_per_cu=0, rounds_per_launch=rounds)
+    assert a == b
+    assert (sa["rounds"], sa["node_evals"], sa["edge_updates"], sa["alg_bytes"]) == \
+        (sb["rounds"], sb["node_evals"], sb["edge_updates"], sb["alg_bytes"])
+    c, sc = _solve_raw(g, 1, expand_blocks_per_cu=0)
+    assert c == a and sc["rounds"] == sa["rounds"] and sc["eval_kernel_ms"] > 0
+    o, d, so = H.oracle_solve_game(g, H.standard_reach(g))
+    assert a == H.canon_lists(o, d) and sa["rounds"] == so["rounds"]
+
+
+@pytest.mark.parametrize("expand_bpc", [4, 0])
+def test_arena_regrow_fused_and_separate(monkeypatch, expand_bpc):
+    # an overflow in the middle of a batch of rounds: the interrupted round runs
+    # again from its start (its half-written slots and queued predecessors are dropped)
+    g = workloads.random_game(3000, 8, mean_degree=8.0, p_ident=0.2, p_dec=0.6)
+    ref, sref = _solve_raw(g, 0)
+    monkeypatch.setenv("GEQ_ARENA_ENTRIES", "64")
+    small, ssmall = _solve_raw(g, 0, expand_blocks_per_cu=expand_bpc, rounds_per_launch=3)
+    assert small == ref
+    # the interrupted round's evaluations ran and are counted too
+    assert ssmall["rounds"] == sref["rounds"] and ssmall["node_evals"] >= sref["node_evals"]
+
+
+@pytest.mark.parametrize("expand_bpc,rounds", [(4, 1), (0, 1), (0, 2)])
+def test_slot_retag_keeps_results(monkeypatch, expand_bpc, rounds):
+    # re-tag every node's two list slots after every batch of rounds (normally
+    # every 16384 rounds): which slot is current must survive it
+    g = workloads.random_game(8000, 31, mean_degree=6.0)
+    ref, sref = _solve_raw(g, 0)
+    monkeypatch.setenv("GEQ_RETAG_ROUNDS", "1")
+    got, sgot = _solve_raw(g, 0, expand_blocks_per_cu=expand_bpc, rounds_per_launch=rounds)
+    assert got == ref and sgot["rounds"] == sref["rounds"]
