@@ -245,7 +245,7 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     if ((e = gm->wl0.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->wl1.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->chg.alloc(V)) != hipSuccess) return oom(e, "changed flags");
-    if ((e = gm->stamp.alloc(V)) != hipSuccess) return oom(e, "stamps");
+    if ((e = gm->queued.alloc(2ull * queue_words(V))) != hipSuccess) return oom(e, "queued bitmaps");
     if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
     if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
     if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
@@ -366,10 +366,10 @@ uint32_t eval_blocks_resident(bool fused, uint32_t w) {
 
 EvalArgs eval_args(geq_game* g) {
     return EvalArgs{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->st.p, g->arena.p, g->ctl.p, g->arena_cap,
-                    g->chg.p, g->conf.max, g->stats.p, g->rrow.p, g->rcol.p, g->stamp.p};
+                    g->chg.p, g->conf.max, g->stats.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V)};
 }
 ExpandArgs expand_args(geq_game* g) {
-    return ExpandArgs{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->rrow.p, g->rcol.p, g->stamp.p, g->ctl.p};
+    return ExpandArgs{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V), g->ctl.p};
 }
 
 geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape) {
@@ -401,8 +401,8 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
 // round that overflowed, nothing in flight): grow the arena and reset what
 // the interrupted round left behind, so that it runs again from its start.
 // Its half-written slots carry its own tag and are ignored by the rerun; its
-// appends to the next worklist are dropped, and the stamps are cleared so the
-// rerun can queue the same predecessors again.
+// appends to the next worklist are dropped with their queued bits (the rerun
+// clears the bits of its own list again as it evaluates it).
 geq_status recover_overflow(geq_game* g) {
     hipStream_t s = g->ctx->stream;
     geq_status st = grow_arena(g, g->h_ctl->arena_top);
@@ -410,7 +410,7 @@ geq_status recover_overflow(geq_game* g) {
     Ctl fix = *g->h_ctl;
     fix.overflow = 0;
     fix.wl_n[(fix.round + 1u) % 3u] = 0;
-    GEQ_HIP(hipMemsetAsync(g->stamp.p, 0, 4ull * g->V, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipMemsetAsync(g->queued.p, 0, 8ull * queue_words(g->V), s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemcpyAsync(g->ctl.p, &fix, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     *g->h_ctl = fix;
@@ -523,7 +523,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         GEQ_HIP(hipMemsetAsync(g->st.p, 0, 32ull * V, s), GEQ_ERR_TRANSFER);
         if (n_reach) k_seed_zero<<<(n_reach + 255) / 256, 256, 0, s>>>(g->st.p, seed, n_reach);
     }
-    GEQ_HIP(hipMemsetAsync(g->stamp.p, 0, 4ull * V, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipMemsetAsync(g->queued.p, 0, 8ull * queue_words(V), s), GEQ_ERR_TRANSFER);
     Ctl c0{};
     c0.arena_top = (uint32_t)h_arena.size();
     g->retag_round = 0;

@@ -104,7 +104,7 @@ geq_status geq::alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_gam
     if ((e = gm->wl0.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->wl1.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->chg.alloc(V)) != hipSuccess) return oom(e, "changed flags");
-    if ((e = gm->stamp.alloc(V)) != hipSuccess) return oom(e, "stamps");
+    if ((e = gm->queued.alloc(2ull * queue_words(V))) != hipSuccess) return oom(e, "queued bitmaps");
     if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
     if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
     gm->arena_cap = initial_arena_entries(E, arena_entries);

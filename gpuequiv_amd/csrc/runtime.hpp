@@ -103,7 +103,8 @@ struct geq_game {
     geq_ctx* ctx = nullptr;
     geq::ConfInfo conf{};
     uint32_t V = 0, E = 0;
-    geq::DevBuf<uint32_t> row, rrow, rcol, wl0, wl1, stamp, seed, lens, offs, out;
+    geq::DevBuf<uint32_t> row, rrow, rcol, wl0, wl1, seed, lens, offs, out;
+    geq::DevBuf<uint32_t> queued;     // two bitmaps of queue_words(V) words
     geq::DevBuf<uint32_t> std_seed;   // standard_reach, once computed
     uint32_t std_n = 0;
     bool std_ready = false;

@@ -36,21 +36,21 @@ using namespace geq;
 
 namespace {
 
-// per-node bytes of the pools (row, rrow, atk, two list slots, wl x2, chg, stamp, job_of) and per-edge bytes (edges, rcol)
-constexpr uint64_t NODE_BYTES = 4 + 4 + 1 + 32 + 4 + 4 + 1 + 4 + 4;
+// per-node bytes of the pools (row, rrow, atk, two list slots, wl x2, chg, job_of, queued bits rounded up) and
+// per-edge bytes (edges, rcol)
+constexpr uint64_t NODE_BYTES = 4 + 4 + 1 + 32 + 4 + 4 + 1 + 4 + 1;
 constexpr uint64_t EDGE_BYTES = 8 + 4;
 
 // Place a built game at node base nb / edge base eb of the pools.  Its
 // slots start empty, tagged so that slot 0 is current from round tag t0 on.
 __global__ void k_place_nodes(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t nb, uint32_t eb, uint32_t job,
-                              uint32_t t0, uint32_t* prow, uint8_t* patk, uint4* st, uint32_t* stamp, uint32_t* job_of) {
+                              uint32_t t0, uint32_t* prow, uint8_t* patk, uint4* st, uint32_t* job_of) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v > V) return;
     prow[nb + v] = eb + row[v];             // v == V: the row end of the job (padding slot)
     patk[nb + v] = v < V ? atk[v] : (uint8_t)1;
     st[2ull * (nb + v)] = make_uint4(0, 0, 0, ((t0 - 1u) & 0xffffu) << 16);
     st[2ull * (nb + v) + 1] = make_uint4(0, 0, 0, ((t0 - 2u) & 0xffffu) << 16);
-    stamp[nb + v] = 0;
     job_of[nb + v] = job;
 }
 // a released range holds no lists (the arena compaction walks every slot)
@@ -451,6 +451,8 @@ geq_status geq_server_create(geq_ctx* ctx, const geq_server_opts* opts, geq_serv
     GEQ_HIP(hipMemcpyAsync(S->pool->ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemsetAsync(S->pool->stats.p, 0, (4ull * STAT_SLOTS + 4) * 8, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemsetAsync(S->active.p, 0, S->max_jobs, s), GEQ_ERR_TRANSFER);
+    // no node is queued; a node's bits are clear again whenever it is on no list
+    GEQ_HIP(hipMemsetAsync(S->pool->queued.p, 0, 8ull * queue_words(S->pool->V), s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     S->h_active.assign(S->max_jobs, 0);
     S->nodes.init((uint32_t)V);
@@ -542,7 +544,7 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         // the builder's stream finished this game before it was queued as BUILT
         const uint32_t t0 = round_tag(g->h_ctl->round);   // the next round to run
         k_place_nodes<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, J->eb, J->slot, t0, g->row.p, g->atk.p, g->st.p,
-                                                           g->stamp.p, S->job_of.p);
+                                                           S->job_of.p);
         if (E) k_place_edges<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, bwt, E, J->nb, J->eb, g->edges.p);
         // reverse CSR of the job's range
         GEQ_TRY(ensure_buf(S->place_cnt, V + 1, s));

@@ -236,22 +236,17 @@ __device__ __forceinline__ bool eval_node(const EvalArgs& a, uint32_t r, uint32_
     return changed;
 }
 
-// Append the not-yet-queued predecessors of the lanes' nodes u with f set
-// (GPURunner::changed_nodes, src/energygame.rs:1467-1479).  The changed nodes'
-// reverse-CSR ranges are loaded together, then their predecessor slots are
-// walked 64 at a time (one lane per slot).  A predecessor is queued once per
-// target list: the first lane to swap its stamp to `stamp_val` appends it.
-// Appends go to the wave's LDS stage; a stage past 192 entries is flushed by
-// the wave with one atomic on *out_n, the rest waits for flush_appends.
-// off / rs: 64-entry LDS scratch of the wave.
-__device__ __forceinline__ void expand_nodes(const uint32_t* __restrict__ rrow, const uint32_t* __restrict__ rcol, uint32_t* stamp,
-                                             uint32_t stamp_val, uint32_t u, bool f, uint32_t lane, uint32_t* off, uint32_t* rs,
-                                             uint32_t* stage, uint32_t& cnt, uint32_t* out, uint32_t* out_n) {
-    uint32_t p0 = 0, pc = 0;
-    if (f) {
-        p0 = rrow[u];
-        pc = rrow[u + 1] - p0;
-    }
+// Append the not-yet-queued predecessors of the lanes' nodes
+// (GPURunner::changed_nodes, src/energygame.rs:1467-1479): lane k's node has
+// its pc predecessors at rcol[p0 ..] (pc = 0: nothing to queue).  The
+// predecessor slots of all lanes are walked 64 at a time (one lane per slot).
+// A predecessor is queued once per list: the lane whose atomicOr sets its bit
+// in the target list's bitmap q appends it.  Appends go to the wave's LDS
+// stage; a stage past 192 entries is flushed by the wave with one atomic on
+// *out_n, the rest waits for flush_appends.  off / rs: 64-entry LDS scratch.
+__device__ __forceinline__ void expand_nodes(const uint32_t* __restrict__ rcol, uint32_t* q, uint32_t p0, uint32_t pc, uint32_t lane,
+                                             uint32_t* off, uint32_t* rs, uint32_t* stage, uint32_t& cnt, uint32_t* out,
+                                             uint32_t* out_n) {
     const uint32_t incl = wave_incl_scan(pc, lane);
     const uint32_t total = uniform(shfl32(incl, 63));
     if (total == 0) return;
@@ -269,7 +264,8 @@ __device__ __forceinline__ void expand_nodes(const uint32_t* __restrict__ rrow, 
                 if (off[mid] <= s) lo = mid; else hi = mid - 1;
             }
             w = rcol[rs[lo] + (s - off[lo])];
-            win = atomicExch(&stamp[w], stamp_val) != stamp_val;
+            const uint32_t bit = 1u << (w & 31u);
+            win = (atomicOr(&q[w >> 5], bit) & bit) == 0u;
         }
         const uint64_t bal = __ballot(win);
         if (win) stage[cnt + mbcnt(bal)] = w;
@@ -345,7 +341,7 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
     for (uint32_t base = uniform((blockIdx.x * nw + wid) * chunk); base < n; base += waves * chunk) {
         const uint32_t cnt = min(chunk, n - base);
         // lane k: worklist entry base + k
-        uint32_t u = 0, r0 = 0, deg = 0, slot = 0;
+        uint32_t u = 0, r0 = 0, deg = 0, slot = 0, p0 = 0, pc = 0;
         bool isatk = false, chg = false;
         uint4 own = make_uint4(0, 0, 0, 0);
         if (lane < cnt) {
@@ -354,6 +350,12 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
             deg = a.row[u + 1] - r0;
             isatk = a.atk[u] != 0;
             own = load_cur(a.st, u, t, slot);
+            if (FUSED) {            // the predecessor range, in case u changes
+                p0 = a.rrow[u];
+                pc = a.rrow[u + 1] - p0;
+            }
+            // u leaves this round's list: clear its queued bit
+            atomicAnd(&a.queued[(r & 1u) * a.qwords + (u >> 5)], ~(1u << (u & 31u)));
         }
         uint32_t k0 = 0;
         while (k0 < cnt) {
@@ -423,7 +425,7 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
         }
         if (FUSED) {
             wave_sync();   // ex / rs of the last run are reused as expansion scratch
-            expand_nodes(a.rrow, a.rcol, a.stamp, r + 2u, u, chg, lane, ex, rs, stage, app, out, out_n);
+            expand_nodes(a.rcol, a.queued + ((r + 1u) & 1u) * a.qwords, p0, chg ? pc : 0u, lane, ex, rs, stage, app, out, out_n);
         }
     }
     if (FUSED) flush_appends(stage, app, s_cnt, out, out_n);
@@ -466,10 +468,9 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_eval(EvalArgs a, uint32_t k) {
 template __global__ void k_eval<false>(EvalArgs, uint32_t);
 template __global__ void k_eval<true>(EvalArgs, uint32_t);
 
-// The predecessors of round r's changed nodes form round r + 1's worklist
-// (stamp r + 2); in seeding mode the seeds' predecessors join round r's own
-// worklist (stamp r + 1, the value its entries already carry).  A wave takes
-// 64 worklist entries at a time.
+// The predecessors of round r's changed nodes form round r + 1's worklist; in
+// seeding mode the seeds' predecessors join round r's own worklist.  A wave
+// takes 64 worklist entries at a time.
 // k: the kernel's round within the current batch of rounds (round = ctl->round + k).
 __global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a, uint32_t k) {
     __shared__ uint32_t s_stage[MAX_WAVES][APPEND_STAGE];
@@ -489,13 +490,15 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a, uint32_t k) 
     const uint32_t gstride = gridDim.x * nw * 64;
     for (uint32_t b = (blockIdx.x * nw + wid) * 64; b < n; b += gstride) {
         const uint32_t i = b + lane;
-        uint32_t u = 0;
-        bool f = false;
+        uint32_t p0 = 0, pc = 0;
         if (i < n) {
-            u = list[i];
-            f = seeding ? true : (a.chg[i] != 0);
+            const uint32_t u = list[i];
+            if (seeding || a.chg[i] != 0) {
+                p0 = a.rrow[u];
+                pc = a.rrow[u + 1] - p0;
+            }
         }
-        expand_nodes(a.rrow, a.rcol, a.stamp, tr + 1u, u, f, lane, s_off[wid], s_rs[wid], s_stage[wid], cnt, out, out_n);
+        expand_nodes(a.rcol, a.queued + (tr & 1u) * a.qwords, p0, pc, lane, s_off[wid], s_rs[wid], s_stage[wid], cnt, out, out_n);
     }
     flush_appends(s_stage[wid], cnt, s_cnt, out, out_n);
 }

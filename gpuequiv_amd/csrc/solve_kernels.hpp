@@ -46,9 +46,10 @@ struct EvalArgs {
     uint8_t* chg;                   // per worklist entry: node changed (read by a separate k_expand)
     uint32_t maxv;                  // EnergyConf::max (<= 3)
     unsigned long long* stats;      // per block: node evals, edge updates, bytes, changed
-    const uint32_t* rrow;           // reverse CSR and stamps: the expansion folded into k_eval
+    const uint32_t* rrow;           // reverse CSR: the expansion folded into k_eval
     const uint32_t* rcol;
-    uint32_t* stamp;
+    uint32_t* queued;               // two bitmaps of qwords words: node queued on the list of a round parity
+    uint32_t qwords;
 };
 
 struct StatAcc {                    // per-lane statistics, flushed once per kernel
@@ -62,9 +63,16 @@ struct ExpandArgs {
     const uint8_t* chg;
     const uint32_t* rrow;
     const uint32_t* rcol;
-    uint32_t* stamp;
+    uint32_t* queued;
+    uint32_t qwords;
     Ctl* ctl;
 };
+
+// Words of one "queued" bitmap for V nodes.  Bit v of bitmap (r & 1) is set
+// while node v is on round r's worklist: set by the append that queues it,
+// cleared by round r's evaluation of v (so the bitmap of a parity is empty
+// again before the round two later starts appending to it).
+__host__ __device__ inline uint32_t queue_words(uint32_t V) { return (V + 31u) / 32u; }
 
 // FUSED: k_eval also appends the predecessors of its changed nodes (no k_expand).
 template <bool FUSED>
