@@ -175,6 +175,23 @@ __device__ __forceinline__ uint32_t shfl32(uint32_t v, uint32_t src) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
 }
 
+// the value of lane `src` (uniform) in every lane, read by the scalar unit
+__device__ __forceinline__ uint32_t lane_val(uint32_t v, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)src);
+}
+
+// DPP-moved value (the control below), 0 where the source lane falls outside
+// the row; as the operand of an OR it folds into one v_or_b32_dpp.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_z(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+constexpr int DPP_QUAD_SHR1 = 0x90;    // quad_perm [0,0,1,2]: lane i of a quad reads lane max(i - 1, 0)
+constexpr int DPP_QUAD_SHR2 = 0x40;    // quad_perm [0,0,0,1]: lane i of a quad reads lane max(i - 2, 0)
+constexpr int DPP_ROW_SHR1 = 0x111;    // row_shr:1
+constexpr int DPP_ROW_SHR4 = 0x114;    // row_shr:4
+constexpr int DPP_ROW_SHR8 = 0x118;    // row_shr:8
+
 // Value of lane (lane - n) within the lane's row of 16 (DPP row_shr:n, no
 // LDS traffic); 0 for the first n lanes of a row.  n in {1, 2, 4, 8}.
 template <int N>
@@ -193,7 +210,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
     v += row_shr<4>(v);
     v += row_shr<8>(v);
     // row sums: lane 15 of each row holds it
-    const uint32_t r0 = shfl32(v, 15), r1 = shfl32(v, 31), r2 = shfl32(v, 47);
+    const uint32_t r0 = lane_val(v, 15), r1 = lane_val(v, 31), r2 = lane_val(v, 47);
     const uint32_t row = lane >> 4;
     if (row >= 1) v += r0;
     if (row >= 2) v += r1;
@@ -211,25 +228,33 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // 4096-point bitmap operations on one u64 word per lane.
 // Point p = 64 * lane + b has coordinates e_k = (p >> 2k) & 3.
 // ---------------------------------------------------------------------------
-constexpr uint64_t D0_LT3 = 0x7777777777777777ull, D0_LT2 = 0x3333333333333333ull;
-constexpr uint64_t D1_LT3 = 0x0FFF0FFF0FFF0FFFull, D1_LT2 = 0x00FF00FF00FF00FFull;
-constexpr uint64_t D2_LT3 = 0x0000FFFFFFFFFFFFull, D2_LT2 = 0x00000000FFFFFFFFull;
 
-// Upward closure: every point >= some member becomes a member.
+// Upward closure: every point >= some member becomes a member.  Each step
+// ORs in the set shifted up by one or two values of one dimension.
+//  * dims 0 and 1 stay inside 16-bit groups: the two 32-bit halves are shifted
+//    independently;
+//  * dim 2 moves whole 16-bit groups (lo.low, lo.high, hi.low, hi.high);
+//  * dim 3 (lanes of a quad) and dim 4 (quads of a row of 16) are DPP quad
+//    permutations and zero-filling row shifts, folded into the OR (a lane that
+//    reads itself adds nothing);
+//  * dim 5 (rows) crosses rows: ds_bpermute, masked below row 1 / row 2.
 __device__ __forceinline__ uint64_t up_close(uint64_t w, uint32_t lane) {
-    w |= (w & D0_LT3) << 1;
-    w |= (w & D0_LT2) << 2;
-    w |= (w & D1_LT3) << 4;
-    w |= (w & D1_LT2) << 8;
-    w |= (w & D2_LT3) << 16;
-    w |= (w & D2_LT2) << 32;
-    // dims 3 and 4 live within a row of 16 lanes (DPP shifts), dim 5 across rows
-    const uint32_t e3 = lane & 3u, e4 = (lane >> 2) & 3u, e5 = lane >> 4;
+    uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+    lo |= (lo & 0x77777777u) << 1;  hi |= (hi & 0x77777777u) << 1;
+    lo |= (lo & 0x33333333u) << 2;  hi |= (hi & 0x33333333u) << 2;
+    lo |= (lo & 0x0FFF0FFFu) << 4;  hi |= (hi & 0x0FFF0FFFu) << 4;
+    lo |= (lo & 0x00FF00FFu) << 8;  hi |= (hi & 0x00FF00FFu) << 8;
+    const uint32_t lo1 = lo | (lo << 16);                  // group 1 |= group 0
+    const uint32_t hi1 = hi | (hi << 16) | (lo >> 16);     // groups 2, 3 |= groups 1, 2
+    lo = lo1;
+    hi = hi1 | lo1;                                        // groups 2, 3 |= groups 0, 1
+    lo |= dpp_z<DPP_QUAD_SHR1>(lo);  hi |= dpp_z<DPP_QUAD_SHR1>(hi);
+    lo |= dpp_z<DPP_QUAD_SHR2>(lo);  hi |= dpp_z<DPP_QUAD_SHR2>(hi);
+    lo |= dpp_z<DPP_ROW_SHR4>(lo);   hi |= dpp_z<DPP_ROW_SHR4>(hi);
+    lo |= dpp_z<DPP_ROW_SHR8>(lo);   hi |= dpp_z<DPP_ROW_SHR8>(hi);
+    w = ((uint64_t)hi << 32) | lo;
+    const uint32_t e5 = lane >> 4;
     uint64_t t;
-    t = row_shr64<1>(w);        if (e3 >= 1) w |= t;
-    t = row_shr64<2>(w);        if (e3 >= 2) w |= t;
-    t = row_shr64<4>(w);        if (e4 >= 1) w |= t;
-    t = row_shr64<8>(w);        if (e4 >= 2) w |= t;
     t = shfl_up64(w, 16, lane); if (e5 >= 1) w |= t;
     t = shfl_up64(w, 32, lane); if (e5 >= 2) w |= t;
     return w;
@@ -237,12 +262,18 @@ __device__ __forceinline__ uint64_t up_close(uint64_t w, uint32_t lane) {
 
 // Minimal points of an upward-closed set: x is minimal iff no x - e_k is a member.
 __device__ __forceinline__ uint64_t minimal_of(uint64_t w, uint32_t lane) {
-    uint64_t nm = ((w & D0_LT3) << 1) | ((w & D1_LT3) << 4) | ((w & D2_LT3) << 16);
-    const uint32_t e3 = lane & 3u, e4 = (lane >> 2) & 3u, e5 = lane >> 4;
-    uint64_t t;
-    t = row_shr64<1>(w);        if (e3 >= 1) nm |= t;
-    t = row_shr64<4>(w);        if (e4 >= 1) nm |= t;
-    t = shfl_up64(w, 16, lane); if (e5 >= 1) nm |= t;
+    const uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+    uint32_t nlo = ((lo & 0x77777777u) << 1) | ((lo & 0x0FFF0FFFu) << 4) | (lo << 16);
+    uint32_t nhi = ((hi & 0x77777777u) << 1) | ((hi & 0x0FFF0FFFu) << 4) | (hi << 16) | (lo >> 16);
+    // dim 3: the lane below, within the quad (read by every lane, kept by lanes 1..3 of a quad)
+    const uint32_t tlo = dpp_z<DPP_ROW_SHR1>(lo), thi = dpp_z<DPP_ROW_SHR1>(hi);
+    nlo |= (lane & 3u) ? tlo : 0u;
+    nhi |= (lane & 3u) ? thi : 0u;
+    nlo |= dpp_z<DPP_ROW_SHR4>(lo);  // dim 4: four lanes below, 0 in the row's first quad
+    nhi |= dpp_z<DPP_ROW_SHR4>(hi);
+    uint64_t nm = ((uint64_t)nhi << 32) | nlo;
+    const uint64_t t = shfl_up64(w, 16, lane);
+    if (lane >= 16u) nm |= t;        // dim 5: the row below
     return w & ~nm;
 }
 

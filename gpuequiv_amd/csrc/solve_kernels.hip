@@ -12,7 +12,7 @@ __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint
                                                  uint32_t arena_cap, uint32_t r) {
     const uint32_t c = (uint32_t)__popcll(M);
     const uint32_t incl = wave_incl_scan(c, lane);
-    const uint32_t total = uniform(shfl32(incl, 63));
+    const uint32_t total = lane_val(incl, 63);
     uint32_t p = incl - c;
     if (total <= INLINE_CAP) {
         uint16_t* h = reinterpret_cast<uint16_t*>(rec);
@@ -31,7 +31,7 @@ __device__ __forceinline__ uint32_t write_sorted(uint64_t M, uint32_t lane, uint
     }
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&ctl->arena_top, total);
-    base = uniform(shfl32(base, 0));
+    base = lane_val(base, 0);
     if ((uint64_t)base + total > arena_cap) {
         if (lane == 0) atomicCAS(&ctl->overflow, 0u, r + 1u);
         return ~0u;
@@ -115,9 +115,9 @@ __device__ __forceinline__ bool eval_node(const EvalArgs& a, uint32_t r, uint32_
             while (longm) {
                 const uint32_t s = (uint32_t)__builtin_ctzll(longm);
                 longm &= longm - 1;
-                const InvUpd lpu = prep_update(uniform(shfl32(w, s)));
-                const uint32_t loff = uniform(shfl32(d.y, s));
-                const uint32_t llen = uniform(shfl32(len, s));
+                const InvUpd lpu = prep_update(lane_val(w, s));
+                const uint32_t loff = lane_val(d.y, s);
+                const uint32_t llen = lane_val(len, s);
                 for (uint32_t j = lane; j < llen; j += 64) bm_set(A, inv_apply(arena[loff + j], lpu, maxrep));
             }
         }
@@ -171,7 +171,7 @@ __device__ __forceinline__ bool eval_node(const EvalArgs& a, uint32_t r, uint32_
                 } else {
                     uint32_t base = 0;
                     if (lane == 0) base = atomicAdd(&a.ctl->arena_top, len);
-                    base = uniform(shfl32(base, 0));
+                    base = lane_val(base, 0);
                     if ((uint64_t)base + len > a.arena_cap) {
                         if (lane == 0) atomicCAS(&a.ctl->overflow, 0u, r + 1u);
                         new_len = ~0u;
@@ -196,12 +196,12 @@ __device__ __forceinline__ bool eval_node(const EvalArgs& a, uint32_t r, uint32_
                 }
                 const uint32_t nk = min(64u, deg - base);
                 for (uint32_t s = 0; s < nk; s++) {
-                    const InvUpd spu = prep_update(uniform(shfl32(w, s)));
+                    const InvUpd spu = prep_update(lane_val(w, s));
                     uint4 ds;
-                    ds.x = uniform(shfl32(d.x, s));
-                    ds.y = uniform(shfl32(d.y, s));
-                    ds.z = uniform(shfl32(d.z, s));
-                    ds.w = uniform(shfl32(d.w, s));
+                    ds.x = lane_val(d.x, s);
+                    ds.y = lane_val(d.y, s);
+                    ds.z = lane_val(d.z, s);
+                    ds.w = lane_val(d.w, s);
                     const uint32_t ls = nl_len(ds);
                     B[lane] = 0u; B[lane + 64] = 0u;
                     wave_sync();
@@ -248,7 +248,7 @@ __device__ __forceinline__ void expand_nodes(const uint32_t* __restrict__ rcol, 
                                              uint32_t* off, uint32_t* rs, uint32_t* stage, uint32_t& cnt, uint32_t* out,
                                              uint32_t* out_n) {
     const uint32_t incl = wave_incl_scan(pc, lane);
-    const uint32_t total = uniform(shfl32(incl, 63));
+    const uint32_t total = lane_val(incl, 63);
     if (total == 0) return;
     off[lane] = incl - pc;
     rs[lane] = p0;
@@ -274,7 +274,7 @@ __device__ __forceinline__ void expand_nodes(const uint32_t* __restrict__ rcol, 
             wave_sync();
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(out_n, cnt);
-            base = uniform(shfl32(base, 0));
+            base = lane_val(base, 0);
             for (uint32_t j = lane; j < cnt; j += 64) out[base + j] = stage[j];
             cnt = 0;
             wave_sync();
@@ -366,20 +366,20 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
             const uint32_t k1 = over ? (uint32_t)__builtin_ctzll(over) : cnt;
             if (k1 == k0) {
                 // node k0 alone has more successors than the staging area
-                const uint32_t vr0 = uniform(shfl32(r0, k0));
-                const uint32_t vdeg = uniform(shfl32(deg, k0));
-                const bool vatk = uniform(shfl32(isatk ? 1u : 0u, k0)) != 0u;
+                const uint32_t vr0 = lane_val(r0, k0);
+                const uint32_t vdeg = lane_val(deg, k0);
+                const bool vatk = lane_val(isatk ? 1u : 0u, k0) != 0u;
                 uint4 vown;
-                vown.x = shfl32(own.x, k0); vown.y = shfl32(own.y, k0); vown.z = shfl32(own.z, k0); vown.w = shfl32(own.w, k0);
+                vown.x = lane_val(own.x, k0); vown.y = lane_val(own.y, k0); vown.z = lane_val(own.z, k0); vown.w = lane_val(own.w, k0);
                 // the node's new list goes to its non-current slot
-                uint4* vout = a.st + 2ull * uniform(shfl32(u, k0)) + (uniform(shfl32(slot, k0)) ^ 1u);
+                uint4* vout = a.st + 2ull * lane_val(u, k0) + (lane_val(slot, k0) ^ 1u);
                 const bool c = eval_node<FUSED>(a, r, A, B, O, lane, base + k0, vdeg, vatk, vown, vout,
                                                 SuccGlobal{a.edges, a.st, vr0, t}, acc, st_bytes_lane);
                 chg |= c && lane == k0;
                 k0++;
                 continue;
             }
-            const uint32_t total = uniform(shfl32(incl, k1 - 1));
+            const uint32_t total = lane_val(incl, k1 - 1);
             if (lane >= k0 && lane < k1) {
                 ex[lane] = incl - d;
                 rs[lane] = r0;
@@ -411,11 +411,11 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
             }
             wave_sync();
             for (uint32_t k = k0; k < k1; k++) {
-                const uint32_t vdeg = uniform(shfl32(deg, k));
-                const bool vatk = uniform(shfl32(isatk ? 1u : 0u, k)) != 0u;
+                const uint32_t vdeg = lane_val(deg, k);
+                const bool vatk = lane_val(isatk ? 1u : 0u, k) != 0u;
                 uint4 vown;
-                vown.x = shfl32(own.x, k); vown.y = shfl32(own.y, k); vown.z = shfl32(own.z, k); vown.w = shfl32(own.w, k);
-                uint4* vout = a.st + 2ull * uniform(shfl32(u, k)) + (uniform(shfl32(slot, k)) ^ 1u);
+                vown.x = lane_val(own.x, k); vown.y = lane_val(own.y, k); vown.z = lane_val(own.z, k); vown.w = lane_val(own.w, k);
+                uint4* vout = a.st + 2ull * lane_val(u, k) + (lane_val(slot, k) ^ 1u);
                 const uint32_t off = uniform(ex[k]);
                 const bool c = eval_node<FUSED>(a, r, A, B, O, lane, base + k, vdeg, vatk, vown, vout, SuccStaged{rec, wts, off},
                                                 acc, st_bytes_lane);
