@@ -161,6 +161,10 @@ def bench_batch(args) -> dict:
     assert edge_updates == plain_edges, "batched and plain paths relaxed different edge counts"
     kms = prof["eval_kernel_ms"]
     achieved = prof["alg_bytes"] / (kms / 1e3) / 1e9 if kms else 0.0
+    avg_us = kms * 1e3 / max(1, int(prof["eval_launches"]))
+    # measured HBM traffic per launch (rocprofv3 summary of this workload) over this run's launch time
+    traffic = measured_traffic("batch_2048_k_eval_pmc.md") if len(graphs) == 2048 else None
+    traffic_gbs = traffic / (avg_us * 1e-6) / 1e9 if traffic and avg_us else None
     lat_ms = np.array(lat) * 1e3
     return {
         "metric": "edge updates/sec + games/sec (batched vs plain), p50/p99 solve latency, 1 GPU",
@@ -180,9 +184,11 @@ def bench_batch(args) -> dict:
         "parity_sample": {"games_compared": len(sample), "mismatches": mismatches},
         "autotune": tune,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_eval",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_eval",
+                     "traffic_source": "profiles/batch_2048_k_eval_pmc.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
+                     "traffic_gbs": traffic_gbs,
                      "launches": int(prof["eval_launches"]),
-                     "avg_launch_us": kms * 1e3 / max(1, int(prof["eval_launches"])),
+                     "avg_launch_us": avg_us,
                      "alg_bytes_per_launch": prof["alg_bytes"] / max(1, int(prof["eval_launches"]))},
     }
 
@@ -561,6 +567,9 @@ def main() -> int:
         launches += int(prof["eval_launches"])
     assert int(prof["edge_updates"]) == edge_updates and int(prof["rounds"]) == int(s["rounds"])
     achieved = alg / (eval_ms / 1e3) / 1e9 if eval_ms > 0 else 0.0
+    avg_us = eval_ms * 1e3 / max(1, launches)
+    traffic = measured_traffic("plain_100k_k_eval_pmc.md") if (args.positions, args.seed) == (100000, 2) else None
+    traffic_gbs = traffic / (avg_us * 1e-6) / 1e9 if traffic and avg_us else None
     phase = {"profiled_solves": n_prof, "k_eval_ms_per_solve": eval_ms / n_prof,
              "k_expand_ms_per_solve": expand_ms / n_prof, "rounds": int(prof["rounds"]),
              "profiled_solve_ms": prof["solve_ms"]}
@@ -584,9 +593,10 @@ def main() -> int:
                    "positions": graph.n_vertices(), "edges": graph.n_edges(), "rounds": int(s["rounds"]),
                    "edge_updates_per_solve": edge_updates, "node_evals_per_solve": int(s["node_evals"])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("plain_100k_k_eval_pmc.md"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": "profiles/plain_100k_k_eval_pmc.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
-                     "kernel": "k_eval", "launches": launches, "avg_launch_us": eval_ms * 1e3 / max(1, launches),
+                     "traffic_gbs": traffic_gbs,
+                     "kernel": "k_eval", "launches": launches, "avg_launch_us": avg_us,
                      "alg_bytes_per_launch": alg / max(1, launches)},
         "per_kernel_profile": phase,
         "device_solve_ms": s["solve_ms"],
