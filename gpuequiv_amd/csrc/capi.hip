@@ -241,7 +241,7 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     if ((e = gm->rcol.alloc(E)) != hipSuccess) return oom(e, "rev_column_indices");
     if ((e = gm->edges.alloc(E)) != hipSuccess) return oom(e, "edges");
     if ((e = gm->atk.alloc(V)) != hipSuccess) return oom(e, "attacker_pos");
-    if ((e = gm->st.alloc(2ull * V)) != hipSuccess) return oom(e, "energy lists");
+    if ((e = gm->st.alloc((size_t)NODE_U4 * V)) != hipSuccess) return oom(e, "node records");
     if ((e = gm->wl0.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->wl1.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->chg.alloc(V)) != hipSuccess) return oom(e, "changed flags");
@@ -257,7 +257,12 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     auto up = [&](void* dst, const void* src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s); };
     if (up(gm->row.p, row.data(), 4ull * (V + 1)) != hipSuccess || up(gm->rrow.p, rrow.data(), 4ull * (V + 1)) != hipSuccess ||
         (E && up(gm->rcol.p, rcol.data(), 4ull * E) != hipSuccess) || (E && up(gm->edges.p, edges.data(), 8ull * E) != hipSuccess) ||
-        up(gm->atk.p, gm->h_atk.data(), V) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        up(gm->atk.p, gm->h_atk.data(), V) != hipSuccess) {
+        delete gm;
+        return fail(GEQ_ERR_TRANSFER, "uploading the game graph failed");
+    }
+    if (V) k_node_hdr<<<(V + 255) / 256, 256, 0, s>>>(gm->row.p, gm->rrow.p, gm->atk.p, V, gm->st.p);
+    if (hipStreamSynchronize(s) != hipSuccess) {
         delete gm;
         return fail(GEQ_ERR_TRANSFER, "uploading the game graph failed");
     }
@@ -291,7 +296,7 @@ namespace {
 // slot 0 (tag 0, current while both slots are untouched) of every node to reach
 __global__ void k_seed_zero(uint4* st, const uint32_t* ids, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) st[2ull * ids[i]] = make_uint4(1u, 0u, 0u, 0u);   // EnergyArray::zero(1): one all-zero energy
+    if (i < n) st[slot_index(ids[i], 0)] = make_uint4(1u, 0u, 0u, 0u);   // EnergyArray::zero(1): one all-zero energy
 }
 
 }  // namespace
@@ -455,9 +460,31 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
 
 // The fixpoint on a resident game whose n_reach nodes to reach are already in g->seed.
 namespace {
-__global__ void k_standard_reach(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t* out, uint32_t* n) {
-    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V && !atk[v] && row[v] == row[v + 1]) out[atomicAdd(n, 1u)] = v;
+// Defender positions without successors.  A block of 256 threads takes
+// 256 * REACH_PER consecutive nodes, counts its matches, reserves their room
+// with one atomic and writes them in node order.
+constexpr uint32_t REACH_PER = 16;
+__global__ __launch_bounds__(256) void k_standard_reach(const uint32_t* row, const uint8_t* atk, uint32_t V, uint32_t* out,
+                                                        uint32_t* n) {
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const uint32_t v0 = (blockIdx.x * 256u + threadIdx.x) * REACH_PER;
+    auto hit = [&](uint32_t v) { return v < V && !atk[v] && row[v] == row[v + 1]; };
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < REACH_PER; k++) c += hit(v0 + k) ? 1u : 0u;
+    const uint32_t incl = wave_incl_scan(c, lane);
+    if (lane == 63) s_wave[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        s_base = tot ? atomicAdd(n, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = s_base + incl - c;
+    for (uint32_t w = 0; w < wid; w++) off += s_wave[w];
+    for (uint32_t k = 0; k < REACH_PER; k++)
+        if (hit(v0 + k)) out[off++] = v0 + k;
 }
 }  // namespace
 
@@ -468,7 +495,7 @@ geq_status geq::standard_seed(geq_game* g) {
     GEQ_HIP(g->std_seed.alloc(std::max<uint32_t>(V, 1) + 1), GEQ_ERR_OUT_OF_MEMORY);
     uint32_t* cnt = g->std_seed.p + std::max<uint32_t>(V, 1);   // the count sits after the list
     GEQ_HIP(hipMemsetAsync(cnt, 0, 4, s), GEQ_ERR_TRANSFER);
-    if (V) k_standard_reach<<<(V + 255) / 256, 256, 0, s>>>(g->row.p, g->atk.p, V, g->std_seed.p, cnt);
+    if (V) k_standard_reach<<<(V + 256 * REACH_PER - 1) / (256 * REACH_PER), 256, 0, s>>>(g->row.p, g->atk.p, V, g->std_seed.p, cnt);
     GEQ_HIP(hipMemcpyAsync(&g->std_n, cnt, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     g->std_ready = true;
@@ -515,12 +542,15 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             geq_status st = grow_arena(g, (uint32_t)h_arena.size());
             if (st != GEQ_OK) return st;
         }
-        GEQ_HIP(hipMemcpyAsync(g->st.p, h_st.data(), 32ull * V, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        // both slots of every node, leaving the headers in place
+        if (V)
+            GEQ_HIP(hipMemcpy2DAsync(g->st.p + 1, 16ull * NODE_U4, h_st.data(), 32, 32, V, hipMemcpyHostToDevice, s),
+                    GEQ_ERR_TRANSFER);
         if (!h_arena.empty())
             GEQ_HIP(hipMemcpyAsync(g->arena.p, h_arena.data(), 2ull * h_arena.size(), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
     if (!initial) {
-        GEQ_HIP(hipMemsetAsync(g->st.p, 0, 32ull * V, s), GEQ_ERR_TRANSFER);
+        if (V) k_clear_slots<<<(V + 255) / 256, 256, 0, s>>>(g->st.p, V);
         if (n_reach) k_seed_zero<<<(n_reach + 255) / 256, 256, 0, s>>>(g->st.p, seed, n_reach);
     }
     GEQ_HIP(hipMemsetAsync(g->queued.p, 0, 8ull * queue_words(V), s), GEQ_ERR_TRANSFER);
@@ -651,7 +681,7 @@ geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_e
     hipStream_t s = g->ctx->stream;
     if (!g->lens.p || g->lens.n < V) GEQ_HIP(g->lens.alloc(V), GEQ_ERR_OUT_OF_MEMORY);
     const uint32_t t = round_tag(g->h_ctl->round);   // the lists after the last round that ran
-    k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + 2ull * first, V, t, g->lens.p);
+    k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, V, t, g->lens.p);
     std::vector<uint32_t> lens(V), offs(V + 1);
     GEQ_HIP(hipMemcpyAsync(lens.data(), g->lens.p, 4ull * V, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -665,7 +695,7 @@ geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_e
     if (!g->offs.p || g->offs.n < V + 1) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
     if (!g->out.p || g->out.n < tot) GEQ_HIP(g->out.alloc(tot), GEQ_ERR_OUT_OF_MEMORY);
     GEQ_HIP(hipMemcpyAsync(g->offs.p, offs.data(), 4ull * (V + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + 2ull * first, g->arena.p, V, t, g->offs.p, g->out.p);
+    k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, g->arena.p, V, t, g->offs.p, g->out.p);
     uint32_t* data = (uint32_t*)std::malloc(4ull * (tot ? tot : 1));
     uint32_t* ooffs = (uint32_t*)std::malloc(4ull * (V + 1));
     if (!data || !ooffs) {

@@ -100,7 +100,7 @@ geq_status geq::alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_gam
     if ((e = gm->rcol.alloc(E)) != hipSuccess) return oom(e, "rev_column_indices");
     if ((e = gm->edges.alloc(E)) != hipSuccess) return oom(e, "edges");
     if ((e = gm->atk.alloc(V)) != hipSuccess) return oom(e, "attacker_pos");
-    if ((e = gm->st.alloc(2ull * V)) != hipSuccess) return oom(e, "energy lists");
+    if ((e = gm->st.alloc((size_t)NODE_U4 * V)) != hipSuccess) return oom(e, "node records");
     if ((e = gm->wl0.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->wl1.alloc(V)) != hipSuccess) return oom(e, "worklist");
     if ((e = gm->chg.alloc(V)) != hipSuccess) return oom(e, "changed flags");
@@ -144,6 +144,7 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
     }
     GEQ_HIP(hipMemcpyAsync(fill.p, gm->rrow.p, 4ull * (V + 1), hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
     k_rev_scatter<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, V, d_col, fill.p, gm->rcol.p);
+    if (V) k_node_hdr<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, gm->rrow.p, gm->atk.p, V, gm->st.p);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     *out = gm;
     return GEQ_OK;

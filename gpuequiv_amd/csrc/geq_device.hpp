@@ -27,7 +27,7 @@ namespace geq {
 //              into the energy arena (u16 units)
 //   h[7]: tag of the round that wrote the slot (round + 1, mod 2^16)
 //
-// Every node owns two slots (32 bytes, st[2v] and st[2v + 1]).  A round
+// Every node owns two slots (32 bytes, in its node record below).  A round
 // writes a node's new list into the slot that is not current, so the other
 // evaluations of the same round keep reading the list as of the round's
 // start (Jacobi) and no commit pass is needed.  At round r (tag t):
@@ -46,6 +46,15 @@ struct __align__(16) NodeList {
     uint16_t h[8];
 };
 
+// Node record: 64 bytes = 4 x uint4, one per node, so that everything a
+// node's own evaluation reads comes in one cache-line piece:
+//   [0] header {first edge, degree | attacker << 31, first reverse edge, reverse degree}
+//   [1], [2] the two list slots
+//   [3] unused
+constexpr uint32_t NODE_U4 = 4;
+constexpr uint32_t ATK_BIT = 0x80000000u;
+__host__ __device__ __forceinline__ size_t slot_index(uint32_t v, uint32_t s) { return (size_t)NODE_U4 * v + 1u + s; }
+
 __host__ __device__ __forceinline__ uint32_t round_tag(uint32_t r) { return (r + 1u) & 0xffffu; }
 __device__ __forceinline__ uint32_t nl_len(const uint4& d) { return d.x & LEN_MASK; }
 __device__ __forceinline__ uint32_t nl_off(const uint4& d) { return d.y; }  // h[2] | h[3] << 16
@@ -60,13 +69,13 @@ __device__ __forceinline__ uint32_t cur_slot(const uint4& a, const uint4& b, uin
     const uint32_t age_a = (t - nl_tag(a)) & 0xffffu, age_b = (t - nl_tag(b)) & 0xffffu;
     return (age_a == 0u || (age_b != 0u && age_b < age_a)) ? 1u : 0u;
 }
-// node v's current record at round tag t (both slots are loaded together)
-__device__ __forceinline__ uint4 load_cur(const uint4* __restrict__ st, uint32_t v, uint32_t t) {
-    const uint4 a = st[2ull * v], b = st[2ull * v + 1];
+// node v's current list slot at round tag t (both slots are loaded together)
+__device__ __forceinline__ uint4 load_cur(const uint4* __restrict__ nodes, uint32_t v, uint32_t t) {
+    const uint4 a = nodes[slot_index(v, 0)], b = nodes[slot_index(v, 1)];
     return cur_slot(a, b, t) ? b : a;
 }
-__device__ __forceinline__ uint4 load_cur(const uint4* __restrict__ st, uint32_t v, uint32_t t, uint32_t& slot) {
-    const uint4 a = st[2ull * v], b = st[2ull * v + 1];
+__device__ __forceinline__ uint4 load_cur(const uint4* __restrict__ nodes, uint32_t v, uint32_t t, uint32_t& slot) {
+    const uint4 a = nodes[slot_index(v, 0)], b = nodes[slot_index(v, 1)];
     slot = cur_slot(a, b, t);
     return slot ? b : a;
 }

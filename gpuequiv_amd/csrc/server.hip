@@ -36,9 +36,9 @@ using namespace geq;
 
 namespace {
 
-// per-node bytes of the pools (row, rrow, atk, two list slots, wl x2, chg, job_of, queued bits rounded up) and
+// per-node bytes of the pools (row, rrow, atk, node record, wl x2, chg, job_of, queued bits rounded up) and
 // per-edge bytes (edges, rcol)
-constexpr uint64_t NODE_BYTES = 4 + 4 + 1 + 32 + 4 + 4 + 1 + 4 + 1;
+constexpr uint64_t NODE_BYTES = 4 + 4 + 1 + 16 * NODE_U4 + 4 + 4 + 1 + 4 + 1;
 constexpr uint64_t EDGE_BYTES = 8 + 4;
 
 // Place a built game at node base nb / edge base eb of the pools.  Its
@@ -49,16 +49,16 @@ __global__ void k_place_nodes(const uint32_t* row, const uint8_t* atk, uint32_t 
     if (v > V) return;
     prow[nb + v] = eb + row[v];             // v == V: the row end of the job (padding slot)
     patk[nb + v] = v < V ? atk[v] : (uint8_t)1;
-    st[2ull * (nb + v)] = make_uint4(0, 0, 0, ((t0 - 1u) & 0xffffu) << 16);
-    st[2ull * (nb + v) + 1] = make_uint4(0, 0, 0, ((t0 - 2u) & 0xffffu) << 16);
+    st[slot_index(nb + v, 0)] = make_uint4(0, 0, 0, ((t0 - 1u) & 0xffffu) << 16);
+    st[slot_index(nb + v, 1)] = make_uint4(0, 0, 0, ((t0 - 2u) & 0xffffu) << 16);
     job_of[nb + v] = job;
 }
 // a released range holds no lists (the arena compaction walks every slot)
 __global__ void k_clear_nodes(uint4* st, uint32_t nb, uint32_t n) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < n) {
-        st[2ull * (nb + v)] = make_uint4(0, 0, 0, 0);
-        st[2ull * (nb + v) + 1] = make_uint4(0, 0, 0, 0);
+        st[slot_index(nb + v, 0)] = make_uint4(0, 0, 0, 0);
+        st[slot_index(nb + v, 1)] = make_uint4(0, 0, 0, 0);
     }
 }
 // arena compaction (t: tag of the next round): arena entries each current
@@ -76,15 +76,15 @@ __global__ void k_arena_move(uint4* st, const uint16_t* src, uint16_t* dst, cons
     if (v >= n) return;
     uint32_t c = 0;
     uint4 d = load_cur(st, v, t, c);
-    uint4 o = st[2ull * v + (c ^ 1u)];
+    uint4 o = st[slot_index(v, c ^ 1u)];
     o.x &= 0xffff0000u;                     // length 0
-    st[2ull * v + (c ^ 1u)] = o;
+    st[slot_index(v, c ^ 1u)] = o;
     const uint32_t len = nl_len(d);
     if (len <= INLINE_CAP) return;
     const uint32_t from = nl_off(d), to = offs[v];
     for (uint32_t j = 0; j < len; j++) dst[to + j] = src[from + j];
     d.y = to;
-    st[2ull * v + c] = d;
+    st[slot_index(v, c)] = d;
 }
 __global__ void k_place_edges(const uint32_t* col, const uint32_t* wt, uint32_t E, uint32_t nb, uint32_t eb, uint2* edges) {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,7 +111,7 @@ __global__ void k_place_reach(const uint32_t* row, const uint8_t* atk, uint32_t 
                               uint32_t* seed, uint32_t* n) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < V && !atk[v] && row[v] == row[v + 1]) {
-        st[2ull * (nb + v)] = make_uint4(1u, 0u, 0u, ((t0 - 1u) & 0xffffu) << 16);   // slot 0 stays current
+        st[slot_index(nb + v, 0)] = make_uint4(1u, 0u, 0u, ((t0 - 1u) & 0xffffu) << 16);   // slot 0 stays current
         seed[atomicAdd(n, 1u)] = nb + v;
     }
 }
@@ -558,6 +558,8 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(S->scan_tmp.p, bytes, S->place_cnt.p, S->place_excl.p, V + 1, s), GEQ_ERR_INTERNAL);
         k_place_rrow<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(S->place_excl.p, V, J->eb, J->nb, g->rrow.p, S->place_fill.p);
         k_place_rcol<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, bcol, V, J->nb, J->eb, S->place_fill.p, g->rcol.p);
+        k_node_hdr<<<(V + tb - 1) / tb, tb, 0, s>>>(g->row.p + J->nb, g->rrow.p + J->nb, g->atk.p + J->nb, V,
+                                                    g->st.p + (size_t)NODE_U4 * J->nb);
         // nodes to reach: zero energy + seeds of the current round
         GEQ_TRY(ensure_buf(g->seed, V, s));
         GEQ_HIP(hipMemsetAsync(S->seed_cnt.p, 0, 4, s), GEQ_ERR_TRANSFER);

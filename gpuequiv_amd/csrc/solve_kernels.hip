@@ -346,14 +346,14 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
         uint4 own = make_uint4(0, 0, 0, 0);
         if (lane < cnt) {
             u = wl[base + lane];
-            r0 = a.row[u];
-            deg = a.row[u + 1] - r0;
-            isatk = a.atk[u] != 0;
+            // header and both slots: one 64-byte record
+            const uint4 hdr = a.st[(size_t)NODE_U4 * u];
             own = load_cur(a.st, u, t, slot);
-            if (FUSED) {            // the predecessor range, in case u changes
-                p0 = a.rrow[u];
-                pc = a.rrow[u + 1] - p0;
-            }
+            r0 = hdr.x;
+            deg = hdr.y & ~ATK_BIT;
+            isatk = (hdr.y & ATK_BIT) != 0u;
+            p0 = hdr.z;             // the predecessor range, in case u changes
+            pc = hdr.w;
             // u leaves this round's list: clear its queued bit
             atomicAnd(&a.queued[(r & 1u) * a.qwords + (u >> 5)], ~(1u << (u & 31u)));
         }
@@ -372,7 +372,7 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
                 uint4 vown;
                 vown.x = lane_val(own.x, k0); vown.y = lane_val(own.y, k0); vown.z = lane_val(own.z, k0); vown.w = lane_val(own.w, k0);
                 // the node's new list goes to its non-current slot
-                uint4* vout = a.st + 2ull * lane_val(u, k0) + (lane_val(slot, k0) ^ 1u);
+                uint4* vout = a.st + slot_index(lane_val(u, k0), lane_val(slot, k0) ^ 1u);
                 const bool c = eval_node<FUSED>(a, r, A, B, O, lane, base + k0, vdeg, vatk, vown, vout,
                                                 SuccGlobal{a.edges, a.st, vr0, t}, acc, st_bytes_lane);
                 chg |= c && lane == k0;
@@ -415,7 +415,7 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
                 const bool vatk = lane_val(isatk ? 1u : 0u, k) != 0u;
                 uint4 vown;
                 vown.x = lane_val(own.x, k); vown.y = lane_val(own.y, k); vown.z = lane_val(own.z, k); vown.w = lane_val(own.w, k);
-                uint4* vout = a.st + 2ull * lane_val(u, k) + (lane_val(slot, k) ^ 1u);
+                uint4* vout = a.st + slot_index(lane_val(u, k), lane_val(slot, k) ^ 1u);
                 const uint32_t off = uniform(ex[k]);
                 const bool c = eval_node<FUSED>(a, r, A, B, O, lane, base + k, vdeg, vatk, vown, vout, SuccStaged{rec, wts, off},
                                                 acc, st_bytes_lane);
@@ -516,10 +516,27 @@ __global__ void k_advance(Ctl* ctl, uint32_t rounds) {
 __global__ void k_retag(uint4* st, uint32_t n, uint32_t t) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const uint4 a = st[2ull * v], b = st[2ull * v + 1];
+    const uint4 a = st[slot_index(v, 0)], b = st[slot_index(v, 1)];
     const uint32_t c = cur_slot(a, b, t);
-    st[2ull * v + c] = nl_with_tag(c ? b : a, t - 1u);
-    st[2ull * v + (c ^ 1u)] = nl_with_tag(c ? a : b, t - 2u);
+    st[slot_index(v, c)] = nl_with_tag(c ? b : a, t - 1u);
+    st[slot_index(v, c ^ 1u)] = nl_with_tag(c ? a : b, t - 2u);
+}
+
+// Node headers from the CSR arrays (after an upload or a placement).
+__global__ void k_node_hdr(const uint32_t* row, const uint32_t* rrow, const uint8_t* atk, uint32_t n, uint4* st) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t r0 = row[v], p0 = rrow[v];
+    st[(size_t)NODE_U4 * v] = make_uint4(r0, (row[v + 1] - r0) | (atk[v] ? ATK_BIT : 0u), p0, rrow[v + 1] - p0);
+    st[(size_t)NODE_U4 * v + 3] = make_uint4(0, 0, 0, 0);
+}
+
+// Both list slots of nodes [0, n) empty with tag 0 (slot 0 current); headers kept.
+__global__ void k_clear_slots(uint4* st, uint32_t n) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    st[slot_index(v, 0)] = make_uint4(0, 0, 0, 0);
+    st[slot_index(v, 1)] = make_uint4(0, 0, 0, 0);
 }
 
 // Sum the per-block statistics slots into out[0..3].
