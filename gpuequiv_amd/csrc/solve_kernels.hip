@@ -328,6 +328,13 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
     uint32_t* stage = O + 128;    // FUSED: staged appends (APPEND_STAGE entries)
 
     const uint32_t t = round_tag(r);
+    // This round's list leaves its queued bitmap: a large list clears the
+    // whole bitmap with plain stores (grid-stride), a short one clears its
+    // nodes' bits one atomic each.
+    uint32_t* const qcur = a.queued + (r & 1u) * a.qwords;
+    const bool clear_all = (uint64_t)a.qwords < 8ull * n;
+    if (clear_all)
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.qwords; i += gridDim.x * blockDim.x) qcur[i] = 0u;
     uint32_t* out = a.wl[(r + 1u) & 1u];
     uint32_t* out_n = &a.ctl->wl_n[(r + 1u) % 3u];
     uint32_t app = 0;             // FUSED: appends staged by this wave
@@ -354,8 +361,7 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
             isatk = (hdr.y & ATK_BIT) != 0u;
             p0 = hdr.z;             // the predecessor range, in case u changes
             pc = hdr.w;
-            // u leaves this round's list: clear its queued bit
-            atomicAnd(&a.queued[(r & 1u) * a.qwords + (u >> 5)], ~(1u << (u & 31u)));
+            if (!clear_all) atomicAnd(&qcur[u >> 5], ~(1u << (u & 31u)));
         }
         uint32_t k0 = 0;
         while (k0 < cnt) {
