@@ -77,6 +77,8 @@ using namespace geq;
 geq_game::~geq_game() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (h_ctl) (void)hipHostFree(h_ctl);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_end) (void)hipEventDestroy(ev_end);
 }
 
 extern "C" {
@@ -560,9 +562,9 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     GEQ_HIP(hipMemcpyAsync(g->ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemsetAsync(g->stats.p, 0, (4ull * STAT_SLOTS + 4) * sizeof(unsigned long long), s), GEQ_ERR_TRANSFER);
 
-    hipEvent_t ev_start, ev_end;
-    GEQ_HIP(hipEventCreate(&ev_start), GEQ_ERR_NO_DEVICE);
-    GEQ_HIP(hipEventCreate(&ev_end), GEQ_ERR_NO_DEVICE);
+    if (!g->ev_start) GEQ_HIP(hipEventCreate(&g->ev_start), GEQ_ERR_NO_DEVICE);
+    if (!g->ev_end) GEQ_HIP(hipEventCreate(&g->ev_end), GEQ_ERR_NO_DEVICE);
+    hipEvent_t ev_start = g->ev_start, ev_end = g->ev_end;
     std::vector<hipEvent_t> prof;   // 4 events per round: around k_eval, around k_expand
     GEQ_HIP(hipEventRecord(ev_start, s), GEQ_ERR_NO_DEVICE);
 
@@ -577,8 +579,11 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         sx.n_seed = n_reach;
         k_expand<<<std::max(1u, std::min(shape.expand_blocks, (n_reach + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx, 0u);
     }
-    GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
-    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    // No readback here: with seeds the first batch of rounds runs (a batch
+    // over an empty worklist only costs its empty launches), without them
+    // nothing does.
+    *g->h_ctl = c0;
+    g->h_ctl->wl_n[0] = n_reach ? 1u : 0u;
 
     uint64_t launches = 0;
     geq_status result = GEQ_OK;
@@ -644,8 +649,6 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         expand_ms += b;
     }
     for (hipEvent_t ev : prof) (void)hipEventDestroy(ev);
-    (void)hipEventDestroy(ev_start);
-    (void)hipEventDestroy(ev_end);
     if (result != GEQ_OK) return result;
     unsigned long long h_stats[4] = {0, 0, 0, 0};
     k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, shape.eval_blocks,

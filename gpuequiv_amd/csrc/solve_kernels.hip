@@ -437,6 +437,23 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
     if (FUSED) flush_appends(stage, app, s_cnt, out, out_n);
 }
 
+// The round of a kernel at position k of a batch, its worklist size (0 after
+// an overflow) and the overflow flag, from one scalar load of the control
+// block's first 32 bytes (round and counts share them).
+struct RoundCtl {
+    uint32_t r, n;
+    bool ovf;
+};
+__device__ __forceinline__ RoundCtl read_round(const Ctl* ctl, uint32_t k) {
+    const uint4* p = reinterpret_cast<const uint4*>(ctl);
+    const uint4 w0 = p[0], w1 = p[1];   // w0: wl_n[0..2], overflow; w1: arena_top, round, last_nonempty, pad
+    const uint32_t r = uniform(w1.y) + k;
+    const uint32_t i = r % 3u;
+    const uint32_t n = uniform(i == 0u ? w0.x : i == 1u ? w0.y : w0.z);
+    const bool ovf = uniform(w0.w) != 0u;
+    return RoundCtl{r, ovf ? 0u : n, ovf};
+}
+
 // Per-block statistics slot (no cross-block atomics): slot b is only ever
 // written by block b of the fixed-size grid, rounds are stream-ordered.
 __device__ __forceinline__ void flush_stats(const StatAcc& acc, unsigned long long* slots) {
@@ -459,9 +476,9 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_eval(EvalArgs a, uint32_t k) {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     __shared__ uint32_t s_cnt[MAX_WAVES + 1];
     StatAcc acc;
-    const uint32_t r = uniform(a.ctl->round) + k;
-    const bool ovf = a.ctl->overflow != 0u;
-    const uint32_t n = ovf ? 0u : uniform(a.ctl->wl_n[r % 3u]);
+    const RoundCtl rc = read_round(a.ctl, k);
+    const uint32_t r = rc.r, n = rc.n;
+    const bool ovf = rc.ovf;
     if (!ovf && blockIdx.x == 0 && threadIdx.x == 0) {
         // round r - 1's list count is consumed (its k_eval and k_expand are
         // done): it becomes the count of round r + 1's appends
@@ -482,9 +499,10 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a, uint32_t k) 
     __shared__ uint32_t s_stage[MAX_WAVES][APPEND_STAGE];
     __shared__ uint32_t s_off[MAX_WAVES][64], s_rs[MAX_WAVES][64];
     __shared__ uint32_t s_cnt[MAX_WAVES + 1];
-    const uint32_t r = uniform(a.ctl->round) + k;
+    const RoundCtl rc = read_round(a.ctl, k);
+    const uint32_t r = rc.r;
     const bool seeding = a.seed != nullptr;
-    const uint32_t n = a.ctl->overflow ? 0u : (seeding ? a.n_seed : uniform(a.ctl->wl_n[r % 3u]));
+    const uint32_t n = rc.ovf ? 0u : (seeding ? a.n_seed : rc.n);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = threadIdx.x >> 6;
     const uint32_t* list = seeding ? a.seed : a.wl[r & 1u];
