@@ -331,6 +331,11 @@ static bool lists_equal_ref(const Ctx& X, const List& a_new, const List& b_old) 
     return true;
 }
 
+// Threads for the evaluations of one round (orc_set_threads).  Rounds are
+// Jacobi, so a round's evaluations are independent and any split over threads
+// gives the same result; the merge stays sequential, in worklist order.
+static int g_threads = 1;
+
 struct SolveStats {
     uint64_t rounds = 0, node_evals = 0, atk_evals = 0, def_evals = 0, edge_updates = 0, bytes = 0,
              def_retries = 0;
@@ -370,9 +375,12 @@ static void solve(const Ctx& X, uint32_t V, const uint32_t* row, const uint32_t*
         def_visit.clear();
         // Jacobi: all evaluations read the state from the start of the round.
         std::vector<AtkResult> ar(atk_jobs.size());
-        for (size_t i = 0; i < atk_jobs.size(); i++) ar[i] = attack_eval(X, atk_jobs[i], row, col, wt, E);
+        const long na = (long)atk_jobs.size(), nd = (long)def_jobs.size();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads) if (g_threads > 1)
+        for (long i = 0; i < na; i++) ar[i] = attack_eval(X, atk_jobs[i], row, col, wt, E);
         std::vector<DefResult> dr(def_jobs.size());
-        for (size_t i = 0; i < def_jobs.size(); i++)
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads) if (g_threads > 1)
+        for (long i = 0; i < nd; i++)
             dr[i] = defend_eval(X, def_jobs[i].first, row, col, wt, E, def_jobs[i].second);
         // accounting: edge updates and algorithmic bytes (attempted evaluations)
         auto account = [&](uint32_t u, const List& old_list, const List* new_list) {
@@ -641,6 +649,8 @@ static void make_reverse(uint32_t V, const std::vector<uint32_t>& row, const std
 // C ABI for ctypes (tests/ and bench.py cpu_baseline only)
 // ===========================================================================
 extern "C" {
+
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 
 struct orc_lists {          // per-node energy lists: offsets in energies (not words)
     uint32_t n_nodes;

@@ -63,8 +63,14 @@ def lib():
         h.orc_signature_refinement.restype = C.c_uint32
         h.orc_bisimilar_minimize.argtypes = [C.c_uint32, u32, u32, i32, u32, u32, u32, i32]
         h.orc_bisimilar_minimize.restype = C.c_uint32
+        h.orc_set_threads.argtypes = [C.c_int]
         _lib = h
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads for the evaluations of one fixpoint round (1: sequential)."""
+    lib().orc_set_threads(int(n))
 
 
 def _u32(a):

@@ -288,3 +288,19 @@ def test_bruteforce_minimal_and_fold_identity():
         mb = np.zeros(U.N, bool); mb[B] = True
         inter = U.up(ma) & U.up(mb)
         assert tuple(sorted(x[0] for x in out)) == U.as_points(U.minimal(inter))
+
+
+def test_oracle_threads_give_identical_results():
+    # the oracle's round evaluations spread over threads (the bench's
+    # all-cores CPU baseline) compute exactly the sequential results
+    from gpuequiv_amd import workloads
+    g = workloads.random_game(6000, 77)
+    reach = H.standard_reach(g)
+    ref = H.oracle_solve_game(g, reach)
+    orc.set_threads(4)
+    try:
+        got = H.oracle_solve_game(g, reach)
+    finally:
+        orc.set_threads(1)
+    assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+    assert ref[2] == got[2]
