@@ -3,6 +3,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -79,6 +80,7 @@ geq_game::~geq_game() {
     if (h_ctl) (void)hipHostFree(h_ctl);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_end) (void)hipEventDestroy(ev_end);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
 }
 
 extern "C" {
@@ -358,17 +360,20 @@ RoundShape round_shape(const geq_game* g) {
 }
 
 // Blocks of k_eval (w waves each, fused or not) that one CU holds at once.
+// (Cached per shape; host threads may race to fill an entry, with the same value.)
 uint32_t eval_blocks_resident(bool fused, uint32_t w) {
-    static int cache[2][MAX_WAVES + 1] = {};
-    int& c = cache[fused ? 1 : 0][w];
-    if (c == 0) {
+    static std::atomic<int> cache[2][MAX_WAVES + 1];
+    std::atomic<int>& c = cache[fused ? 1 : 0][w];
+    int v = c.load(std::memory_order_relaxed);
+    if (v == 0) {
         int per_cu = 0;
         const void* fn = fused ? reinterpret_cast<const void*>(&k_eval<true>) : reinterpret_cast<const void*>(&k_eval<false>);
         const size_t lds = (size_t)w * (fused ? EVAL_FUSED_LDS_PER_WAVE : EVAL_LDS_PER_WAVE);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)(64 * w), lds) != hipSuccess || per_cu <= 0) per_cu = 64;
-        c = per_cu;
+        v = per_cu;
+        c.store(v, std::memory_order_relaxed);
     }
-    return (uint32_t)c;
+    return (uint32_t)v;
 }
 
 EvalArgs eval_args(geq_game* g) {
@@ -379,8 +384,13 @@ ExpandArgs expand_args(geq_game* g) {
     return ExpandArgs{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V), g->ctl.p};
 }
 
+// The rounds are captured on the game's own stream, never on the context's
+// shared one: host threads solving other games may enqueue on that stream
+// meanwhile, and a capture would take their work in.  (Capturing records;
+// nothing runs on the capture stream.)
 geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape) {
-    hipStream_t s = g->ctx->stream;
+    if (!g->cap_stream) GEQ_HIP(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking), GEQ_ERR_NO_DEVICE);
+    hipStream_t s = g->cap_stream;
     if (g->graph_exec) {
         (void)hipGraphExecDestroy(g->graph_exec);
         g->graph_exec = nullptr;

@@ -128,6 +128,7 @@ struct geq_game {
     std::vector<uint32_t> game_base;      // batched games: first node of every game (+ total)
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds
     hipEvent_t ev_start = nullptr, ev_end = nullptr;   // around each solve's rounds
+    hipStream_t cap_stream = nullptr;                  // graph capture only
     void* graph_arena = nullptr;          // arena pointer baked into graph_exec
     ~geq_game();
 };

@@ -273,3 +273,28 @@ def test_slot_retag_keeps_results(monkeypatch, expand_bpc, rounds):
     monkeypatch.setenv("GEQ_RETAG_ROUNDS", "1")
     got, sgot = _solve_raw(g, 0, expand_blocks_per_cu=expand_bpc, rounds_per_launch=rounds)
     assert got == ref and sgot["rounds"] == sref["rounds"]
+
+
+def test_concurrent_host_threads_share_the_context():
+    # the crate's types are Send + Sync (Cargo.toml:18-21): games solved from
+    # several host threads at once through one context give their own results
+    import threading
+    graphs = [workloads.random_game(v, 500 + i) for i, v in enumerate((8000, 300, 15000, 2000, 9000, 50))]
+    want = [_canon(_run(g)[1]) for g in graphs]
+    got = [None] * len(graphs)
+    errors = []
+
+    def work(i):
+        try:
+            for _ in range(3):
+                got[i] = _canon(_run(graphs[i])[1])
+        except Exception as e:   # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(len(graphs))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=100)
+    assert not errors, errors
+    assert got == want
