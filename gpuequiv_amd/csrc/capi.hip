@@ -297,6 +297,25 @@ void geq_game_free(geq_game* g) {
 
 namespace {
 
+// The per-solve reset in one launch: empty list slots (unless initial lists
+// were uploaded), no node queued, zero statistics, a fresh control block.
+__global__ void k_solve_init(uint4* st, uint32_t V, int clear_slots, uint32_t* queued, uint32_t qwords2,
+                             unsigned long long* stats, uint32_t nstats, Ctl* ctl, uint32_t arena_top) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    if (clear_slots)
+        for (uint32_t v = tid; v < V; v += nt) {
+            st[slot_index(v, 0)] = make_uint4(0, 0, 0, 0);
+            st[slot_index(v, 1)] = make_uint4(0, 0, 0, 0);
+        }
+    for (uint32_t i = tid; i < qwords2; i += nt) queued[i] = 0u;
+    for (uint32_t i = tid; i < nstats; i += nt) stats[i] = 0ull;
+    if (tid == 0) {
+        Ctl c{};
+        c.arena_top = arena_top;
+        *ctl = c;
+    }
+}
+
 // slot 0 (tag 0, current while both slots are untouched) of every node to reach
 __global__ void k_seed_zero(uint4* st, const uint32_t* ids, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -561,16 +580,16 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         if (!h_arena.empty())
             GEQ_HIP(hipMemcpyAsync(g->arena.p, h_arena.data(), 2ull * h_arena.size(), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
-    if (!initial) {
-        if (V) k_clear_slots<<<(V + 255) / 256, 256, 0, s>>>(g->st.p, V);
-        if (n_reach) k_seed_zero<<<(n_reach + 255) / 256, 256, 0, s>>>(g->st.p, seed, n_reach);
-    }
-    GEQ_HIP(hipMemsetAsync(g->queued.p, 0, 8ull * queue_words(V), s), GEQ_ERR_TRANSFER);
     Ctl c0{};
     c0.arena_top = (uint32_t)h_arena.size();
     g->retag_round = 0;
-    GEQ_HIP(hipMemcpyAsync(g->ctl.p, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    GEQ_HIP(hipMemsetAsync(g->stats.p, 0, (4ull * STAT_SLOTS + 4) * sizeof(unsigned long long), s), GEQ_ERR_TRANSFER);
+    {
+        const uint32_t q2 = 2u * queue_words(V), ns = 4u * STAT_SLOTS + 4u;
+        const uint32_t most = std::max(std::max(V, q2), ns);
+        k_solve_init<<<std::min(4096u, (most + 255) / 256), 256, 0, s>>>(g->st.p, V, initial ? 0 : 1, g->queued.p, q2, g->stats.p,
+                                                                         ns, g->ctl.p, c0.arena_top);
+    }
+    if (!initial && n_reach) k_seed_zero<<<(n_reach + 255) / 256, 256, 0, s>>>(g->st.p, seed, n_reach);
 
     if (!g->ev_start) GEQ_HIP(hipEventCreate(&g->ev_start), GEQ_ERR_NO_DEVICE);
     if (!g->ev_end) GEQ_HIP(hipEventCreate(&g->ev_end), GEQ_ERR_NO_DEVICE);
@@ -647,7 +666,14 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     }
     const uint64_t rounds = g->h_ctl->last_nonempty;
     GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
-    GEQ_HIP(hipEventSynchronize(ev_end), GEQ_ERR_NO_DEVICE);
+    // the statistics come back with the same synchronisation
+    unsigned long long h_stats[4] = {0, 0, 0, 0};
+    if (result == GEQ_OK) {
+        k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, shape.eval_blocks, g->stats.p + 4ull * STAT_SLOTS);
+        GEQ_HIP(hipMemcpyAsync(h_stats, g->stats.p + 4ull * STAT_SLOTS, sizeof(h_stats), hipMemcpyDeviceToHost, s),
+                GEQ_ERR_TRANSFER);
+    }
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev_start, ev_end);
     double eval_ms = 0, expand_ms = 0;
@@ -660,11 +686,6 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     }
     for (hipEvent_t ev : prof) (void)hipEventDestroy(ev);
     if (result != GEQ_OK) return result;
-    unsigned long long h_stats[4] = {0, 0, 0, 0};
-    k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, shape.eval_blocks,
-                                     g->stats.p + 4ull * STAT_SLOTS);
-    GEQ_HIP(hipMemcpyAsync(h_stats, g->stats.p + 4ull * STAT_SLOTS, sizeof(h_stats), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
-    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->rounds = rounds;

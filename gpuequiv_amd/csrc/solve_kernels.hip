@@ -555,13 +555,6 @@ __global__ void k_node_hdr(const uint32_t* row, const uint32_t* rrow, const uint
     st[(size_t)NODE_U4 * v + 3] = make_uint4(0, 0, 0, 0);
 }
 
-// Both list slots of nodes [0, n) empty with tag 0 (slot 0 current); headers kept.
-__global__ void k_clear_slots(uint4* st, uint32_t n) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    st[slot_index(v, 0)] = make_uint4(0, 0, 0, 0);
-    st[slot_index(v, 1)] = make_uint4(0, 0, 0, 0);
-}
 
 // Sum the per-block statistics slots into out[0..3].
 __global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots, unsigned long long* out) {
