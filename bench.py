@@ -305,13 +305,15 @@ def bench_gamebuild(args) -> dict:
     for _ in range(args.warmup):
         GameBuild(lts, pairs, max_positions=args.max_positions).close()
     builds, solves = [], []
-    for _ in range(args.steps):
+    for i in range(args.steps):
         b = GameBuild(lts, pairs, max_positions=args.max_positions)
         bs = b.build_stats
         builds.append(bs["build_ms"])
-        if not bs["truncated"]:
-            st = b.solve()
-            solves.append(st)
+        if not bs["truncated"] or i == args.steps - 1:
+            # a capped game is still a well-formed energy game (the positions of
+            # its last level have no successors): its solve is timed once
+            b.solve()
+            solves.append(b.solve())
         b.close()
     V, E = int(bs["positions"]), int(bs["edges"])
     bms = float(np.median(builds))
@@ -328,8 +330,23 @@ def bench_gamebuild(args) -> dict:
     }
     if solves:
         s = solves[-1]
-        line["solve"] = {"rounds": int(s["rounds"]), "edge_updates": int(s["edge_updates"]), "solve_ms": s["solve_ms"],
-                         "edge_updates_per_s": int(s["edge_updates"]) / (s["solve_ms"] / 1e3)}
+        line["solve"] = {"of": "the capped game" if bs["truncated"] else "the game",
+                         "rounds": int(s["rounds"]), "edge_updates": int(s["edge_updates"]), "solve_ms": s["solve_ms"],
+                         "edge_updates_per_s": int(s["edge_updates"]) / (s["solve_ms"] / 1e3),
+                         "alg_bytes_per_s": int(s["alg_bytes"]) / (s["solve_ms"] / 1e3)}
+    if not args.no_cpu:
+        # the oracle's breadth-first build of the same LTS and pairs, capped lower (bounded sample)
+        from oracle import oracle as orc
+        cap = min(args.max_positions, 2000000)
+        t0 = time.perf_counter()
+        og = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs, max_positions=cap,
+                           positions_wanted=False)
+        cpu_s = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": og["V"] / cpu_s, "unit": "positions built/s", "cores": 1, "kind": "port",
+                                "sample": f"the oracle's build of the same LTS and pairs capped at {cap} positions: "
+                                          f"{og['V']} positions, {og['E']} edges in {cpu_s:.1f} s, single thread, "
+                                          f"{cpu_model()}"}
+        line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
     return line
 
 

@@ -560,11 +560,25 @@ struct Build {
         map[x] = idx;
         return idx;
     }
-    void build_internal(const Lts& L) {               // gamebuild.rs:229-239, :242-252
+    // max_positions (0: none) is this build's cap, not the reference's: the
+    // walk stops after the breadth-first level (the nodes created by the
+    // previous level) that leaves more than max_positions nodes; the nodes
+    // created by that level keep empty rows (truncated = true).
+    bool truncated = false;
+    void build_internal(const Lts& L, uint64_t max_positions = 0) {   // gamebuild.rs:229-239, :242-252
         n_start = (uint32_t)nodes.size();
         std::vector<Pos> succ;
         std::vector<uint32_t> w;
+        size_t level_end = nodes.size();
         for (size_t idx = 0; idx < nodes.size(); idx++) {
+            if (idx == level_end) {
+                if (max_positions && nodes.size() > max_positions) {
+                    truncated = true;
+                    for (; idx < nodes.size(); idx++) row.push_back((uint32_t)col.size());
+                    break;
+                }
+                level_end = nodes.size();
+            }
             Pos cur = nodes[idx];
             successors(L, cur, succ, w);
             row.push_back((uint32_t)col.size());
@@ -769,12 +783,13 @@ struct orc_game {
 // GameBuild seeded with an explicit list of (p, q) attack positions
 // (GameBuild::compare :165-170 for one pair; compare_multiple / compare_all
 // :176-215 with the caller choosing the seed order).
-orc_game* orc_gamebuild(uint32_t n_states, const uint32_t* lrow, const uint32_t* ldst, const int32_t* llbl,
-                        uint32_t n_pairs, const uint32_t* pairs) {
+orc_game* orc_gamebuild_capped(uint32_t n_states, const uint32_t* lrow, const uint32_t* ldst, const int32_t* llbl,
+                               uint32_t n_pairs, const uint32_t* pairs, uint64_t max_positions, int* truncated) {
     Lts L = lts_from_csr(n_states, lrow, ldst, llbl, 1);
     Build B;
     for (uint32_t i = 0; i < n_pairs; i++) B.new_node(mk_attack(pairs[2 * i], {pairs[2 * i + 1]}));
-    B.build_internal(L);
+    B.build_internal(L, max_positions);
+    if (truncated) *truncated = B.truncated ? 1 : 0;
     orc_game* g = (orc_game*)std::calloc(1, sizeof(orc_game));
     uint32_t V = (uint32_t)B.nodes.size(), E = (uint32_t)B.col.size();
     g->V = V; g->E = E; g->n_start = B.n_start;
@@ -797,6 +812,11 @@ orc_game* orc_gamebuild(uint32_t n_states, const uint32_t* lrow, const uint32_t*
     qo[V] = xo[V] = (uint32_t)pool.size();
     g->p = dup32(p); g->q1 = dup32(q1); g->q_off = dup32(qo); g->qx_off = dup32(xo); g->pool = dup32(pool);
     return g;
+}
+
+orc_game* orc_gamebuild(uint32_t n_states, const uint32_t* lrow, const uint32_t* ldst, const int32_t* llbl,
+                        uint32_t n_pairs, const uint32_t* pairs) {
+    return orc_gamebuild_capped(n_states, lrow, ldst, llbl, n_pairs, pairs, 0, nullptr);
 }
 
 void orc_game_free(orc_game* g) {

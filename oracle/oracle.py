@@ -59,6 +59,8 @@ def lib():
         h.orc_gamebuild.argtypes = [C.c_uint32, u32, u32, i32, C.c_uint32, u32]
         h.orc_gamebuild.restype = P(orc_game)
         h.orc_game_free.argtypes = [P(orc_game)]
+        h.orc_gamebuild_capped.argtypes = [C.c_uint32, u32, u32, i32, C.c_uint32, u32, C.c_uint64, P(C.c_int)]
+        h.orc_gamebuild_capped.restype = P(orc_game)
         h.orc_signature_refinement.argtypes = [C.c_uint32, u32, u32, i32, u32]
         h.orc_signature_refinement.restype = C.c_uint32
         h.orc_bisimilar_minimize.argtypes = [C.c_uint32, u32, u32, i32, u32, u32, u32, i32]
@@ -182,13 +184,17 @@ def solve(elements: int, mx: int, row, col, wt, atk, reach, init_off=None, init_
     return offs, data, dict(zip(keys, [int(x) for x in stats]))
 
 
-def gamebuild(n_states: int, lrow, ldst, llbl, pairs):
-    """Build the spectroscopy game seeded with the given (p, q) attack positions."""
+def gamebuild(n_states: int, lrow, ldst, llbl, pairs, max_positions: int = 0, positions_wanted: bool = True):
+    """Build the spectroscopy game seeded with the given (p, q) attack positions.
+    max_positions > 0: stop after the breadth-first level that passes it (the
+    nodes that level created keep empty rows; "truncated" in the result).
+    positions_wanted=False skips the Python list of position tuples."""
     lrow, lrp = _u32(lrow)
     ldst, ldp = _u32(ldst if len(ldst) else [0])
     llbl, llp = _i32(llbl if len(llbl) else [0])
     pr, prp = _u32(np.asarray(pairs, dtype=np.uint32).reshape(-1) if len(pairs) else [0])
-    g = lib().orc_gamebuild(n_states, lrp, ldp, llp, len(pairs), prp)
+    trunc = C.c_int(0)
+    g = lib().orc_gamebuild_capped(n_states, lrp, ldp, llp, len(pairs), prp, int(max_positions), C.byref(trunc))
     try:
         c = g.contents
         V, E = c.V, c.E
@@ -200,7 +206,7 @@ def gamebuild(n_states: int, lrow, ldst, llbl, pairs):
         ps = arr(c.p, V)
         q1 = arr(c.q1, V)
         positions = []
-        for i in range(V):
+        for i in (range(V) if positions_wanted else ()):
             k = int(kinds[i])
             if k == 0:
                 positions.append(("A", int(ps[i]), [int(x) for x in pool[qo[i]:xo[i]]]))
@@ -212,7 +218,7 @@ def gamebuild(n_states: int, lrow, ldst, llbl, pairs):
         return {
             "V": V, "E": E, "n_start": c.n_start,
             "row": arr(c.row, V + 1), "col": arr(c.col, E), "wt": arr(c.wt, E), "atk": arr(c.atk, V),
-            "positions": positions,
+            "positions": positions, "truncated": bool(trunc.value),
         }
     finally:
         lib().orc_game_free(g)

@@ -142,6 +142,12 @@ def test_truncated_build_is_well_formed():
     V = b.n_positions()
     assert V > 2000 and h["row"][-1] == len(h["col"]) and np.all(np.diff(h["row"].astype(np.int64)) >= 0)
     assert h["col"].max() < V
+    # the same cut as the oracle's capped breadth-first walk: identical game
+    og = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs, max_positions=2000,
+                       positions_wanted=False)
+    assert og["truncated"] and og["V"] == V
+    for k in ("row", "col", "wt"):
+        assert np.array_equal(h[k], og[k]), k
 
 
 def test_full_spectroscopy_verdict_matrix():
