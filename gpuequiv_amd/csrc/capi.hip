@@ -116,6 +116,7 @@ geq_status geq_ctx_create(int device, geq_ctx** out) {
 void geq_ctx_destroy(geq_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    ctx->build_ws.reset();   // its device buffers, before the stream goes
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

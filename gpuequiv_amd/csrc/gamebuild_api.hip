@@ -38,7 +38,7 @@ __global__ void k_rev_scatter(const uint32_t* row, uint32_t V, const uint32_t* c
 template <class T>
 geq_status ensure(DevBuf<T>& b, size_t need, hipStream_t s, bool keep, const char* what) {
     if (b.p && b.n >= need) return GEQ_OK;
-    size_t cap = std::max<size_t>(need, b.n ? b.n + b.n / 2 : 1024);
+    size_t cap = std::max<size_t>(need, b.n ? 2 * b.n : 1024);
     T* p = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * cap);
     if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("gamebuild: allocating ") + what + ": " + hipGetErrorString(e));
@@ -324,8 +324,18 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     };
 
     const uint32_t tb = 256;
-    GB_TRY(node_cap(std::max<uint32_t>(n_pairs, 1)));
-    GB_TRY(ensure(B->spool, std::max<uint32_t>(n_pairs, 1) + 1024, s, false, "set pool"));
+    // a standalone build's arrays start at the size of the context's last
+    // build (repeated builds of similar games then allocate once)
+    const uint64_t hint_cap = 1ull << 28;
+    const uint64_t hv = jt ? 0 : std::min<uint64_t>(ctx->hint_positions.load(), hint_cap);
+    const uint64_t he = jt ? 0 : std::min<uint64_t>(ctx->hint_edges.load(), hint_cap);
+    const uint64_t hw = jt ? 0 : std::min<uint64_t>(ctx->hint_words.load(), hint_cap);
+    GB_TRY(node_cap(std::max<uint64_t>(std::max<uint32_t>(n_pairs, 1), hv + 1)));
+    GB_TRY(ensure(B->spool, std::max<uint64_t>((uint64_t)std::max<uint32_t>(n_pairs, 1) + 1024, hw + 1), s, false, "set pool"));
+    if (he) {
+        GB_TRY(ensure(B->col, he + 1, s, false, "edges"));
+        GB_TRY(ensure(B->wt, he + 1, s, false, "edges"));
+    }
     GB_TRY(table_cap(std::max<uint32_t>(n_pairs, 1)));
     if (n_pairs) {
         GB_TRY(ensure(d_pairs, 2ull * n_pairs, s, false, "start pairs"));
@@ -586,13 +596,26 @@ geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs
     *out = nullptr;
     geq_status st = validate_job(lts, pairs, n_pairs);
     if (st != GEQ_OK) return st;
-    BuildScratch ws;
+    // the context's scratch when no other build holds it
+    std::unique_lock<std::mutex> lk(ctx->build_mu, std::try_to_lock);
+    std::unique_ptr<BuildScratch> own;
+    BuildScratch* ws = nullptr;
+    if (lk.owns_lock()) {
+        if (!ctx->build_ws) ctx->build_ws = std::make_shared<BuildScratch>();
+        ws = ctx->build_ws.get();
+    } else {
+        own = std::make_unique<BuildScratch>();
+        ws = own.get();
+    }
     geq_built* B = new geq_built();
-    st = build_core(ctx, lts, pairs, n_pairs, max_positions, nullptr, B, ws, stats);
+    st = build_core(ctx, lts, pairs, n_pairs, max_positions, nullptr, B, *ws, stats);
     if (st != GEQ_OK) {
         delete B;
         return st;
     }
+    ctx->hint_positions = B->V;
+    ctx->hint_edges = B->E;
+    ctx->hint_words = B->spool_top;
     *out = B;
     return GEQ_OK;
 }

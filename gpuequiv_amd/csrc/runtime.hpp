@@ -2,7 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -92,11 +95,23 @@ uint32_t retag_interval();
 }  // namespace geq
 
 struct geq_game;
+namespace geq {
+namespace gb {
+struct BuildScratch;
+}
+}  // namespace geq
+
 struct geq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string name;
     int n_cu = 0;
+    // geq_gamebuild's device scratch, kept between builds (one build at a
+    // time uses it; a concurrent build takes scratch of its own), and the
+    // sizes of the last build, which size the next one's arrays up front
+    std::mutex build_mu;
+    std::shared_ptr<geq::gb::BuildScratch> build_ws;
+    std::atomic<uint64_t> hint_positions{0}, hint_edges{0}, hint_words{0};
 };
 
 struct geq_game {
