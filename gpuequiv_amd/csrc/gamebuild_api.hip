@@ -351,10 +351,23 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         k_job_count<<<nb, tb, 0, s>>>(B->p.p, 0, n_pairs, d_job_of_state.p, n_jobs, d_jcnt.p);
     }
     uint64_t E = 0, levels = 0;
-    uint32_t lo = 0, hi = n_pairs;
+    uint32_t lo = 0, hi = n_pairs;   // [lo, hi): the rest of the current level's frontier
     bool truncated = false;
+    // Merged builds take a level in chunks of frontier positions and check the
+    // per-job caps after each chunk, so a job that passes its cap stops within
+    // the level instead of finishing it.  Chunks of one level, in parent
+    // order, create exactly the positions, ids and edges the whole level would
+    // (dedup across chunks goes through the position table).
+    uint32_t chunk = (jt && jt->cap) ? (uint32_t)std::min<uint64_t>(std::max<uint64_t>(32768, jt->cap / 8), 0x7fffffffu)
+                                     : 0xffffffffu;
+    if (jt)
+        if (const char* env = std::getenv("GEQ_BUILD_CHUNK")) {   // small chunks exercise the path in tests
+            const long long v = std::atoll(env);
+            if (v >= 1) chunk = (uint32_t)std::min<long long>(v, 0x7fffffffll);
+        }
     while (lo < hi) {
-        const uint32_t nf = hi - lo;
+        const uint32_t c_hi = hi - lo > chunk ? lo + chunk : hi;
+        const uint32_t nf = c_hi - lo;
         GB_TRY(ensure(ub_c, nf + 1, s, false, "frontier"));
         GB_TRY(ensure(ub_s, nf + 1, s, false, "frontier"));
         GB_TRY(ensure(c_off, nf + 1, s, false, "frontier"));
@@ -402,9 +415,12 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         E += n_edges;
         B->spool_top += n_words;
         B->V += n_new;
-        lo = hi;
-        hi = B->V;
-        levels++;
+        lo = c_hi;
+        const bool level_done = lo == hi;
+        if (level_done) {        // the next level is every position this one created
+            hi = B->V;
+            levels++;
+        }
         if (jt) {
             // positions of every job so far (read with the counts above); jobs past the cap stop growing
             bool newly_dead = false;
@@ -417,7 +433,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             }
             if (newly_dead)
                 GB_HIP(hipMemcpyAsync(d_dead.p, jt->dead.data(), n_jobs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-        } else if (max_positions && B->V > max_positions) {
+        } else if (level_done && max_positions && B->V > max_positions) {
             truncated = true;
             break;
         }

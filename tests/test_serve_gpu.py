@@ -11,7 +11,7 @@ from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
-NODE_BYTES, EDGE_BYTES = 62, 12          # per-slot bytes of the position / edge pools
+NODE_BYTES, EDGE_BYTES = 87, 12          # per-slot bytes of the position / edge pools
 
 
 def _budget(positions, edges_per_position):
@@ -146,4 +146,37 @@ def test_invalid_and_empty_jobs_finish_alone():
     assert H.canon_lists(*res[ids[0]][1]) == jobs[0][2]
     assert H.canon_lists(*res[ids[3]][1]) == jobs[2][2]
     assert srv.stats()["builds"] >= 1
+    srv.close()
+
+
+def test_chunked_levels_and_the_position_cap(monkeypatch):
+    # Tiny chunks: every level of the merged builds runs in pieces of 37
+    # frontier positions, with the per-job caps checked after each piece.  The
+    # jobs within the cap give exactly their standalone results; a job past it
+    # fails alone.
+    print("serve: chunked levels, one job over the position cap", flush=True)
+    monkeypatch.setenv("GEQ_BUILD_CHUNK", "37")
+    jobs = _jobs(10, 8)
+    cap = max(size[0] for *_, size in jobs)
+    rng = np.random.default_rng(99)
+    big = None
+    for k in range(40):
+        n, src, dst, lbl = workloads.random_lts(120, 360, 4, 9000 + k)
+        lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+        pairs = [(int(p), int(q)) for p, q in rng.integers(0, n, (8, 2)) if p != q]
+        if pairs and _alone(lts, pairs, cap)[0] is None:
+            big = (lts, pairs)
+            break
+    if big is None:
+        pytest.skip("no game over the cap found")
+    srv = Server(budget_bytes=256 << 20, rounds_per_step=2, max_positions=cap)
+    ids = [srv.submit(lts, pairs) for lts, pairs, _, _ in jobs]
+    big_id = srv.submit(*big)
+    res = srv.run_until_done(ids + [big_id])
+    assert res[big_id][0]["state_name"] == "failed"
+    for k, j in enumerate(ids):
+        info, r = res[j]
+        assert info["state_name"] == "done", info
+        assert (info["positions"], info["edges"]) == jobs[k][3], k
+        assert H.canon_lists(*r) == jobs[k][2], k
     srv.close()
