@@ -304,9 +304,10 @@ __global__ void k_solve_init(uint4* st, uint32_t V, int clear_slots, uint32_t* q
                              unsigned long long* stats, uint32_t nstats, Ctl* ctl, uint32_t arena_top) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
     if (clear_slots)
-        for (uint32_t v = tid; v < V; v += nt) {
+        for (uint32_t v = tid; v < V; v += nt) {   // the record after its header: both slots and the spare piece
             st[slot_index(v, 0)] = make_uint4(0, 0, 0, 0);
             st[slot_index(v, 1)] = make_uint4(0, 0, 0, 0);
+            st[slot_index(v, 2)] = make_uint4(0, 0, 0, 0);
         }
     for (uint32_t i = tid; i < qwords2; i += nt) queued[i] = 0u;
     for (uint32_t i = tid; i < nstats; i += nt) stats[i] = 0ull;
@@ -587,7 +588,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     {
         const uint32_t q2 = 2u * queue_words(V), ns = 4u * STAT_SLOTS + 4u;
         const uint32_t most = std::max(std::max(V, q2), ns);
-        k_solve_init<<<std::min(4096u, (most + 255) / 256), 256, 0, s>>>(g->st.p, V, initial ? 0 : 1, g->queued.p, q2, g->stats.p,
+        k_solve_init<<<std::min(65536u, (most + 255) / 256), 256, 0, s>>>(g->st.p, V, initial ? 0 : 1, g->queued.p, q2, g->stats.p,
                                                                          ns, g->ctl.p, c0.arena_top);
     }
     if (!initial && n_reach) k_seed_zero<<<(n_reach + 255) / 256, 256, 0, s>>>(g->st.p, seed, n_reach);
