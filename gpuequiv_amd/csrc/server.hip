@@ -313,8 +313,11 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
                 pos += B->n_pos[k];
                 dead += B->dead[k];
             }
-            std::fprintf(stderr, "[geq server] t=%.1f ms build of %zu jobs: %llu positions, %u levels, %llu over the cap, %.1f ms (status %d)\n",
-                         tb0, build.size(), (unsigned long long)pos, B->levels, (unsigned long long)dead, S->now_ms() - tb0, (int)st);
+            std::fprintf(stderr,
+                         "[geq server] t=%.1f ms build of %zu jobs: %llu positions, %u levels, %llu over the cap, %.1f ms "
+                         "(level loop %.1f ms; status %d)\n",
+                         tb0, build.size(), (unsigned long long)pos, B->levels, (unsigned long long)dead, S->now_ms() - tb0,
+                         B->build_ms, (int)st);
         }
         std::lock_guard<std::mutex> lk(S->mu);
         const double now = S->now_ms();
