@@ -5,8 +5,8 @@
  * Gobbel2000/gpuequiv v1.0.0.  It replaces the crate's wgpu device / buffer /
  * pipeline layer (src/energygame/gpuutils.rs, the private GPURunner,
  * AttackShader and DefendIterShader of src/energygame.rs and the four WGSL
- * kernels).  A Rust GPURunner replacement would bind these functions with
- * `extern "C"`; the binding is shown in INTEGRATION.md.
+ * kernels).  A Rust GPURunner replacement binds these functions through an
+ * `extern "C"` block with the same signatures and #[repr(C)] structs.
  *
  * Conventions
  *  - All arrays are plain host pointers + sizes; the caller keeps ownership of
