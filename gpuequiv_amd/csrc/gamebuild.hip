@@ -166,8 +166,29 @@ __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, u
     ub_spool[i] = s;
 }
 
+// Dynamic LDS: 0, or 4 * lts_words(...) bytes for a copy of the LTS, which
+// the block then reads instead of global memory.
 __global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint32_t* cand_off,
                            const uint32_t* spool_off, uint64_t spool_base, Cand* cands, uint32_t* nsucc) {
+    extern __shared__ uint32_t lts_lds[];
+    if (lts_words(L.n, L.m, L.n_en) <= LTS_LDS_WORDS) {   // the host then passed LDS for the copy
+        uint32_t* r = lts_lds;
+        uint32_t* d = r + (L.n + 1);
+        int32_t* lb = reinterpret_cast<int32_t*>(d + L.m);
+        uint32_t* er = reinterpret_cast<uint32_t*>(lb + L.m);
+        int32_t* el = reinterpret_cast<int32_t*>(er + (L.n + 1));
+        for (uint32_t k = threadIdx.x; k <= L.n; k += blockDim.x) {
+            r[k] = L.row[k];
+            er[k] = L.en_row[k];
+        }
+        for (uint32_t k = threadIdx.x; k < L.m; k += blockDim.x) {
+            d[k] = L.dst[k];
+            lb[k] = L.lbl[k];
+        }
+        for (uint32_t k = threadIdx.x; k < L.n_en; k += blockDim.x) el[k] = L.en_lbl[k];
+        __syncthreads();
+        L.row = r; L.dst = d; L.lbl = lb; L.en_row = er; L.en_lbl = el;
+    }
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == n_front) nsucc[i] = 0;   // trailing zero: the exclusive scan yields the edge total
     if (i >= n_front) return;

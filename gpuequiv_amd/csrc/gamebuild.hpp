@@ -24,7 +24,13 @@ struct LtsArgs {
     // stopped growing (NULL for a single game)
     const uint32_t* job_of_state;
     const uint8_t* job_dead;
+    // sizes for a copy into LDS (k_generate): states, transitions, enabled labels
+    uint32_t n, m, n_en;
 };
+// k_generate copies the LTS (row, dst, lbl, en_row, en_lbl) into LDS when it
+// takes at most this many words.
+constexpr uint32_t LTS_LDS_WORDS = 12 * 1024;
+__host__ __device__ inline uint32_t lts_words(uint32_t n, uint32_t m, uint32_t n_en) { return 2u * (n + 1u) + 2u * m + n_en; }
 
 // Position records (structure of arrays).  Sets Q / Q* live in the set pool:
 // Attack(p, Q) = (ATTACK, p, Q, -); Clause(p, q) = (CLAUSE, p, [q], -);

@@ -286,7 +286,11 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         GB_HIP(hipMemsetAsync(d_jcnt.p, 0, 4ull * n_jobs, s), GEQ_ERR_TRANSFER);
         GB_HIP(hipMemcpyAsync(d_dead.p, jt->dead.data(), n_jobs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
-    LtsArgs L{l_row.p, l_dst.p, l_lbl.p, l_enrow.p, l_enlbl.p, jt ? d_job_of_state.p : nullptr, jt ? d_dead.p : nullptr};
+    LtsArgs L{l_row.p, l_dst.p, l_lbl.p, l_enrow.p, l_enlbl.p, jt ? d_job_of_state.p : nullptr, jt ? d_dead.p : nullptr,
+              n, m, (uint32_t)en_lbl.size()};
+    // a small LTS is read from LDS by k_generate (its walks are chains of dependent reads)
+    const uint32_t gen_lds = lts_words(n, m, (uint32_t)en_lbl.size()) <= LTS_LDS_WORDS
+                                 ? 4u * lts_words(n, m, (uint32_t)en_lbl.size()) : 0u;
     std::vector<uint32_t> h_jcnt(n_jobs);
 
     // node, edge and set-pool capacities grow level by level
@@ -389,7 +393,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (E + n_cand >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 edges"));
         GB_TRY(ensure(B->col, E + n_cand + 1, s, true, "edges"));
         GB_TRY(ensure(B->wt, E + n_cand + 1, s, true, "edges"));
-        k_generate<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, c_off.p, s_off.p, B->spool_top, cands.p, nsucc.p);
+        k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, c_off.p, s_off.p, B->spool_top, cands.p, nsucc.p);
         GB_TRY(exclusive_scan<uint32_t>(nsucc.p, rowl.p, nf + 1, tmp, s));
         if (n_cand) {
             k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
