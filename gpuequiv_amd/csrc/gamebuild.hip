@@ -60,6 +60,17 @@ __device__ uint32_t pos_hash(uint32_t kind, uint32_t p, const uint32_t* __restri
     return h == 0xffffffffu ? 0x7fffffffu : h;
 }
 
+// The same hash of a position with set Q in LDS (entry k at q[k * st]) and no Q*.
+__device__ uint32_t pos_hash_lds(uint32_t kind, uint32_t p, const uint32_t* q, uint32_t ql, uint32_t st) {
+    uint32_t h = mix(0x811c9dc5u, kind);
+    h = mix(h, p);
+    h = mix(h, ql);
+    for (uint32_t k = 0; k < ql; k++) h = mix(h, q[k * st]);
+    h = mix(h, 0u);
+    h ^= h >> 16;
+    return h == 0xffffffffu ? 0x7fffffffu : h;
+}
+
 __device__ bool sets_equal(const uint32_t* __restrict__ spool, uint32_t ao, uint32_t bo, uint32_t len) {
     for (uint32_t k = 0; k < len; k++)
         if (spool[ao + k] != spool[bo + k]) return false;
@@ -167,10 +178,19 @@ __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, u
 }
 
 // Dynamic LDS: 0, or 4 * lts_words(...) bytes for a copy of the LTS, which
-// the block then reads instead of global memory.
-__global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint32_t* cand_off,
-                           const uint32_t* spool_off, uint64_t spool_base, Cand* cands, uint32_t* nsucc) {
+// the block then reads instead of global memory.  Each thread also keeps the
+// successor set Q' of one label group in LDS while it has at most GEN_LDS
+// entries (entry k of thread t at qbuf[k * GEN_BLOCK + t]): it is built
+// sorted and duplicate-free there, probed and hashed there, and written to the
+// set pool once.  Larger sets take the same steps in the set pool.
+constexpr uint32_t GEN_BLOCK = 256, GEN_LDS = 16;
+
+__global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L,
+                                                        const uint32_t* cand_off, const uint32_t* spool_off, uint64_t spool_base,
+                                                        Cand* cands, uint32_t* nsucc) {
     extern __shared__ uint32_t lts_lds[];
+    __shared__ uint32_t s_qbuf[GEN_LDS * GEN_BLOCK];
+    uint32_t* const qb = s_qbuf + threadIdx.x;
     if (lts_words(L.n, L.m, L.n_en) <= LTS_LDS_WORDS) {   // the host then passed LDS for the copy
         uint32_t* r = lts_lds;
         uint32_t* d = r + (L.n + 1);
@@ -237,34 +257,87 @@ __global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L,
             const int32_t a = L.lbl[t];
             uint32_t g_end = t;
             while (g_end < t_end && L.lbl[g_end] == a) g_end++;
-            uint32_t T = 0;
+            // Q' = the union of the label-a successors of Q: sorted and
+            // duplicate-free in LDS while it fits, else gathered in the pool
+            uint32_t nq = 0, T = 0;
+            bool spill = false;
             for (uint32_t k = 0; k < ql; k++) {
                 uint32_t lo_q, hi_q;
                 label_range(L, spool[qo + k], a, lo_q, hi_q);
-                for (uint32_t x = lo_q; x < hi_q; x++) spool[sp + T++] = L.dst[x];
+                for (uint32_t x = lo_q; x < hi_q; x++) {
+                    const uint32_t v = L.dst[x];
+                    if (spill) {
+                        spool[sp + T++] = v;
+                        continue;
+                    }
+                    uint32_t j = nq;
+                    while (j > 0 && qb[(j - 1) * GEN_BLOCK] > v) j--;
+                    if (j > 0 && qb[(j - 1) * GEN_BLOCK] == v) continue;
+                    if (nq == GEN_LDS) {          // full: continue in the pool
+                        for (uint32_t y = 0; y < nq; y++) spool[sp + y] = qb[y * GEN_BLOCK];
+                        T = nq;
+                        spool[sp + T++] = v;
+                        spill = true;
+                        continue;
+                    }
+                    for (uint32_t y = nq; y > j; y--) qb[y * GEN_BLOCK] = qb[(y - 1) * GEN_BLOCK];
+                    qb[j * GEN_BLOCK] = v;
+                    nq++;
+                }
             }
-            uint32_t* qa = spool + sp;
-            sort_u32(qa, T);
-            uint32_t Tu = 0;
-            for (uint32_t x = 0; x < T; x++)
-                if (Tu == 0 || qa[Tu - 1] != qa[x]) qa[Tu++] = qa[x];
-            for (uint32_t tt = t; tt < g_end; tt++) {
-                const uint32_t pp = L.dst[tt];
-                if (!contains_sorted(qa, Tu, pp)) emit(K_ATTACK, pp, (uint32_t)sp, Tu, 0, 0, W_OBS);
+            if (!spill) {
+                for (uint32_t y = 0; y < nq; y++) spool[sp + y] = qb[y * GEN_BLOCK];
+                for (uint32_t tt = t; tt < g_end; tt++) {
+                    const uint32_t pp = L.dst[tt];
+                    bool in = false;
+                    for (uint32_t y = 0; y < nq; y++) in |= qb[y * GEN_BLOCK] == pp;
+                    if (!in) {
+                        Cand c;
+                        c.kind_p = (K_ATTACK << 30) | pp;
+                        c.q_off = (uint32_t)sp; c.q_len = nq; c.x_off = 0; c.x_len = 0;
+                        c.weight = W_OBS;
+                        c.hash = pos_hash_lds(K_ATTACK, pp, qb, nq, GEN_BLOCK);
+                        c.res = 0;
+                        c.parent = i;
+                        c.slot = slot;
+                        cands[cbase + slot] = c;
+                        slot++;
+                    }
+                }
+                sp += nq;
+            } else {
+                uint32_t* qa = spool + sp;
+                sort_u32(qa, T);
+                uint32_t Tu = 0;
+                for (uint32_t x = 0; x < T; x++)
+                    if (Tu == 0 || qa[Tu - 1] != qa[x]) qa[Tu++] = qa[x];
+                for (uint32_t tt = t; tt < g_end; tt++) {
+                    const uint32_t pp = L.dst[tt];
+                    if (!contains_sorted(qa, Tu, pp)) emit(K_ATTACK, pp, (uint32_t)sp, Tu, 0, 0, W_OBS);
+                }
+                sp += Tu;
             }
-            sp += Tu;
             t = g_end;
         }
         emit(K_DEFEND, p, qo, ql, 0, 0, W_CHAL);     // Q* = {}
         // conjunction challenges with partitioned Q* (gamepos.rs:329-375)
+        // per q: subset / superset flags, in two registers for |Q| <= 32, else in the pool
         const uint64_t fl = sbase + sd + 3ull * ql;  // per q: bit0 subset, bit1 superset
+        uint32_t sub_m = 0, sup_m = 0;
         for (uint32_t k = 0; k < ql; k++) {
             bool sub, sup;
             compare_enabled(L, p, spool[qo + k], sub, sup);
-            spool[fl + k] = (sub ? 1u : 0u) | (sup ? 2u : 0u);
+            if (ql <= 32) {
+                sub_m |= (sub ? 1u : 0u) << k;
+                sup_m |= (sup ? 1u : 0u) << k;
+            } else {
+                spool[fl + k] = (sub ? 1u : 0u) | (sup ? 2u : 0u);
+            }
         }
         auto in_x = [&](uint32_t which, uint32_t k) -> bool {
-            uint32_t f = spool[fl + k];
+            uint32_t f;
+            if (ql <= 32) f = ((sub_m >> k) & 1u) | (((sup_m >> k) & 1u) << 1);
+            else f = spool[fl + k];
             return which == 0 ? (f & 1u) : which == 1 ? (f & 2u) : (f == 3u);
         };
         uint64_t pb = sbase + sd;
