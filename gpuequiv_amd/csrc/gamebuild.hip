@@ -148,12 +148,11 @@ __device__ __forceinline__ bool contains_sorted(const uint32_t* a, uint32_t n, u
 
 __device__ __forceinline__ uint32_t deg_of(const LtsArgs& L, uint32_t s) { return L.row[s + 1] - L.row[s]; }
 
-__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint32_t* ub_cand, uint32_t* ub_spool) {
+__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n_front) return;
-    if (i == n_front) {        // trailing zero so the exclusive scans yield the totals
-        ub_cand[i] = 0;
-        ub_spool[i] = 0;
+    if (i == n_front) {        // trailing zero so the exclusive scan yields the totals
+        ub[i] = make_uint2(0u, 0u);
         return;
     }
     const uint32_t u = lo + i;
@@ -173,8 +172,7 @@ __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, u
         for (uint32_t k = 0; k < ql; k++) sd += deg_of(L, N.spool[N.q_off[u] + k]);
         s = sd + 4 * ql;
     }
-    ub_cand[i] = c;
-    ub_spool[i] = s;
+    ub[i] = make_uint2(c, s);
 }
 
 // Dynamic LDS: 0, or 4 * lts_words(...) bytes for a copy of the LTS, which
@@ -186,7 +184,7 @@ __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, u
 constexpr uint32_t GEN_BLOCK = 256, GEN_LDS = 16;
 
 __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L,
-                                                        const uint32_t* cand_off, const uint32_t* spool_off, uint64_t spool_base,
+                                                        const uint2* off, uint64_t spool_base,
                                                         Cand* cands, uint32_t* nsucc) {
     extern __shared__ uint32_t lts_lds[];
     __shared__ uint32_t s_qbuf[GEN_LDS * GEN_BLOCK];
@@ -215,8 +213,9 @@ __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo,
     const uint32_t u = lo + i;
     const uint32_t kind = N.kind[u], p = N.p[u], qo = N.q_off[u], ql = N.q_len[u], xo = N.x_off[u], xl = N.x_len[u];
     uint32_t* spool = N.spool;
-    const uint32_t cbase = cand_off[i], cend = cand_off[i + 1];
-    const uint64_t sbase = spool_base + spool_off[i];
+    const uint2 o0 = off[i];
+    const uint32_t cbase = o0.x, cend = off[i + 1].x;
+    const uint64_t sbase = spool_base + o0.y;
     uint32_t slot = 0;
     if (L.job_dead && L.job_dead[L.job_of_state[p]]) {   // k_bounds gave it no room
         nsucc[i] = 0;

@@ -56,9 +56,12 @@ struct Cand {               // one successor candidate of a frontier position
     uint32_t slot;          // successor index within the parent
 };
 
-__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint32_t* ub_cand, uint32_t* ub_spool);
-__global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint32_t* cand_off,
-                           const uint32_t* spool_off, uint64_t spool_base, Cand* cands, uint32_t* nsucc);
+// Per frontier position: upper bounds on its candidates (x) and on the set-pool
+// words its successor sets take (y); entry n_front is zero.
+__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub);
+// `off`: the exclusive scan of k_bounds' pairs (candidate and set-pool offsets).
+__global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint2* off, uint64_t spool_base,
+                           Cand* cands, uint32_t* nsucc);
 __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask);
 __global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag);
 __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,
@@ -75,8 +78,26 @@ __global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk);
 
 // Device buffers of one build (level loop and job extraction), kept by a
 // caller that builds repeatedly so that steady-state builds allocate nothing.
+// Pinned host words that kernels write directly (the level loop's counts).
+struct HostWords {
+    uint32_t* h = nullptr;   // host view
+    uint32_t* d = nullptr;   // device view of the same memory
+    size_t n = 0;
+    HostWords() = default;
+    HostWords(const HostWords&) = delete;
+    HostWords& operator=(const HostWords&) = delete;
+    ~HostWords() { release(); }
+    void release() {
+        if (h) (void)hipHostFree(h);
+        h = d = nullptr;
+        n = 0;
+    }
+};
+
 struct BuildScratch {
-    DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, ub_c, ub_s, c_off, s_off, nsucc, rowl, flag, rank, keymin, d_jcnt;
+    DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, nsucc, rowl, flag, rank, keymin, d_jcnt;
+    DevBuf<uint2> ub, off;
+    HostWords rb;
     DevBuf<int32_t> l_lbl, l_enlbl;
     DevBuf<uint8_t> d_insert, tmp, d_dead;
     DevBuf<Cand> cands;

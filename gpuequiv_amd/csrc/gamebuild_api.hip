@@ -66,17 +66,27 @@ geq_status exclusive_scan(const T* in, T* out, size_t n, DevBuf<uint8_t>& tmp, h
     return GEQ_OK;
 }
 
-// several device words with one stream synchronisation
-void read_u32s(std::initializer_list<std::pair<const uint32_t*, uint32_t*>> words, hipStream_t s) {
-    for (const auto& w : words) (void)hipMemcpyAsync(w.second, w.first, 4, hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
+struct PairSum {
+    __host__ __device__ uint2 operator()(const uint2& a, const uint2& b) const { return make_uint2(a.x + b.x, a.y + b.y); }
+};
+
+// Exclusive scan of (candidates, set-pool words) pairs: one scan for both.
+geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<uint8_t>& tmp, hipStream_t s) {
+    size_t bytes = 0;
+    GEQ_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, in, out, PairSum{}, make_uint2(0u, 0u), n, s),
+            GEQ_ERR_INTERNAL);
+    geq_status st = ensure(tmp, bytes + 16, s, false, "scan scratch");
+    if (st != GEQ_OK) return st;
+    GEQ_HIP(hipcub::DeviceScan::ExclusiveScan(tmp.p, bytes, in, out, PairSum{}, make_uint2(0u, 0u), n, s),
+            GEQ_ERR_INTERNAL);
+    return GEQ_OK;
 }
 
-uint32_t read_u32(const uint32_t* d, hipStream_t s) {
-    uint32_t v = 0;
-    (void)hipMemcpyAsync(&v, d, 4, hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    return v;
+// Writes *a, *b and cnt[0 .. n_cnt) to out[0], out[1], out[2 ..] (pinned host
+// memory): a level's counts reach the host with one launch and one wait.
+__global__ void k_readback(const uint32_t* a, const uint32_t* b, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out) {
+    for (uint32_t i = threadIdx.x; i < 2u + n_cnt; i += blockDim.x)
+        out[i] = i == 0u ? *a : i == 1u ? *b : cnt[i - 2u];
 }
 
 }  // namespace
@@ -245,7 +255,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     B->V = B->E = 0;
     B->spool_top = 0;
     auto& l_row = ws.l_row; auto& l_dst = ws.l_dst; auto& l_enrow = ws.l_enrow; auto& d_pairs = ws.d_pairs;
-    auto& ub_c = ws.ub_c; auto& ub_s = ws.ub_s; auto& c_off = ws.c_off; auto& s_off = ws.s_off; auto& nsucc = ws.nsucc;
+    auto& ub = ws.ub; auto& off = ws.off; auto& nsucc = ws.nsucc;
     auto& rowl = ws.rowl; auto& flag = ws.flag; auto& rank = ws.rank; auto& keymin = ws.keymin;
     auto& l_lbl = ws.l_lbl; auto& l_enlbl = ws.l_enlbl; auto& d_insert = ws.d_insert; auto& tmp = ws.tmp;
     auto& cands = ws.cands; auto& table = ws.table;
@@ -291,7 +301,21 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     // a small LTS is read from LDS by k_generate (its walks are chains of dependent reads)
     const uint32_t gen_lds = lts_words(n, m, (uint32_t)en_lbl.size()) <= LTS_LDS_WORDS
                                  ? 4u * lts_words(n, m, (uint32_t)en_lbl.size()) : 0u;
-    std::vector<uint32_t> h_jcnt(n_jobs);
+    // the level loop's counts come back through pinned words written by k_readback
+    if (ws.rb.n < 2u + n_jobs) {
+        ws.rb.release();
+        const size_t words = std::max<size_t>(64, 2u + n_jobs);
+        GB_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws.rb.h), 4 * words, hipHostMallocMapped | hipHostMallocCoherent),
+               GEQ_ERR_OUT_OF_MEMORY);
+        ws.rb.n = words;
+        GB_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ws.rb.d), ws.rb.h, 0), GEQ_ERR_INTERNAL);
+    }
+    const uint32_t* const h_rb = ws.rb.h;
+    auto readback = [&](const uint32_t* a, const uint32_t* b, const uint32_t* cnt, uint32_t n_cnt) -> geq_status {
+        k_readback<<<1, 256, 0, s>>>(a, b, cnt, n_cnt, ws.rb.d);
+        GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+        return GEQ_OK;
+    };
 
     // node, edge and set-pool capacities grow level by level
     auto node_cap = [&](size_t need) -> geq_status {
@@ -372,17 +396,14 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     while (lo < hi) {
         const uint32_t c_hi = hi - lo > chunk ? lo + chunk : hi;
         const uint32_t nf = c_hi - lo;
-        GB_TRY(ensure(ub_c, nf + 1, s, false, "frontier"));
-        GB_TRY(ensure(ub_s, nf + 1, s, false, "frontier"));
-        GB_TRY(ensure(c_off, nf + 1, s, false, "frontier"));
-        GB_TRY(ensure(s_off, nf + 1, s, false, "frontier"));
+        GB_TRY(ensure(ub, nf + 1, s, false, "frontier"));
+        GB_TRY(ensure(off, nf + 1, s, false, "frontier"));
         GB_TRY(ensure(nsucc, nf + 1, s, false, "frontier"));
         GB_TRY(ensure(rowl, nf + 1, s, false, "frontier"));
-        k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub_c.p, ub_s.p);
-        GB_TRY(exclusive_scan<uint32_t>(ub_c.p, c_off.p, nf + 1, tmp, s));
-        GB_TRY(exclusive_scan<uint32_t>(ub_s.p, s_off.p, nf + 1, tmp, s));
-        uint32_t n_cand = 0, n_words = 0;
-        read_u32s({{c_off.p + nf, &n_cand}, {s_off.p + nf, &n_words}}, s);
+        k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub.p);
+        GB_TRY(exclusive_scan_pairs(ub.p, off.p, nf + 1, tmp, s));
+        GB_TRY(readback(&off.p[nf].x, &off.p[nf].y, nullptr, 0));
+        const uint32_t n_cand = h_rb[0], n_words = h_rb[1];
         if ((uint64_t)B->spool_top + n_words >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: set pool exceeds 2^32 words"));
         GB_TRY(ensure(B->spool, B->spool_top + n_words + 1, s, true, "set pool"));
         GB_TRY(ensure(cands, std::max<uint32_t>(n_cand, 1), s, false, "candidates"));
@@ -393,7 +414,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (E + n_cand >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 edges"));
         GB_TRY(ensure(B->col, E + n_cand + 1, s, true, "edges"));
         GB_TRY(ensure(B->wt, E + n_cand + 1, s, true, "edges"));
-        k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, c_off.p, s_off.p, B->spool_top, cands.p, nsucc.p);
+        k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, off.p, B->spool_top, cands.p, nsucc.p);
         GB_TRY(exclusive_scan<uint32_t>(nsucc.p, rowl.p, nf + 1, tmp, s));
         if (n_cand) {
             k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
@@ -413,9 +434,8 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         }
         k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p);
         // one synchronisation per level for the edge and position counts (and the per-job counts)
-        uint32_t n_edges = 0, n_new = 0;
-        if (jt) GB_HIP(hipMemcpyAsync(h_jcnt.data(), d_jcnt.p, 4ull * n_jobs, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
-        read_u32s({{rowl.p + nf, &n_edges}, {rank.p + n_cand, &n_new}}, s);
+        GB_TRY(readback(rowl.p + nf, rank.p + n_cand, jt ? d_jcnt.p : nullptr, n_jobs));
+        const uint32_t n_edges = h_rb[0], n_new = h_rb[1];
         E += n_edges;
         B->spool_top += n_words;
         B->V += n_new;
@@ -429,8 +449,8 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             // positions of every job so far (read with the counts above); jobs past the cap stop growing
             bool newly_dead = false;
             for (uint32_t j = 0; j < n_jobs; j++) {
-                jt->count[j] = h_jcnt[j];
-                if (!jt->dead[j] && jt->cap && h_jcnt[j] > jt->cap) {
+                jt->count[j] = h_rb[2 + j];
+                if (!jt->dead[j] && jt->cap && jt->count[j] > jt->cap) {
                     jt->dead[j] = 1;
                     newly_dead = true;
                 }
