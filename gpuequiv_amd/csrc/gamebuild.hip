@@ -148,14 +148,7 @@ __device__ __forceinline__ bool contains_sorted(const uint32_t* a, uint32_t n, u
 
 __device__ __forceinline__ uint32_t deg_of(const LtsArgs& L, uint32_t s) { return L.row[s + 1] - L.row[s]; }
 
-__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n_front) return;
-    if (i == n_front) {        // trailing zero so the exclusive scan yields the totals
-        ub[i] = make_uint2(0u, 0u);
-        return;
-    }
-    const uint32_t u = lo + i;
+__device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, uint32_t u) {
     const uint32_t kind = N.kind[u], p = N.p[u], ql = N.q_len[u], xl = N.x_len[u];
     uint32_t c = 0, s = 0;
     if (L.job_dead && L.job_dead[L.job_of_state[p]]) {
@@ -172,7 +165,22 @@ __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, u
         for (uint32_t k = 0; k < ql; k++) sd += deg_of(L, N.spool[N.q_off[u] + k]);
         s = sd + 4 * ql;
     }
-    ub[i] = make_uint2(c, s);
+    return make_uint2(c, s);
+}
+
+__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n_front) return;
+    // trailing zero so the exclusive scan yields the totals
+    ub[i] = i == n_front ? make_uint2(0u, 0u) : bounds_of(N, L, lo + i);
+}
+
+__global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, uint32_t add, uint32_t n_slots, LtsArgs L,
+                              uint2* ub) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    const uint32_t n_front = *n_new + add;
+    ub[i] = i < n_front ? bounds_of(N, L, lo + i) : make_uint2(0u, 0u);
 }
 
 // Dynamic LDS: 0, or 4 * lts_words(...) bytes for a copy of the LTS, which

@@ -59,6 +59,12 @@ struct Cand {               // one successor candidate of a frontier position
 // Per frontier position: upper bounds on its candidates (x) and on the set-pool
 // words its successor sets take (y); entry n_front is zero.
 __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub);
+// The same for a frontier of *n_new + add positions from `lo` whose size only
+// the device knows: entries [n_front, n_slots) are zero, so the exclusive scan
+// of all n_slots entries has the frontier's offsets in front and the totals
+// from entry n_front on.
+__global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, uint32_t add, uint32_t n_slots, LtsArgs L,
+                              uint2* ub);
 // `off`: the exclusive scan of k_bounds' pairs (candidate and set-pool offsets).
 __global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint2* off, uint64_t spool_base,
                            Cand* cands, uint32_t* nsucc);
@@ -76,8 +82,6 @@ __global__ void k_fill_u64(unsigned long long* a, uint64_t n, unsigned long long
 __global__ void k_fill_u32(uint32_t* a, uint64_t n, uint32_t v);
 __global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk);
 
-// Device buffers of one build (level loop and job extraction), kept by a
-// caller that builds repeatedly so that steady-state builds allocate nothing.
 // Pinned host words that kernels write directly (the level loop's counts).
 struct HostWords {
     uint32_t* h = nullptr;   // host view
@@ -94,6 +98,8 @@ struct HostWords {
     }
 };
 
+// Device buffers of one build (level loop and job extraction), kept by a
+// caller that builds repeatedly so that steady-state builds allocate nothing.
 struct BuildScratch {
     DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, nsucc, rowl, flag, rank, keymin, d_jcnt;
     DevBuf<uint2> ub, off;

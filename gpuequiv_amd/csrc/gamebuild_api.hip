@@ -82,11 +82,42 @@ geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<ui
     return GEQ_OK;
 }
 
-// Writes *a, *b and cnt[0 .. n_cnt) to out[0], out[1], out[2 ..] (pinned host
-// memory): a level's counts reach the host with one launch and one wait.
-__global__ void k_readback(const uint32_t* a, const uint32_t* b, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out) {
-    for (uint32_t i = threadIdx.x; i < 2u + n_cnt; i += blockDim.x)
-        out[i] = i == 0u ? *a : i == 1u ? *b : cnt[i - 2u];
+// Writes *a and *b to out[0] and out[1] (pinned host memory): a chunk's
+// candidate and set-pool totals reach the host with one launch and one wait.
+__global__ void k_readback(const uint32_t* a, const uint32_t* b, uint32_t* out) {
+    if (threadIdx.x == 0) out[0] = *a;
+    if (threadIdx.x == 1) out[1] = *b;
+}
+
+// The counts at the end of a chunk, in pinned host memory: out[0] its edges,
+// out[1] its new positions, out[4 ..] the per-job position counts cnt.  With
+// `next` (the offsets k_bounds_next scanned for the next level, whose
+// *n_new + add positions are taken in chunks of `chunk`), out[2] and out[3] are
+// the candidate and set-pool totals of the next level's first chunk.
+__global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
+                               const uint32_t* cnt, uint32_t n_cnt, uint32_t* out) {
+    for (uint32_t i = threadIdx.x; i < 4u + n_cnt; i += blockDim.x) {
+        uint32_t v = 0;
+        if (i == 0u) {
+            v = *n_edges;
+        } else if (i == 1u) {
+            v = *n_new;
+        } else if (i < 4u) {
+            if (next) {
+                const uint2 t = next[min(*n_new + add, chunk)];
+                v = i == 2u ? t.x : t.y;
+            }
+        } else {
+            v = cnt[i - 4u];
+        }
+        out[i] = v;
+    }
+}
+
+// Jobs whose position count passed the cap get no further successors.
+__global__ void k_mark_dead(const uint32_t* cnt, uint32_t n_jobs, uint64_t cap, uint8_t* dead) {
+    for (uint32_t j = threadIdx.x; j < n_jobs; j += blockDim.x)
+        if (cnt[j] > cap) dead[j] = 1;
 }
 
 }  // namespace
@@ -302,17 +333,16 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     const uint32_t gen_lds = lts_words(n, m, (uint32_t)en_lbl.size()) <= LTS_LDS_WORDS
                                  ? 4u * lts_words(n, m, (uint32_t)en_lbl.size()) : 0u;
     // the level loop's counts come back through pinned words written by k_readback
-    if (ws.rb.n < 2u + n_jobs) {
+    if (ws.rb.n < 4u + n_jobs) {
         ws.rb.release();
-        const size_t words = std::max<size_t>(64, 2u + n_jobs);
+        const size_t words = std::max<size_t>(64, 4u + n_jobs);
         GB_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws.rb.h), 4 * words, hipHostMallocMapped | hipHostMallocCoherent),
                GEQ_ERR_OUT_OF_MEMORY);
         ws.rb.n = words;
         GB_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ws.rb.d), ws.rb.h, 0), GEQ_ERR_INTERNAL);
     }
     const uint32_t* const h_rb = ws.rb.h;
-    auto readback = [&](const uint32_t* a, const uint32_t* b, const uint32_t* cnt, uint32_t n_cnt) -> geq_status {
-        k_readback<<<1, 256, 0, s>>>(a, b, cnt, n_cnt, ws.rb.d);
+    auto wait = [&]() -> geq_status {
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
         return GEQ_OK;
     };
@@ -393,17 +423,27 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             const long long v = std::atoll(env);
             if (v >= 1) chunk = (uint32_t)std::min<long long>(v, 0x7fffffffll);
         }
+    // The last chunk of a level also bounds and scans the next level before
+    // its wait, so the next level's first chunk starts without one.
+    bool have_next = false;
+    uint32_t next_cand = 0, next_words = 0;
     while (lo < hi) {
         const uint32_t c_hi = hi - lo > chunk ? lo + chunk : hi;
         const uint32_t nf = c_hi - lo;
-        GB_TRY(ensure(ub, nf + 1, s, false, "frontier"));
-        GB_TRY(ensure(off, nf + 1, s, false, "frontier"));
         GB_TRY(ensure(nsucc, nf + 1, s, false, "frontier"));
         GB_TRY(ensure(rowl, nf + 1, s, false, "frontier"));
-        k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub.p);
-        GB_TRY(exclusive_scan_pairs(ub.p, off.p, nf + 1, tmp, s));
-        GB_TRY(readback(&off.p[nf].x, &off.p[nf].y, nullptr, 0));
-        const uint32_t n_cand = h_rb[0], n_words = h_rb[1];
+        uint32_t n_cand = next_cand, n_words = next_words;
+        if (!have_next) {
+            GB_TRY(ensure(ub, nf + 1, s, false, "frontier"));
+            GB_TRY(ensure(off, nf + 1, s, false, "frontier"));
+            k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub.p);
+            GB_TRY(exclusive_scan_pairs(ub.p, off.p, nf + 1, tmp, s));
+            k_readback<<<1, 64, 0, s>>>(&off.p[nf].x, &off.p[nf].y, ws.rb.d);
+            GB_TRY(wait());
+            n_cand = h_rb[0];
+            n_words = h_rb[1];
+        }
+        have_next = false;
         if ((uint64_t)B->spool_top + n_words >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: set pool exceeds 2^32 words"));
         GB_TRY(ensure(B->spool, B->spool_top + n_words + 1, s, true, "set pool"));
         GB_TRY(ensure(cands, std::max<uint32_t>(n_cand, 1), s, false, "candidates"));
@@ -433,9 +473,31 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             }
         }
         k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p);
-        // one synchronisation per level for the edge and position counts (and the per-job counts)
-        GB_TRY(readback(rowl.p + nf, rank.p + n_cand, jt ? d_jcnt.p : nullptr, n_jobs));
+        if (jt && jt->cap) k_mark_dead<<<1, 256, 0, s>>>(d_jcnt.p, n_jobs, jt->cap, d_dead.p);
+        // After a level's last chunk, the next level is the positions the level
+        // created: `add` by its earlier chunks and *(rank + n_cand) by this one.
+        const bool last_chunk = c_hi == hi;
+        const uint32_t add = B->V - hi;
+        if (last_chunk) {
+            const uint64_t slots = (uint64_t)add + n_cand + 1;
+            if (slots > 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 positions"));
+            GB_TRY(ensure(ub, slots, s, false, "frontier"));
+            GB_TRY(ensure(off, slots, s, false, "frontier"));
+            k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots, L,
+                                                                           ub.p);
+            GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots, tmp, s));
+        }
+        // one synchronisation per chunk for the edge and position counts, the per-job
+        // counts and the next level's first totals
+        k_chunk_counts<<<1, 256, 0, s>>>(rowl.p + nf, rank.p + n_cand, add, chunk, last_chunk ? off.p : nullptr,
+                                         jt ? d_jcnt.p : nullptr, n_jobs, ws.rb.d);
+        GB_TRY(wait());
         const uint32_t n_edges = h_rb[0], n_new = h_rb[1];
+        if (last_chunk) {
+            have_next = true;
+            next_cand = h_rb[2];
+            next_words = h_rb[3];
+        }
         E += n_edges;
         B->spool_top += n_words;
         B->V += n_new;
@@ -446,17 +508,12 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             levels++;
         }
         if (jt) {
-            // positions of every job so far (read with the counts above); jobs past the cap stop growing
-            bool newly_dead = false;
+            // positions of every job so far (read with the counts above); jobs past
+            // the cap stop growing (k_mark_dead set the same flags on the device)
             for (uint32_t j = 0; j < n_jobs; j++) {
-                jt->count[j] = h_rb[2 + j];
-                if (!jt->dead[j] && jt->cap && jt->count[j] > jt->cap) {
-                    jt->dead[j] = 1;
-                    newly_dead = true;
-                }
+                jt->count[j] = h_rb[4 + j];
+                if (jt->cap && jt->count[j] > jt->cap) jt->dead[j] = 1;
             }
-            if (newly_dead)
-                GB_HIP(hipMemcpyAsync(d_dead.p, jt->dead.data(), n_jobs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
         } else if (level_done && max_positions && B->V > max_positions) {
             truncated = true;
             break;
