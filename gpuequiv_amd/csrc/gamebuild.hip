@@ -148,9 +148,14 @@ __device__ __forceinline__ bool contains_sorted(const uint32_t* a, uint32_t n, u
 
 __device__ __forceinline__ uint32_t deg_of(const LtsArgs& L, uint32_t s) { return L.row[s + 1] - L.row[s]; }
 
-__device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, uint32_t u) {
+// The bounds of position u; `heavy` tells whether k_generate_heavy takes it:
+// an attack position whose serial walk (one binary search per member of Q and
+// label group of p, the successor sets gathered and sorted) weighs at least
+// heavy_min (0: none), and whose sets fit the block's LDS.
+__device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, uint32_t u, uint32_t heavy_min, bool& heavy) {
     const uint32_t kind = N.kind[u], p = N.p[u], ql = N.q_len[u], xl = N.x_len[u];
     uint32_t c = 0, s = 0;
+    heavy = false;
     if (L.job_dead && L.job_dead[L.job_of_state[p]]) {
         // a job past its position cap: no successors
     } else if (kind == K_CLAUSE) {
@@ -164,23 +169,35 @@ __device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, 
         uint32_t sd = 0;
         for (uint32_t k = 0; k < ql; k++) sd += deg_of(L, N.spool[N.q_off[u] + k]);
         s = sd + 4 * ql;
+        const uint32_t ng = L.en_row[p + 1] - L.en_row[p];
+        heavy = heavy_min != 0u && sd <= HV_CAP && ql <= HV_CAP && (uint64_t)ql * (ng + 1u) + sd >= heavy_min;
     }
     return make_uint2(c, s);
 }
 
-__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub) {
+__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub, HeavyArgs H) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n_front) return;
+    bool heavy = false;
     // trailing zero so the exclusive scan yields the totals
-    ub[i] = i == n_front ? make_uint2(0u, 0u) : bounds_of(N, L, lo + i);
+    ub[i] = i == n_front ? make_uint2(0u, 0u) : bounds_of(N, L, lo + i, H.heavy_min, heavy);
+    if (i < n_front) {
+        H.flag[i] = heavy ? 1 : 0;
+        if (heavy) H.list[atomicAdd(H.count, 1u)] = i;
+    }
 }
 
-__global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, uint32_t add, uint32_t n_slots, LtsArgs L,
-                              uint2* ub) {
+__global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, uint32_t add, uint32_t n_slots, uint32_t chunk,
+                              LtsArgs L, uint2* ub, HeavyArgs H) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_slots) return;
     const uint32_t n_front = *n_new + add;
-    ub[i] = i < n_front ? bounds_of(N, L, lo + i) : make_uint2(0u, 0u);
+    bool heavy = false;
+    ub[i] = i < n_front ? bounds_of(N, L, lo + i, H.heavy_min, heavy) : make_uint2(0u, 0u);
+    if (i < min(n_front, chunk)) {      // the flags and list cover the first chunk
+        H.flag[i] = heavy ? 1 : 0;
+        if (heavy) H.list[atomicAdd(H.count, 1u)] = i;
+    }
 }
 
 // Dynamic LDS: 0, or 4 * lts_words(...) bytes for a copy of the LTS, which
@@ -192,7 +209,7 @@ __global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, ui
 constexpr uint32_t GEN_BLOCK = 256, GEN_LDS = 16;
 
 __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L,
-                                                        const uint2* off, uint64_t spool_base,
+                                                        const uint2* off, const uint8_t* heavy, uint64_t spool_base,
                                                         Cand* cands, uint32_t* nsucc) {
     extern __shared__ uint32_t lts_lds[];
     __shared__ uint32_t s_qbuf[GEN_LDS * GEN_BLOCK];
@@ -217,7 +234,7 @@ __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo,
     }
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == n_front) nsucc[i] = 0;   // trailing zero: the exclusive scan yields the edge total
-    if (i >= n_front) return;
+    if (i >= n_front || heavy[i]) return;   // k_generate_heavy writes the heavy positions
     const uint32_t u = lo + i;
     const uint32_t kind = N.kind[u], p = N.p[u], qo = N.q_off[u], ql = N.q_len[u], xo = N.x_off[u], xl = N.x_len[u];
     uint32_t* spool = N.spool;
@@ -373,6 +390,233 @@ __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo,
     }
     nsucc[i] = slot;
     for (uint32_t j = cbase + slot; j < cend; j++) cands[j].kind_p = INVALID;
+}
+
+namespace {
+
+constexpr uint32_t HV_BLOCK = 256;
+
+__device__ __forceinline__ uint32_t hash_finish(uint32_t h) {
+    h ^= h >> 16;
+    return h == 0xffffffffu ? 0x7fffffffu : h;
+}
+
+// Exclusive prefix sum of one value per thread over the block (threads in
+// order); `total` gets the block's sum.  ws: HV_BLOCK / 64 words of LDS.
+__device__ uint32_t block_scan(uint32_t v, uint32_t* ws, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63u) ws[w] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (uint32_t k = 0; k < HV_BLOCK / 64u; k++) {
+        const uint32_t s = ws[k];
+        pre += k < w ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + x - v;
+}
+
+// Thread t's contiguous share [a, b) of n items (thread order = item order).
+__device__ __forceinline__ void share(uint32_t n, uint32_t& a, uint32_t& b) {
+    const uint32_t per = (n + HV_BLOCK - 1u) / HV_BLOCK;
+    a = min(n, threadIdx.x * per);
+    b = min(n, a + per);
+}
+
+// Ascending bitonic sort of a[0..n) in LDS, n a power of two, by the block.
+__device__ void bitonic_lds(uint32_t* a, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t x = threadIdx.x; x < n; x += HV_BLOCK) {
+                const uint32_t y = x ^ j;
+                if (y > x) {
+                    const uint32_t u = a[x], v = a[y];
+                    if ((u > v) == ((x & k) == 0u)) {
+                        a[x] = v;
+                        a[y] = u;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+__device__ __forceinline__ bool in_part(uint32_t which, uint8_t f) {
+    return which == 0u ? (f & 1u) != 0u : which == 1u ? (f & 2u) != 0u : f == 3u;
+}
+
+}  // namespace
+
+// One block per heavy attack position (H.list): the same candidates, set-pool
+// layout and hashes as k_generate's walk, with the work of each label group
+// spread over the block: the members of Q search their label ranges in
+// parallel, Q' is sorted and deduplicated in LDS, the group's transitions are
+// tested and emitted in parallel (ranks from a block scan keep the slot order),
+// and the conjunction challenges partition Q with block scans.
+__global__ __launch_bounds__(HV_BLOCK) void k_generate_heavy(NodeArgs N, uint32_t lo, LtsArgs L, const uint2* off,
+                                                             uint64_t spool_base, HeavyArgs H, Cand* cands, uint32_t* nsucc) {
+    __shared__ uint32_t qs[HV_CAP];
+    __shared__ uint8_t fl[HV_CAP];
+    __shared__ uint32_t ws[HV_BLOCK / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n_heavy = *H.count;
+    uint32_t* spool = N.spool;
+    for (uint32_t h = blockIdx.x; h < n_heavy; h += gridDim.x) {
+        const uint32_t i = H.list[h];
+        const uint32_t u = lo + i;
+        const uint32_t p = N.p[u], qo = N.q_off[u], ql = N.q_len[u];
+        const uint2 o0 = off[i];
+        const uint32_t cbase = o0.x, cend = off[i + 1].x;
+        const uint64_t sbase = spool_base + o0.y;
+        uint32_t a0, a1;
+        share(ql, a0, a1);
+        uint32_t part = 0;
+        for (uint32_t k = a0; k < a1; k++) part += deg_of(L, spool[qo + k]);
+        uint32_t sd;
+        (void)block_scan(part, ws, sd);
+        uint32_t slot = 0;
+        uint64_t sp = sbase;
+        // observation moves, one label group of p's transitions at a time (p's
+        // distinct labels in order)
+        for (uint32_t gi = L.en_row[p]; gi < L.en_row[p + 1]; gi++) {
+            const int32_t a = L.en_lbl[gi];
+            uint32_t t, g_end;
+            label_range(L, p, a, t, g_end);
+            uint32_t cnt = 0;
+            for (uint32_t k = a0; k < a1; k++) {
+                uint32_t x0, x1;
+                label_range(L, spool[qo + k], a, x0, x1);
+                cnt += x1 - x0;
+            }
+            uint32_t T;
+            uint32_t w = block_scan(cnt, ws, T);   // T <= sd <= HV_CAP
+            for (uint32_t k = a0; k < a1; k++) {
+                uint32_t x0, x1;
+                label_range(L, spool[qo + k], a, x0, x1);
+                for (uint32_t x = x0; x < x1; x++) qs[w++] = L.dst[x];
+            }
+            uint32_t n2 = 1;
+            while (n2 < T) n2 <<= 1;
+            for (uint32_t x = T + tid; x < n2; x += HV_BLOCK) qs[x] = 0xffffffffu;
+            __syncthreads();
+            bitonic_lds(qs, n2);
+            // Q' = the distinct values in order, written to the pool at sp
+            uint32_t b0, b1;
+            share(T, b0, b1);
+            uint32_t nd = 0;
+            for (uint32_t x = b0; x < b1; x++) nd += (x == 0u || qs[x] != qs[x - 1]) ? 1u : 0u;
+            uint32_t Tu;
+            uint32_t r = block_scan(nd, ws, Tu);
+            for (uint32_t x = b0; x < b1; x++)
+                if (x == 0u || qs[x] != qs[x - 1]) spool[sp + r++] = qs[x];
+            // an observation move to every target of the group outside Q', in transition order
+            uint32_t c0, c1;
+            share(g_end - t, c0, c1);
+            uint32_t ne = 0;
+            for (uint32_t x = c0; x < c1; x++) ne += contains_sorted(qs, T, L.dst[t + x]) ? 0u : 1u;
+            uint32_t n_emit;
+            uint32_t rr = block_scan(ne, ws, n_emit);
+            for (uint32_t x = c0; x < c1; x++) {
+                const uint32_t pp = L.dst[t + x];
+                if (contains_sorted(qs, T, pp)) continue;
+                uint32_t hh = mix(mix(mix(0x811c9dc5u, K_ATTACK), pp), Tu);
+                for (uint32_t y = 0; y < T; y++)
+                    if (y == 0u || qs[y] != qs[y - 1]) hh = mix(hh, qs[y]);
+                Cand c;
+                c.kind_p = (K_ATTACK << 30) | pp;
+                c.q_off = (uint32_t)sp; c.q_len = Tu; c.x_off = 0; c.x_len = 0;
+                c.weight = W_OBS;
+                c.hash = hash_finish(mix(hh, 0u));
+                c.res = 0;
+                c.parent = i;
+                c.slot = slot + rr;
+                cands[cbase + slot + rr] = c;
+                rr++;
+            }
+            slot += n_emit;
+            sp += Tu;
+            __syncthreads();   // qs is rewritten by the next group
+        }
+        // Q and its enabledness flags (bit 0 subset, bit 1 superset) in LDS
+        for (uint32_t k = a0; k < a1; k++) {
+            const uint32_t q = spool[qo + k];
+            bool sub, sup;
+            compare_enabled(L, p, q, sub, sup);
+            qs[k] = q;
+            fl[k] = (sub ? 1u : 0u) | (sup ? 2u : 0u);
+        }
+        __syncthreads();
+        if (tid == 0) {                 // Defend(p, Q, {})
+            uint32_t hh = mix(mix(mix(0x811c9dc5u, K_DEFEND), p), ql);
+            for (uint32_t k = 0; k < ql; k++) hh = mix(hh, qs[k]);
+            Cand c;
+            c.kind_p = (K_DEFEND << 30) | p;
+            c.q_off = qo; c.q_len = ql; c.x_off = 0; c.x_len = 0;
+            c.weight = W_CHAL;
+            c.hash = hash_finish(mix(hh, 0u));
+            c.res = 0;
+            c.parent = i;
+            c.slot = slot;
+            cands[cbase + slot] = c;
+        }
+        slot++;
+        // conjunction challenges with partitioned Q* (gamepos.rs:329-375)
+        uint64_t pb = sbase + sd;
+        uint32_t emitted[3];
+        uint32_t n_emitted = 0;
+        for (uint32_t which = 0; which < 3u; which++) {
+            uint32_t local = 0;
+            for (uint32_t k = a0; k < a1; k++) local += in_part(which, fl[k]) ? 1u : 0u;
+            uint32_t cnt;
+            uint32_t rin = block_scan(local, ws, cnt);
+            if (cnt == 0u || cnt == ql) continue;
+            bool dup = false;
+            for (uint32_t e = 0; e < n_emitted; e++) {
+                uint32_t diff = 0;
+                for (uint32_t k = a0; k < a1; k++) diff += in_part(which, fl[k]) != in_part(emitted[e], fl[k]) ? 1u : 0u;
+                uint32_t n_diff;
+                (void)block_scan(diff, ws, n_diff);
+                dup |= n_diff == 0u;
+            }
+            if (dup) continue;
+            // stable partition: the members outside the part, then those inside
+            const uint32_t na = ql - cnt;
+            for (uint32_t k = a0; k < a1; k++) {
+                if (in_part(which, fl[k])) spool[pb + na + rin++] = qs[k];
+                else spool[pb + (k - rin)] = qs[k];
+            }
+            if (tid == 0) {
+                uint32_t hh = mix(mix(mix(0x811c9dc5u, K_DEFEND), p), na);
+                for (uint32_t k = 0; k < ql; k++)
+                    if (!in_part(which, fl[k])) hh = mix(hh, qs[k]);
+                hh = mix(hh, cnt);
+                for (uint32_t k = 0; k < ql; k++)
+                    if (in_part(which, fl[k])) hh = mix(hh, qs[k]);
+                Cand c;
+                c.kind_p = (K_DEFEND << 30) | p;
+                c.q_off = (uint32_t)pb; c.q_len = na; c.x_off = (uint32_t)(pb + na); c.x_len = cnt;
+                c.weight = W_CHAL;
+                c.hash = hash_finish(hh);
+                c.res = 0;
+                c.parent = i;
+                c.slot = slot;
+                cands[cbase + slot] = c;
+            }
+            slot++;
+            emitted[n_emitted++] = which;
+            pb += ql;
+        }
+        if (tid == 0) nsucc[i] = slot;
+        for (uint32_t j = cbase + slot + tid; j < cend; j += HV_BLOCK) cands[j].kind_p = INVALID;
+        __syncthreads();   // qs and fl are rewritten for the next position
+    }
 }
 
 __device__ bool cand_equals_cand(const Cand& a, const Cand& b, const uint32_t* spool) {

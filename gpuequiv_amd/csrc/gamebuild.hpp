@@ -56,18 +56,34 @@ struct Cand {               // one successor candidate of a frontier position
     uint32_t slot;          // successor index within the parent
 };
 
+// Heavy frontier positions: attack positions whose serial successor walk is
+// long (see bounds_of) are generated by one block each (k_generate_heavy)
+// instead of one thread.  HV_CAP bounds their |Q| and sum of member degrees
+// (the block's LDS).
+constexpr uint32_t HV_CAP = 8192;
+struct HeavyArgs {
+    uint8_t* flag;       // per frontier position: 1 if heavy
+    uint32_t* list;      // the heavy positions' frontier indices (any order)
+    uint32_t* count;     // their number (zero before k_bounds)
+    uint32_t heavy_min;  // weight from which a position is heavy (0: none)
+};
+
 // Per frontier position: upper bounds on its candidates (x) and on the set-pool
-// words its successor sets take (y); entry n_front is zero.
-__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub);
+// words its successor sets take (y); entry n_front is zero.  Also the heavy flags and list.
+__global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub, HeavyArgs H);
 // The same for a frontier of *n_new + add positions from `lo` whose size only
 // the device knows: entries [n_front, n_slots) are zero, so the exclusive scan
 // of all n_slots entries has the frontier's offsets in front and the totals
-// from entry n_front on.
-__global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, uint32_t add, uint32_t n_slots, LtsArgs L,
-                              uint2* ub);
+// from entry n_front on.  The heavy flags and list cover the frontier's first
+// min(n_front, chunk) positions.
+__global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, uint32_t add, uint32_t n_slots, uint32_t chunk,
+                              LtsArgs L, uint2* ub, HeavyArgs H);
 // `off`: the exclusive scan of k_bounds' pairs (candidate and set-pool offsets).
-__global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint2* off, uint64_t spool_base,
-                           Cand* cands, uint32_t* nsucc);
+// Positions flagged in `heavy` are left to k_generate_heavy.
+__global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint2* off, const uint8_t* heavy,
+                           uint64_t spool_base, Cand* cands, uint32_t* nsucc);
+__global__ void k_generate_heavy(NodeArgs N, uint32_t lo, LtsArgs L, const uint2* off, uint64_t spool_base, HeavyArgs H,
+                                 Cand* cands, uint32_t* nsucc);
 __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask);
 __global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag);
 __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,
@@ -103,6 +119,8 @@ struct HostWords {
 struct BuildScratch {
     DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, nsucc, rowl, flag, rank, keymin, d_jcnt;
     DevBuf<uint2> ub, off;
+    DevBuf<uint8_t> hv_flag;          // heavy positions of a frontier (k_bounds)
+    DevBuf<uint32_t> hv_list, hv_count;
     HostWords rb;
     DevBuf<int32_t> l_lbl, l_enlbl;
     DevBuf<uint8_t> d_insert, tmp, d_dead;

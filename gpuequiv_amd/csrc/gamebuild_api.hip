@@ -423,6 +423,15 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             const long long v = std::atoll(env);
             if (v >= 1) chunk = (uint32_t)std::min<long long>(v, 0x7fffffffll);
         }
+    // Attack positions whose serial walk weighs at least heavy_min are
+    // generated by a block each (k_generate_heavy); 0 turns this off.
+    uint32_t heavy_min = 2048;
+    if (const char* env = std::getenv("GEQ_BUILD_HEAVY_MIN")) {   // small values exercise the path in tests
+        const long long v = std::atoll(env);
+        if (v >= 0) heavy_min = (uint32_t)std::min<long long>(v, 0xffffffffll);
+    }
+    GB_TRY(ensure(ws.hv_count, 1, s, false, "heavy positions"));
+    auto heavy_args = [&]() { return HeavyArgs{ws.hv_flag.p, ws.hv_list.p, ws.hv_count.p, heavy_min}; };
     // The last chunk of a level also bounds and scans the next level before
     // its wait, so the next level's first chunk starts without one.
     bool have_next = false;
@@ -436,7 +445,10 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (!have_next) {
             GB_TRY(ensure(ub, nf + 1, s, false, "frontier"));
             GB_TRY(ensure(off, nf + 1, s, false, "frontier"));
-            k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub.p);
+            GB_TRY(ensure(ws.hv_flag, std::max<uint32_t>(nf, 1), s, false, "heavy positions"));
+            GB_TRY(ensure(ws.hv_list, std::max<uint32_t>(nf, 1), s, false, "heavy positions"));
+            GB_HIP(hipMemsetAsync(ws.hv_count.p, 0, 4, s), GEQ_ERR_TRANSFER);
+            k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub.p, heavy_args());
             GB_TRY(exclusive_scan_pairs(ub.p, off.p, nf + 1, tmp, s));
             k_readback<<<1, 64, 0, s>>>(&off.p[nf].x, &off.p[nf].y, ws.rb.d);
             GB_TRY(wait());
@@ -454,7 +466,11 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (E + n_cand >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 edges"));
         GB_TRY(ensure(B->col, E + n_cand + 1, s, true, "edges"));
         GB_TRY(ensure(B->wt, E + n_cand + 1, s, true, "edges"));
-        k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, off.p, B->spool_top, cands.p, nsucc.p);
+        k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, off.p, ws.hv_flag.p, B->spool_top, cands.p,
+                                                               nsucc.p);
+        if (heavy_min && nf)
+            k_generate_heavy<<<std::min<uint32_t>(nf, 1024), 256, 0, s>>>(nodes(), lo, L, off.p, B->spool_top, heavy_args(),
+                                                                          cands.p, nsucc.p);
         GB_TRY(exclusive_scan<uint32_t>(nsucc.p, rowl.p, nf + 1, tmp, s));
         if (n_cand) {
             k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
@@ -483,8 +499,11 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             if (slots > 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 positions"));
             GB_TRY(ensure(ub, slots, s, false, "frontier"));
             GB_TRY(ensure(off, slots, s, false, "frontier"));
-            k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots, L,
-                                                                           ub.p);
+            GB_TRY(ensure(ws.hv_flag, slots, s, false, "heavy positions"));
+            GB_TRY(ensure(ws.hv_list, slots, s, false, "heavy positions"));
+            GB_HIP(hipMemsetAsync(ws.hv_count.p, 0, 4, s), GEQ_ERR_TRANSFER);
+            k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots,
+                                                                           chunk, L, ub.p, heavy_args());
             GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots, tmp, s));
         }
         // one synchronisation per chunk for the edge and position counts, the per-job

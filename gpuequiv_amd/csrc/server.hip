@@ -417,6 +417,8 @@ geq_status geq_server_create(geq_ctx* ctx, const geq_server_opts* opts, geq_serv
     S->rounds_per_step = opts && opts->rounds_per_step ? opts->rounds_per_step : 4;
     S->compact_entries = opts ? opts->compact_entries : 0;
     S->trace = std::getenv("GEQ_SERVER_TRACE") != nullptr;
+    if (const char* env = std::getenv("GEQ_SERVER_MAX_BATCH"))   // jobs per merged build (the per-job counters hold 1024)
+        S->max_batch = (uint32_t)std::min(std::max(std::atoi(env), 1), 1024);
     S->max_positions = opts && opts->max_positions ? std::min<uint64_t>(opts->max_positions, V) : V;
     const uint32_t n_builders = opts && opts->builders ? std::min(opts->builders, 16u) : 2u;
     geq_status st = GEQ_OK;

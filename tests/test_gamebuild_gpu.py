@@ -115,6 +115,31 @@ def test_compare_all_matches_oracle(n, m, labels, seed):
     _assert_same_energies(b, og)
 
 
+@pytest.mark.parametrize("heavy_min", ["1", "40", "0"])
+@pytest.mark.parametrize("n,m,labels,seed", [(60, 200, 4, 3), (300, 1200, 8, 1)])
+def test_block_generated_positions_match_oracle(monkeypatch, heavy_min, n, m, labels, seed):
+    # Attack positions weighing at least GEQ_BUILD_HEAVY_MIN are generated by
+    # a block each (1: every one with a non-empty Q; 0: none); the game and
+    # its energies are the oracle's either way, capped builds included.
+    print("block-generated positions, heavy_min", heavy_min, "lts", n, m, labels, seed, flush=True)
+    monkeypatch.setenv("GEQ_BUILD_HEAVY_MIN", heavy_min)
+    n, src, dst, lbl = workloads.random_lts(n, m, labels, seed)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    if n >= 300:
+        lts, _ = lts.bisimilar_minimize()
+    pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q]
+    b = GameBuild(lts, pairs)
+    og = _assert_same_game(b, lts, pairs)
+    _assert_same_energies(b, og)
+    b2 = GameBuild(lts, pairs[:40], max_positions=3000)
+    og2 = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs[:40], max_positions=3000,
+                        positions_wanted=False)
+    h = b2._download()
+    assert bool(b2.build_stats["truncated"]) == og2["truncated"] and b2.n_positions() == og2["V"]
+    for k in ("row", "col", "wt"):
+        assert np.array_equal(h[k], og2[k]), k
+
+
 def test_vlts_peterson_matches_oracle():
     lts = TransitionSystem.from_csv_file(os.path.join(GOLDEN, "peterson_mutex_weak.csv"))
     lts, _ = lts.bisimilar_minimize()
