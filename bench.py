@@ -507,7 +507,7 @@ def main() -> int:
                     help="serve: per-job game size limit (2M); gamebuild: build cap (200M)")
     ap.add_argument("--rounds-per-step", type=int, default=4, help="serve: rounds per server step")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM admission budget (GB)")
-    ap.add_argument("--builders", type=int, default=2, help="serve: game-building threads")
+    ap.add_argument("--builders", type=int, default=3, help="serve: game-building threads")
     ap.add_argument("--positions", type=int, default=100000)
     ap.add_argument("--games", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=None)

@@ -425,7 +425,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         }
     // Attack positions whose serial walk weighs at least heavy_min are
     // generated by a block each (k_generate_heavy); 0 turns this off.
-    uint32_t heavy_min = 2048;
+    uint32_t heavy_min = 1024;
     if (const char* env = std::getenv("GEQ_BUILD_HEAVY_MIN")) {   // small values exercise the path in tests
         const long long v = std::atoll(env);
         if (v >= 0) heavy_min = (uint32_t)std::min<long long>(v, 0xffffffffll);

@@ -420,7 +420,7 @@ geq_status geq_server_create(geq_ctx* ctx, const geq_server_opts* opts, geq_serv
     if (const char* env = std::getenv("GEQ_SERVER_MAX_BATCH"))   // jobs per merged build (the per-job counters hold 1024)
         S->max_batch = (uint32_t)std::min(std::max(std::atoi(env), 1), 1024);
     S->max_positions = opts && opts->max_positions ? std::min<uint64_t>(opts->max_positions, V) : V;
-    const uint32_t n_builders = opts && opts->builders ? std::min(opts->builders, 16u) : 2u;
+    const uint32_t n_builders = opts && opts->builders ? std::min(opts->builders, 16u) : 3u;
     geq_status st = GEQ_OK;
     for (uint32_t i = 0; i < n_builders; i++) {
         geq_ctx* c = nullptr;

@@ -24,7 +24,7 @@ STATES = {0: "queued", 1: "building", 2: "built", 3: "resident", 4: "done", 5: "
 class Server:
     def __init__(self, budget_bytes: int = 4 << 30, max_jobs: int = 4096, rounds_per_step: int = 4,
                  edges_per_position: float = 4.0, compact_entries: int = 0, max_positions: int = 0,
-                 builders: int = 2):
+                 builders: int = 3):
         self.ctx = _lib.Context.get()
         o = _lib.geq_server_opts(int(budget_bytes), float(edges_per_position), int(max_jobs), int(rounds_per_step),
                                  int(compact_entries), int(max_positions), int(builders), 0)

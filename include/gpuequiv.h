@@ -251,7 +251,7 @@ typedef struct {
                                       many u16 entries (default 0: three quarters of its size) */
     uint64_t max_positions;        /* a job whose game grows past this many positions fails with
                                       GEQ_ERR_OUT_OF_MEMORY (0: no limit but the budget) */
-    uint32_t builders;             /* game-building threads, each with its own stream (default 2) */
+    uint32_t builders;             /* game-building threads, each with its own stream (default 3) */
     uint32_t pad;
 } geq_server_opts;
 typedef struct {

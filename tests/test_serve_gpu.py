@@ -149,7 +149,7 @@ def test_invalid_and_empty_jobs_finish_alone():
     srv.close()
 
 
-@pytest.mark.parametrize("heavy_min", ["2048", "1"])
+@pytest.mark.parametrize("heavy_min", ["1024", "1"])
 def test_chunked_levels_and_the_position_cap(monkeypatch, heavy_min):
     # Tiny chunks: every level of the merged builds runs in pieces of 37
     # frontier positions, with the per-job caps checked after each piece.  The
