@@ -462,7 +462,8 @@ def bench_serve(args) -> dict:
         plain_lat = _fifo_latency(arrivals, service)[np.array(ok)] * 1e3
         span = wall - arrivals[0]
         sweep.append({
-            "rate_jobs_per_s": rate, "jobs": len(jobs), "failed_over_size_limit": failed,
+            "rate_jobs_per_s": rate, "saturation": len(jobs) / rate < 0.01,   # all jobs within 10 ms "jobs": len(jobs),
+            "failed_over_size_limit": failed,
             "served_jobs_per_s": len(jobs) / span, "served_wall_s": wall,
             "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
             "plain_fifo_latency_ms_p50": float(np.percentile(plain_lat, 50)),
@@ -502,7 +503,9 @@ def main() -> int:
     ap.add_argument("--states", type=int, default=None, help="spectroscopy / gamebuild: LTS states")
     ap.add_argument("--pairs", type=int, default=4096, help="gamebuild: sampled pairs")
     ap.add_argument("--jobs", type=int, default=400, help="serve: jobs per arrival rate")
-    ap.add_argument("--rates", default="50,200,1000,2000", help="serve: arrival rates (jobs/s)")
+    ap.add_argument("--rates", default="50,200,1000,2000,100000",
+                    help="serve: arrival rates (jobs/s); 100000 submits all jobs at once, after the other rates have "
+                         "warmed the server (its saturation throughput)")
     ap.add_argument("--max-positions", type=int, default=None,
                     help="serve: per-job game size limit (2M); gamebuild: build cap (200M)")
     ap.add_argument("--rounds-per-step", type=int, default=4, help="serve: rounds per server step")
