@@ -726,10 +726,13 @@ __global__ void k_starts(const uint32_t* pairs, uint32_t n, const uint8_t* inser
     while (atomicCAS(&table[s], EMPTY, mine) != EMPTY) s = (s + 1) & mask;
 }
 
-// re-insert every node into a larger table
-__global__ void k_rehash(const uint32_t* hash, uint32_t n, unsigned long long* table, uint32_t mask) {
+// re-insert every node into a larger table, except the start positions that
+// k_starts left out (an earlier copy of a repeated start pair: the table keeps
+// the last id, as GameBuild::new_node does)
+__global__ void k_rehash(const uint32_t* hash, uint32_t n, const uint8_t* insert, uint32_t n_start, unsigned long long* table,
+                         uint32_t mask) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (i < n_start && !insert[i])) return;
     const uint32_t h = hash[i];
     uint32_t s = h & mask;
     const unsigned long long mine = ((unsigned long long)h << 32) | i;

@@ -93,7 +93,8 @@ __global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl
 __global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row);
 __global__ void k_starts(const uint32_t* pairs, uint32_t n, const uint8_t* insert, NodeArgs N, unsigned long long* table,
                          uint32_t mask);
-__global__ void k_rehash(const uint32_t* hash, uint32_t n, unsigned long long* table, uint32_t mask);
+__global__ void k_rehash(const uint32_t* hash, uint32_t n, const uint8_t* insert, uint32_t n_start, unsigned long long* table,
+                         uint32_t mask);
 __global__ void k_fill_u64(unsigned long long* a, uint64_t n, unsigned long long v);
 __global__ void k_fill_u32(uint32_t* a, uint64_t n, uint32_t v);
 __global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk);

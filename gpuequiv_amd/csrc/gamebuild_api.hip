@@ -374,7 +374,9 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         const uint32_t tb = 256;
         k_fill_u64<<<(uint32_t)((cap + tb - 1) / tb), tb, 0, s>>>(table.p, cap, EMPTY_ENTRY);
         k_fill_u32<<<(uint32_t)((cap + tb - 1) / tb), tb, 0, s>>>(keymin.p, cap, 0xffffffffu);
-        if (B->V) k_rehash<<<(B->V + tb - 1) / tb, tb, 0, s>>>(B->hash.p, B->V, table.p, table_mask);
+        if (B->V)
+            k_rehash<<<(B->V + tb - 1) / tb, tb, 0, s>>>(B->hash.p, B->V, d_insert.p, std::min<uint32_t>(n_pairs, B->V), table.p,
+                                                        table_mask);
         return GEQ_OK;
     };
     auto nodes = [&]() {

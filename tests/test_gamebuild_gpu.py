@@ -157,6 +157,20 @@ def test_duplicate_start_pairs_keep_the_last_id():
     _assert_same_game(b, lts, pairs)
 
 
+def test_repeated_start_pair_survives_table_growth():
+    # The table of a game with thousands of positions grows (and is rebuilt)
+    # several times; the earlier copy of a repeated start pair stays out of it,
+    # so its successors' edges lead to the last copy's id, as in the oracle.
+    n, src, dst, lbl = workloads.random_lts(60, 200, 4, 3)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q]
+    pairs = [pairs[1], pairs[0]] + pairs
+    print("repeated start pairs, table growth:", len(pairs), "pairs", flush=True)
+    b = GameBuild(lts, pairs)
+    assert b.n_positions() > 2048
+    _assert_same_game(b, lts, pairs)
+
+
 def test_truncated_build_is_well_formed():
     n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
     lts = TransitionSystem.from_arrays(n, src, dst, lbl)
