@@ -332,7 +332,8 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     // a small LTS is read from LDS by k_generate (its walks are chains of dependent reads)
     const uint32_t gen_lds = lts_words(n, m, (uint32_t)en_lbl.size()) <= LTS_LDS_WORDS
                                  ? 4u * lts_words(n, m, (uint32_t)en_lbl.size()) : 0u;
-    // the level loop's counts come back through pinned words written by k_readback
+    // the level loop's counts come back through pinned words written by
+    // k_readback and k_chunk_counts
     if (ws.rb.n < 4u + n_jobs) {
         ws.rb.release();
         const size_t words = std::max<size_t>(64, 4u + n_jobs);
