@@ -79,6 +79,10 @@ __device__ bool sets_equal(const uint32_t* __restrict__ spool, uint32_t ao, uint
 
 // label range [lo, hi) of `a` in state s's label-sorted transitions
 __device__ __forceinline__ void label_range(const LtsArgs& L, uint32_t s, int32_t a, uint32_t& lo, uint32_t& hi) {
+    if (L.en_mask && !((L.en_mask[s] >> (a & 63)) & 1ull)) {   // s has no a-transition
+        lo = hi = 0;
+        return;
+    }
     uint32_t b = L.row[s], e = L.row[s + 1];
     uint32_t l = b, r = e;
     while (l < r) {
@@ -99,6 +103,12 @@ __device__ void compare_enabled(const LtsArgs& L, uint32_t p, uint32_t q, bool& 
     subset = true;
     superset = true;
     if (p == q) return;
+    if (L.en_mask) {        // the label bit sets: two words instead of the list walk
+        const unsigned long long mp = L.en_mask[p], mq = L.en_mask[q];
+        subset = (mq & ~mp) == 0ull;
+        superset = (mp & ~mq) == 0ull;
+        return;
+    }
     uint32_t pi = L.en_row[p], pe = L.en_row[p + 1], qi = L.en_row[q], qe = L.en_row[q + 1];
     while (subset || superset) {
         if (pi == pe && qi == qe) break;
@@ -231,6 +241,7 @@ __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo,
         for (uint32_t k = threadIdx.x; k < L.n_en; k += blockDim.x) el[k] = L.en_lbl[k];
         __syncthreads();
         L.row = r; L.dst = d; L.lbl = lb; L.en_row = er; L.en_lbl = el;
+        L.en_mask = nullptr;   // searches in LDS are cheap
     }
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == n_front) nsucc[i] = 0;   // trailing zero: the exclusive scan yields the edge total

@@ -26,6 +26,9 @@ struct LtsArgs {
     const uint8_t* job_dead;
     // sizes for a copy into LDS (k_generate): states, transitions, enabled labels
     uint32_t n, m, n_en;
+    // when every label is in [0, 64): per state the bit set of its labels
+    // (bit a: label a), else NULL
+    const unsigned long long* en_mask;
 };
 // k_generate copies the LTS (row, dst, lbl, en_row, en_lbl) into LDS when it
 // takes at most this many words.
@@ -119,6 +122,7 @@ struct HostWords {
 // caller that builds repeatedly so that steady-state builds allocate nothing.
 struct BuildScratch {
     DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, nsucc, rowl, flag, rank, keymin, d_jcnt;
+    DevBuf<unsigned long long> l_mask;
     DevBuf<uint2> ub, off;
     DevBuf<uint8_t> hv_flag;          // heavy positions of a frontier (k_bounds)
     DevBuf<uint32_t> hv_list, hv_count;

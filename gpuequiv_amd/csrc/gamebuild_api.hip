@@ -327,8 +327,20 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         GB_HIP(hipMemsetAsync(d_jcnt.p, 0, 4ull * n_jobs, s), GEQ_ERR_TRANSFER);
         GB_HIP(hipMemcpyAsync(d_dead.p, jt->dead.data(), n_jobs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
+    // label bit sets when every label is in [0, 64) (label_range, compare_enabled)
+    bool use_mask = !en_lbl.empty();
+    for (const int32_t a : en_lbl) use_mask = use_mask && a >= 0 && a < 64;
+    if (const char* env = std::getenv("GEQ_BUILD_LABEL_MASK")) use_mask = use_mask && std::atoi(env) != 0;
+    std::vector<unsigned long long> mask;   // alive until the build's last wait
+    if (use_mask) {
+        mask.assign(n, 0ull);
+        for (uint32_t st = 0; st < n; st++)
+            for (uint32_t k = en_row[st]; k < en_row[st + 1]; k++) mask[st] |= 1ull << en_lbl[k];
+        GB_TRY(ensure(ws.l_mask, std::max<uint32_t>(n, 1), s, false, "LTS"));
+        GB_HIP(hipMemcpyAsync(ws.l_mask.p, mask.data(), 8ull * n, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    }
     LtsArgs L{l_row.p, l_dst.p, l_lbl.p, l_enrow.p, l_enlbl.p, jt ? d_job_of_state.p : nullptr, jt ? d_dead.p : nullptr,
-              n, m, (uint32_t)en_lbl.size()};
+              n, m, (uint32_t)en_lbl.size(), use_mask ? ws.l_mask.p : nullptr};
     // a small LTS is read from LDS by k_generate (its walks are chains of dependent reads)
     const uint32_t gen_lds = lts_words(n, m, (uint32_t)en_lbl.size()) <= LTS_LDS_WORDS
                                  ? 4u * lts_words(n, m, (uint32_t)en_lbl.size()) : 0u;

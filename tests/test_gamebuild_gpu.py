@@ -115,14 +115,18 @@ def test_compare_all_matches_oracle(n, m, labels, seed):
     _assert_same_energies(b, og)
 
 
-@pytest.mark.parametrize("heavy_min", ["1", "40", "0"])
+@pytest.mark.parametrize("heavy_min,label_mask", [("1", "1"), ("40", "1"), ("0", "1"), ("1", "0"), ("0", "0")])
 @pytest.mark.parametrize("n,m,labels,seed", [(60, 200, 4, 3), (300, 1200, 8, 1)])
-def test_block_generated_positions_match_oracle(monkeypatch, heavy_min, n, m, labels, seed):
+def test_block_generated_positions_match_oracle(monkeypatch, heavy_min, label_mask, n, m, labels, seed):
     # Attack positions weighing at least GEQ_BUILD_HEAVY_MIN are generated by
-    # a block each (1: every one with a non-empty Q; 0: none); the game and
-    # its energies are the oracle's either way, capped builds included.
-    print("block-generated positions, heavy_min", heavy_min, "lts", n, m, labels, seed, flush=True)
+    # a block each (1: every one with a non-empty Q; 0: none).  label_mask 0
+    # turns off the per-state label bit sets (the label walks search the rows
+    # instead).  The game and its energies are the oracle's either way,
+    # capped builds included.
+    print("block-generated positions, heavy_min", heavy_min, "label_mask", label_mask, "lts", n, m, labels, seed,
+          flush=True)
     monkeypatch.setenv("GEQ_BUILD_HEAVY_MIN", heavy_min)
+    monkeypatch.setenv("GEQ_BUILD_LABEL_MASK", label_mask)
     n, src, dst, lbl = workloads.random_lts(n, m, labels, seed)
     lts = TransitionSystem.from_arrays(n, src, dst, lbl)
     if n >= 300:
@@ -138,6 +142,28 @@ def test_block_generated_positions_match_oracle(monkeypatch, heavy_min, n, m, la
     assert bool(b2.build_stats["truncated"]) == og2["truncated"] and b2.n_positions() == og2["V"]
     for k in ("row", "col", "wt"):
         assert np.array_equal(h[k], og2[k]), k
+
+
+@pytest.mark.parametrize("heavy_min,label_mask", [("1024", "1"), ("1024", "0"), ("1", "1")])
+def test_large_lts_capped_build_matches_oracle(monkeypatch, heavy_min, label_mask):
+    # An LTS too large for k_generate's LDS copy: its walks read global memory,
+    # where the per-state label bit sets apply (label_mask 0 turns them off).
+    # A capped build of 300 pairs equals the oracle's capped breadth-first walk.
+    print("large lts, capped build, heavy_min", heavy_min, "label_mask", label_mask, flush=True)
+    monkeypatch.setenv("GEQ_BUILD_HEAVY_MIN", heavy_min)
+    monkeypatch.setenv("GEQ_BUILD_LABEL_MASK", label_mask)
+    n, src, dst, lbl = workloads.random_lts(3000, 12000, 4, 7)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    rng = np.random.default_rng(7)
+    pairs = [(int(p), int(q)) for p, q in rng.integers(0, n, (600, 2)) if p != q][:300]
+    cap = 200000
+    b = GameBuild(lts, pairs, max_positions=cap)
+    og = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs, max_positions=cap,
+                       positions_wanted=False)
+    h = b._download()
+    assert og["truncated"] and bool(b.build_stats["truncated"]) and b.n_positions() == og["V"]
+    for k in ("row", "col", "wt"):
+        assert np.array_equal(h[k], og[k]), k
 
 
 def test_vlts_peterson_matches_oracle():
