@@ -240,8 +240,7 @@ __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo,
         }
         for (uint32_t k = threadIdx.x; k < L.n_en; k += blockDim.x) el[k] = L.en_lbl[k];
         __syncthreads();
-        L.row = r; L.dst = d; L.lbl = lb; L.en_row = er; L.en_lbl = el;
-        L.en_mask = nullptr;   // searches in LDS are cheap
+        L.row = r; L.dst = d; L.lbl = lb; L.en_row = er; L.en_lbl = el;   // en_mask stays in global memory
     }
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == n_front) nsucc[i] = 0;   // trailing zero: the exclusive scan yields the edge total
