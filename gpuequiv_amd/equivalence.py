@@ -58,7 +58,8 @@ class Equivalence:
     def preorder_matrix(self, equivalences: Sequence[Energy]):
         """preorder(p, q, e) for every ordered pair of processes and every e
         in `equivalences`, as a bool array [n, n, len(equivalences)]:
-        vectorised over the budgets of all seeded pairs."""
+        vectorised over the packed budgets of all seeded pairs (a pair's
+        preorder holds unless one of its budgets lies below e, :125-132)."""
         import numpy as np
         n, k = self.start_info.n_processes, len(equivalences)
         conf = equivalences[0].conf if k else None
@@ -66,20 +67,34 @@ class Equivalence:
         sp = self.start_info.starting_points
         m = (max(self.minimization) + 1) if self.minimization is not None else n
         red = np.zeros((max(m, 1), max(m, 1), k), dtype=bool)
-        rows, owner = [], []
-        for i, (pt, e) in enumerate(zip(sp, self.energies)):
-            if e.conf != conf:
-                raise ValueError("Incompatible Energy configurations")
-            for v in e:
-                rows.append(v.to_vec())
-                owner.append(i)
         below_any = np.zeros((len(sp), k), dtype=bool)
-        if rows:
-            vals = np.array(rows, dtype=np.int64)                       # [r, elements]
-            below = np.all(vals[:, None, :] <= eqs[None, :, :], axis=2)  # [r, k]
-            np.logical_or.at(below_any, np.array(owner), below)
-        for i, pt in enumerate(sp):
-            red[pt.p, pt.q[0]] = ~below_any[i]
+        if sp and k:
+            for e in self.energies:
+                if e.conf != conf:
+                    raise ValueError("Incompatible Energy configurations")
+            lens = np.array([e.n_energies() for e in self.energies], dtype=np.int64)
+            if lens.sum():
+                data = np.concatenate([e.array for e in self.energies]).astype(np.int64)   # [r, words]
+                # the fields' word and shift, as Energy's packing lays them out
+                bits, mask = conf.energy_bits(), conf.energy_mask()
+                vals = np.empty((data.shape[0], conf.elements), dtype=np.int64)
+                shift = word = 0
+                for f in range(conf.elements):
+                    vals[:, f] = (data[:, word] >> shift) & mask
+                    shift += bits
+                    if shift >= 32:
+                        shift, word = 0, word + 1
+                below = np.all(vals[:, None, :] <= eqs[None, :, :], axis=2).astype(np.uint8)   # [r, k]
+                starts = np.concatenate(([0], np.cumsum(lens)[:-1]))
+                has = lens > 0
+                below_any[has] = np.maximum.reduceat(below, starts[has], axis=0).astype(bool)
+        if sp:
+            P = np.fromiter((pt.p for pt in sp), dtype=np.int64, count=len(sp))
+            Q = np.fromiter((pt.q[0] for pt in sp), dtype=np.int64, count=len(sp))
+            # a repeated pair takes its last entry, as the per-pair loop would
+            _, first_rev = np.unique((P * max(m, 1) + Q)[::-1], return_index=True)
+            last = len(sp) - 1 - first_rev
+            red[P[last], Q[last]] = ~below_any[last]
         idx = np.arange(m)
         red[idx, idx] = True
         part = np.asarray(self.minimization if self.minimization is not None else range(n), dtype=np.int64)
