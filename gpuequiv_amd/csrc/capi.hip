@@ -618,7 +618,9 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
 
     uint64_t launches = 0;
     geq_status result = GEQ_OK;
+    bool stats_in_ctl = false;   // h_ctl->totals hold the statistics of every round that ran
     while (g->h_ctl->wl_n[g->h_ctl->round % 3u] != 0) {
+        stats_in_ctl = false;
         if (profile_kernels) {
             // one round per launch; each kernel's dispatch carries its own
             // start / end events (hipExtLaunchKernelGGL), so the times are the
@@ -654,6 +656,11 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             }
             GEQ_HIP(hipGraphLaunch(g->graph_exec, s), GEQ_ERR_NO_DEVICE);
             launches += shape.rounds_per_launch;
+            // the statistics so far travel with the control block: if this
+            // batch was the last, the solve needs no further wait
+            GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
+            k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, shape.eval_blocks, g->ctl.p->totals);
+            stats_in_ctl = true;
         }
         GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -667,15 +674,19 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         if (result != GEQ_OK) break;
     }
     const uint64_t rounds = g->h_ctl->last_nonempty;
-    GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
-    // the statistics come back with the same synchronisation
     unsigned long long h_stats[4] = {0, 0, 0, 0};
-    if (result == GEQ_OK) {
-        k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, shape.eval_blocks, g->stats.p + 4ull * STAT_SLOTS);
-        GEQ_HIP(hipMemcpyAsync(h_stats, g->stats.p + 4ull * STAT_SLOTS, sizeof(h_stats), hipMemcpyDeviceToHost, s),
-                GEQ_ERR_TRANSFER);
+    if (stats_in_ctl && result == GEQ_OK) {
+        for (int k = 0; k < 4; k++) h_stats[k] = g->h_ctl->totals[k];
+    } else {
+        // the statistics come back with one more synchronisation
+        GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
+        if (result == GEQ_OK) {
+            k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, shape.eval_blocks, g->stats.p + 4ull * STAT_SLOTS);
+            GEQ_HIP(hipMemcpyAsync(h_stats, g->stats.p + 4ull * STAT_SLOTS, sizeof(h_stats), hipMemcpyDeviceToHost, s),
+                    GEQ_ERR_TRANSFER);
+        }
+        GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     }
-    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev_start, ev_end);
     double eval_ms = 0, expand_ms = 0;

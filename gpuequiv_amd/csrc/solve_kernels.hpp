@@ -29,8 +29,11 @@ struct Ctl {                        // small device control block, read back bet
     uint32_t arena_top;             // bump pointer of the energy arena (u16 units)
     uint32_t round;                 // current round; its worklist is wl[round & 1]
     uint32_t last_nonempty;         // 1 + the last round whose worklist was not empty
-    uint32_t pad[9];
+    uint32_t pad;
+    unsigned long long totals[4];   // k_reduce_stats after a batch of rounds: node evaluations, edge
+                                    // updates, algorithmic bytes, changed nodes (read back with the block)
 };
+static_assert(sizeof(Ctl) == 64, "Ctl is one 64-byte block");
 
 // Kernels read the round's worklist and size from Ctl, so a round needs no
 // host-side state and can be captured once in a hipGraph and replayed.
