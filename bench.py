@@ -571,7 +571,9 @@ def main() -> int:
     st = _lib.geq_stats()
 
     def solve(profile: int = 0):
-        _lib.check(lib.geq_game_solve(handle, _lib.u32p(reach), len(reach), None, profile, C.byref(st)))
+        # EnergyGame::standard_reach: the defender dead ends (the `reach` list
+        # above), found on the device at the first solve and kept with the game
+        _lib.check(lib.geq_game_solve_standard(handle, profile, C.byref(st)))
         return st.as_dict()
 
     for _ in range(args.warmup):

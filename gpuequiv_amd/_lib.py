@@ -171,6 +171,7 @@ _SIGS = {
     "geq_energies_free": (None, [P(geq_energies)]),
     "geq_game_upload": (C.c_int, [C.c_void_p, P(geq_graph), P(C.c_void_p)]),
     "geq_game_solve": (C.c_int, [C.c_void_p, P(C.c_uint32), C.c_uint32, P(geq_energies), C.c_int, P(geq_stats)]),
+    "geq_game_solve_standard": (C.c_int, [C.c_void_p, C.c_int, P(geq_stats)]),
     "geq_game_energies": (C.c_int, [C.c_void_p, P(geq_energies)]),
     "geq_game_free": (None, [C.c_void_p]),
     "geq_standard_reach": (C.c_int, [P(geq_graph), P(C.c_uint32), P(C.c_uint32)]),

@@ -491,6 +491,14 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
     return geq::solve_core(g, n_reach, initial, profile_kernels, stats);
 }
 
+extern "C" geq_status geq_game_solve_standard(geq_game* g, int profile_kernels, geq_stats* stats) {
+    if (!g) return fail(GEQ_ERR_INVALID_ARG, "game is NULL");
+    (void)hipSetDevice(g->ctx->device);
+    const geq_status st = geq::standard_seed(g);
+    if (st != GEQ_OK) return st;
+    return geq::solve_core(g, g->std_n, nullptr, profile_kernels, stats, g->std_seed.p);
+}
+
 // The fixpoint on a resident game whose n_reach nodes to reach are already in g->seed.
 namespace {
 // Defender positions without successors.  A block of 256 threads takes

@@ -122,6 +122,11 @@ void geq_energies_free(geq_energies* e);
 geq_status geq_game_upload(geq_ctx* ctx, const geq_graph* graph, geq_game** out);
 geq_status geq_game_solve(geq_game* game, const uint32_t* to_reach, uint32_t n_reach, const geq_energies* initial,
                           int profile_kernels, geq_stats* stats);
+/* The same with EnergyGame::standard_reach (src/energygame.rs:347-352): the
+ * defender positions without successors start at {0}.  They are found on the
+ * device at the first such call and kept with the game, so repeated solves
+ * upload nothing. */
+geq_status geq_game_solve_standard(geq_game* game, int profile_kernels, geq_stats* stats);
 geq_status geq_game_energies(geq_game* game, geq_energies* out);
 void geq_game_free(geq_game* game);
 

@@ -206,6 +206,40 @@ def _solve_raw(graph, profile, **tune):
         lib.geq_game_free(h)
 
 
+def test_standard_reach_solve_matches_explicit_reach():
+    # geq_game_solve_standard seeds with the defender dead ends found on the
+    # device (once per game); the result equals the explicit standard_reach
+    # list's, on repeated solves of the same resident game and with profiling.
+    import ctypes as C
+    from gpuequiv_amd import _lib
+    lib = _lib.lib()
+    ctx = _lib.Context.get()
+    for V, seed in ((3000, 4), (50000, 9)):
+        graph = workloads.random_game(V, seed)
+        print("standard reach solve", V, seed, flush=True)
+        want, sw = _solve_raw(graph, 0)
+        keep = []
+        gs = graph._c_struct(keep)
+        h = C.c_void_p()
+        _lib.check(lib.geq_game_upload(ctx.handle, C.byref(gs), C.byref(h)))
+        try:
+            for profile in (0, 0, 1):
+                st = _lib.geq_stats()
+                _lib.check(lib.geq_game_solve_standard(h, profile, C.byref(st)))
+                out = _lib.geq_energies()
+                _lib.check(lib.geq_game_energies(h, C.byref(out)))
+                try:
+                    offs, data = _lib.energies_to_numpy(out)
+                finally:
+                    lib.geq_energies_free(C.byref(out))
+                assert H.canon_lists(offs, data) == want
+                s = st.as_dict()
+                assert (s["rounds"], s["edge_updates"], s["node_evals"]) == (sw["rounds"], sw["edge_updates"],
+                                                                            sw["node_evals"])
+        finally:
+            lib.geq_game_free(h)
+
+
 def test_profiled_launches_match_graph_replay():
     g = workloads.random_game(20000, 17)
     a, sa = _solve_raw(g, 0)
