@@ -488,6 +488,9 @@ def bench_serve(args) -> dict:
                                f"5% tau, 1..256 pairs; jobs over {max_pos} positions rejected; seed {args.seed}",
                    "positions": V, "edges": E, "budget_bytes": budget},
         "plain_path": {"jobs_per_s": len(jobs) / plain_s, "total_s": plain_s, "edge_updates": plain_edges,
+                       # the jobs over the size limit: standalone builds check it at level ends
+                       "over_limit_s": float(service[~np.array(ok)].sum()),
+                       "over_limit_positions": int(sum(v for (v, _), k in zip(sizes, ok) if not k)),
                        "service_ms_p50": float(np.percentile(service * 1e3, 50)),
                        "service_ms_p99": float(np.percentile(service * 1e3, 99))},
         "served_over_plain": best["served_jobs_per_s"] / (len(jobs) / plain_s),
