@@ -98,7 +98,7 @@ class Equivalence:
         idx = np.arange(m)
         red[idx, idx] = True
         part = np.asarray(self.minimization if self.minimization is not None else range(n), dtype=np.int64)
-        return red[part[:, None], part[None, :]]
+        return red.take(part, axis=0).take(part, axis=1)
 
     def relation(self, equivalence: Energy) -> "EquivalenceRelation":
         """Classes of processes equivalent under `equivalence` (:154-170)."""
