@@ -85,3 +85,30 @@ def test_preorder_matrix_matches_preorder_calls():
         for q in range(4):
             for k, e in enumerate(eqs):
                 assert mat[p, q, k] == eq.preorder(p, q, e), (p, q, k)
+
+
+def test_preorder_matrix_random_budgets_and_repeated_pairs():
+    # the vectorised matrix against per-pair preorder() calls: random budgets
+    # (some pairs with none), a minimisation that merges states, and a pair
+    # seeded twice (its last entry counts, as in the pair lookup)
+    import numpy as np
+    from gpuequiv_amd import EnergyArray, EnergyConf, std_equivalences
+    from gpuequiv_amd.equivalence import Equivalence, StartInfo
+    from gpuequiv_amd.gamebuild import AttackPosition
+    STD = EnergyConf.STANDARD
+    rng = np.random.default_rng(11)
+    m = 40
+    pairs = [(int(p), int(q)) for p, q in rng.integers(0, m, (300, 2)) if p != q]
+    pairs.append(pairs[5])
+    budgets = [rng.integers(0, 4, (int(rng.integers(0, 6)), 6)).tolist() for _ in pairs]
+    info = StartInfo([AttackPosition(p, (q,)) for p, q in pairs], 50)
+    eq = Equivalence(info, [EnergyArray.from_conf(b, STD) for b in budgets])
+    # every reduced state is the image of some state (as for a real partition)
+    eq.set_minimization(list(range(m)) + [int(x) for x in rng.integers(0, m, 50 - m)])
+    eqs = [getattr(std_equivalences, nm)() for nm in std_equivalences.NAMES]
+    mat = eq.preorder_matrix(eqs)
+    assert mat.shape == (50, 50, len(eqs))
+    for p in range(50):
+        for q in range(50):
+            for k, e in enumerate(eqs):
+                assert mat[p, q, k] == eq.preorder(p, q, e), (p, q, k)
