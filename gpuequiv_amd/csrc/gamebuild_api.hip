@@ -351,8 +351,11 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         const size_t words = std::max<size_t>(64, 4u + n_jobs);
         GB_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws.rb.h), 4 * words, hipHostMallocMapped | hipHostMallocCoherent),
                GEQ_ERR_OUT_OF_MEMORY);
+        if (hipHostGetDevicePointer(reinterpret_cast<void**>(&ws.rb.d), ws.rb.h, 0) != hipSuccess || !ws.rb.d) {
+            ws.rb.release();   // usable only with both views
+            return bail(fail(GEQ_ERR_INTERNAL, "gamebuild: no device view of the pinned count words"));
+        }
         ws.rb.n = words;
-        GB_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ws.rb.d), ws.rb.h, 0), GEQ_ERR_INTERNAL);
     }
     const uint32_t* const h_rb = ws.rb.h;
     auto wait = [&]() -> geq_status {
