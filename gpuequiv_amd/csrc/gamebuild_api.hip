@@ -82,33 +82,41 @@ geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<ui
     return GEQ_OK;
 }
 
-// Writes *a and *b to out[0] and out[1] (pinned host memory): a chunk's
-// candidate and set-pool totals reach the host with one launch and one wait.
-__global__ void k_readback(const uint32_t* a, const uint32_t* b, uint32_t* out) {
-    if (threadIdx.x == 0) out[0] = *a;
-    if (threadIdx.x == 1) out[1] = *b;
+// Words of the pinned readback block: a chunk's totals, the heavy positions
+// of the chunk bounded last, then the per-job position counts.
+constexpr uint32_t RB_EDGES = 0, RB_NEW = 1, RB_CAND = 2, RB_WORDS = 3, RB_HEAVY = 4, RB_JOBS = 5;
+
+// Writes a chunk's candidate and set-pool totals (*a, *b) and its heavy
+// positions (*h) to pinned host memory: they reach the host with one launch
+// and one wait.
+__global__ void k_readback(const uint32_t* a, const uint32_t* b, const uint32_t* h, uint32_t* out) {
+    if (threadIdx.x == 0) out[RB_CAND] = *a;
+    if (threadIdx.x == 1) out[RB_WORDS] = *b;
+    if (threadIdx.x == 2) out[RB_HEAVY] = *h;
 }
 
-// The counts at the end of a chunk, in pinned host memory: out[0] its edges,
-// out[1] its new positions, out[4 ..] the per-job position counts cnt.  With
-// `next` (the offsets k_bounds_next scanned for the next level, whose
-// *n_new + add positions are taken in chunks of `chunk`), out[2] and out[3] are
-// the candidate and set-pool totals of the next level's first chunk.
+// The counts at the end of a chunk, in pinned host memory: its edges, its new
+// positions and the per-job position counts cnt.  With `next` (the offsets
+// k_bounds_next scanned for the next level, whose *n_new + add positions are
+// taken in chunks of `chunk`), also the candidate and set-pool totals and the
+// heavy positions (*heavy) of the next level's first chunk.
 __global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
-                               const uint32_t* cnt, uint32_t n_cnt, uint32_t* out) {
-    for (uint32_t i = threadIdx.x; i < 4u + n_cnt; i += blockDim.x) {
+                               const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out) {
+    for (uint32_t i = threadIdx.x; i < RB_JOBS + n_cnt; i += blockDim.x) {
         uint32_t v = 0;
-        if (i == 0u) {
+        if (i == RB_EDGES) {
             v = *n_edges;
-        } else if (i == 1u) {
+        } else if (i == RB_NEW) {
             v = *n_new;
-        } else if (i < 4u) {
+        } else if (i == RB_CAND || i == RB_WORDS) {
             if (next) {
                 const uint2 t = next[min(*n_new + add, chunk)];
-                v = i == 2u ? t.x : t.y;
+                v = i == RB_CAND ? t.x : t.y;
             }
+        } else if (i == RB_HEAVY) {
+            v = next ? *heavy : 0u;
         } else {
-            v = cnt[i - 4u];
+            v = cnt[i - RB_JOBS];
         }
         out[i] = v;
     }
@@ -346,9 +354,9 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
                                  ? 4u * lts_words(n, m, (uint32_t)en_lbl.size()) : 0u;
     // the level loop's counts come back through pinned words written by
     // k_readback and k_chunk_counts
-    if (ws.rb.n < 4u + n_jobs) {
+    if (ws.rb.n < RB_JOBS + n_jobs) {
         ws.rb.release();
-        const size_t words = std::max<size_t>(64, 4u + n_jobs);
+        const size_t words = std::max<size_t>(64, RB_JOBS + n_jobs);
         GB_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws.rb.h), 4 * words, hipHostMallocMapped | hipHostMallocCoherent),
                GEQ_ERR_OUT_OF_MEMORY);
         if (hipHostGetDevicePointer(reinterpret_cast<void**>(&ws.rb.d), ws.rb.h, 0) != hipSuccess || !ws.rb.d) {
@@ -453,13 +461,13 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     // The last chunk of a level also bounds and scans the next level before
     // its wait, so the next level's first chunk starts without one.
     bool have_next = false;
-    uint32_t next_cand = 0, next_words = 0;
+    uint32_t next_cand = 0, next_words = 0, next_heavy = 0;
     while (lo < hi) {
         const uint32_t c_hi = hi - lo > chunk ? lo + chunk : hi;
         const uint32_t nf = c_hi - lo;
         GB_TRY(ensure(nsucc, nf + 1, s, false, "frontier"));
         GB_TRY(ensure(rowl, nf + 1, s, false, "frontier"));
-        uint32_t n_cand = next_cand, n_words = next_words;
+        uint32_t n_cand = next_cand, n_words = next_words, n_heavy = next_heavy;
         if (!have_next) {
             GB_TRY(ensure(ub, nf + 1, s, false, "frontier"));
             GB_TRY(ensure(off, nf + 1, s, false, "frontier"));
@@ -468,10 +476,11 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             GB_HIP(hipMemsetAsync(ws.hv_count.p, 0, 4, s), GEQ_ERR_TRANSFER);
             k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub.p, heavy_args());
             GB_TRY(exclusive_scan_pairs(ub.p, off.p, nf + 1, tmp, s));
-            k_readback<<<1, 64, 0, s>>>(&off.p[nf].x, &off.p[nf].y, ws.rb.d);
+            k_readback<<<1, 64, 0, s>>>(&off.p[nf].x, &off.p[nf].y, ws.hv_count.p, ws.rb.d);
             GB_TRY(wait());
-            n_cand = h_rb[0];
-            n_words = h_rb[1];
+            n_cand = h_rb[RB_CAND];
+            n_words = h_rb[RB_WORDS];
+            n_heavy = h_rb[RB_HEAVY];
         }
         have_next = false;
         if ((uint64_t)B->spool_top + n_words >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: set pool exceeds 2^32 words"));
@@ -486,8 +495,8 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         GB_TRY(ensure(B->wt, E + n_cand + 1, s, true, "edges"));
         k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, off.p, ws.hv_flag.p, B->spool_top, cands.p,
                                                                nsucc.p);
-        if (heavy_min && nf)
-            k_generate_heavy<<<std::min<uint32_t>(nf, 1024), 256, 0, s>>>(nodes(), lo, L, off.p, B->spool_top, heavy_args(),
+        if (n_heavy)   // k_bounds listed them (their count came back with the totals)
+            k_generate_heavy<<<std::min<uint32_t>(n_heavy, 1024), 256, 0, s>>>(nodes(), lo, L, off.p, B->spool_top, heavy_args(),
                                                                           cands.p, nsucc.p);
         GB_TRY(exclusive_scan<uint32_t>(nsucc.p, rowl.p, nf + 1, tmp, s));
         if (n_cand) {
@@ -527,13 +536,14 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         // one synchronisation per chunk for the edge and position counts, the per-job
         // counts and the next level's first totals
         k_chunk_counts<<<1, 256, 0, s>>>(rowl.p + nf, rank.p + n_cand, add, chunk, last_chunk ? off.p : nullptr,
-                                         jt ? d_jcnt.p : nullptr, n_jobs, ws.rb.d);
+                                         ws.hv_count.p, jt ? d_jcnt.p : nullptr, n_jobs, ws.rb.d);
         GB_TRY(wait());
-        const uint32_t n_edges = h_rb[0], n_new = h_rb[1];
+        const uint32_t n_edges = h_rb[RB_EDGES], n_new = h_rb[RB_NEW];
         if (last_chunk) {
             have_next = true;
-            next_cand = h_rb[2];
-            next_words = h_rb[3];
+            next_cand = h_rb[RB_CAND];
+            next_words = h_rb[RB_WORDS];
+            next_heavy = h_rb[RB_HEAVY];
         }
         E += n_edges;
         B->spool_top += n_words;
@@ -548,7 +558,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             // positions of every job so far (read with the counts above); jobs past
             // the cap stop growing (k_mark_dead set the same flags on the device)
             for (uint32_t j = 0; j < n_jobs; j++) {
-                jt->count[j] = h_rb[4 + j];
+                jt->count[j] = h_rb[RB_JOBS + j];
                 if (jt->cap && jt->count[j] > jt->cap) jt->dead[j] = 1;
             }
         } else if (level_done && max_positions && B->V > max_positions) {
