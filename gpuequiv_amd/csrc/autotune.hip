@@ -62,11 +62,22 @@ geq_status autotune(geq_game* g, const std::vector<uint32_t>* reach, uint32_t re
         uint32_t Tune::*field;
         std::vector<uint32_t> values;
     };
+    // the game's own round count is a candidate batch of rounds too: one
+    // replay then covers the whole fixpoint with no empty rounds after it
+    std::vector<uint32_t> batches = {1, 2, 4, 8, 16, 32};
+    {
+        geq_stats st{};
+        const geq_status s = reach ? geq_game_solve(g, reach->data(), (uint32_t)reach->size(), nullptr, 0, &st)
+                                   : solve_core(g, g->std_n, nullptr, 0, &st, g->std_seed.p);
+        if (s != GEQ_OK) return s;
+        if (st.rounds >= 1 && st.rounds <= 256 && std::find(batches.begin(), batches.end(), (uint32_t)st.rounds) == batches.end())
+            batches.push_back((uint32_t)st.rounds);
+    }
     const std::vector<Dim> dims = {
         {&Tune::waves_per_block, {1, 2, 4, 8}},
         {&Tune::eval_blocks_per_cu, {1, 2, 4, 8, 16, 32}},
         {&Tune::expand_blocks_per_cu, {0, 1, 2, 4, 8, 16}},   // 0: expansion inside k_eval
-        {&Tune::rounds_per_launch, {1, 2, 4, 8, 16, 32}},
+        {&Tune::rounds_per_launch, batches},
     };
     uint64_t peak = 0;
     uint32_t candidates = 0;
