@@ -409,14 +409,9 @@ ExpandArgs expand_args(geq_game* g) {
 // shared one: host threads solving other games may enqueue on that stream
 // meanwhile, and a capture would take their work in.  (Capturing records;
 // nothing runs on the capture stream.)
-geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape) {
-    if (!g->cap_stream) GEQ_HIP(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking), GEQ_ERR_NO_DEVICE);
-    hipStream_t s = g->cap_stream;
-    if (g->graph_exec) {
-        (void)hipGraphExecDestroy(g->graph_exec);
-        g->graph_exec = nullptr;
-    }
-    GEQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), GEQ_ERR_NO_DEVICE);
+// One batch of rounds_per_launch rounds and the k_advance after them, on s:
+// what a round graph holds.
+void launch_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape, hipStream_t s) {
     for (uint32_t k = 0; k < shape.rounds_per_launch; k++) {
         if (shape.fused) {
             k_eval<true><<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_FUSED_LDS_PER_WAVE, s>>>(ea, k);
@@ -426,6 +421,17 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
         }
     }
     k_advance<<<1, 64, 0, s>>>(g->ctl.p, shape.rounds_per_launch);
+}
+
+geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape) {
+    if (!g->cap_stream) GEQ_HIP(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking), GEQ_ERR_NO_DEVICE);
+    hipStream_t s = g->cap_stream;
+    if (g->graph_exec) {
+        (void)hipGraphExecDestroy(g->graph_exec);
+        g->graph_exec = nullptr;
+    }
+    GEQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), GEQ_ERR_NO_DEVICE);
+    launch_rounds(g, ea, xa, shape, s);
     hipGraph_t graph = nullptr;
     GEQ_HIP(hipStreamEndCapture(s, &graph), GEQ_ERR_NO_DEVICE);
     hipError_t e = hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0);
@@ -657,12 +663,19 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             result = fail(GEQ_ERR_UNSUPPORTED, "the cooperative solve mode (GEQ_SOLVE_MODE=0) is disabled");
             break;
         } else {
-            // rounds_per_launch captured rounds per graph launch
-            if (!g->graph_exec || g->graph_arena != g->arena.p) {
-                result = capture_rounds(g, ea, xa, shape);
-                if (result != GEQ_OK) break;
+            // rounds_per_launch captured rounds per graph launch.  A game's
+            // first solve launches the same kernels directly: a game solved
+            // once does not pay for a capture; its next solve captures.
+            if (!g->graph_exec && g->solves == 0) {
+                launch_rounds(g, ea, xa, shape, s);
+                GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
+            } else {
+                if (!g->graph_exec || g->graph_arena != g->arena.p) {
+                    result = capture_rounds(g, ea, xa, shape);
+                    if (result != GEQ_OK) break;
+                }
+                GEQ_HIP(hipGraphLaunch(g->graph_exec, s), GEQ_ERR_NO_DEVICE);
             }
-            GEQ_HIP(hipGraphLaunch(g->graph_exec, s), GEQ_ERR_NO_DEVICE);
             launches += shape.rounds_per_launch;
             // the statistics so far travel with the control block: if this
             // batch was the last, the solve needs no further wait
@@ -682,6 +695,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         if (result != GEQ_OK) break;
     }
     const uint64_t rounds = g->h_ctl->last_nonempty;
+    if (result == GEQ_OK) g->solves++;
     unsigned long long h_stats[4] = {0, 0, 0, 0};
     if (stats_in_ctl && result == GEQ_OK) {
         for (int k = 0; k < 4; k++) h_stats[k] = g->h_ctl->totals[k];

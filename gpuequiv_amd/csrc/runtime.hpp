@@ -142,6 +142,7 @@ struct geq_game {
     } tune;
     std::vector<uint32_t> game_base;      // batched games: first node of every game (+ total)
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds
+    uint64_t solves = 0;                  // solves finished (the first launches its rounds directly)
     hipEvent_t ev_start = nullptr, ev_end = nullptr;   // around each solve's rounds
     hipStream_t cap_stream = nullptr;                  // graph capture only
     void* graph_arena = nullptr;          // arena pointer baked into graph_exec
