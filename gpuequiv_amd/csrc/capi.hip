@@ -117,6 +117,8 @@ void geq_ctx_destroy(geq_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     ctx->build_ws.reset();   // its device buffers, before the stream goes
+    if (ctx->spare_game) geq_game_free(ctx->spare_game);
+    ctx->spare_game = nullptr;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -539,7 +541,8 @@ geq_status geq::standard_seed(geq_game* g) {
     if (g->std_ready) return GEQ_OK;
     hipStream_t s = g->ctx->stream;
     const uint32_t V = g->V;
-    GEQ_HIP(g->std_seed.alloc(std::max<uint32_t>(V, 1) + 1), GEQ_ERR_OUT_OF_MEMORY);
+    if (g->std_seed.n < std::max<uint32_t>(V, 1) + 1ull)
+        GEQ_HIP(g->std_seed.alloc(std::max<uint32_t>(V, 1) + 1), GEQ_ERR_OUT_OF_MEMORY);
     uint32_t* cnt = g->std_seed.p + std::max<uint32_t>(V, 1);   // the count sits after the list
     GEQ_HIP(hipMemsetAsync(cnt, 0, 4, s), GEQ_ERR_TRANSFER);
     if (V) k_standard_reach<<<(V + 256 * REACH_PER - 1) / (256 * REACH_PER), 256, 0, s>>>(g->row.p, g->atk.p, V, g->std_seed.p, cnt);

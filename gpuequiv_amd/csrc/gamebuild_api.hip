@@ -165,21 +165,72 @@ geq_status geq::alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_gam
     return GEQ_OK;
 }
 
+namespace {
+// Whether a cached game's buffers hold a game of V nodes and E edges.
+bool game_fits(const geq_game* g, uint32_t V, uint32_t E) {
+    return g->row.n >= V + 1ull && g->rrow.n >= V + 1ull && g->rcol.n >= E && g->edges.n >= E && g->atk.n >= V &&
+           g->st.n >= (size_t)NODE_U4 * V && g->wl0.n >= V && g->wl1.n >= V && g->chg.n >= V &&
+           g->queued.n >= 2ull * queue_words(V) && g->arena_cap >= initial_arena_entries(E, 0) && g->h_ctl && g->ctl.p &&
+           g->stats.p;
+}
+}  // namespace
+
+void geq::release_built_game(geq_game* g) {
+    geq_ctx* ctx = g->ctx;
+    geq_game* drop = g;
+    {
+        // keep the game with more node room: it fits more of the next games
+        std::lock_guard<std::mutex> lk(ctx->game_mu);
+        if (!ctx->spare_game || ctx->spare_game->st.n <= g->st.n) {
+            drop = ctx->spare_game;
+            ctx->spare_game = g;
+        }
+    }
+    if (drop) geq_game_free(drop);
+}
+
 geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,
                                  const uint32_t* d_wt, const uint8_t* d_atk, geq_game** out) {
     hipStream_t s = ctx->stream;
     geq_game* gm = nullptr;
-    geq_status st0 = alloc_game_buffers(ctx, V, E, &gm);
-    if (st0 != GEQ_OK) return st0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->game_mu);
+        if (ctx->spare_game && game_fits(ctx->spare_game, V, E)) {
+            gm = ctx->spare_game;
+            ctx->spare_game = nullptr;
+        }
+    }
+    if (gm) {
+        // the buffers of a freed built game: everything of the old game is reset
+        // (its lists and control block are reset by every solve)
+        gm->V = V;
+        gm->E = E;
+        gm->game_base = {0, V};
+        gm->tune = geq_game::Tune{};
+        if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
+        gm->std_ready = false;
+        gm->std_n = 0;
+        gm->retag_round = 0;
+        gm->solves = 0;
+        if (gm->graph_exec) (void)hipGraphExecDestroy(gm->graph_exec);
+        gm->graph_exec = nullptr;
+        gm->graph_arena = nullptr;
+        gm->h_atk.clear();
+    } else {
+        geq_status st0 = alloc_game_buffers(ctx, V, E, &gm);
+        if (st0 != GEQ_OK) return st0;
+    }
     auto oom = [&](hipError_t e, const char* what) -> geq_status {
         delete gm;
         return fail(e == hipErrorOutOfMemory ? GEQ_ERR_OUT_OF_MEMORY : GEQ_ERR_NO_DEVICE,
                     std::string("allocating ") + what + ": " + hipGetErrorString(e));
     };
     hipError_t e;
-    DevBuf<uint32_t> cnt, fill;
-    DevBuf<uint8_t> tmp;
-    if ((e = cnt.alloc(V + 1)) != hipSuccess || (e = fill.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");
+    DevBuf<uint32_t>& cnt = gm->rev_cnt;
+    DevBuf<uint32_t>& fill = gm->rev_fill;
+    DevBuf<uint8_t>& tmp = gm->rev_tmp;
+    if (cnt.n < V + 1ull && (e = cnt.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");
+    if (fill.n < V + 1ull && (e = fill.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");
     const uint32_t tb = 256;
     GEQ_HIP(hipMemcpyAsync(gm->row.p, d_row, 4ull * (V + 1), hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemcpyAsync(gm->atk.p, d_atk, V, hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);

@@ -60,6 +60,9 @@ geq_status solve_core(geq_game* g, uint32_t n_reach, const geq_energies* initial
 // the defender positions without successors, found on the device once and
 // kept in g->std_seed / g->std_n.
 geq_status standard_seed(geq_game* g);
+// A built game's buffers go to its context's cache (one game) for the next
+// built game, or are freed when the cache is taken.
+void release_built_game(geq_game* g);
 geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_energies* out);
 // Device-resident game from device arrays (STANDARD conf): rows, columns,
 // packed updates, attacker flags; the reverse CSR is derived on the device.
@@ -112,6 +115,10 @@ struct geq_ctx {
     std::mutex build_mu;
     std::shared_ptr<geq::gb::BuildScratch> build_ws;
     std::atomic<uint64_t> hint_positions{0}, hint_edges{0}, hint_words{0};
+    // the device buffers of the last freed built game, for the next built game
+    // that fits in them (game_from_device); guarded by game_mu
+    std::mutex game_mu;
+    geq_game* spare_game = nullptr;
 };
 
 struct geq_game {
@@ -126,6 +133,8 @@ struct geq_game {
     geq::DevBuf<uint2> edges;
     geq::DevBuf<uint8_t> atk, chg;
     geq::DevBuf<uint4> st;            // two list slots per node
+    geq::DevBuf<uint32_t> rev_cnt, rev_fill;   // reverse-CSR scratch of game_from_device
+    geq::DevBuf<uint8_t> rev_tmp;
     geq::DevBuf<uint16_t> arena;
     geq::DevBuf<geq::Ctl> ctl;
     geq::DevBuf<unsigned long long> stats;
@@ -160,6 +169,6 @@ struct geq_built {
     geq_game* game = nullptr;      // created on the first solve
     uint32_t n_reach = 0;
     ~geq_built() {
-        if (game) geq_game_free(game);
+        if (game) geq::release_built_game(game);
     }
 };

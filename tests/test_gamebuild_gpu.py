@@ -197,6 +197,22 @@ def test_repeated_start_pair_survives_table_growth():
     _assert_same_game(b, lts, pairs)
 
 
+def test_built_games_reuse_freed_buffers():
+    # A freed built game's device buffers serve the next built game that fits
+    # in them: bigger, smaller and equal games in a row each give the oracle's
+    # game and energies, also when the reused buffers still hold a larger game.
+    print("built games in a row (buffer reuse)", flush=True)
+    shapes = [(60, 200, 4, 3), (12, 30, 2, 1), (60, 200, 4, 3), (30, 90, 3, 2), (60, 200, 4, 3), (12, 30, 2, 1)]
+    for n, m, labels, seed in shapes:
+        n2, src, dst, lbl = workloads.random_lts(n, m, labels, seed)
+        lts = TransitionSystem.from_arrays(n2, src, dst, lbl)
+        pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q]
+        b = GameBuild(lts, pairs)
+        og = _assert_same_game(b, lts, pairs)
+        _assert_same_energies(b, og)
+        b.close()
+
+
 def test_truncated_build_is_well_formed():
     n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
     lts = TransitionSystem.from_arrays(n, src, dst, lbl)
