@@ -313,10 +313,11 @@ def bench_gamebuild(args) -> dict:
         bs = b.build_stats
         builds.append(bs["build_ms"])
         if not bs["truncated"] or i == args.steps - 1:
-            # a capped game is still a well-formed energy game (the positions of
-            # its last level have no successors): its solve is timed once
-            b.solve()
-            solves.append(b.solve())
+            # a capped game is solved as a game of its own (its unexpanded
+            # frontier is never a position to reach): timed once, and reported
+            # as the capped game's solve, not the reference's answer
+            b.solve(capped=True)
+            solves.append(b.solve(capped=True))
         b.close()
     V, E = int(bs["positions"]), int(bs["edges"])
     bms = float(np.median(builds))

@@ -19,18 +19,41 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    sources = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           # packed FP32 instructions stay off (no kernel here needs them)
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-function", "-Wno-unused-parameter", "-Wno-unused-variable",
-           "-o", OUT + ".tmp"] + sources
+def _flags() -> list:
+    return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+            # packed FP32 instructions stay off (no kernel here needs them)
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wall", "-Wno-unused-function", "-Wno-unused-parameter", "-Wno-unused-variable"]
+
+
+def _run(cmd: list, verbose: bool) -> subprocess.CompletedProcess:
     if verbose:
         print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    msgs = [ln for ln in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in ln]
-    if r.returncode != 0:
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every source to an object in parallel, then link the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sources = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
+    obj_dir = os.path.join(HERE, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.basename(src) + ".o") for src in sources]
+    jobs = max(1, min(len(sources), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(jobs) as pool:
+        results = list(pool.map(lambda so: _run([hipcc()] + _flags() + ["-c", so[0], "-o", so[1]], verbose),
+                                zip(sources, objs)))
+    msgs = []
+    failed = False
+    for r in results:
+        msgs += [ln for ln in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in ln]
+        failed |= r.returncode != 0
+    if not failed:
+        r = _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs, verbose)
+        msgs += [ln for ln in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in ln]
+        failed = r.returncode != 0
+    if failed:
         sys.stderr.write("\n".join(msgs) + "\n")
         raise RuntimeError("hipcc failed building libgpuequiv_hip.so")
     if verbose and msgs:

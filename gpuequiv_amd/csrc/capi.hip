@@ -15,6 +15,38 @@ namespace geq {
 
 static thread_local std::string g_last_error;
 
+// Every live context, so that an allocation that runs out of device memory
+// can free the game buffers the contexts keep for reuse.
+static std::mutex g_ctx_mu;
+static std::vector<geq_ctx*> g_ctxs;
+
+bool drop_spare_games(int device) {
+    std::vector<geq_game*> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        for (geq_ctx* c : g_ctxs) {
+            if (c->device != device) continue;
+            std::lock_guard<std::mutex> gl(c->game_mu);
+            if (c->spare_game) drop.push_back(c->spare_game);
+            c->spare_game = nullptr;
+        }
+    }
+    for (geq_game* g : drop) delete g;
+    return !drop.empty();
+}
+
+hipError_t device_malloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && drop_spare_games(dev)) {
+            (void)hipGetLastError();   // the failed attempt's error is handled here
+            e = hipMalloc(p, bytes);
+        }
+    }
+    return e;
+}
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 geq_status fail(geq_status s, const std::string& msg) {
     g_last_error = msg;
@@ -109,12 +141,20 @@ geq_status geq_ctx_create(int device, geq_ctx** out) {
         delete c;
         return fail(GEQ_ERR_NO_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        g_ctxs.push_back(c);
+    }
     *out = c;
     return GEQ_OK;
 }
 
 void geq_ctx_destroy(geq_ctx* ctx) {
     if (!ctx) return;
+    {
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        g_ctxs.erase(std::remove(g_ctxs.begin(), g_ctxs.end(), ctx), g_ctxs.end());
+    }
     (void)hipSetDevice(ctx->device);
     ctx->build_ws.reset();   // its device buffers, before the stream goes
     if (ctx->spare_game) geq_game_free(ctx->spare_game);
@@ -344,7 +384,7 @@ geq_status geq::grow_arena(geq_game* g, uint32_t needed_top) {
     cap = std::min<uint64_t>(cap, 0xffffffffull);
     if (cap <= g->arena_cap) return fail(GEQ_ERR_OUT_OF_MEMORY, "energy arena exceeds 2^32 entries");
     uint16_t* p = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), 2ull * cap);
+    hipError_t e = device_malloc(reinterpret_cast<void**>(&p), 2ull * cap);
     if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("growing the energy arena: ") + hipGetErrorString(e));
     e = hipMemcpyAsync(p, g->arena.p, 2ull * g->arena_cap, hipMemcpyDeviceToDevice, g->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
@@ -545,7 +585,8 @@ geq_status geq::standard_seed(geq_game* g) {
         GEQ_HIP(g->std_seed.alloc(std::max<uint32_t>(V, 1) + 1), GEQ_ERR_OUT_OF_MEMORY);
     uint32_t* cnt = g->std_seed.p + std::max<uint32_t>(V, 1);   // the count sits after the list
     GEQ_HIP(hipMemsetAsync(cnt, 0, 4, s), GEQ_ERR_TRANSFER);
-    if (V) k_standard_reach<<<(V + 256 * REACH_PER - 1) / (256 * REACH_PER), 256, 0, s>>>(g->row.p, g->atk.p, V, g->std_seed.p, cnt);
+    const uint32_t Vr = std::min(V, g->reach_limit);   // the nodes that may be reached
+    if (Vr) k_standard_reach<<<(Vr + 256 * REACH_PER - 1) / (256 * REACH_PER), 256, 0, s>>>(g->row.p, g->atk.p, Vr, g->std_seed.p, cnt);
     GEQ_HIP(hipMemcpyAsync(&g->std_n, cnt, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     g->std_ready = true;

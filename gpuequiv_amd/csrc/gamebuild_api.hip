@@ -40,7 +40,7 @@ geq_status ensure(DevBuf<T>& b, size_t need, hipStream_t s, bool keep, const cha
     if (b.p && b.n >= need) return GEQ_OK;
     size_t cap = std::max<size_t>(need, b.n ? 2 * b.n : 1024);
     T* p = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * cap);
+    hipError_t e = device_malloc(reinterpret_cast<void**>(&p), sizeof(T) * cap);
     if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("gamebuild: allocating ") + what + ": " + hipGetErrorString(e));
     if (keep && b.p && b.n) {
         e = hipMemcpyAsync(p, b.p, sizeof(T) * b.n, hipMemcpyDeviceToDevice, s);
@@ -210,6 +210,7 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
         if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
         gm->std_ready = false;
         gm->std_n = 0;
+        gm->reach_limit = 0xffffffffu;
         gm->retag_round = 0;
         gm->solves = 0;
         if (gm->graph_exec) (void)hipGraphExecDestroy(gm->graph_exec);
@@ -220,8 +221,9 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
         geq_status st0 = alloc_game_buffers(ctx, V, E, &gm);
         if (st0 != GEQ_OK) return st0;
     }
+    // freed on every error path below (the cached buffers included)
+    std::unique_ptr<geq_game> own(gm);
     auto oom = [&](hipError_t e, const char* what) -> geq_status {
-        delete gm;
         return fail(e == hipErrorOutOfMemory ? GEQ_ERR_OUT_OF_MEMORY : GEQ_ERR_NO_DEVICE,
                     std::string("allocating ") + what + ": " + hipGetErrorString(e));
     };
@@ -238,15 +240,12 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
     k_fill_u32<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(cnt.p, V + 1, 0u);
     if (E) k_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(d_col, E, cnt.p);
     geq_status st = exclusive_scan<uint32_t>(cnt.p, gm->rrow.p, V + 1, tmp, s);
-    if (st != GEQ_OK) {
-        delete gm;
-        return st;
-    }
+    if (st != GEQ_OK) return st;
     GEQ_HIP(hipMemcpyAsync(fill.p, gm->rrow.p, 4ull * (V + 1), hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
     k_rev_scatter<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, V, d_col, fill.p, gm->rcol.p);
     if (V) k_node_hdr<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, gm->rrow.p, gm->atk.p, V, gm->st.p);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
-    *out = gm;
+    *out = own.release();
     return GEQ_OK;
 }
 
@@ -627,6 +626,8 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         GB_HIP(hipMemcpyAsync(B->row.p + B->V, &e32, 4, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
     B->E = (uint32_t)E;
+    B->truncated = truncated;
+    B->frontier = truncated ? lo : B->V;
     GB_TRY(ensure(B->atk, std::max<uint32_t>(B->V, 1), s, false, "attacker flags"));
     if (B->V) k_attacker_flags<<<(B->V + tb - 1) / tb, tb, 0, s>>>(B->kind.p, B->V, B->atk.p);
     GB_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -873,17 +874,46 @@ void geq_built_game_free(geq_built_game* o) {
     std::memset(o, 0, sizeof(*o));
 }
 
-geq_status geq_built_solve(geq_built* B, int profile_kernels, geq_stats* stats) {
-    if (!B) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+namespace {
+geq_status built_solve(geq_built* B, int profile_kernels, geq_stats* stats) {
     (void)hipSetDevice(B->ctx->device);
     if (!B->game) {
-        geq_status st = game_from_device(B->ctx, B->V, B->E, B->row.p, B->col.p, B->wt.p, B->atk.p, &B->game);
+        geq_game* g = nullptr;
+        geq_status st = game_from_device(B->ctx, B->V, B->E, B->row.p, B->col.p, B->wt.p, B->atk.p, &g);
         if (st != GEQ_OK) return st;
-        st = standard_seed(B->game);
-        if (st != GEQ_OK) return st;
-        B->n_reach = B->game->std_n;
+        // the frontier of a capped build is never a position to reach
+        g->reach_limit = B->frontier;
+        st = standard_seed(g);
+        if (st != GEQ_OK) {
+            release_built_game(g);
+            return st;
+        }
+        B->game = g;
+        B->n_reach = g->std_n;
     }
     return solve_core(B->game, B->n_reach, nullptr, profile_kernels, stats, B->game->std_seed.p);
+}
+}  // namespace
+
+geq_status geq_built_solve(geq_built* B, int profile_kernels, geq_stats* stats) {
+    if (!B) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    if (B->truncated)
+        return fail(GEQ_ERR_INVALID_ARG, "the build stopped at max_positions: positions " + std::to_string(B->frontier) +
+                                             ".. were never expanded, so this is not the complete game "
+                                             "(geq_built_solve_capped solves it with that frontier left out)");
+    return built_solve(B, profile_kernels, stats);
+}
+
+geq_status geq_built_solve_capped(geq_built* B, int profile_kernels, geq_stats* stats) {
+    if (!B) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    return built_solve(B, profile_kernels, stats);
+}
+
+geq_status geq_built_frontier(geq_built* B, uint32_t* truncated, uint32_t* frontier) {
+    if (!B) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    if (truncated) *truncated = B->truncated ? 1u : 0u;
+    if (frontier) *frontier = B->frontier;
+    return GEQ_OK;
 }
 
 geq_status geq_built_energies(geq_built* B, uint32_t first, uint32_t count, geq_energies* out) {

@@ -35,6 +35,13 @@ bool energy_to_canon(const ConfInfo& c, uint32_t packed, uint32_t& canon);
 uint32_t energy_from_canon(const ConfInfo& c, uint32_t canon);
 bool update_to_canon(const ConfInfo& c, uint32_t packed, uint32_t& canon);
 
+// hipMalloc that, when the device is out of memory, frees the game buffers
+// every context on the current device keeps for reuse (release_built_game)
+// and tries once more.  Every device allocation of the library goes through it.
+hipError_t device_malloc(void** p, size_t bytes);
+// Free the cached spare game of every context on `device`; true if one was freed.
+bool drop_spare_games(int device);
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -48,7 +55,7 @@ struct DevBuf {
     hipError_t alloc(size_t count) {
         release();
         n = count;
-        return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (count ? count : 1));
+        return device_malloc(reinterpret_cast<void**>(&p), sizeof(T) * (count ? count : 1));
     }
 };
 
@@ -130,6 +137,7 @@ struct geq_game {
     geq::DevBuf<uint32_t> std_seed;   // standard_reach, once computed
     uint32_t std_n = 0;
     bool std_ready = false;
+    uint32_t reach_limit = 0xffffffffu;   // standard_reach looks at the nodes below this id only
     geq::DevBuf<uint2> edges;
     geq::DevBuf<uint8_t> atk, chg;
     geq::DevBuf<uint4> st;            // two list slots per node
@@ -168,6 +176,10 @@ struct geq_built {
     geq::DevBuf<uint32_t> p, q_off, q_len, x_off, x_len, hash, spool, row, col, wt;
     geq_game* game = nullptr;      // created on the first solve
     uint32_t n_reach = 0;
+    // max_positions stopped the build: the positions from `frontier` on were
+    // never expanded (their rows are empty although they may have successors)
+    bool truncated = false;
+    uint32_t frontier = 0;
     ~geq_built() {
         if (game) geq::release_built_game(game);
     }

@@ -363,7 +363,7 @@ geq_status compact_arena(geq_server* S) {
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(S->arena_min, 2ull * live + (1u << 20)), 0xffffffffull);
     uint16_t* p = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), 2ull * cap);
+    hipError_t e = device_malloc(reinterpret_cast<void**>(&p), 2ull * cap);
     if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("compacting the energy arena: ") + hipGetErrorString(e));
     k_arena_move<<<(n + tb - 1) / tb, tb, 0, s>>>(g->st.p, g->arena.p, p, S->place_excl.p, n, t);
     e = hipMemcpyAsync(&g->ctl.p->arena_top, &live, 4, hipMemcpyHostToDevice, s);

@@ -106,10 +106,26 @@ class GameBuild:
     def n_positions(self) -> int:
         return int(self.build_stats["positions"])
 
+    def truncated(self) -> bool:
+        """max_positions stopped the build: the last level was never expanded."""
+        return bool(self.build_stats["truncated"])
+
+    def frontier(self) -> int:
+        """First unexpanded position of a capped build (the position count otherwise)."""
+        t, f = C.c_uint32(), C.c_uint32()
+        _lib.check(_lib.lib().geq_built_frontier(self.handle, C.byref(t), C.byref(f)))
+        return int(f.value)
+
     # -------------------------------------------------------------- device solve
-    def solve(self, profile: bool = False) -> dict:
+    def solve(self, profile: bool = False, capped: bool = False) -> dict:
+        """EnergyGame::standard_reach(game).run() on the device.  A capped build
+        raises (it is not the complete game) unless ``capped`` asks for the
+        solve of the capped game as a game of its own: its unexpanded frontier
+        is then never a position to reach, and the budgets are not the
+        reference's."""
         st = _lib.geq_stats()
-        _lib.check(_lib.lib().geq_built_solve(self.handle, 1 if profile else 0, C.byref(st)))
+        fn = _lib.lib().geq_built_solve_capped if capped else _lib.lib().geq_built_solve
+        _lib.check(fn(self.handle, 1 if profile else 0, C.byref(st)))
         self.last_stats = st.as_dict()
         return self.last_stats
 

@@ -228,8 +228,22 @@ geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs
                          geq_built** out, geq_build_stats* stats);
 geq_status geq_built_download(geq_built* built, geq_built_game* out);
 void geq_built_game_free(geq_built_game* g);
-/* standard_reach solve of the built game on the device */
+/* standard_reach solve of the built game on the device.  A build stopped by
+ * max_positions (geq_build_stats.truncated) is refused with
+ * GEQ_ERR_INVALID_ARG: its last level's positions were never expanded, and
+ * solving it as a complete game would count them as defender dead ends the
+ * attacker wins at zero energy. */
 geq_status geq_built_solve(geq_built* built, int profile_kernels, geq_stats* stats);
+/* The solve of a capped build as a game of its own: the unexpanded frontier
+ * positions are left out of the positions to reach (the attacker cannot win
+ * through them), every other rule as geq_built_solve.  Each budget set is then
+ * contained in the complete game's up-closure, so these are NOT the reference's
+ * answers; for a complete build it equals geq_built_solve.  No counterpart in
+ * the reference, which always builds the whole game. */
+geq_status geq_built_solve_capped(geq_built* built, int profile_kernels, geq_stats* stats);
+/* truncated: 1 if max_positions stopped the build; frontier: the first
+ * unexpanded position (the position count when complete) */
+geq_status geq_built_frontier(geq_built* built, uint32_t* truncated, uint32_t* frontier);
 /* energies of positions [first, first + count) after geq_built_solve */
 geq_status geq_built_energies(geq_built* built, uint32_t first, uint32_t count, geq_energies* out);
 void geq_built_free(geq_built* built);

@@ -761,12 +761,15 @@ public:
     const geq_build_stats& build_stats() const { return stats_; }
     uint64_t n_positions() const { return stats_.positions; }
 
-    // EnergyGame::standard_reach(game).run() on the device-resident game
-    geq_stats solve() {
+    // EnergyGame::standard_reach(game).run() on the device-resident game; a
+    // build stopped by max_positions throws unless `capped` asks for the solve
+    // of the capped game (its unexpanded frontier never reached)
+    geq_stats solve(bool capped = false) {
         geq_stats st{};
-        detail::check(geq_built_solve(built_.get(), 0, &st));
+        detail::check(capped ? geq_built_solve_capped(built_.get(), 0, &st) : geq_built_solve(built_.get(), 0, &st));
         return st;
     }
+    bool truncated() const { return stats_.truncated != 0; }
     std::vector<EnergyArray> energies(uint32_t first, uint32_t count) const {
         detail::EnergiesGuard out;
         detail::check(geq_built_energies(built_.get(), first, count, &out.e));

@@ -231,6 +231,44 @@ def test_truncated_build_is_well_formed():
         assert np.array_equal(h[k], og[k]), k
 
 
+def test_truncated_build_refuses_a_complete_solve():
+    # A capped build is not the complete game: its last level's positions were
+    # never expanded.  geq_built_solve refuses it; the capped solve leaves that
+    # frontier out of the positions to reach and equals the oracle's solve of
+    # the cut game with the same seeds.
+    from gpuequiv_amd._lib import Error
+    n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q][:50]
+    b = GameBuild(lts, pairs, max_positions=2000)
+    assert b.truncated()
+    front = b.frontier()
+    assert 0 < front < b.n_positions()
+    with pytest.raises(Error) as ei:
+        b.solve()
+    assert ei.value.code == 4
+    h = b._download()
+    dead = [v for v in range(b.n_positions()) if not h["atk"][v] and h["row"][v] == h["row"][v + 1]]
+    reach = [v for v in dead if v < front]
+    assert len(reach) < len(dead)          # the frontier holds "dead ends" that are only unexpanded
+    offs, data, st = orc.solve(6, 3, h["row"], h["col"], h["wt"], h["atk"], reach)
+    s = b.solve(capped=True)
+    o, d = b.energies_raw(0, b.n_positions())
+    assert H.canon_lists(o, d) == H.canon_lists(offs, data)
+    assert s["rounds"] == st["rounds"] and s["edge_updates"] == st["edge_updates"]
+    b.close()
+    # a complete build: the capped solve is the ordinary one
+    b = GameBuild(lts, pairs[:3])
+    assert not b.truncated() and b.frontier() == b.n_positions()
+    og = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs[:3], positions_wanted=False)
+    reach = [v for v in range(og["V"]) if not og["atk"][v] and og["row"][v] == og["row"][v + 1]]
+    offs, data, _ = orc.solve(6, 3, og["row"], og["col"], og["wt"], og["atk"], reach)
+    b.solve(capped=True)
+    o, d = b.energies_raw(0, og["V"])
+    assert H.canon_lists(o, d) == H.canon_lists(offs, data)
+    b.close()
+
+
 def test_full_spectroscopy_verdict_matrix():
     """300 states, 1,200 transitions, 8 labels: the 300 x 300 x 13 preorder
     matrix of equivalences() equals the oracle's."""
