@@ -297,14 +297,7 @@ def bench_gamebuild(args) -> dict:
 
     n, src, dst, lbl = workloads.powerlaw_lts(args.states, 5 * args.states, 60, args.seed)
     lts = TransitionSystem.from_arrays(n, src, dst, lbl)
-    rng = np.random.default_rng(args.seed)
-    pairs = []
-    seen = set()
-    while len(pairs) < args.pairs:
-        p, q = (int(x) for x in rng.integers(0, n, 2))
-        if p != q and (p, q) not in seen:
-            seen.add((p, q))
-            pairs.append((p, q))
+    pairs = workloads.sample_pairs(n, args.pairs, args.seed)
     for _ in range(args.warmup):
         GameBuild(lts, pairs, max_positions=args.max_positions).close()
     builds, solves = [], []

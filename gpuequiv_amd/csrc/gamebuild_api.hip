@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -66,19 +67,37 @@ geq_status exclusive_scan(const T* in, T* out, size_t n, DevBuf<uint8_t>& tmp, h
     return GEQ_OK;
 }
 
+// Sum of (candidates, set-pool words) bound pairs, saturating at `lim`
+// (saturating addition of non-negative values is associative, so the scan
+// stays exact below lim).  A level whose running total reaches lim is refused
+// before any candidate is written: its offsets no longer fit 32 bits, and a
+// wrapped total would size the buffers below the offsets k_generate writes at.
 struct PairSum {
-    __host__ __device__ uint2 operator()(const uint2& a, const uint2& b) const { return make_uint2(a.x + b.x, a.y + b.y); }
+    uint32_t lim;
+    __host__ __device__ uint2 operator()(const uint2& a, const uint2& b) const {
+        const uint64_t x = (uint64_t)a.x + b.x, y = (uint64_t)a.y + b.y;
+        return make_uint2(x < lim ? (uint32_t)x : lim, y < lim ? (uint32_t)y : lim);
+    }
 };
+
+// The saturation point of the bound scans: 2^32 - 1, or GEQ_BUILD_LEVEL_LIMIT
+// (lower values exercise the refusal in tests).
+uint32_t level_limit() {
+    if (const char* env = std::getenv("GEQ_BUILD_LEVEL_LIMIT")) {
+        const long long v = std::atoll(env);
+        if (v >= 1 && v < 0xffffffffll) return (uint32_t)v;
+    }
+    return 0xffffffffu;
+}
 
 // Exclusive scan of (candidates, set-pool words) pairs: one scan for both.
 geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<uint8_t>& tmp, hipStream_t s) {
     size_t bytes = 0;
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, in, out, PairSum{}, make_uint2(0u, 0u), n, s),
-            GEQ_ERR_INTERNAL);
+    const PairSum op{level_limit()};
+    GEQ_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, in, out, op, make_uint2(0u, 0u), n, s), GEQ_ERR_INTERNAL);
     geq_status st = ensure(tmp, bytes + 16, s, false, "scan scratch");
     if (st != GEQ_OK) return st;
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveScan(tmp.p, bytes, in, out, PairSum{}, make_uint2(0u, 0u), n, s),
-            GEQ_ERR_INTERNAL);
+    GEQ_HIP(hipcub::DeviceScan::ExclusiveScan(tmp.p, bytes, in, out, op, make_uint2(0u, 0u), n, s), GEQ_ERR_INTERNAL);
     return GEQ_OK;
 }
 
@@ -486,6 +505,10 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     }
     uint64_t E = 0, levels = 0;
     uint32_t lo = 0, hi = n_pairs;   // [lo, hi): the rest of the current level's frontier
+    uint32_t level_lo = 0;           // first position of the current level
+    // GEQ_BUILD_TRACE=1: one stderr line per level (frontier, new positions, totals, time)
+    const char* trace_env = std::getenv("GEQ_BUILD_TRACE");
+    const bool trace = trace_env && std::atoi(trace_env) != 0;
     bool truncated = false;
     // Merged builds take a level in chunks of frontier positions and check the
     // per-job caps after each chunk, so a job that passes its cap stops within
@@ -533,6 +556,9 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             n_heavy = h_rb[RB_HEAVY];
         }
         have_next = false;
+        if (n_cand >= level_limit() || n_words >= level_limit())
+            return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: a level of " + std::to_string(nf) +
+                                                        " positions needs 2^32 or more candidates or set-pool words"));
         if ((uint64_t)B->spool_top + n_words >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: set pool exceeds 2^32 words"));
         GB_TRY(ensure(B->spool, B->spool_top + n_words + 1, s, true, "set pool"));
         GB_TRY(ensure(cands, std::max<uint32_t>(n_cand, 1), s, false, "candidates"));
@@ -601,6 +627,12 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         lo = c_hi;
         const bool level_done = lo == hi;
         if (level_done) {        // the next level is every position this one created
+            if (trace)
+                std::fprintf(stderr, "gamebuild: level %llu: %u frontier -> %u new positions, %llu positions, %llu edges, %.1f ms\n",
+                             (unsigned long long)levels, hi - level_lo, B->V - hi, (unsigned long long)B->V,
+                             (unsigned long long)E,
+                             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            level_lo = hi;
             hi = B->V;
             levels++;
         }

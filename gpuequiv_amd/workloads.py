@@ -108,6 +108,18 @@ def powerlaw_lts(n_states: int, n_transitions: int, n_labels: int, seed: int, al
     return n_states, src, dst, lbl
 
 
+def sample_pairs(n_states: int, k: int, seed: int) -> list:
+    """k distinct ordered pairs (p, q), p != q, drawn uniformly (seeded)."""
+    rng = np.random.default_rng(seed)
+    pairs, seen = [], set()
+    while len(pairs) < k:
+        p, q = (int(x) for x in rng.integers(0, n_states, 2))
+        if p != q and (p, q) not in seen:
+            seen.add((p, q))
+            pairs.append((p, q))
+    return pairs
+
+
 def serve_job(rng: np.random.Generator, n_min: int = 100, n_max: int = 20000, per_state: int = 4,
               labels_min: int = 4, labels_max: int = 64, p_tau: float = 0.05, pairs_max: int = 256):
     """One spectroscopy job of the serving workload: states log-uniform in

@@ -288,3 +288,25 @@ def test_full_spectroscopy_verdict_matrix():
     assert np.array_equal(got, want)
     assert np.array_equal(eq.preorder_matrix(eqs), want)
     assert want.sum() > n * 13          # verdicts beyond the diagonal
+
+
+def test_level_past_the_scan_limit_is_refused(monkeypatch):
+    # A level whose candidate or set-pool-word bounds add up past 32 bits is
+    # refused with OUT_OF_MEMORY before any candidate is written (the bound
+    # scan saturates instead of wrapping).  GEQ_BUILD_LEVEL_LIMIT lowers that
+    # limit so a small game reaches it; the context builds normally afterwards.
+    from gpuequiv_amd._lib import Error
+    print("level past the bound-scan limit", flush=True)
+    n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q][:50]
+    for limit in ("40", "600"):          # refused at the first level / at a later one
+        monkeypatch.setenv("GEQ_BUILD_LEVEL_LIMIT", limit)
+        with pytest.raises(Error) as ei:
+            GameBuild(lts, pairs)
+        assert ei.value.code == 5 and "2^32" in str(ei.value)
+    monkeypatch.delenv("GEQ_BUILD_LEVEL_LIMIT")
+    b = GameBuild(lts, pairs[:5])
+    og = _assert_same_game(b, lts, pairs[:5])
+    _assert_same_energies(b, og)
+    b.close()
