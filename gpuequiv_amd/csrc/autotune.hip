@@ -30,7 +30,7 @@ geq_tune_params params_of(const geq_game* g) {
     p.eval_blocks_per_cu = g->tune.eval_blocks_per_cu;
     p.expand_blocks_per_cu = g->tune.expand_blocks_per_cu;
     p.rounds_per_launch = g->tune.rounds_per_launch;
-    p.arena_entries = g->arena_cap;
+    p.arena_entries = g->conf.generic ? g->garena_cap : g->arena_cap;   // u16 entries, or words of a generic conf
     return p;
 }
 
@@ -133,7 +133,7 @@ geq_status autotune(geq_game* g, const std::vector<uint32_t>* reach, uint32_t re
     apply(g, best);
     // antichain capacity: the arena holds the largest solve seen without regrowing
     const uint64_t want = peak + peak / 4 + 1024;
-    if (want > g->arena_cap) {
+    if (want > (g->conf.generic ? g->garena_cap : g->arena_cap)) {
         s = grow_arena(g, (uint32_t)std::min<uint64_t>(want, 0xffffffffull));
         if (s != GEQ_OK) return s;
     }
@@ -171,7 +171,7 @@ geq_status geq_game_set_tune(geq_game* g, const geq_tune_params* p) {
     t.expand_blocks_per_cu = p->expand_blocks_per_cu;
     t.rounds_per_launch = p->rounds_per_launch;
     apply(g, t);
-    if (p->arena_entries > g->arena_cap) {
+    if (p->arena_entries > (g->conf.generic ? g->garena_cap : g->arena_cap)) {
         geq_status s = grow_arena(g, (uint32_t)std::min<uint64_t>(p->arena_entries, 0xffffffffull));
         if (s != GEQ_OK) return s;
     }

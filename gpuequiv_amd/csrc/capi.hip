@@ -62,16 +62,71 @@ static uint32_t bitlen(uint32_t x) { return x == 0 ? 0 : 32u - (uint32_t)__built
 
 // EnergyConf arithmetic of src/energy.rs:48-78
 geq_status check_conf(const geq_conf& c, ConfInfo& o) {
-    if (c.elements == 0 || c.elements > 6 || c.max > 3)
-        return fail(GEQ_ERR_UNSUPPORTED, "energy configuration outside the kernels (need 1 <= elements <= 6, max <= 3): elements=" +
-                                             std::to_string(c.elements) + " max=" + std::to_string(c.max));
+    if (c.elements == 0)
+        return fail(GEQ_ERR_UNSUPPORTED, "energy configuration without elements");
+    o = ConfInfo{};
     o.elements = c.elements;
     o.max = c.max;
     o.ebits = next_pow2(bitlen(c.max));
     o.emask = 0xffffffffu >> (32 - o.ebits);
     o.ubits = next_pow2(bitlen(c.elements + 2));
     o.umask = 0xffffffffu >> (32 - o.ubits);
+    const uint64_t esz = ((uint64_t)o.ebits * c.elements + 31) / 32, usz = ((uint64_t)o.ubits * c.elements + 31) / 32;
+    if (esz > GEN_MAX_WORDS)
+        return fail(GEQ_ERR_UNSUPPORTED, "energy configuration needs " + std::to_string(esz) + " words per energy (at most " +
+                                             std::to_string(GEN_MAX_WORDS) + "): elements=" + std::to_string(c.elements) +
+                                             " max=" + std::to_string(c.max));
+    o.esz = (uint32_t)esz;
+    o.usz = (uint32_t)usz;
+    o.generic = !(c.elements <= 6 && c.max <= 3);
     return GEQ_OK;
+}
+
+GConf gconf_of(const ConfInfo& c) {
+    GConf g{};
+    g.elements = c.elements;
+    g.maxv = c.max;
+    g.ebits = c.ebits;
+    g.emask = c.emask;
+    g.ubits = c.ubits;
+    g.umask = c.umask;
+    g.esz = c.esz;
+    g.usz = c.usz;
+    if (c.ebits < 32) {
+        uint32_t E = 0, low = 0;
+        for (uint32_t f = 0; f < 32; f += 2 * c.ebits) {   // the even fields of a word
+            E |= c.emask << f;
+            low |= 1u << f;
+        }
+        g.E = E;
+        g.G = low << c.ebits;                              // lowest bit of every odd field
+    }
+    return g;
+}
+
+// Field k of a packed value of `bits`-wide fields (fields never straddle words).
+static uint32_t field(const uint32_t* w, uint32_t k, uint32_t bits, uint32_t mask) {
+    const uint64_t b = (uint64_t)k * bits;
+    return (w[b >> 5] >> (b & 31)) & mask;
+}
+// The bits of the last word above the last field are zero.
+static bool tail_clear(const uint32_t* w, uint32_t n_fields, uint32_t bits, uint32_t words) {
+    const uint64_t used = (uint64_t)n_fields * bits - 32ull * (words - 1);   // bits used in the last word
+    return used >= 32 || (w[words - 1] >> used) == 0;
+}
+
+bool energy_valid(const ConfInfo& c, const uint32_t* w) {
+    if (!tail_clear(w, c.elements, c.ebits, c.esz)) return false;
+    for (uint32_t k = 0; k < c.elements; k++)
+        if (field(w, k, c.ebits, c.emask) > c.max) return false;
+    return true;
+}
+
+bool update_valid(const ConfInfo& c, const uint32_t* w) {
+    if (!tail_clear(w, c.elements, c.ubits, c.usz)) return false;
+    for (uint32_t k = 0; k < c.elements; k++)
+        if (field(w, k, c.ubits, c.umask) > c.elements + 1) return false;   // Min(k) needs k <= elements (src/energy.rs:551-556)
+    return true;
 }
 
 bool energy_to_canon(const ConfInfo& c, uint32_t packed, uint32_t& canon) {
@@ -196,9 +251,10 @@ void geq_energies_free(geq_energies* e) {
 namespace {
 
 // Validate one graph and append it to the merged host arrays at node offset vb / edge offset eb.
+// gwt (generic confs only): the update words of every edge, usz per edge.
 geq_status append_graph(const geq_graph* g, const ConfInfo& ci, uint32_t vb, uint64_t eb, std::vector<uint32_t>& row,
                         std::vector<uint2>& edges, std::vector<uint32_t>& rrow, std::vector<uint32_t>& rcol,
-                        std::vector<uint8_t>& atk) {
+                        std::vector<uint8_t>& atk, std::vector<uint32_t>* gwt) {
     const uint32_t V = g->n_vertices, E = g->n_edges;
     if (V == 0) return fail(GEQ_ERR_INVALID_ARG, "empty game graph");
     if (!g->row_offsets || !g->attacker_pos || (E && (!g->column_indices || !g->weights)))
@@ -211,7 +267,14 @@ geq_status append_graph(const geq_graph* g, const ConfInfo& ci, uint32_t vb, uin
         uint32_t c = g->column_indices[k];
         if (c >= V) return fail(GEQ_ERR_INVALID_ARG, "column index out of range at edge " + std::to_string(k));
         uint32_t w;
-        if (!update_to_canon(ci, g->weights[k], w)) return fail(GEQ_ERR_INVALID_ARG, "invalid update at edge " + std::to_string(k));
+        if (ci.generic) {
+            const uint32_t* uw = g->weights + (size_t)k * ci.usz;
+            if (!update_valid(ci, uw)) return fail(GEQ_ERR_INVALID_ARG, "invalid update at edge " + std::to_string(k));
+            std::copy(uw, uw + ci.usz, gwt->begin() + (eb + k) * ci.usz);
+            w = uw[0];
+        } else if (!update_to_canon(ci, g->weights[k], w)) {
+            return fail(GEQ_ERR_INVALID_ARG, "invalid update at edge " + std::to_string(k));
+        }
         edges[eb + k] = make_uint2(vb + c, w);
     }
     for (uint32_t v = 0; v < V; v++) atk[vb + v] = g->attacker_pos[v] ? 1 : 0;
@@ -259,9 +322,10 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     std::vector<uint32_t> row(V + 1, 0), rrow(V + 1, 0), rcol(E);
     std::vector<uint2> edges(E);
     std::vector<uint8_t> atkv(V);
+    std::vector<uint32_t> gwt(ci.generic ? (size_t)E * ci.usz : 0);
     uint64_t eb = 0;
     for (uint32_t i = 0; i < n_games; i++) {
-        st = append_graph(&graphs[i], ci, base[i], eb, row, edges, rrow, rcol, atkv);
+        st = append_graph(&graphs[i], ci, base[i], eb, row, edges, rrow, rcol, atkv, &gwt);
         if (st != GEQ_OK) {
             if (n_games > 1) set_error("game " + std::to_string(i) + ": " + geq_last_error());
             return st;
@@ -296,11 +360,27 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
     if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
     if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
-    gm->arena_cap = initial_arena_entries(E, 0);
-    if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
+    if (ci.generic) {
+        // the word arena: the zero energy, then room for about one list per two edges
+        gm->garena_cap = (uint32_t)std::min<uint64_t>(
+            GEN_ZERO_WORDS + std::max<uint64_t>(64, (uint64_t)initial_arena_entries(E, 0) / 2) * ci.esz, 0x7fffffffull);
+        if ((e = gm->garena.alloc(gm->garena_cap)) != hipSuccess) return oom(e, "energy arena");
+        if ((e = gm->gwt.alloc(std::max<size_t>((size_t)E * ci.usz, 1))) != hipSuccess) return oom(e, "updates");
+        gm->arena_cap = 0;
+    } else {
+        gm->arena_cap = initial_arena_entries(E, 0);
+        if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
+    }
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
         return oom(e, "pinned control block");
     hipStream_t s = ctx->stream;
+    if (ci.generic) {
+        if ((E && hipMemcpyAsync(gm->gwt.p, gwt.data(), 4ull * E * ci.usz, hipMemcpyHostToDevice, s) != hipSuccess) ||
+            hipMemsetAsync(gm->garena.p, 0, 4ull * GEN_ZERO_WORDS, s) != hipSuccess) {
+            delete gm;
+            return fail(GEQ_ERR_TRANSFER, "uploading the game graph failed");
+        }
+    }
     auto up = [&](void* dst, const void* src, size_t bytes) { return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s); };
     if (up(gm->row.p, row.data(), 4ull * (V + 1)) != hipSuccess || up(gm->rrow.p, rrow.data(), 4ull * (V + 1)) != hipSuccess ||
         (E && up(gm->rcol.p, rcol.data(), 4ull * E) != hipSuccess) || (E && up(gm->edges.p, edges.data(), 8ull * E) != hipSuccess) ||
@@ -380,6 +460,25 @@ uint32_t geq::initial_arena_entries(uint64_t E, uint64_t want) {
 }
 
 geq_status geq::grow_arena(geq_game* g, uint32_t needed_top) {
+    if (g->conf.generic) {   // the word arena of the list kernels
+        uint64_t cap = std::max<uint64_t>(2ull * g->garena_cap, (uint64_t)needed_top + (1u << 18));
+        cap = std::min<uint64_t>(cap, 0xffffffffull);
+        if (cap <= g->garena_cap) return fail(GEQ_ERR_OUT_OF_MEMORY, "energy arena exceeds 2^32 words");
+        uint32_t* p = nullptr;
+        hipError_t e = device_malloc(reinterpret_cast<void**>(&p), 4ull * cap);
+        if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("growing the energy arena: ") + hipGetErrorString(e));
+        e = hipMemcpyAsync(p, g->garena.p, 4ull * g->garena_cap, hipMemcpyDeviceToDevice, g->ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return fail(GEQ_ERR_TRANSFER, std::string("copying the energy arena: ") + hipGetErrorString(e));
+        }
+        (void)hipFree(g->garena.p);
+        g->garena.p = p;
+        g->garena.n = cap;
+        g->garena_cap = (uint32_t)cap;
+        return GEQ_OK;
+    }
     uint64_t cap = std::max<uint64_t>(2ull * g->arena_cap, (uint64_t)needed_top + (1u << 20));
     cap = std::min<uint64_t>(cap, 0xffffffffull);
     if (cap <= g->arena_cap) return fail(GEQ_ERR_OUT_OF_MEMORY, "energy arena exceeds 2^32 entries");
@@ -407,6 +506,18 @@ namespace geq {
 RoundShape round_shape(const geq_game* g) {
     const uint32_t ncu = (uint32_t)std::max(1, g->ctx->n_cu);
     RoundShape r;
+    if (g->conf.generic) {
+        // list kernels: GEN_BLOCK threads, one node per wave at a time, and a separate k_expand
+        r.threads = GEN_BLOCK;
+        r.fused = false;
+        const uint32_t per_cu = std::min(g->tune.eval_blocks_per_cu, list_blocks_resident(g->conf.esz));
+        r.eval_blocks = std::max(1u, std::min(ncu * per_cu, (g->V + GEN_BLOCK / 64 - 1) / (GEN_BLOCK / 64)));
+        r.eval_blocks = std::min(r.eval_blocks, STAT_SLOTS);
+        const uint32_t xpc = g->tune.expand_blocks_per_cu ? g->tune.expand_blocks_per_cu : 4u;
+        r.expand_blocks = std::max(1u, std::min(ncu * xpc, (g->V + GEN_BLOCK - 1) / GEN_BLOCK));
+        r.rounds_per_launch = std::max(1u, g->tune.rounds_per_launch);
+        return r;
+    }
     const uint32_t w = std::max(1u, std::min<uint32_t>(g->tune.waves_per_block, MAX_WAVES));
     r.threads = 64u * w;
     r.fused = g->tune.expand_blocks_per_cu == 0;
@@ -439,6 +550,67 @@ uint32_t eval_blocks_resident(bool fused, uint32_t w) {
     return (uint32_t)v;
 }
 
+// The k_eval_list instantiation for energies of esz words.
+const void* list_kernel(uint32_t esz) {
+    return esz == 1 ? reinterpret_cast<const void*>(&k_eval_list<1>)
+                    : esz <= 4 ? reinterpret_cast<const void*>(&k_eval_list<4>) : reinterpret_cast<const void*>(&k_eval_list<32>);
+}
+
+uint32_t list_blocks_resident(uint32_t esz) {
+    static std::atomic<int> cache[3];
+    std::atomic<int>& c = cache[esz == 1 ? 0 : esz <= 4 ? 1 : 2];
+    int v = c.load(std::memory_order_relaxed);
+    if (v == 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, list_kernel(esz), (int)GEN_BLOCK, 0) != hipSuccess || per_cu <= 0)
+            per_cu = 8;
+        v = per_cu;
+        c.store(v, std::memory_order_relaxed);
+    }
+    return (uint32_t)v;
+}
+
+void launch_list_eval(const RoundShape& shape, const GenArgs& ga, uint32_t k, hipStream_t s) {
+    const uint32_t esz = ga.c.esz;
+    if (esz == 1) k_eval_list<1><<<shape.eval_blocks, GEN_BLOCK, 0, s>>>(ga, k);
+    else if (esz <= 4) k_eval_list<4><<<shape.eval_blocks, GEN_BLOCK, 0, s>>>(ga, k);
+    else k_eval_list<32><<<shape.eval_blocks, GEN_BLOCK, 0, s>>>(ga, k);
+}
+
+GenArgs gen_args(geq_game* g) {
+    GenArgs a{};
+    a.wl[0] = g->wl0.p;
+    a.wl[1] = g->wl1.p;
+    a.edges = g->edges.p;
+    a.wt = g->gwt.p;
+    a.st = g->st.p;
+    a.arena = g->garena.p;
+    a.ctl = g->ctl.p;
+    a.arena_cap = g->garena_cap;
+    a.chg = g->chg.p;
+    a.stats = g->stats.p;
+    a.queued = g->queued.p;
+    a.qwords = queue_words(g->V);
+    a.scr = g->gscr.p;
+    a.scr_words = g->scr_words;
+    a.c = gconf_of(g->conf);
+    return a;
+}
+
+// The list kernels' scratch: scr_words words for every wave of the grid.
+geq_status ensure_scratch(geq_game* g, const RoundShape& shape, uint64_t words) {
+    words = std::max<uint64_t>(std::max<uint64_t>(words, 4096), g->scr_words);
+    if (words > 0x7fffffffull) return fail(GEQ_ERR_OUT_OF_MEMORY, "a node's candidate lists need more than 2^31 words");
+    const uint64_t total = words * shape.eval_blocks * (GEN_BLOCK / 64);
+    if (g->gscr.p && g->gscr.n >= total && g->scr_words >= words) return GEQ_OK;
+    g->gscr.release();
+    hipError_t e = g->gscr.alloc(total);
+    if (e != hipSuccess)
+        return fail(GEQ_ERR_OUT_OF_MEMORY, "allocating " + std::to_string(4 * total) + " bytes of list scratch: " + hipGetErrorString(e));
+    g->scr_words = (uint32_t)words;
+    return GEQ_OK;
+}
+
 EvalArgs eval_args(geq_game* g) {
     return EvalArgs{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->st.p, g->arena.p, g->ctl.p, g->arena_cap,
                     g->chg.p, g->conf.max, g->stats.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V)};
@@ -454,8 +626,12 @@ ExpandArgs expand_args(geq_game* g) {
 // One batch of rounds_per_launch rounds and the k_advance after them, on s:
 // what a round graph holds.
 void launch_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape, hipStream_t s) {
+    const GenArgs ga = g->conf.generic ? gen_args(g) : GenArgs{};
     for (uint32_t k = 0; k < shape.rounds_per_launch; k++) {
-        if (shape.fused) {
+        if (g->conf.generic) {
+            launch_list_eval(shape, ga, k, s);
+            k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa, k);
+        } else if (shape.fused) {
             k_eval<true><<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_FUSED_LDS_PER_WAVE, s>>>(ea, k);
         } else {
             k_eval<false><<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea, k);
@@ -479,7 +655,9 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
     hipError_t e = hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(GEQ_ERR_NO_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    g->graph_arena = g->arena.p;
+    g->graph_arena = g->conf.generic ? (void*)g->garena.p : (void*)g->arena.p;
+    g->graph_scr = g->gscr.p;
+    g->graph_scr_words = g->scr_words;
     return GEQ_OK;
 }
 
@@ -491,10 +669,17 @@ geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa,
 // clears the bits of its own list again as it evaluates it).
 geq_status recover_overflow(geq_game* g) {
     hipStream_t s = g->ctx->stream;
-    geq_status st = grow_arena(g, g->h_ctl->arena_top);
+    geq_status st = GEQ_OK;
+    if (!g->conf.generic || g->h_ctl->arena_top > g->garena_cap) st = grow_arena(g, g->h_ctl->arena_top);
     if (st != GEQ_OK) return st;
+    if (g->conf.generic && g->h_ctl->scr_need > g->scr_words) {
+        // a node's candidates did not fit its wave's scratch
+        st = ensure_scratch(g, round_shape(g), std::max<uint64_t>(g->h_ctl->scr_need, 2ull * g->scr_words));
+        if (st != GEQ_OK) return st;
+    }
     Ctl fix = *g->h_ctl;
     fix.overflow = 0;
+    fix.scr_need = 0;
     fix.wl_n[(fix.round + 1u) % 3u] = 0;
     GEQ_HIP(hipMemsetAsync(g->queued.p, 0, 8ull * queue_words(g->V), s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemcpyAsync(g->ctl.p, &fix, sizeof(Ctl), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
@@ -602,7 +787,37 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
 
     // ---- initial state: EnergyGame::with_reach (src/energygame.rs:355-365) or explicit lists
     std::vector<uint16_t> h_arena;
-    if (initial) {
+    std::vector<uint32_t> h_words;   // generic confs: the initial lists' words
+    if (initial && g->conf.generic) {
+        const uint32_t esz = g->conf.esz;
+        if (initial->n_nodes != V || !initial->offsets || (initial->offsets[V] && !initial->data))
+            return fail(GEQ_ERR_INVALID_ARG, "initial energies must cover every node");
+        if (initial->energy_size != esz)
+            return fail(GEQ_ERR_INVALID_ARG, "initial energies: energy_size must be " + std::to_string(esz) + " for this conf");
+        std::vector<uint4> h_st(2ull * V, make_uint4(0, 0, 0, 0));   // slot 0 current (tags 0), slot 1 empty
+        for (uint32_t v = 0; v < V; v++) {
+            const uint32_t b = initial->offsets[v], e = initial->offsets[v + 1];
+            if (e < b || e - b > GLEN_MASK) return fail(GEQ_ERR_INVALID_ARG, "initial energies: bad offsets");
+            const uint64_t off = GEN_ZERO_WORDS + h_words.size();
+            if (off + (uint64_t)(e - b) * esz > 0x7fffffffull) return fail(GEQ_ERR_OUT_OF_MEMORY, "initial energies too large");
+            for (uint32_t j = b; j < e; j++) {
+                const uint32_t* w = initial->data + (size_t)j * esz;
+                if (!energy_valid(g->conf, w)) return fail(GEQ_ERR_INVALID_ARG, "initial energies: invalid energy");
+                h_words.insert(h_words.end(), w, w + esz);
+            }
+            h_st[2ull * v] = make_uint4(e - b, (uint32_t)off, 0u, 0u);
+        }
+        if (GEN_ZERO_WORDS + h_words.size() + (1u << 16) > g->garena_cap) {
+            geq_status st = grow_arena(g, (uint32_t)(GEN_ZERO_WORDS + h_words.size()));
+            if (st != GEQ_OK) return st;
+        }
+        if (V)
+            GEQ_HIP(hipMemcpy2DAsync(g->st.p + 1, 16ull * NODE_U4, h_st.data(), 32, 32, V, hipMemcpyHostToDevice, s),
+                    GEQ_ERR_TRANSFER);
+        if (!h_words.empty())
+            GEQ_HIP(hipMemcpyAsync(g->garena.p + GEN_ZERO_WORDS, h_words.data(), 4ull * h_words.size(), hipMemcpyHostToDevice, s),
+                    GEQ_ERR_TRANSFER);
+    } else if (initial) {
         if (initial->n_nodes != V || !initial->offsets || (initial->offsets[V] && !initial->data))
             return fail(GEQ_ERR_INVALID_ARG, "initial energies must cover every node");
         if (initial->energy_size != 1) return fail(GEQ_ERR_INVALID_ARG, "initial energies: energy_size must be 1 for this conf");
@@ -641,7 +856,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             GEQ_HIP(hipMemcpyAsync(g->arena.p, h_arena.data(), 2ull * h_arena.size(), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
     Ctl c0{};
-    c0.arena_top = (uint32_t)h_arena.size();
+    c0.arena_top = g->conf.generic ? (uint32_t)(GEN_ZERO_WORDS + h_words.size()) : (uint32_t)h_arena.size();
     g->retag_round = 0;
     {
         const uint32_t q2 = 2u * queue_words(V), ns = 4u * STAT_SLOTS + 4u;
@@ -660,6 +875,10 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     const RoundShape shape = round_shape(g);
     EvalArgs ea = eval_args(g);
     ExpandArgs xa = expand_args(g);
+    if (g->conf.generic) {
+        const geq_status st = ensure_scratch(g, shape, g->scr_words ? g->scr_words : 16384u);
+        if (st != GEQ_OK) return st;
+    }
 
     // seed: predecessors of the nodes to reach (initialize_visit_lists, :1461-1465)
     if (n_reach) {
@@ -685,7 +904,16 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             // kernels' own, without the gaps between dispatches
             hipEvent_t ev[4];
             for (hipEvent_t& e : ev) GEQ_HIP(hipEventCreate(&e), GEQ_ERR_NO_DEVICE);
-            if (shape.fused) {
+            if (g->conf.generic) {
+                const GenArgs ga = gen_args(g);
+                if (ga.c.esz == 1)
+                    hipExtLaunchKernelGGL(k_eval_list<1>, dim3(shape.eval_blocks), dim3(GEN_BLOCK), 0, s, ev[0], ev[1], 0, ga, 0u);
+                else if (ga.c.esz <= 4)
+                    hipExtLaunchKernelGGL(k_eval_list<4>, dim3(shape.eval_blocks), dim3(GEN_BLOCK), 0, s, ev[0], ev[1], 0, ga, 0u);
+                else
+                    hipExtLaunchKernelGGL(k_eval_list<32>, dim3(shape.eval_blocks), dim3(GEN_BLOCK), 0, s, ev[0], ev[1], 0, ga, 0u);
+                hipExtLaunchKernelGGL(k_expand, dim3(shape.expand_blocks), dim3(shape.threads), 0, s, ev[2], ev[3], 0, xa, 0u);
+            } else if (shape.fused) {
                 hipExtLaunchKernelGGL(k_eval<true>, dim3(shape.eval_blocks), dim3(shape.threads),
                                       (shape.threads / 64) * EVAL_FUSED_LDS_PER_WAVE, s, ev[0], ev[1], 0, ea, 0u);
                 GEQ_HIP(hipEventRecord(ev[2], s), GEQ_ERR_NO_DEVICE);
@@ -714,7 +942,10 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
                 launch_rounds(g, ea, xa, shape, s);
                 GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
             } else {
-                if (!g->graph_exec || g->graph_arena != g->arena.p) {
+                const bool stale = g->conf.generic ? (g->graph_arena != g->garena.p || g->graph_scr != g->gscr.p ||
+                                                      g->graph_scr_words != g->scr_words)
+                                                   : g->graph_arena != g->arena.p;
+                if (!g->graph_exec || stale) {
                     result = capture_rounds(g, ea, xa, shape);
                     if (result != GEQ_OK) break;
                 }
@@ -787,13 +1018,75 @@ extern "C" geq_status geq_game_energies(geq_game* g, geq_energies* out) {
     return geq::download_energies(g, 0, g->V, out);
 }
 
+namespace {
+// download_energies for a generic conf: every node's list in the crate's
+// packing, esz words per energy, energies in ascending lexicographic order
+// of their words (word 0 first).
+geq_status download_lists(geq_game* g, uint32_t first, uint32_t V, uint32_t t, geq_energies* out) {
+    hipStream_t s = g->ctx->stream;
+    const uint32_t esz = g->conf.esz;
+    if (V) k_list_lens_g<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, V, t, g->lens.p);
+    std::vector<uint32_t> lens(V), offs(V + 1);
+    if (V) GEQ_HIP(hipMemcpyAsync(lens.data(), g->lens.p, 4ull * V, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    uint64_t tot = 0;
+    for (uint32_t v = 0; v < V; v++) {
+        offs[v] = (uint32_t)tot;
+        tot += lens[v];
+    }
+    if (tot > 0xffffffffull || tot * esz > 0xffffffffull) return fail(GEQ_ERR_OUT_OF_MEMORY, "result exceeds 2^32 words");
+    offs[V] = (uint32_t)tot;
+    if (!g->offs.p || g->offs.n < V + 1ull) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
+    if (!g->out.p || g->out.n < tot * esz) GEQ_HIP(g->out.alloc(std::max<uint64_t>(tot * esz, 1)), GEQ_ERR_OUT_OF_MEMORY);
+    GEQ_HIP(hipMemcpyAsync(g->offs.p, offs.data(), 4ull * (V + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    if (V) k_gather_g<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, g->garena.p, V, t, esz, g->offs.p, g->out.p);
+    uint32_t* data = (uint32_t*)std::malloc(4ull * std::max<uint64_t>(tot * esz, 1));
+    uint32_t* ooffs = (uint32_t*)std::malloc(4ull * (V + 1));
+    if (!data || !ooffs) {
+        std::free(data);
+        std::free(ooffs);
+        return fail(GEQ_ERR_OUT_OF_MEMORY, "host allocation of the result");
+    }
+    hipError_t e = tot ? hipMemcpyAsync(data, g->out.p, 4ull * tot * esz, hipMemcpyDeviceToHost, s) : hipSuccess;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        std::free(data);
+        std::free(ooffs);
+        return fail(GEQ_ERR_TRANSFER, std::string("downloading energies: ") + hipGetErrorString(e));
+    }
+    std::memcpy(ooffs, offs.data(), 4ull * (V + 1));
+    // ascending order per node (rows compared word by word)
+    std::vector<uint32_t> idx, tmp;
+    for (uint32_t v = 0; v < V; v++) {
+        const uint32_t n = lens[v];
+        if (n < 2) continue;
+        uint32_t* rows = data + (size_t)offs[v] * esz;
+        idx.resize(n);
+        for (uint32_t i = 0; i < n; i++) idx[i] = i;
+        std::sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
+            return std::lexicographical_compare(rows + (size_t)x * esz, rows + (size_t)(x + 1) * esz, rows + (size_t)y * esz,
+                                                rows + (size_t)(y + 1) * esz);
+        });
+        tmp.assign(rows, rows + (size_t)n * esz);
+        for (uint32_t i = 0; i < n; i++) std::copy(tmp.begin() + (size_t)idx[i] * esz, tmp.begin() + (size_t)(idx[i] + 1) * esz,
+                                                  rows + (size_t)i * esz);
+    }
+    out->n_nodes = V;
+    out->energy_size = esz;
+    out->offsets = ooffs;
+    out->data = data;
+    return GEQ_OK;
+}
+}  // namespace
+
 // Energies of the nodes [first, first + V) as packed words, ascending per node.
 geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_energies* out) {
     if (!g || !out || (uint64_t)first + V > g->V) return fail(GEQ_ERR_INVALID_ARG, "node range out of bounds");
     (void)hipSetDevice(g->ctx->device);
     hipStream_t s = g->ctx->stream;
-    if (!g->lens.p || g->lens.n < V) GEQ_HIP(g->lens.alloc(V), GEQ_ERR_OUT_OF_MEMORY);
+    if (!g->lens.p || g->lens.n < V) GEQ_HIP(g->lens.alloc(std::max<uint32_t>(V, 1)), GEQ_ERR_OUT_OF_MEMORY);
     const uint32_t t = round_tag(g->h_ctl->round);   // the lists after the last round that ran
+    if (g->conf.generic) return download_lists(g, first, V, t, out);
     k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, V, t, g->lens.p);
     std::vector<uint32_t> lens(V), offs(V + 1);
     GEQ_HIP(hipMemcpyAsync(lens.data(), g->lens.p, 4ull * V, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
@@ -875,20 +1168,21 @@ extern "C" geq_status geq_batch_energies(geq_game* b, geq_energies* outs) {
     geq_status st = geq_game_energies(b, &all);
     if (st != GEQ_OK) return st;
     const uint32_t n_games = (uint32_t)b->game_base.size() - 1;
+    const uint32_t esz = all.energy_size;
     for (uint32_t i = 0; i < n_games; i++) {
         const uint32_t v0 = b->game_base[i], v1 = b->game_base[i + 1], n = v1 - v0;
         const uint32_t e0 = all.offsets[v0], e1 = all.offsets[v1];
         outs[i].n_nodes = n;
-        outs[i].energy_size = 1;
+        outs[i].energy_size = esz;
         outs[i].offsets = (uint32_t*)std::malloc(4ull * (n + 1));
-        outs[i].data = (uint32_t*)std::malloc(4ull * std::max<uint32_t>(1, e1 - e0));
+        outs[i].data = (uint32_t*)std::malloc(4ull * std::max<uint64_t>(1, (uint64_t)(e1 - e0) * esz));
         if (!outs[i].offsets || !outs[i].data) {
             for (uint32_t j = 0; j <= i; j++) geq_energies_free(&outs[j]);
             geq_energies_free(&all);
             return fail(GEQ_ERR_OUT_OF_MEMORY, "host allocation of batch results");
         }
         for (uint32_t v = 0; v <= n; v++) outs[i].offsets[v] = all.offsets[v0 + v] - e0;
-        if (e1 > e0) std::memcpy(outs[i].data, all.data + e0, 4ull * (e1 - e0));
+        if (e1 > e0) std::memcpy(outs[i].data, all.data + (size_t)e0 * esz, 4ull * (e1 - e0) * esz);
     }
     geq_energies_free(&all);
     return GEQ_OK;

@@ -24,12 +24,23 @@ geq_status fail(geq_status s, const std::string& msg);
             return ::geq::fail((code), std::string(#expr) + ": " + hipGetErrorString(_e));   \
     } while (0)
 
-// Energy configuration accepted by the kernels: elements <= 6, max <= 3.
+// Energy configuration of a game.  elements <= 6 and max <= 3 run on the
+// bitmap kernels (2-bit canonical fields); every other conf whose energies
+// take at most GEN_MAX_WORDS words runs on the list kernels in the crate's
+// own packing (`generic`).
 struct ConfInfo {
     uint32_t elements, max;
     uint32_t ebits, emask, ubits, umask;
+    uint32_t esz = 1, usz = 1;   // words per energy / per update (EnergyConf::energy_size, update_size)
+    bool generic = false;
 };
 geq_status check_conf(const geq_conf& c, ConfInfo& out);
+// the list kernels' view of a generic conf
+GConf gconf_of(const ConfInfo& c);
+// a packed energy / update of esz / usz words is well formed for the conf
+// (fields within range, unused bits zero)
+bool energy_valid(const ConfInfo& c, const uint32_t* words);
+bool update_valid(const ConfInfo& c, const uint32_t* words);
 // conf packing <-> canonical (2-bit energies, 4-bit updates)
 bool energy_to_canon(const ConfInfo& c, uint32_t packed, uint32_t& canon);
 uint32_t energy_from_canon(const ConfInfo& c, uint32_t canon);
@@ -92,10 +103,20 @@ struct RoundShape {
 };
 RoundShape round_shape(const geq_game* g);
 uint32_t eval_blocks_resident(bool fused, uint32_t waves_per_block);
+// list kernels: the k_eval_list instantiation for esz-word energies, its
+// blocks per CU, one launch of it, and the per-wave scratch (words per wave)
+const void* list_kernel(uint32_t esz);
+uint32_t list_blocks_resident(uint32_t esz);
+void launch_list_eval(const RoundShape& shape, const GenArgs& ga, uint32_t k, hipStream_t s);
+geq_status ensure_scratch(geq_game* g, const RoundShape& shape, uint64_t words);
 EvalArgs eval_args(geq_game* g);
 ExpandArgs expand_args(geq_game* g);
 geq_status capture_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const RoundShape& shape);
+// Grow the energy arena (u16 entries, or words for a generic conf) past needed_top, keeping its content.
 geq_status grow_arena(geq_game* g, uint32_t needed_top);
+// words of the generic arena kept for the zero energy (its start)
+constexpr uint32_t GEN_ZERO_WORDS = GEN_MAX_WORDS;
+GenArgs gen_args(geq_game* g);
 // ctl->overflow was raised: grow the arena and make the round run again
 geq_status recover_overflow(geq_game* g);
 // between batches of rounds: k_retag when due (slot tags stay unambiguous)
@@ -144,6 +165,13 @@ struct geq_game {
     geq::DevBuf<uint32_t> rev_cnt, rev_fill;   // reverse-CSR scratch of game_from_device
     geq::DevBuf<uint8_t> rev_tmp;
     geq::DevBuf<uint16_t> arena;
+    // generic confs (list kernels): update words per edge, the word arena (its
+    // first GEN_ZERO_WORDS words hold the zero energy), the per-wave scratch
+    geq::DevBuf<uint32_t> gwt, garena, gscr;
+    uint32_t garena_cap = 0;          // words
+    uint32_t scr_words = 0;           // scratch words per wave
+    void* graph_scr = nullptr;        // scratch pointer and size baked into graph_exec
+    uint32_t graph_scr_words = 0;
     geq::DevBuf<geq::Ctl> ctl;
     geq::DevBuf<unsigned long long> stats;
     geq::Ctl* h_ctl = nullptr;        // pinned

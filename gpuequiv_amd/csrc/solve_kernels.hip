@@ -437,39 +437,6 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
     if (FUSED) flush_appends(stage, app, s_cnt, out, out_n);
 }
 
-// The round of a kernel at position k of a batch, its worklist size (0 after
-// an overflow) and the overflow flag, from one scalar load of the control
-// block's first 32 bytes (round and counts share them).
-struct RoundCtl {
-    uint32_t r, n;
-    bool ovf;
-};
-__device__ __forceinline__ RoundCtl read_round(const Ctl* ctl, uint32_t k) {
-    const uint4* p = reinterpret_cast<const uint4*>(ctl);
-    const uint4 w0 = p[0], w1 = p[1];   // w0: wl_n[0..2], overflow; w1: arena_top, round, last_nonempty, pad
-    const uint32_t r = uniform(w1.y) + k;
-    const uint32_t i = r % 3u;
-    const uint32_t n = uniform(i == 0u ? w0.x : i == 1u ? w0.y : w0.z);
-    const bool ovf = uniform(w0.w) != 0u;
-    return RoundCtl{r, ovf ? 0u : n, ovf};
-}
-
-// Per-block statistics slot (no cross-block atomics): slot b is only ever
-// written by block b of the fixed-size grid, rounds are stream-ordered.
-__device__ __forceinline__ void flush_stats(const StatAcc& acc, unsigned long long* slots) {
-    __shared__ unsigned long long s_red[4];
-    if (threadIdx.x < 4) s_red[threadIdx.x] = 0ull;
-    __syncthreads();
-    if (acc.bytes) atomicAdd(&s_red[2], (unsigned long long)acc.bytes);
-    if ((threadIdx.x & 63u) == 0) {
-        if (acc.evals) atomicAdd(&s_red[0], (unsigned long long)acc.evals);
-        if (acc.edges) atomicAdd(&s_red[1], (unsigned long long)acc.edges);
-        if (acc.changed) atomicAdd(&s_red[3], (unsigned long long)acc.changed);
-    }
-    __syncthreads();
-    if (threadIdx.x < 4 && s_red[threadIdx.x]) slots[4ull * blockIdx.x + threadIdx.x] += s_red[threadIdx.x];
-}
-
 // k: the kernel's round within the current batch of rounds (round = ctl->round + k).
 template <bool FUSED>
 __global__ __launch_bounds__(MAX_BLOCK) void k_eval(EvalArgs a, uint32_t k) {

@@ -29,7 +29,7 @@ struct Ctl {                        // small device control block, read back bet
     uint32_t arena_top;             // bump pointer of the energy arena (u16 units)
     uint32_t round;                 // current round; its worklist is wl[round & 1]
     uint32_t last_nonempty;         // 1 + the last round whose worklist was not empty
-    uint32_t pad;
+    uint32_t scr_need;              // list kernels: scratch words per wave a node needed (0: enough)
     unsigned long long totals[4];   // k_reduce_stats after a batch of rounds: node evaluations, edge
                                     // updates, algorithmic bytes, changed nodes (read back with the block)
 };
@@ -76,6 +76,75 @@ struct ExpandArgs {
 // cleared by round r's evaluation of v (so the bitmap of a parity is empty
 // again before the round two later starts appending to it).
 __host__ __device__ inline uint32_t queue_words(uint32_t V) { return (V + 31u) / 32u; }
+
+// Energy configurations beyond the bitmap layout (elements > 6 or max > 3):
+// the crate's packing, `ebits`-wide fields (a power of two), at most
+// GEN_MAX_WORDS words per energy.  E masks the even fields of a word, G holds
+// the lowest bit of every odd field (the guard bits of word_le / word_max).
+constexpr uint32_t GEN_MAX_WORDS = 32;
+constexpr uint32_t GEN_BLOCK = 256;           // k_eval_list: 4 waves, one node per wave at a time
+constexpr uint32_t GLEN_MASK = 0x7fffffffu;   // list length in a list kernel's slot (x)
+struct GConf {
+    uint32_t elements, maxv, ebits, emask, ubits, umask, esz, usz;
+    uint32_t E, G;
+};
+// List slot of the list kernels (16 B, two per node like the bitmap slots):
+// x = length, y = word offset into the arena, z = 0, w = round tag << 16.
+struct GenArgs {
+    uint32_t* wl[2];
+    const uint2* edges;             // (column index, first update word) per edge
+    const uint32_t* wt;             // usz update words per edge
+    uint4* st;                      // node records (header + two list slots)
+    uint32_t* arena;                // energies, esz words each; words [0, esz) hold the zero energy
+    Ctl* ctl;
+    uint32_t arena_cap;             // words
+    uint8_t* chg;                   // per worklist entry: node changed (read by k_expand)
+    unsigned long long* stats;
+    uint32_t* queued;
+    uint32_t qwords;
+    uint32_t* scr;                  // per wave scr_words words of candidate scratch
+    uint32_t scr_words;
+    GConf c;
+};
+template <int ESZ>
+__global__ void k_eval_list(GenArgs a, uint32_t k);
+__global__ void k_list_lens_g(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens);
+__global__ void k_gather_g(const uint4* st, const uint32_t* arena, uint32_t n, uint32_t t, uint32_t esz, const uint32_t* offs,
+                           uint32_t* out);
+
+// The round of a kernel at position k of a batch, its worklist size (0 after
+// an overflow) and the overflow flag, from one scalar load of the control
+// block's first 32 bytes (round and counts share them).
+struct RoundCtl {
+    uint32_t r, n;
+    bool ovf;
+};
+__device__ __forceinline__ RoundCtl read_round(const Ctl* ctl, uint32_t k) {
+    const uint4* p = reinterpret_cast<const uint4*>(ctl);
+    const uint4 w0 = p[0], w1 = p[1];   // w0: wl_n[0..2], overflow; w1: arena_top, round, last_nonempty, pad
+    const uint32_t r = uniform(w1.y) + k;
+    const uint32_t i = r % 3u;
+    const uint32_t n = uniform(i == 0u ? w0.x : i == 1u ? w0.y : w0.z);
+    const bool ovf = uniform(w0.w) != 0u;
+    return RoundCtl{r, ovf ? 0u : n, ovf};
+}
+
+// Per-block statistics slot (no cross-block atomics): slot b is only ever
+// written by block b of the fixed-size grid, rounds are stream-ordered.
+__device__ __forceinline__ void flush_stats(const StatAcc& acc, unsigned long long* slots) {
+    __shared__ unsigned long long s_red[4];
+    if (threadIdx.x < 4) s_red[threadIdx.x] = 0ull;
+    __syncthreads();
+    if (acc.bytes) atomicAdd(&s_red[2], (unsigned long long)acc.bytes);
+    if ((threadIdx.x & 63u) == 0) {
+        if (acc.evals) atomicAdd(&s_red[0], (unsigned long long)acc.evals);
+        if (acc.edges) atomicAdd(&s_red[1], (unsigned long long)acc.edges);
+        if (acc.changed) atomicAdd(&s_red[3], (unsigned long long)acc.changed);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && s_red[threadIdx.x]) slots[4ull * blockIdx.x + threadIdx.x] += s_red[threadIdx.x];
+}
+
 
 // FUSED: k_eval also appends the predecessors of its changed nodes (no k_expand).
 template <bool FUSED>

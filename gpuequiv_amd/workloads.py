@@ -60,6 +60,53 @@ def random_game(n_vertices: int, seed: int, p_attacker: float = 0.5, mean_degree
     return GameGraph(EnergyConf.STANDARD, row.astype(np.uint32), col, w.reshape(E, 1), atk)
 
 
+def pack_fields(values: np.ndarray, bits: int, words: int) -> np.ndarray:
+    """Rows of field values (n, fields) packed like the crate (src/energy.rs:143-162):
+    32 // bits fields per word, field k of a word at bits [k * bits, (k + 1) * bits)."""
+    n, f = values.shape
+    per = 32 // bits
+    out = np.zeros((n, words), dtype=np.uint64)
+    for k in range(f):
+        out[:, k // per] |= values[:, k].astype(np.uint64) << np.uint64((k % per) * bits)
+    return out.astype(np.uint32)
+
+
+def random_game_conf(n_vertices: int, seed: int, conf: EnergyConf, p_attacker: float = 0.5, mean_degree: float = 4.0,
+                     p_ident: float = 0.5, p_dec: float = 0.35, any_updates: bool = False) -> GameGraph:
+    """``random_game`` for any EnergyConf: the same degree and target
+    distributions; each update is the identity, a Decrement at one uniform
+    component, or Min(j) at component i with j uniform over the others (over
+    all components when there is only one), packed in the conf's update
+    fields.  any_updates: every field drawn from Zero, Decrement and
+    Min(1..elements)."""
+    rng = np.random.default_rng(seed)
+    V = int(n_vertices)
+    atk = (rng.random(V) < p_attacker).astype(np.uint8)
+    deg = rng.geometric(1.0 / (mean_degree + 1.0), V) - 1
+    row = np.zeros(V + 1, dtype=np.uint64)
+    np.cumsum(deg, out=row[1:])
+    E = int(row[-1])
+    col = rng.integers(0, V, E, dtype=np.uint32)
+    el = conf.elements
+    f = np.zeros((E, el), dtype=np.int64)               # packed field values: 0 Zero, 1 Decrement, k + 1 Min(k)
+    if any_updates:
+        f[:] = rng.integers(0, el + 2, (E, el))
+    else:
+        r = rng.random(E)
+        comp = rng.integers(0, el, E)
+        if el > 1:
+            other = rng.integers(0, el - 1, E)
+            other = other + (other >= comp)
+        else:
+            other = np.zeros(E, dtype=np.int64)
+        dec = (r >= p_ident) & (r < p_ident + p_dec)
+        mn = r >= p_ident + p_dec
+        f[np.nonzero(dec)[0], comp[dec]] = 1
+        f[np.nonzero(mn)[0], comp[mn]] = other[mn] + 2
+    w = pack_fields(f, conf.update_bits(), conf.update_size())
+    return GameGraph(conf, row.astype(np.uint32), col, w, atk)
+
+
 def game_batch(n_games: int, seed: int, v_min: int = 1000, v_max: int = 200000):
     """``n_games`` random games with log-uniform position counts in [v_min, v_max]."""
     rng = np.random.default_rng(seed)

@@ -18,9 +18,14 @@
  *    :143-162, :543-565).  Energies are returned sorted ascending per node as
  *    packed words (the crate's EnergyArray compares order-insensitively,
  *    src/energy.rs:334-369, so this is its canonical form).
- *  - Supported energy configurations: elements <= 6 and max <= 3, which
- *    includes EnergyConf::STANDARD (the conf gamebuild uses,
- *    src/gamebuild.rs:152).  Others return GEQ_ERR_UNSUPPORTED.
+ *  - Energy configurations: every EnergyConf whose energies take at most 32
+ *    words (EnergyConf::energy_size).  elements <= 6 and max <= 3 (which
+ *    includes EnergyConf::STANDARD, the conf gamebuild uses,
+ *    src/gamebuild.rs:152) run on bitmap kernels; the others, among them
+ *    SILENT_STEP and ONE_DIMENSION (src/energy.rs:45-46), on list kernels in
+ *    the crate's own packing.  Energies of several words come back in
+ *    ascending lexicographic order of their words (word 0 first).  More than
+ *    32 words, or elements == 0, returns GEQ_ERR_UNSUPPORTED.
  */
 #ifndef GPUEQUIV_H
 #define GPUEQUIV_H
@@ -42,7 +47,7 @@ typedef enum {
     GEQ_ERR_TRANSFER = 3,      /* host<->device copy failed (BufferMap)    */
     GEQ_ERR_INVALID_ARG = 4,   /* malformed graph / energies / updates     */
     GEQ_ERR_OUT_OF_MEMORY = 5, /* device allocation or budget exhausted    */
-    GEQ_ERR_UNSUPPORTED = 6,   /* energy configuration outside the kernels */
+    GEQ_ERR_UNSUPPORTED = 6,   /* energy configuration of more than 32 words */
     GEQ_ERR_INTERNAL = 7
 } geq_status;
 

@@ -76,8 +76,8 @@ static void pack_fields(const uint32_t* v, uint32_t n, uint32_t bits, uint32_t n
     }
 }
 
-constexpr int MAXW = 8;   // max energy words handled by the oracle
-constexpr int MAXEL = 64; // max elements
+constexpr int MAXW = 32;     // max energy words handled by the oracle
+constexpr int MAXEL = 1024;  // max elements
 
 struct Energy {
     uint32_t w[MAXW];

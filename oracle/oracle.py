@@ -101,7 +101,7 @@ def pack_energy(elements: int, mx: int, vals) -> list | None:
 
 
 def unpack_energy(elements: int, mx: int, words) -> list:
-    w, wp = _u32(list(words) + [0] * 8)
+    w, wp = _u32(list(words) + [0] * 32)
     out, op = _u32(np.zeros(elements))
     lib().orc_unpack_energy(elements, mx, wp, op)
     return [int(x) for x in out]
@@ -118,9 +118,9 @@ def pack_update(elements: int, mx: int, ints) -> list | None:
 
 def inv_update(elements: int, mx: int, e_words, u_words) -> list:
     esz = conf_info(elements, mx)[0]
-    e, ep = _u32(list(e_words) + [0] * 8)
-    u, up = _u32(list(u_words) + [0] * 8)
-    out, op = _u32(np.zeros(8))
+    e, ep = _u32(list(e_words) + [0] * 32)
+    u, up = _u32(list(u_words) + [0] * 1024)
+    out, op = _u32(np.zeros(32))
     lib().orc_inv_update(elements, mx, ep, up, op)
     return [int(x) for x in out[:esz]]
 

@@ -3,7 +3,7 @@ known answers, golden games and an independent up-set fixpoint.  CPU only."""
 import numpy as np
 import pytest
 
-from gpuequiv_amd import Energy, EnergyArray, EnergyConf, Upd, Update, UpdateArray, std_equivalences, update
+from gpuequiv_amd import Energy, EnergyArray, EnergyConf, GameGraph, Upd, Update, UpdateArray, std_equivalences, update
 from gpuequiv_amd import workloads
 from oracle import oracle as orc
 from tests import helpers as H
@@ -304,3 +304,105 @@ def test_oracle_threads_give_identical_results():
         orc.set_threads(1)
     assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
     assert ref[2] == got[2]
+
+
+# ---------------------------------------------------------------- energy confs beyond STANDARD
+def _edge_fields(g):
+    """Per edge: its update's field values (0 Zero, 1 Decrement, v >= 2 Min(v - 1))."""
+    c = g.conf
+    return [[0 if u.kind == "zero" else 1 if u.kind == "dec" else u.idx + 1 for u in Update(g.weights.array[k], c).to_vec()]
+            for k in range(g.n_edges())]
+
+
+def _rows(offs, data, esz, v):
+    return sorted(tuple(int(x) for x in data[(int(offs[v]) + i) * esz:(int(offs[v]) + i + 1) * esz])
+                  for i in range(int(offs[v + 1] - offs[v])))
+
+
+@pytest.mark.parametrize("elements,mx,V,seed,anyu", [(8, 2, 14, 1, False), (8, 2, 30, 2, True), (2, 7, 40, 3, True),
+                                                    (3, 5, 40, 4, False), (3, 5, 30, 5, True)])
+def test_oracle_matches_grid_fixpoint_small_confs(elements, mx, V, seed, anyu):
+    # SILENT_STEP (src/energy.rs:45) and other small confs: the oracle's lists
+    # against a Kleene iteration over every point of {0..max}^elements
+    conf = EnergyConf(elements, mx)
+    g = workloads.random_game_conf(V, seed, conf, any_updates=anyu)
+    reach = H.standard_reach(g)
+    offs, data, _ = H.oracle_solve_game(g, reach)
+    grid = U.Grid(elements, mx)
+    fields = _edge_fields(g)
+    W = U.kleene_grid(grid, g.row_offsets, g.column_indices, fields, g.attacker_pos, reach)
+    esz = conf.energy_size()
+    for v in range(V):
+        pts = [grid.index(Energy(r, conf).to_vec()) for r in _rows(offs, data, esz, v)]
+        mask = np.zeros(grid.n, dtype=bool)
+        mask[pts] = True
+        deg = int(g.row_offsets[v + 1] - g.row_offsets[v])
+        assert np.array_equal(grid.up(mask), W[v]), v
+        if g.attacker_pos[v] or deg != 1:
+            assert sorted(pts) == sorted(np.nonzero(grid.minimal(W[v]))[0].tolist()), v
+
+
+@pytest.mark.parametrize("seed,anyu", [(1, False), (2, True), (3, False)])
+def test_oracle_one_dimension_closed_form(seed, anyu):
+    # ONE_DIMENSION (src/energy.rs:46): 32-bit fields, no saturation in reach
+    conf = EnergyConf.ONE_DIMENSION
+    g = workloads.random_game_conf(400, seed, conf, mean_degree=2.0, p_ident=0.1, p_dec=0.9, p_attacker=0.7,
+                                   any_updates=anyu)
+    reach = H.standard_reach(g)
+    offs, data, _ = H.oracle_solve_game(g, reach)
+    val = U.one_dimension_values(g.row_offsets, g.column_indices, _edge_fields(g), g.attacker_pos, reach)
+    for v in range(g.n_vertices()):
+        got = [int(x) for x in data[offs[v]:offs[v + 1]]]
+        assert got == ([] if val[v] is None else [val[v]]), v
+    if not anyu:
+        assert max(x for x in val if x is not None) > 3    # budgets past STANDARD's saturation
+
+
+def test_oracle_one_dimension_wraps_like_wgsl():
+    # inv_update adds the Decrement in u32 arithmetic before the min with max
+    # (shadergen.rs:103-112): a field at u32::MAX wraps to 0
+    c = EnergyConf.ONE_DIMENSION
+    assert orc.inv_update(1, 0xFFFFFFFF, [0xFFFFFFFF], orc.pack_update(1, 0xFFFFFFFF, [-1])) == [0]
+    assert orc.inv_update(1, 0xFFFFFFFF, [41], orc.pack_update(1, 0xFFFFFFFF, [-1])) == [42]
+    assert orc.inv_update(4, 0xFFFFFFFF, [7, 0xFFFFFFFF, 3, 0], orc.pack_update(4, 0xFFFFFFFF, [0, -1, 1, -1])) == [7, 0, 3, 1]
+    assert c.energy_size() == 1 and c.update_bits() == 2
+
+
+def _embed_game(g, conf, comps):
+    """The STANDARD game g with its 6 components placed at `comps` of conf
+    (Min indices renamed alike)."""
+    fields = np.zeros((g.n_edges(), conf.elements), dtype=np.int64)
+    for k, f in enumerate(_edge_fields(g)):
+        for i, u in enumerate(f):
+            fields[k, comps[i]] = u if u < 2 else comps[u - 2] + 2
+    w = workloads.pack_fields(fields, conf.update_bits(), conf.update_size())
+    return GameGraph(conf, g.row_offsets, g.column_indices, w, g.attacker_pos)
+
+
+def _embed_energy(e_std, conf, comps):
+    vals = [0] * conf.elements
+    for i, x in enumerate(Energy([e_std], EnergyConf.STANDARD).to_vec()):
+        vals[comps[i]] = x
+    return tuple(Energy.new(vals, conf).data)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_oracle_multiword_embedding(seed):
+    # Components are interchangeable: a STANDARD game (one Min field per
+    # update at most, so the field order of inv_update does not matter)
+    # renamed into 6 of the 20 components of EnergyConf{20, 3}, straddling its
+    # two words, gives the STANDARD lists renamed alike.  This pins the
+    # multi-word compares, suprema and updates of the oracle against the
+    # single-word STANDARD path and its golden vectors.
+    conf = EnergyConf(20, 3)
+    comps = [13, 14, 15, 16, 17, 4]
+    games = [workloads.random_game(300, seed)] + [H.fixture_game(d) for d in H.load("energygame_games.json")["games"]]
+    for g in games:
+        reach = H.standard_reach(g)
+        o1, d1, s1 = H.oracle_solve_game(g, reach)
+        ge = _embed_game(g, conf, comps)
+        o2, d2, s2 = H.oracle_solve_game(ge, reach)
+        for v in range(g.n_vertices()):
+            want = sorted(_embed_energy(int(x), conf, comps) for x in d1[o1[v]:o1[v + 1]])
+            assert _rows(o2, d2, 2, v) == want, v
+        assert s1["rounds"] == s2["rounds"] and s1["edge_updates"] == s2["edge_updates"]

@@ -120,3 +120,123 @@ def kleene(row, col, wt, atk, reach, maxv: int = 3, variant: str = "ok", max_ite
 
 def as_points(mask: np.ndarray) -> tuple:
     return tuple(int(x) for x in np.nonzero(mask)[0])
+
+
+# ---------------------------------------------------------------------------
+# Any small EnergyConf: the points of {0..max}^elements enumerated explicitly.
+# ---------------------------------------------------------------------------
+class Grid:
+    """All points of a conf with at most a few thousand points; point i has
+    coordinates coord[:, i] (mixed radix max + 1, component 0 fastest)."""
+
+    def __init__(self, elements: int, maxv: int):
+        self.elements, self.maxv = elements, maxv
+        r = maxv + 1
+        self.n = r ** elements
+        idx = np.arange(self.n, dtype=np.int64)
+        self.coord = np.stack([(idx // r ** k) % r for k in range(elements)])       # (elements, n)
+        self.radix = r
+        m = np.ones((self.n, self.n), dtype=bool)
+        for k in range(elements):
+            m &= self.coord[k][:, None] <= self.coord[k][None, :]
+        self.leq = m
+
+    def index(self, coords) -> int:
+        return int(sum(int(c) * self.radix ** k for k, c in enumerate(coords)))
+
+    def image_table(self, fields) -> np.ndarray:
+        """inv_update of every point for an update given as field values
+        (0 Zero, 1 Decrement, v >= 2 Min(v - 1)): decrements first, then the
+        Min fields in index order."""
+        e = self.coord.copy()
+        for k, u in enumerate(fields):
+            if u == 1:
+                e[k] = np.minimum(e[k] + 1, self.maxv)
+        for i, u in enumerate(fields):
+            if u >= 2:
+                j = u - 2
+                e[j] = np.maximum(e[j], e[i])
+        return sum(e[k] * self.radix ** k for k in range(self.elements)).astype(np.int64)
+
+    def up(self, mask):
+        return self.leq[mask].any(axis=0) if mask.any() else np.zeros(self.n, dtype=bool)
+
+    def minimal(self, mask):
+        idx = np.nonzero(mask)[0]
+        out = np.zeros(self.n, dtype=bool)
+        if idx.size:
+            sub = self.leq[np.ix_(idx, idx)] & ~np.eye(idx.size, dtype=bool)
+            out[idx[~sub.any(axis=0)]] = True
+        return out
+
+
+def kleene_grid(grid: Grid, row, col, upd_fields, atk, reach, max_iter: int = 10000):
+    """Least fixpoint on up-sets of the grid's points (reach nodes start with
+    everything).  upd_fields: per edge, the list of its field values."""
+    V = len(row) - 1
+    W = np.zeros((V, grid.n), dtype=bool)
+    for r in reach:
+        W[int(r)] = True
+    tabs = [grid.image_table(f) for f in upd_fields]
+
+    def img(mask, t):
+        out = np.zeros(grid.n, dtype=bool)
+        out[t[mask]] = True
+        return out
+
+    for _ in range(max_iter):
+        changed = False
+        for u in range(V):
+            a, b = int(row[u]), int(row[u + 1])
+            if a == b:
+                continue
+            if atk[u]:
+                new = W[u].copy()
+                for k in range(a, b):
+                    new |= grid.up(img(W[int(col[k])], tabs[k]))
+            else:
+                if any(not W[int(col[k])].any() for k in range(a, b)):
+                    continue
+                new = np.ones(grid.n, dtype=bool)
+                for k in range(a, b):
+                    new &= grid.up(img(W[int(col[k])], tabs[k]))
+            if not np.array_equal(new, W[u]):
+                W[u] = new
+                changed = True
+        if not changed:
+            return W
+    raise RuntimeError("no fixpoint")
+
+
+def one_dimension_values(row, col, upd_fields, atk, reach, max_iter: int = 100000):
+    """The one-component game in closed form: a node's minimal budgets are one
+    value or none (a chain has one minimum).  Attacker: the least over its
+    successors' values after the update (+1 for a Decrement, unchanged
+    otherwise; Min(1) on the only component is the identity); defender with
+    every successor valued: the greatest of them.  Least fixpoint by value
+    iteration from the nodes to reach at 0.  Returns values (None: no budget)."""
+    V = len(row) - 1
+    val = [None] * V
+    for r in reach:
+        val[int(r)] = 0
+    for _ in range(max_iter):
+        changed = False
+        for u in range(V):
+            a, b = int(row[u]), int(row[u + 1])
+            if a == b:
+                continue
+            imgs = [None if val[int(col[k])] is None else val[int(col[k])] + (1 if upd_fields[k][0] == 1 else 0)
+                    for k in range(a, b)]
+            if atk[u]:
+                c = [x for x in imgs if x is not None] + ([] if val[u] is None else [val[u]])
+                new = min(c) if c else None
+            else:
+                new = None if any(x is None for x in imgs) else max(imgs)
+                if new is not None and val[u] is not None:
+                    new = min(new, val[u])
+            if new != val[u]:
+                val[u] = new
+                changed = True
+        if not changed:
+            return val
+    raise RuntimeError("no fixpoint")
