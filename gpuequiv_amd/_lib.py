@@ -169,6 +169,8 @@ _SIGS = {
     "geq_ctx_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, P(C.c_int)]),
     "geq_solve": (C.c_int, [C.c_void_p, P(geq_graph), P(C.c_uint32), C.c_uint32, P(geq_energies), P(geq_energies), P(geq_stats)]),
     "geq_energies_free": (None, [P(geq_energies)]),
+    "geq_solve_noalloc": (C.c_int, [C.c_void_p, P(geq_graph), P(C.c_uint32), C.c_uint32, P(C.c_uint32), P(C.c_uint32),
+                                    C.c_uint64, P(C.c_uint64), P(geq_stats)]),
     "geq_game_upload": (C.c_int, [C.c_void_p, P(geq_graph), P(C.c_void_p)]),
     "geq_game_solve": (C.c_int, [C.c_void_p, P(C.c_uint32), C.c_uint32, P(geq_energies), C.c_int, P(geq_stats)]),
     "geq_game_solve_standard": (C.c_int, [C.c_void_p, C.c_int, P(geq_stats)]),

@@ -196,6 +196,9 @@ geq_status geq_ctx_create(int device, geq_ctx** out) {
         delete c;
         return fail(GEQ_ERR_NO_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    // the static scratch of geq_solve_noalloc on this device (a failure here
+    // only disables that entry point, which then reports it)
+    (void)noalloc_setup(device);
     {
         std::lock_guard<std::mutex> lk(g_ctx_mu);
         g_ctxs.push_back(c);

@@ -52,6 +52,9 @@ bool update_to_canon(const ConfInfo& c, uint32_t packed, uint32_t& canon);
 hipError_t device_malloc(void** p, size_t bytes);
 // Free the cached spare game of every context on `device`; true if one was freed.
 bool drop_spare_games(int device);
+// geq_solve_noalloc's set-up on the current device (at context creation):
+// the device address of its static scratch array, once per device.
+geq_status noalloc_setup(int device);
 
 template <class T>
 struct DevBuf {

@@ -120,6 +120,23 @@ geq_status geq_solve(geq_ctx* ctx, const geq_graph* graph, const uint32_t* to_re
 
 void geq_energies_free(geq_energies* e);
 
+/* geq_solve without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every device
+ * buffer of the solve lives in one static device array of 64 MiB per device
+ * (held from library load to process end); games that do not fit it (about
+ * 350,000 positions at 4 edges each) return GEQ_ERR_OUT_OF_MEMORY, as does an
+ * energy arena that runs out.  Energy configurations: elements <= 6 and
+ * max <= 3 (others GEQ_ERR_UNSUPPORTED).  Inputs as geq_solve (no initial
+ * energies: EnergyGame::with_reach); the caller passes the output arrays:
+ * out_offsets (n_vertices + 1 entries, in energies) and out_data (out_cap
+ * packed energies, one word each, ascending per node).  *out_n (may be NULL)
+ * receives the result's energy count; a result larger than out_cap returns
+ * GEQ_ERR_INVALID_ARG with *out_n set.  Calls on one device share its array:
+ * the library serialises them with a per-device lock. */
+geq_status geq_solve_noalloc(geq_ctx* ctx, const geq_graph* graph, const uint32_t* to_reach, uint32_t n_reach,
+                             uint32_t* out_offsets, uint32_t* out_data, uint64_t out_cap, uint64_t* out_n, geq_stats* stats);
+
 /* ---- device-resident game (the pieces of geq_solve, for repeated solves) ----
  * geq_game_upload copies the graph to HBM; geq_game_solve resets the energies
  * (to_reach / initial as above) and runs the fixpoint with every input already
