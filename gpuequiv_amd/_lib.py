@@ -167,6 +167,7 @@ _SIGS = {
     "geq_ctx_create": (C.c_int, [C.c_int, P(C.c_void_p)]),
     "geq_ctx_destroy": (None, [C.c_void_p]),
     "geq_ctx_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, P(C.c_int)]),
+    "geq_ctx_memory": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]),
     "geq_solve": (C.c_int, [C.c_void_p, P(geq_graph), P(C.c_uint32), C.c_uint32, P(geq_energies), P(geq_energies), P(geq_stats)]),
     "geq_energies_free": (None, [P(geq_energies)]),
     "geq_solve_noalloc": (C.c_int, [C.c_void_p, P(geq_graph), P(C.c_uint32), C.c_uint32, P(C.c_uint32), P(C.c_uint32),
@@ -264,6 +265,12 @@ class Context:
             if cls._instance is None:
                 cls._instance = Context(int(os.environ.get("GEQ_DEVICE", "0")))
             return cls._instance
+
+    def memory(self) -> tuple:
+        """(free, total) bytes of device memory."""
+        f, t = C.c_uint64(), C.c_uint64()
+        check(lib().geq_ctx_memory(self.handle, C.byref(f), C.byref(t)))
+        return int(f.value), int(t.value)
 
     def info(self) -> tuple:
         buf = C.create_string_buffer(128)

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "runtime.hpp"
+#include "scan.hpp"
 
 namespace geq {
 
@@ -165,6 +166,7 @@ using namespace geq;
 geq_game::~geq_game() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (h_ctl) (void)hipHostFree(h_ctl);
+    if (h_stage) (void)hipHostFree(h_stage);   // (d_stage is its device view)
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_end) (void)hipEventDestroy(ev_end);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -228,6 +230,16 @@ geq_status geq_ctx_info(geq_ctx* ctx, char* name, size_t name_len, int* n_cu) {
         name[name_len - 1] = 0;
     }
     if (n_cu) *n_cu = ctx->n_cu;
+    return GEQ_OK;
+}
+
+geq_status geq_ctx_memory(geq_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!ctx) return fail(GEQ_ERR_INVALID_ARG, "ctx is NULL");
+    (void)hipSetDevice(ctx->device);
+    size_t f = 0, t = 0;
+    GEQ_HIP(hipMemGetInfo(&f, &t), GEQ_ERR_NO_DEVICE);
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
     return GEQ_OK;
 }
 
@@ -376,6 +388,7 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     }
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
         return oom(e, "pinned control block");
+    *gm->h_ctl = Ctl{};
     hipStream_t s = ctx->stream;
     if (ci.generic) {
         if ((E && hipMemcpyAsync(gm->gwt.p, gwt.data(), 4ull * E * ci.usz, hipMemcpyHostToDevice, s) != hipSuccess) ||
@@ -441,6 +454,15 @@ __global__ void k_solve_init(uint4* st, uint32_t V, int clear_slots, uint32_t* q
         c.arena_top = arena_top;
         *ctl = c;
     }
+}
+
+// The download's offsets (n + 1) and lists (offs[n] words) into the staging.
+__global__ void k_stage_out(const uint32_t* offs, uint32_t n, const uint32_t* data, uint64_t cap, uint32_t* h_offs,
+                            uint32_t* h_data) {
+    const uint64_t tot = min((uint64_t)offs[n], cap);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i <= n; i += stride) h_offs[i] = offs[i];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += stride) h_data[i] = data[i];
 }
 
 // slot 0 (tag 0, current while both slots are untouched) of every node to reach
@@ -1090,36 +1112,47 @@ geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_e
     if (!g->lens.p || g->lens.n < V) GEQ_HIP(g->lens.alloc(std::max<uint32_t>(V, 1)), GEQ_ERR_OUT_OF_MEMORY);
     const uint32_t t = round_tag(g->h_ctl->round);   // the lists after the last round that ran
     if (g->conf.generic) return download_lists(g, first, V, t, out);
-    k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, V, t, g->lens.p);
-    std::vector<uint32_t> lens(V), offs(V + 1);
-    GEQ_HIP(hipMemcpyAsync(lens.data(), g->lens.p, 4ull * V, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
-    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
-    uint64_t tot = 0;
-    for (uint32_t v = 0; v < V; v++) {
-        offs[v] = (uint32_t)tot;
-        tot += lens[v];
+    // lengths -> offsets (device scan) -> lists gathered, ascending; offsets and
+    // lists come back through the game's pinned staging (two waits)
+    if (!g->offs.p || g->offs.n < V + 1ull) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
+    if (V) k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, V, t, g->lens.p);
+    geq_status sst = exclusive_scan_dev<uint32_t, SumU32>(g->lens.p, g->offs.p, V, g->scan_tmp, s, SumU32{});
+    if (sst != GEQ_OK) return sst;
+    // room for the gathered lists: every list is inline (<= INLINE_CAP) or in the arena
+    const uint64_t bound = (uint64_t)INLINE_CAP * V + std::min<uint64_t>((uint64_t)g->h_ctl->arena_top, (uint64_t)g->arena_cap);
+    if (!g->out.p || g->out.n < bound) GEQ_HIP(g->out.alloc(std::max<uint64_t>(bound, 1)), GEQ_ERR_OUT_OF_MEMORY);
+    if (V) k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, g->arena.p, V, t, g->offs.p, g->out.p, bound);
+    // offsets and lists written by one copy kernel into mapped pinned memory
+    // (room for the bound), then one wait
+    const size_t words = V + 1ull + bound;
+    if (!g->h_stage || g->h_stage_n < words) {
+        if (g->h_stage) (void)hipHostFree(g->h_stage);
+        g->h_stage = g->d_stage = nullptr;
+        g->h_stage_n = 0;
+        const size_t cap = std::max<size_t>(words, 1024);
+        GEQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->h_stage), 4 * cap, hipHostMallocMapped | hipHostMallocCoherent),
+                GEQ_ERR_OUT_OF_MEMORY);
+        if (hipHostGetDevicePointer(reinterpret_cast<void**>(&g->d_stage), g->h_stage, 0) != hipSuccess || !g->d_stage) {
+            (void)hipHostFree(g->h_stage);
+            g->h_stage = g->d_stage = nullptr;
+            return fail(GEQ_ERR_INTERNAL, "no device view of the download staging");
+        }
+        g->h_stage_n = cap;
     }
-    if (tot > 0xffffffffull) return fail(GEQ_ERR_OUT_OF_MEMORY, "result exceeds 2^32 energies");
-    offs[V] = (uint32_t)tot;
-    if (!g->offs.p || g->offs.n < V + 1) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
-    if (!g->out.p || g->out.n < tot) GEQ_HIP(g->out.alloc(tot), GEQ_ERR_OUT_OF_MEMORY);
-    GEQ_HIP(hipMemcpyAsync(g->offs.p, offs.data(), 4ull * (V + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, g->arena.p, V, t, g->offs.p, g->out.p);
-    uint32_t* data = (uint32_t*)std::malloc(4ull * (tot ? tot : 1));
+    k_stage_out<<<std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((words + 255) / 256), 2048)), 256, 0, s>>>(
+        g->offs.p, V, g->out.p, bound, g->d_stage, g->d_stage + V + 1);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    const uint64_t tot = g->h_stage[V];
+    if (tot > bound) return fail(GEQ_ERR_INTERNAL, "energy download: lists exceed their bound");
     uint32_t* ooffs = (uint32_t*)std::malloc(4ull * (V + 1));
+    uint32_t* data = (uint32_t*)std::malloc(4ull * (tot ? tot : 1));
     if (!data || !ooffs) {
         std::free(data);
         std::free(ooffs);
         return fail(GEQ_ERR_OUT_OF_MEMORY, "host allocation of the result");
     }
-    hipError_t e = hipMemcpyAsync(data, g->out.p, 4ull * tot, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-        std::free(data);
-        std::free(ooffs);
-        return fail(GEQ_ERR_TRANSFER, std::string("downloading energies: ") + hipGetErrorString(e));
-    }
-    std::memcpy(ooffs, offs.data(), 4ull * (V + 1));
+    std::memcpy(ooffs, g->h_stage, 4ull * (V + 1));
+    if (tot) std::memcpy(data, g->h_stage + V + 1, 4ull * tot);
     const bool canon = (g->conf.ebits == 2);   // 2-bit fields pack exactly like the canonical layout
     if (!canon)
         for (uint64_t i = 0; i < tot; i++) data[i] = energy_from_canon(g->conf, data[i]);

@@ -179,6 +179,7 @@ geq_status geq::alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_gam
     if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
     if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
         return oom(e, "pinned control block");
+    *gm->h_ctl = Ctl{};
     if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
     *out = gm;
     return GEQ_OK;

@@ -294,7 +294,7 @@ extern "C" geq_status geq_solve_noalloc(geq_ctx* ctx, const geq_graph* g, const 
         return fail(GEQ_ERR_INVALID_ARG, "out_data holds " + std::to_string(out_cap) + " energies, the result has " + std::to_string(tot) +
                                              " (*out_n)");
     uint32_t* stage = rcol;
-    k_gather<<<(V + tb - 1) / tb, tb, 0, s>>>(nodes, arena, V, t, offs, stage);
+    k_gather<<<(V + tb - 1) / tb, tb, 0, s>>>(nodes, arena, V, t, offs, stage, staging);
     if (tot) GEQ_HIP(hipMemcpyAsync(out_data, stage, 4ull * tot, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     if (ci.ebits != 2)

@@ -178,6 +178,10 @@ struct geq_game {
     geq::DevBuf<geq::Ctl> ctl;
     geq::DevBuf<unsigned long long> stats;
     geq::Ctl* h_ctl = nullptr;        // pinned
+    geq::DevBuf<uint32_t> scan_tmp;   // block totals of the download's offset scan
+    uint32_t* h_stage = nullptr;      // mapped pinned staging of the energy download (host view)
+    uint32_t* d_stage = nullptr;      // its device view
+    size_t h_stage_n = 0;             // its words
     uint32_t arena_cap = 0;
     uint32_t retag_round = 0;         // round of the last k_retag (or of the initialisation)
     std::vector<uint8_t> h_atk;       // kept for standard_reach / reach validation

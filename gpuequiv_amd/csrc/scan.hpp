@@ -1,0 +1,134 @@
+// Device-wide exclusive scan (reduce-then-scan, three launches): used for the
+// energy download's offsets and for gamebuild's per-level offsets.
+//   k_scan_reduce : every block of SCAN_ITEMS consecutive items -> its total
+//   k_scan_tops   : one block scans the block totals in place (and the grand total)
+//   k_scan_apply  : every block scans its items again from its block's offset
+// Wave scans are lane shuffles; the combine operator is a template (plain sums,
+// or the saturating pair sums of the gamebuild bounds).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "runtime.hpp"
+
+namespace geq {
+
+constexpr uint32_t SCAN_THREADS = 256;
+constexpr uint32_t SCAN_PER_THREAD = 16;
+constexpr uint32_t SCAN_ITEMS = SCAN_THREADS * SCAN_PER_THREAD;   // items per block
+constexpr uint32_t SCAN_TOP_THREADS = 1024;
+
+struct SumU32 {
+    __host__ __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+    __host__ __device__ static uint32_t identity() { return 0u; }
+};
+
+namespace detail {
+
+__device__ __forceinline__ uint32_t shfl_up_t(uint32_t v, uint32_t d) { return (uint32_t)__shfl_up((int)v, d, 64); }
+__device__ __forceinline__ uint2 shfl_up_t(uint2 v, uint32_t d) {
+    return make_uint2((uint32_t)__shfl_up((int)v.x, d, 64), (uint32_t)__shfl_up((int)v.y, d, 64));
+}
+
+// inclusive scan of one value per lane over the wave (op applied by shuffles)
+template <class T, class Op>
+__device__ __forceinline__ T wave_scan_incl(T v, uint32_t lane, Op op) {
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const T y = shfl_up_t(v, d);
+        if (lane >= d) v = op(y, v);
+    }
+    return v;
+}
+
+// exclusive scan of one value per thread over the block; `total` gets the block's combination
+template <class T, class Op, uint32_t NT>
+__device__ __forceinline__ T block_scan_excl(T v, Op op, T& total) {
+    __shared__ T s_w[NT / 64];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const T incl = wave_scan_incl(v, lane, op);
+    if (lane == 63u) s_w[w] = incl;
+    __syncthreads();
+    T pre = Op::identity(), tot = Op::identity();
+    for (uint32_t k = 0; k < NT / 64; k++) {
+        if (k < w) pre = op(pre, s_w[k]);
+        tot = op(tot, s_w[k]);
+    }
+    __syncthreads();
+    total = tot;
+    // exclusive value of this thread: pre (+) the wave's exclusive part
+    const T wex = shfl_up_t(incl, 1);
+    return lane == 0 ? pre : op(pre, wex);
+}
+
+}  // namespace detail
+
+template <class T, class Op>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const T* in, size_t n, T* tops, Op op) {
+    const size_t b0 = (size_t)blockIdx.x * SCAN_ITEMS + (size_t)threadIdx.x * SCAN_PER_THREAD;
+    T acc = Op::identity();
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++)
+        if (b0 + k < n) acc = op(acc, in[b0 + k]);
+    T total;
+    (void)detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total);
+    if (threadIdx.x == 0) tops[blockIdx.x] = total;
+}
+
+// tops[0..nb) scanned in place (exclusive); tops[nb] = the grand total
+template <class T, class Op>
+__global__ __launch_bounds__(SCAN_TOP_THREADS) void k_scan_tops(T* tops, uint32_t nb, Op op) {
+    const uint32_t per = (nb + SCAN_TOP_THREADS - 1) / SCAN_TOP_THREADS;
+    const uint32_t b = min(nb, threadIdx.x * per), e = min(nb, b + per);
+    T acc = Op::identity();
+    for (uint32_t i = b; i < e; i++) acc = op(acc, tops[i]);
+    T total;
+    T run = detail::block_scan_excl<T, Op, SCAN_TOP_THREADS>(acc, op, total);
+    for (uint32_t i = b; i < e; i++) {
+        const T x = tops[i];
+        tops[i] = run;
+        run = op(run, x);
+    }
+    if (threadIdx.x == 0) tops[nb] = total;
+}
+
+template <class T, class Op>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T* in, T* out, size_t n, const T* tops, Op op) {
+    const size_t b0 = (size_t)blockIdx.x * SCAN_ITEMS + (size_t)threadIdx.x * SCAN_PER_THREAD;
+    T v[SCAN_PER_THREAD];
+    T acc = Op::identity();
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
+        v[k] = b0 + k < n ? in[b0 + k] : Op::identity();
+        acc = op(acc, v[k]);
+    }
+    T total;
+    T run = op(tops[blockIdx.x], detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total));
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++)
+        if (b0 + k < n) {
+            out[b0 + k] = run;
+            run = op(run, v[k]);
+        }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tops[gridDim.x];   // the grand total
+}
+
+// Exclusive scan of in[0..n) into out[0..n), the total into out[n] (out holds
+// n + 1 entries; in == out is allowed).  tmp: block totals, grown as needed.
+template <class T, class Op>
+geq_status exclusive_scan_dev(const T* in, T* out, size_t n, DevBuf<T>& tmp, hipStream_t s, Op op) {
+    const size_t nb = std::max<size_t>(1, (n + SCAN_ITEMS - 1) / SCAN_ITEMS);
+    if (nb > 0x7fffffffull) return fail(GEQ_ERR_INVALID_ARG, "scan of more than 2^31 blocks");
+    if (!tmp.p || tmp.n < nb + 1) {
+        tmp.release();
+        GEQ_HIP(tmp.alloc(nb + 1), GEQ_ERR_OUT_OF_MEMORY);
+    }
+    k_scan_reduce<T, Op><<<(uint32_t)nb, SCAN_THREADS, 0, s>>>(in, n, tmp.p, op);
+    k_scan_tops<T, Op><<<1, SCAN_TOP_THREADS, 0, s>>>(tmp.p, (uint32_t)nb, op);
+    k_scan_apply<T, Op><<<(uint32_t)nb, SCAN_THREADS, 0, s>>>(in, out, n, tmp.p, op);
+    GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
+    return GEQ_OK;
+}
+
+}  // namespace geq

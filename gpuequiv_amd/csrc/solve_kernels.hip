@@ -549,11 +549,13 @@ __global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* l
 // Copy every node's list into the packed output, ascending.  Lists produced
 // by the single-successor defender copy keep the successor's order and are
 // sorted here (insertion sort; such lists are short antichain images).
-__global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out) {
+__global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out,
+                         uint64_t cap) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const uint4 d = load_cur(st, v, t);
     const uint32_t len = nl_len(d);
+    if ((uint64_t)offs[v] + len > cap) return;   // the caller checks the total against cap
     uint32_t* o = out + offs[v];
     for (uint32_t j = 0; j < len; j++) o[j] = nl_get(d, arena, j);
     if (d.x & UNSORTED) {

@@ -155,6 +155,9 @@ __global__ void k_retag(uint4* st, uint32_t n, uint32_t t);
 __global__ void k_node_hdr(const uint32_t* row, const uint32_t* rrow, const uint8_t* atk, uint32_t n, uint4* st);
 __global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots, unsigned long long* out);
 __global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens);
-__global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out);
+// out holds cap words; a list that would pass it is skipped (the caller has
+// checked offs[n] <= cap)
+__global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out,
+                         uint64_t cap);
 
 }  // namespace geq

@@ -109,6 +109,9 @@ void geq_ctx_destroy(geq_ctx* ctx);
 const char* geq_last_error(void);
 /* HIP device name / CU count of the context's device. */
 geq_status geq_ctx_info(geq_ctx* ctx, char* name, size_t name_len, int* n_cu);
+/* Free and total device memory of the context's device, in bytes (for sizing
+ * a serving budget).  No counterpart in the reference. */
+geq_status geq_ctx_memory(geq_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* ---- one-shot solve: EnergyGame::run (src/energygame.rs:379-383) ----
  * to_reach: the nodes initialised with the zero energy (EnergyGame::with_reach,
