@@ -49,10 +49,15 @@ extern "C" {
 
 geq_status geq_signature_refinement(const geq_lts* lts, uint32_t* partition, uint32_t* n_classes) {
     if (!lts || !partition || !n_classes || !lts->row_offsets) return geq::fail(GEQ_ERR_INVALID_ARG, "NULL argument");
-    const uint32_t n = lts->n_states;
+    const uint32_t n = lts->n_states, m = lts->n_transitions;
+    // the CSR shape first: every row range must lie inside the transition arrays
+    if (m && (!lts->targets || !lts->labels)) return geq::fail(GEQ_ERR_INVALID_ARG, "missing LTS transition arrays");
+    if (lts->row_offsets[0] != 0 || lts->row_offsets[n] != m)
+        return geq::fail(GEQ_ERR_INVALID_ARG, "LTS row_offsets must start at 0 and end at n_transitions");
     for (uint32_t s = 0; s < n; s++)
-        for (uint32_t k = lts->row_offsets[s]; k < lts->row_offsets[s + 1]; k++)
-            if (lts->targets[k] >= n) return geq::fail(GEQ_ERR_INVALID_ARG, "LTS target out of range");
+        if (lts->row_offsets[s + 1] < lts->row_offsets[s]) return geq::fail(GEQ_ERR_INVALID_ARG, "LTS row_offsets not monotone");
+    for (uint32_t k = 0; k < m; k++)
+        if (lts->targets[k] >= n) return geq::fail(GEQ_ERR_INVALID_ARG, "LTS target out of range");
     std::vector<uint32_t> part(n, 0), next(n);
     std::vector<uint64_t> sig;                 // (label << 32 | class) runs, per state sorted + unique
     std::vector<uint32_t> off(n + 1);

@@ -332,3 +332,13 @@ def test_concurrent_host_threads_share_the_context():
         t.join(timeout=100)
     assert not errors, errors
     assert got == want
+
+
+def test_json_example_game_matches_oracle():
+    # the game of examples/json.rs, written as SerdeGameGraph JSON and read back
+    from tests.test_json import json_example_graph
+    g = GameGraph.from_json(json_example_graph().to_json())
+    offs, data, st = H.oracle_solve_game(g, H.standard_reach(g))
+    game, out = _run(g)
+    assert _canon(out) == H.canon_lists(offs, data)
+    assert game.last_stats["rounds"] == st["rounds"]
