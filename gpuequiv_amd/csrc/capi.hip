@@ -316,6 +316,59 @@ geq_status append_graph(const geq_graph* g, const ConfInfo& ci, uint32_t vb, uin
     return GEQ_OK;
 }
 
+// Renumber the positions of each game (ids base[i] .. base[i+1]) in reverse
+// breadth-first order from its defender dead ends (the standard reach): the
+// positions a round of the fixpoint evaluates then sit in a few dense id
+// ranges, so node records, edge runs, reverse runs and queued-bitmap words
+// are read in whole cache lines instead of one random line each.  Positions
+// the walk does not reach keep their order after the reached ones.  Rewrites
+// every array in place; inv[old id] = new id.  Results are the same under any
+// numbering (each game's fixpoint is unique).
+void renumber(const std::vector<uint32_t>& base, uint32_t usz, std::vector<uint32_t>& row, std::vector<uint2>& edges,
+              std::vector<uint32_t>& rrow, std::vector<uint32_t>& rcol, std::vector<uint8_t>& atk, std::vector<uint32_t>& gwt,
+              std::vector<uint32_t>& inv) {
+    const uint32_t V = (uint32_t)atk.size(), E = (uint32_t)edges.size();
+    std::vector<uint32_t> perm(V);     // new id -> old id
+    inv.assign(V, 0xffffffffu);
+    for (size_t gi = 0; gi + 1 < base.size(); gi++) {
+        const uint32_t b0 = base[gi], b1 = base[gi + 1];
+        uint32_t head = b0, tail = b0;
+        auto visit = [&](uint32_t v) {
+            inv[v] = tail;
+            perm[tail++] = v;
+        };
+        for (uint32_t v = b0; v < b1; v++)
+            if (!atk[v] && row[v] == row[v + 1]) visit(v);
+        while (head < tail) {
+            const uint32_t v = perm[head++];
+            for (uint32_t k = rrow[v]; k < rrow[v + 1]; k++)
+                if (inv[rcol[k]] == 0xffffffffu) visit(rcol[k]);
+        }
+        for (uint32_t v = b0; v < b1; v++)
+            if (inv[v] == 0xffffffffu) visit(v);
+    }
+    std::vector<uint32_t> row2(V + 1, 0), rrow2(V + 1, 0), rcol2(E);
+    std::vector<uint2> edges2(E);
+    std::vector<uint8_t> atk2(V);
+    std::vector<uint32_t> gwt2(usz ? (size_t)E * usz : 0);
+    for (uint32_t i = 0; i < V; i++) {
+        const uint32_t o = perm[i];
+        const uint32_t e0 = row[o], d = row[o + 1] - e0, r0 = rrow[o], rd = rrow[o + 1] - r0;
+        row2[i + 1] = row2[i] + d;
+        rrow2[i + 1] = rrow2[i] + rd;
+        for (uint32_t k = 0; k < d; k++) edges2[row2[i] + k] = make_uint2(inv[edges[e0 + k].x], edges[e0 + k].y);
+        if (usz) std::copy(gwt.begin() + (size_t)e0 * usz, gwt.begin() + (size_t)(e0 + d) * usz, gwt2.begin() + (size_t)row2[i] * usz);
+        for (uint32_t k = 0; k < rd; k++) rcol2[rrow2[i] + k] = inv[rcol[r0 + k]];
+        atk2[i] = atk[o];
+    }
+    row.swap(row2);
+    rrow.swap(rrow2);
+    rcol.swap(rcol2);
+    edges.swap(edges2);
+    atk.swap(atk2);
+    gwt.swap(gwt2);
+}
+
 geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs, geq_game** out) {
     if (!ctx || !graphs || !out || n_games == 0) return fail(GEQ_ERR_INVALID_ARG, "NULL argument or no games");
     *out = nullptr;
@@ -347,6 +400,11 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
         }
         eb += graphs[i].n_edges;
     }
+    // internal ids in reverse breadth-first order (GEQ_RENUMBER=0 keeps the caller's)
+    std::vector<uint32_t> inv;
+    const char* renum_env = std::getenv("GEQ_RENUMBER");
+    if (!(renum_env && std::atoi(renum_env) == 0) && V > 1)
+        renumber(base, ci.generic ? ci.usz : 0, row, edges, rrow, rcol, atkv, gwt, inv);
 
     (void)hipSetDevice(ctx->device);
     geq_game* gm = new geq_game();
@@ -356,6 +414,7 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     gm->E = E;
     gm->game_base = base;
     gm->h_atk = std::move(atkv);
+    gm->inv = std::move(inv);
     auto oom = [&](hipError_t e, const char* what) -> geq_status {
         delete gm;
         return fail(e == hipErrorOutOfMemory ? GEQ_ERR_OUT_OF_MEMORY : GEQ_ERR_NO_DEVICE,
@@ -375,6 +434,7 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
     if ((e = gm->ctl.alloc(1)) != hipSuccess) return oom(e, "control block");
     if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
     if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
+    gm->tune.ordered = worklist_order_default();
     if (ci.generic) {
         // the word arena: the zero energy, then room for about one list per two edges
         gm->garena_cap = (uint32_t)std::min<uint64_t>(
@@ -403,6 +463,13 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
         up(gm->atk.p, gm->h_atk.data(), V) != hipSuccess) {
         delete gm;
         return fail(GEQ_ERR_TRANSFER, "uploading the game graph failed");
+    }
+    if (!gm->inv.empty()) {
+        if ((e = gm->inv_map.alloc(V)) != hipSuccess) return oom(e, "id map");
+        if (up(gm->inv_map.p, gm->inv.data(), 4ull * V) != hipSuccess) {
+            delete gm;
+            return fail(GEQ_ERR_TRANSFER, "uploading the game graph failed");
+        }
     }
     if (V) k_node_hdr<<<(V + 255) / 256, 256, 0, s>>>(gm->row.p, gm->rrow.p, gm->atk.p, V, gm->st.p);
     if (hipStreamSynchronize(s) != hipSuccess) {
@@ -541,6 +608,8 @@ RoundShape round_shape(const geq_game* g) {
         const uint32_t xpc = g->tune.expand_blocks_per_cu ? g->tune.expand_blocks_per_cu : 4u;
         r.expand_blocks = std::max(1u, std::min(ncu * xpc, (g->V + GEN_BLOCK - 1) / GEN_BLOCK));
         r.rounds_per_launch = std::max(1u, g->tune.rounds_per_launch);
+        r.ordered = false;
+        r.collect_blocks = 0;
         return r;
     }
     const uint32_t w = std::max(1u, std::min<uint32_t>(g->tune.waves_per_block, MAX_WAVES));
@@ -551,6 +620,9 @@ RoundShape round_shape(const geq_game* g) {
     const uint32_t xpc = r.fused ? 4u : g->tune.expand_blocks_per_cu;
     r.expand_blocks = std::max(1u, std::min(ncu * xpc, (g->V + 64u * w - 1) / (64u * w)));
     r.rounds_per_launch = std::max(1u, g->tune.rounds_per_launch);
+    r.ordered = r.fused && (g->tune.ordered == 1 || (g->tune.ordered == 2 && g->V >= ORDERED_MIN_V));
+    r.collect_blocks = (uint32_t)std::max<uint64_t>(
+        1, ((uint64_t)queue_words(g->V) + COLLECT_THREADS * COLLECT_WORDS - 1) / (COLLECT_THREADS * COLLECT_WORDS));
     // k_eval's grid-stride loop covers any worklist: a grid larger than what
     // is resident at once only adds a second, mostly idle wave of blocks
     r.eval_blocks = std::min(r.eval_blocks, ncu * eval_blocks_resident(r.fused, w));
@@ -638,7 +710,16 @@ geq_status ensure_scratch(geq_game* g, const RoundShape& shape, uint64_t words) 
 
 EvalArgs eval_args(geq_game* g) {
     return EvalArgs{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->st.p, g->arena.p, g->ctl.p, g->arena_cap,
-                    g->chg.p, g->conf.max, g->stats.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V)};
+                    g->chg.p, g->conf.max, g->stats.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V),
+                    round_shape(g).ordered ? 1u : 0u};
+}
+
+uint32_t worklist_order_default() {
+    if (const char* env = std::getenv("GEQ_WORKLIST_ORDER")) {
+        const int v = std::atoi(env);
+        if (v == 0 || v == 1) return (uint32_t)v;
+    }
+    return 2;
 }
 ExpandArgs expand_args(geq_game* g) {
     return ExpandArgs{{g->wl0.p, g->wl1.p}, nullptr, 0u, g->chg.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V), g->ctl.p};
@@ -658,6 +739,9 @@ void launch_rounds(geq_game* g, const EvalArgs& ea, const ExpandArgs& xa, const 
             k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa, k);
         } else if (shape.fused) {
             k_eval<true><<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_FUSED_LDS_PER_WAVE, s>>>(ea, k);
+            if (shape.ordered)
+                k_collect<<<shape.collect_blocks, COLLECT_THREADS, 0, s>>>(CollectArgs{{g->wl0.p, g->wl1.p}, g->queued.p,
+                                                                                        queue_words(g->V), g->ctl.p}, k);
         } else {
             k_eval<false><<<shape.eval_blocks, shape.threads, (shape.threads / 64) * EVAL_LDS_PER_WAVE, s>>>(ea, k);
             k_expand<<<shape.expand_blocks, shape.threads, 0, s>>>(xa, k);
@@ -742,9 +826,16 @@ extern "C" geq_status geq_game_solve(geq_game* g, const uint32_t* to_reach, uint
     for (uint32_t i = 0; i < n_reach; i++)
         if (to_reach[i] >= g->V) return fail(GEQ_ERR_INVALID_ARG, "to_reach id out of range");
     (void)hipSetDevice(g->ctx->device);
+    std::vector<uint32_t> mapped;     // the internal ids of a renumbered game
+    if (n_reach && !g->inv.empty()) {
+        mapped.resize(n_reach);
+        for (uint32_t i = 0; i < n_reach; i++) mapped[i] = g->inv[to_reach[i]];
+        to_reach = mapped.data();
+    }
     if (n_reach) {
         if (!g->seed.p || g->seed.n < n_reach) GEQ_HIP(g->seed.alloc(n_reach), GEQ_ERR_OUT_OF_MEMORY);
         GEQ_HIP(hipMemcpyAsync(g->seed.p, to_reach, 4ull * n_reach, hipMemcpyHostToDevice, g->ctx->stream), GEQ_ERR_TRANSFER);
+        if (!mapped.empty()) GEQ_HIP(hipStreamSynchronize(g->ctx->stream), GEQ_ERR_TRANSFER);   // `mapped` is freed on return
     }
     return geq::solve_core(g, n_reach, initial, profile_kernels, stats);
 }
@@ -830,7 +921,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
                 if (!energy_valid(g->conf, w)) return fail(GEQ_ERR_INVALID_ARG, "initial energies: invalid energy");
                 h_words.insert(h_words.end(), w, w + esz);
             }
-            h_st[2ull * v] = make_uint4(e - b, (uint32_t)off, 0u, 0u);
+            h_st[2ull * (g->inv.empty() ? v : g->inv[v])] = make_uint4(e - b, (uint32_t)off, 0u, 0u);   // the node's internal id
         }
         if (GEN_ZERO_WORDS + h_words.size() + (1u << 16) > g->garena_cap) {
             geq_status st = grow_arena(g, (uint32_t)(GEN_ZERO_WORDS + h_words.size()));
@@ -867,7 +958,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
                 h[2] = (uint16_t)(off & 0xffff);   // second dword of the record = arena offset
                 h[3] = (uint16_t)(off >> 16);
             }
-            std::memcpy(&h_st[2ull * v], h, 16);
+            std::memcpy(&h_st[2ull * (g->inv.empty() ? v : g->inv[v])], h, 16);   // the node's internal id
         }
         if (h_arena.size() + (1u << 20) > g->arena_cap) {
             geq_status st = grow_arena(g, (uint32_t)h_arena.size());
@@ -942,6 +1033,9 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
                 hipExtLaunchKernelGGL(k_eval<true>, dim3(shape.eval_blocks), dim3(shape.threads),
                                       (shape.threads / 64) * EVAL_FUSED_LDS_PER_WAVE, s, ev[0], ev[1], 0, ea, 0u);
                 GEQ_HIP(hipEventRecord(ev[2], s), GEQ_ERR_NO_DEVICE);
+                if (shape.ordered)   // (its time goes with the expansion's)
+                    k_collect<<<shape.collect_blocks, COLLECT_THREADS, 0, s>>>(
+                        CollectArgs{{g->wl0.p, g->wl1.p}, g->queued.p, queue_words(g->V), g->ctl.p}, 0u);
                 GEQ_HIP(hipEventRecord(ev[3], s), GEQ_ERR_NO_DEVICE);
             } else {
                 hipExtLaunchKernelGGL(k_eval<false>, dim3(shape.eval_blocks), dim3(shape.threads),
@@ -1050,7 +1144,9 @@ namespace {
 geq_status download_lists(geq_game* g, uint32_t first, uint32_t V, uint32_t t, geq_energies* out) {
     hipStream_t s = g->ctx->stream;
     const uint32_t esz = g->conf.esz;
-    if (V) k_list_lens_g<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, V, t, g->lens.p);
+    const uint4* stb = g->inv_map.p ? g->st.p : g->st.p + (size_t)NODE_U4 * first;
+    const uint32_t* map = g->inv_map.p ? g->inv_map.p + first : nullptr;
+    if (V) k_list_lens_g<<<(V + 255) / 256, 256, 0, s>>>(stb, V, t, g->lens.p, map);
     std::vector<uint32_t> lens(V), offs(V + 1);
     if (V) GEQ_HIP(hipMemcpyAsync(lens.data(), g->lens.p, 4ull * V, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -1064,7 +1160,7 @@ geq_status download_lists(geq_game* g, uint32_t first, uint32_t V, uint32_t t, g
     if (!g->offs.p || g->offs.n < V + 1ull) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
     if (!g->out.p || g->out.n < tot * esz) GEQ_HIP(g->out.alloc(std::max<uint64_t>(tot * esz, 1)), GEQ_ERR_OUT_OF_MEMORY);
     GEQ_HIP(hipMemcpyAsync(g->offs.p, offs.data(), 4ull * (V + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    if (V) k_gather_g<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, g->garena.p, V, t, esz, g->offs.p, g->out.p);
+    if (V) k_gather_g<<<(V + 255) / 256, 256, 0, s>>>(stb, g->garena.p, V, t, esz, g->offs.p, g->out.p, map);
     uint32_t* data = (uint32_t*)std::malloc(4ull * std::max<uint64_t>(tot * esz, 1));
     uint32_t* ooffs = (uint32_t*)std::malloc(4ull * (V + 1));
     if (!data || !ooffs) {
@@ -1115,13 +1211,16 @@ geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_e
     // lengths -> offsets (device scan) -> lists gathered, ascending; offsets and
     // lists come back through the game's pinned staging (two waits)
     if (!g->offs.p || g->offs.n < V + 1ull) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
-    if (V) k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, V, t, g->lens.p);
+    // renumbered games: output entry v is internal node inv_map[first + v]
+    const uint4* stb = g->inv_map.p ? g->st.p : g->st.p + (size_t)NODE_U4 * first;
+    const uint32_t* map = g->inv_map.p ? g->inv_map.p + first : nullptr;
+    if (V) k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(stb, V, t, g->lens.p, map);
     geq_status sst = exclusive_scan_dev<uint32_t, SumU32>(g->lens.p, g->offs.p, V, g->scan_tmp, s, SumU32{});
     if (sst != GEQ_OK) return sst;
     // room for the gathered lists: every list is inline (<= INLINE_CAP) or in the arena
     const uint64_t bound = (uint64_t)INLINE_CAP * V + std::min<uint64_t>((uint64_t)g->h_ctl->arena_top, (uint64_t)g->arena_cap);
     if (!g->out.p || g->out.n < bound) GEQ_HIP(g->out.alloc(std::max<uint64_t>(bound, 1)), GEQ_ERR_OUT_OF_MEMORY);
-    if (V) k_gather<<<(V + 255) / 256, 256, 0, s>>>(g->st.p + (size_t)NODE_U4 * first, g->arena.p, V, t, g->offs.p, g->out.p, bound);
+    if (V) k_gather<<<(V + 255) / 256, 256, 0, s>>>(stb, g->arena.p, V, t, g->offs.p, g->out.p, bound, map);
     // offsets and lists written by one copy kernel into mapped pinned memory
     // (room for the bound), then one wait
     const size_t words = V + 1ull + bound;

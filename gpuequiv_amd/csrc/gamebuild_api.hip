@@ -181,6 +181,7 @@ geq_status geq::alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_gam
         return oom(e, "pinned control block");
     *gm->h_ctl = Ctl{};
     if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
+    gm->tune.ordered = worklist_order_default();
     *out = gm;
     return GEQ_OK;
 }
@@ -228,6 +229,7 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
         gm->game_base = {0, V};
         gm->tune = geq_game::Tune{};
         if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
+        gm->tune.ordered = worklist_order_default();
         gm->std_ready = false;
         gm->std_n = 0;
         gm->reach_limit = 0xffffffffu;

@@ -429,18 +429,18 @@ template __global__ void k_eval_list<4>(GenArgs, uint32_t);
 template __global__ void k_eval_list<32>(GenArgs, uint32_t);
 
 // t: tag of the round after the last one that ran
-__global__ void k_list_lens_g(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens) {
+__global__ void k_list_lens_g(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens, const uint32_t* map) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) lens[v] = load_cur(st, v, t).x & GLEN_MASK;
+    if (v < n) lens[v] = load_cur(st, map ? map[v] : v, t).x & GLEN_MASK;
 }
 
 // Every node's current list, esz words per energy, at offs[v] (in energies);
 // the host puts each list in ascending order.
 __global__ void k_gather_g(const uint4* st, const uint32_t* arena, uint32_t n, uint32_t t, uint32_t esz, const uint32_t* offs,
-                           uint32_t* out) {
+                           uint32_t* out, const uint32_t* map) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const uint4 d = load_cur(st, v, t);
+    const uint4 d = load_cur(st, map ? map[v] : v, t);
     const uint64_t words = (uint64_t)(d.x & GLEN_MASK) * esz;
     uint32_t* o = out + (size_t)offs[v] * esz;
     for (uint64_t k = 0; k < words; k++) o[k] = arena[d.y + k];

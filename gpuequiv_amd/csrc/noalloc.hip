@@ -280,7 +280,7 @@ extern "C" geq_status geq_solve_noalloc(geq_ctx* ctx, const geq_graph* g, const 
     // ---- output: list lengths scanned into offsets, lists gathered into the
     // regions the solve no longer needs (from rcol on), ascending per node
     const uint32_t t = round_tag(h.round);
-    k_list_lens<<<(V + tb - 1) / tb, tb, 0, s>>>(nodes, V, t, offs);
+    k_list_lens<<<(V + tb - 1) / tb, tb, 0, s>>>(nodes, V, t, offs, nullptr);
     k_scan_one_block<<<1, 1024, 0, s>>>(offs, V);
     GEQ_HIP(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemcpyAsync(out_offsets, offs, 4ull * (V + 1), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
@@ -294,7 +294,7 @@ extern "C" geq_status geq_solve_noalloc(geq_ctx* ctx, const geq_graph* g, const 
         return fail(GEQ_ERR_INVALID_ARG, "out_data holds " + std::to_string(out_cap) + " energies, the result has " + std::to_string(tot) +
                                              " (*out_n)");
     uint32_t* stage = rcol;
-    k_gather<<<(V + tb - 1) / tb, tb, 0, s>>>(nodes, arena, V, t, offs, stage, staging);
+    k_gather<<<(V + tb - 1) / tb, tb, 0, s>>>(nodes, arena, V, t, offs, stage, staging, nullptr);
     if (tot) GEQ_HIP(hipMemcpyAsync(out_data, stage, 4ull * tot, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     if (ci.ebits != 2)

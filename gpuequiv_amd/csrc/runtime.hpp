@@ -103,7 +103,13 @@ struct RoundShape {
     uint32_t eval_blocks, expand_blocks, rounds_per_launch;
     uint32_t threads;                 // block size of k_eval / k_expand
     bool fused;                       // k_eval expands too: one kernel per round
+    bool ordered;                     // (fused only) k_collect lists the next round in id order
+    uint32_t collect_blocks;
 };
+// games from this many positions list their rounds in id order by default
+constexpr uint32_t ORDERED_MIN_V = 1u << 20;
+// the worklist order of a new game: GEQ_WORKLIST_ORDER (0 / 1), else by size
+uint32_t worklist_order_default();
 RoundShape round_shape(const geq_game* g);
 uint32_t eval_blocks_resident(bool fused, uint32_t waves_per_block);
 // list kernels: the k_eval_list instantiation for esz-word energies, its
@@ -185,12 +191,20 @@ struct geq_game {
     uint32_t arena_cap = 0;
     uint32_t retag_round = 0;         // round of the last k_retag (or of the initialisation)
     std::vector<uint8_t> h_atk;       // kept for standard_reach / reach validation
+    // Positions renumbered at upload (reverse breadth-first order from the
+    // defender dead ends, per game): internal id = inv[caller's id]; empty when
+    // the ids are the caller's.  inv_map is the same map on the device.
+    std::vector<uint32_t> inv;
+    geq::DevBuf<uint32_t> inv_map;
     struct Tune {                     // launch shape, set by the autotuner or defaults
         uint32_t mode = 1;            // 1: hipGraph of captured rounds (0, the cooperative launch, is switched off)
         uint32_t eval_blocks_per_cu = 8;     // capped at the blocks a CU holds at once
         uint32_t expand_blocks_per_cu = 0;   // 0: the expansion runs inside k_eval
         uint32_t rounds_per_launch = 8;
         uint32_t waves_per_block = 8;     // k_eval / k_expand workgroup: 1..8 waves
+        // worklist order: 0 append order, 1 id order (k_collect after each
+        // k_eval), 2 id order for games of at least ORDERED_MIN_V positions
+        uint32_t ordered = 2;
     } tune;
     std::vector<uint32_t> game_base;      // batched games: first node of every game (+ total)
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds

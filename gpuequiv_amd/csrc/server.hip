@@ -438,6 +438,9 @@ geq_status geq_server_create(geq_ctx* ctx, const geq_server_opts* opts, geq_serv
     }
     S->pool->tune.rounds_per_launch = S->rounds_per_step;
     S->pool->tune.mode = 1;
+    // append-order worklists: the pool's bitmaps span every slot of the pool,
+    // most of them free, so listing a round from its bitmap would read it whole
+    S->pool->tune.ordered = 0;
     S->arena_min = S->pool->arena_cap;
     auto oom = [&](hipError_t e) {
         delete S;

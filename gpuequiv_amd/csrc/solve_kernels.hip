@@ -246,7 +246,7 @@ __device__ __forceinline__ bool eval_node(const EvalArgs& a, uint32_t r, uint32_
 // *out_n, the rest waits for flush_appends.  off / rs: 64-entry LDS scratch.
 __device__ __forceinline__ void expand_nodes(const uint32_t* __restrict__ rcol, uint32_t* q, uint32_t p0, uint32_t pc, uint32_t lane,
                                              uint32_t* off, uint32_t* rs, uint32_t* stage, uint32_t& cnt, uint32_t* out,
-                                             uint32_t* out_n) {
+                                             uint32_t* out_n, bool bits_only = false) {
     const uint32_t incl = wave_incl_scan(pc, lane);
     const uint32_t total = lane_val(incl, 63);
     if (total == 0) return;
@@ -267,6 +267,7 @@ __device__ __forceinline__ void expand_nodes(const uint32_t* __restrict__ rcol, 
             const uint32_t bit = 1u << (w & 31u);
             win = (atomicOr(&q[w >> 5], bit) & bit) == 0u;
         }
+        if (bits_only) continue;   // k_collect lists the queued nodes after the round
         const uint64_t bal = __ballot(win);
         if (win) stage[cnt + mbcnt(bal)] = w;
         cnt += (uint32_t)__popcll(bal);
@@ -431,7 +432,8 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
         }
         if (FUSED) {
             wave_sync();   // ex / rs of the last run are reused as expansion scratch
-            expand_nodes(a.rcol, a.queued + ((r + 1u) & 1u) * a.qwords, p0, chg ? pc : 0u, lane, ex, rs, stage, app, out, out_n);
+            expand_nodes(a.rcol, a.queued + ((r + 1u) & 1u) * a.qwords, p0, chg ? pc : 0u, lane, ex, rs, stage, app, out, out_n,
+                         a.ordered != 0u);
         }
     }
     if (FUSED) flush_appends(stage, app, s_cnt, out, out_n);
@@ -494,6 +496,48 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_expand(ExpandArgs a, uint32_t k) 
     flush_appends(s_stage[wid], cnt, s_cnt, out, out_n);
 }
 
+// After round r's evaluations (ordered worklists): the nodes queued for round
+// r + 1 (set bits of bitmap (r + 1) & 1) become its worklist in id order.
+// Block b lists the nodes of bitmap words [b * W, (b + 1) * W), W =
+// COLLECT_THREADS * COLLECT_WORDS, with one atomic for the block's room.
+__global__ __launch_bounds__(COLLECT_THREADS) void k_collect(CollectArgs a, uint32_t k) {
+    __shared__ uint32_t s_w[COLLECT_THREADS / 64 + 1];
+    const RoundCtl rc = read_round(a.ctl, k);
+    if (rc.ovf) return;
+    const uint32_t tr = rc.r + 1u;
+    const uint32_t* __restrict__ q = a.queued + (tr & 1u) * a.qwords;
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    const size_t w0 = ((size_t)blockIdx.x * COLLECT_THREADS + threadIdx.x) * COLLECT_WORDS;
+    uint32_t bits[COLLECT_WORDS];
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < COLLECT_WORDS; i++) {
+        bits[i] = w0 + i < a.qwords ? q[w0 + i] : 0u;
+        c += (uint32_t)__popc(bits[i]);
+    }
+    const uint32_t incl = wave_incl_scan(c, lane);
+    if (lane == 63u) s_w[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < COLLECT_THREADS / 64; w++) tot += s_w[w];
+        s_w[COLLECT_THREADS / 64] = tot ? atomicAdd(&a.ctl->wl_n[tr % 3u], tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = s_w[COLLECT_THREADS / 64] + incl - c;
+    for (uint32_t w = 0; w < wid; w++) pos += s_w[w];
+    uint32_t* out = a.wl[tr & 1u];
+#pragma unroll
+    for (uint32_t i = 0; i < COLLECT_WORDS; i++) {
+        uint32_t m = bits[i];
+        while (m) {
+            const uint32_t b = (uint32_t)__builtin_ctz(m);
+            m &= m - 1u;
+            out[pos++] = (uint32_t)((w0 + i) * 32u + b);
+        }
+    }
+}
+
 // End of a batch of `rounds` rounds: the round advances past them, or, after
 // an arena overflow, stops at the round that overflowed (it runs again).
 __global__ void k_advance(Ctl* ctl, uint32_t rounds) {
@@ -541,19 +585,19 @@ __global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots,
 }
 
 // t: tag of the round after the last one that ran
-__global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens) {
+__global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens, const uint32_t* map) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) lens[v] = nl_len(load_cur(st, v, t));
+    if (v < n) lens[v] = nl_len(load_cur(st, map ? map[v] : v, t));
 }
 
 // Copy every node's list into the packed output, ascending.  Lists produced
 // by the single-successor defender copy keep the successor's order and are
 // sorted here (insertion sort; such lists are short antichain images).
 __global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out,
-                         uint64_t cap) {
+                         uint64_t cap, const uint32_t* map) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const uint4 d = load_cur(st, v, t);
+    const uint4 d = load_cur(st, map ? map[v] : v, t);
     const uint32_t len = nl_len(d);
     if ((uint64_t)offs[v] + len > cap) return;   // the caller checks the total against cap
     uint32_t* o = out + offs[v];

@@ -53,7 +53,19 @@ struct EvalArgs {
     const uint32_t* rcol;
     uint32_t* queued;               // two bitmaps of qwords words: node queued on the list of a round parity
     uint32_t qwords;
+    uint32_t ordered;               // 1: the expansion only sets queued bits; k_collect builds the next list in id order
 };
+
+// The next round's worklist from its queued bitmap, in id order (each block a
+// contiguous range of bitmap words, one atomic per block).
+struct CollectArgs {
+    uint32_t* wl[2];
+    const uint32_t* queued;
+    uint32_t qwords;
+    Ctl* ctl;
+};
+__global__ void k_collect(CollectArgs a, uint32_t k);
+constexpr uint32_t COLLECT_THREADS = 256, COLLECT_WORDS = 4;   // bitmap words per thread
 
 struct StatAcc {                    // per-lane statistics, flushed once per kernel
     uint64_t evals = 0, edges = 0, bytes = 0, changed = 0;
@@ -108,9 +120,9 @@ struct GenArgs {
 };
 template <int ESZ>
 __global__ void k_eval_list(GenArgs a, uint32_t k);
-__global__ void k_list_lens_g(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens);
+__global__ void k_list_lens_g(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens, const uint32_t* map);
 __global__ void k_gather_g(const uint4* st, const uint32_t* arena, uint32_t n, uint32_t t, uint32_t esz, const uint32_t* offs,
-                           uint32_t* out);
+                           uint32_t* out, const uint32_t* map);
 
 // The round of a kernel at position k of a batch, its worklist size (0 after
 // an overflow) and the overflow flag, from one scalar load of the control
@@ -154,10 +166,11 @@ __global__ void k_advance(Ctl* ctl, uint32_t rounds);
 __global__ void k_retag(uint4* st, uint32_t n, uint32_t t);
 __global__ void k_node_hdr(const uint32_t* row, const uint32_t* rrow, const uint8_t* atk, uint32_t n, uint4* st);
 __global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots, unsigned long long* out);
-__global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens);
+// Output download: entry v of the output is node map[v] (node v when map is NULL).
+__global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens, const uint32_t* map);
 // out holds cap words; a list that would pass it is skipped (the caller has
 // checked offs[n] <= cap)
 __global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out,
-                         uint64_t cap);
+                         uint64_t cap, const uint32_t* map);
 
 }  // namespace geq

@@ -103,3 +103,19 @@ def test_set_tune_rejects_out_of_range():
     with pytest.raises(Error):
         b.set_tune(expand_blocks_per_cu=65)
     b.close()
+
+
+def test_batch_in_id_order_with_explicit_reach(monkeypatch):
+    # a batch whose rounds are listed from the queued bitmap, seeded with
+    # explicit per-game reach lists (mapped through the upload's renumbering)
+    monkeypatch.setenv("GEQ_WORKLIST_ORDER", "1")
+    print("batch, id-ordered rounds, explicit reach", flush=True)
+    games = [workloads.random_game(int(v), 300 + i) for i, v in enumerate([800, 3000, 120, 9000])]
+    reach = [np.array(H.standard_reach(g), np.uint32) for g in games]
+    b = GameBatch(games, reach=reach)
+    b.solve()
+    outs = b.raw_energies()
+    for g, r, (o, d) in zip(games, reach, outs):
+        offs, data, _ = H.oracle_solve_game(g, r.tolist())
+        assert H.canon_lists(o, d) == H.canon_lists(offs, data)
+    b.close()

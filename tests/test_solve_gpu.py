@@ -342,3 +342,21 @@ def test_json_example_game_matches_oracle():
     game, out = _run(g)
     assert _canon(out) == H.canon_lists(offs, data)
     assert game.last_stats["rounds"] == st["rounds"]
+
+
+@pytest.mark.parametrize("order,renumber", [("1", "1"), ("0", "0"), ("1", "0")])
+def test_worklist_order_and_renumbering_keep_results(monkeypatch, order, renumber):
+    # Rounds listed in id order from the queued bitmap (GEQ_WORKLIST_ORDER=1,
+    # the default from 2^20 positions) or in append order (0), positions
+    # renumbered in reverse breadth-first order at upload (the default) or not
+    # (GEQ_RENUMBER=0): the same sets, and the same Jacobi schedule counters.
+    monkeypatch.setenv("GEQ_WORKLIST_ORDER", order)
+    monkeypatch.setenv("GEQ_RENUMBER", renumber)
+    for V, seed in [(5000, 21), (30000, 22)]:
+        print("order", order, "renumber", renumber, "game", V, seed, flush=True)
+        g = workloads.random_game(V, seed)
+        offs, data, st = H.oracle_solve_game(g, H.standard_reach(g))
+        game, out = _run(g)
+        assert _canon(out) == H.canon_lists(offs, data)
+        for k in ("rounds", "node_evals", "edge_updates", "alg_bytes"):
+            assert game.last_stats[k] == st[k], k
