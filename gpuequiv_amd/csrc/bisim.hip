@@ -4,19 +4,19 @@
 //
 //   signature(s) = the sorted set of (label, class of target) over s's transitions
 //
-// Per round: keys (label, class[target]) per transition; a segmented radix
-// sort orders every state's keys; one thread per state drops duplicates and
+// Per round: keys (label, class[target]) per transition; a segmented sort
+// orders every state's keys (one thread per short row, one block per long row); one thread per state drops duplicates and
 // hashes the run; a hash table with full comparison of the runs finds, for
 // every state, the first state with the same signature (atomicMin); a scan of
 // the "I am first" flags numbers the classes in order of first occurrence,
 // exactly as the reference's sigmap does.  Rounds repeat until the class count
 // stops changing.
-#include <hipcub/hipcub.hpp>
-
 #include <string>
 #include <vector>
 
 #include "runtime.hpp"
+#include "scan.hpp"
+#include "sort.hpp"
 
 namespace geq {
 void quotient_lts(const geq_lts* lts, const uint32_t* partition, uint32_t c, uint32_t* out_row, uint32_t* out_targets,
@@ -122,10 +122,10 @@ geq_status refine(geq_ctx* ctx, const geq_lts* lts, uint32_t* partition, uint32_
     const uint32_t n = lts->n_states, m = lts->n_transitions;
     hipStream_t s = ctx->stream;
     const uint32_t tb = 256;
-    DevBuf<uint32_t> d_row, d_dst, d_part, d_len, d_hash, d_keymin, d_slot, d_rep, d_flag, d_cls;
+    DevBuf<uint32_t> d_row, d_dst, d_part, d_len, d_hash, d_keymin, d_slot, d_rep, d_flag, d_cls, scan_tmp;
     DevBuf<int32_t> d_lbl;
-    DevBuf<unsigned long long> keys_a, keys_b, table;
-    DevBuf<uint8_t> tmp;
+    DevBuf<unsigned long long> keys, table;
+    SegSortPlan plan;
     uint32_t cap = 16;
     while (cap < 2 * n + 2) cap <<= 1;
     hipError_t e;
@@ -133,7 +133,7 @@ geq_status refine(geq_ctx* ctx, const geq_lts* lts, uint32_t* partition, uint32_
         (e = d_lbl.alloc(std::max(m, 1u))) != hipSuccess || (e = d_part.alloc(n)) != hipSuccess || (e = d_len.alloc(n)) != hipSuccess ||
         (e = d_hash.alloc(n)) != hipSuccess || (e = d_slot.alloc(n)) != hipSuccess || (e = d_rep.alloc(n)) != hipSuccess ||
         (e = d_flag.alloc(n + 1)) != hipSuccess || (e = d_cls.alloc(n + 1)) != hipSuccess ||
-        (e = keys_a.alloc(std::max(m, 1u))) != hipSuccess || (e = keys_b.alloc(std::max(m, 1u))) != hipSuccess ||
+        (e = keys.alloc(std::max(m, 1u))) != hipSuccess ||
         (e = table.alloc(cap)) != hipSuccess || (e = d_keymin.alloc(cap)) != hipSuccess)
         return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("bisimulation buffers: ") + hipGetErrorString(e));
     GEQ_HIP(hipMemcpyAsync(d_row.p, lts->row_offsets, 4ull * (n + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
@@ -142,30 +142,24 @@ geq_status refine(geq_ctx* ctx, const geq_lts* lts, uint32_t* partition, uint32_
         GEQ_HIP(hipMemcpyAsync(d_lbl.p, lts->labels, 4ull * m, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
     k_fill32<<<(n + tb - 1) / tb, tb, 0, s>>>(d_part.p, n, 0u);
-    size_t sort_bytes = 0, scan_bytes = 0;
-    GEQ_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, sort_bytes, keys_a.p, keys_b.p, (int)m, (int)n, d_row.p,
-                                                      d_row.p + 1, 0, 64, s),
-            GEQ_ERR_INTERNAL);
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, d_flag.p, d_cls.p, n + 1, s), GEQ_ERR_INTERNAL);
-    GEQ_HIP(tmp.alloc(std::max(sort_bytes, scan_bytes) + 16), GEQ_ERR_OUT_OF_MEMORY);
+    geq_status pst = seg_sort_plan(lts->row_offsets, n, plan, s);
+    if (pst != GEQ_OK) return pst;
     uint32_t prev = 0, cnt = 1, r = 0;
     while (prev != cnt) {
         prev = cnt;
         r++;
         if (m) {
-            k_sig_keys<<<(m + tb - 1) / tb, tb, 0, s>>>(d_dst.p, d_lbl.p, d_part.p, m, keys_a.p);
-            size_t b = sort_bytes;
-            GEQ_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, b, keys_a.p, keys_b.p, (int)m, (int)n, d_row.p, d_row.p + 1,
-                                                              0, 64, s),
-                    GEQ_ERR_INTERNAL);
+            k_sig_keys<<<(m + tb - 1) / tb, tb, 0, s>>>(d_dst.p, d_lbl.p, d_part.p, m, keys.p);
+            geq_status sst = seg_sort_u64(d_row.p, n, keys.p, plan, s);
+            if (sst != GEQ_OK) return sst;
         }
-        k_sig_dedup<<<(n + tb - 1) / tb, tb, 0, s>>>(d_row.p, keys_b.p, n, d_len.p, d_hash.p);
+        k_sig_dedup<<<(n + tb - 1) / tb, tb, 0, s>>>(d_row.p, keys.p, n, d_len.p, d_hash.p);
         k_fill64<<<(cap + tb - 1) / tb, tb, 0, s>>>(table.p, cap, SIG_EMPTY);
         k_fill32<<<(cap + tb - 1) / tb, tb, 0, s>>>(d_keymin.p, cap, 0xffffffffu);
-        k_sig_insert<<<(n + tb - 1) / tb, tb, 0, s>>>(d_row.p, keys_b.p, d_len.p, d_hash.p, n, table.p, d_keymin.p, cap - 1, d_slot.p);
+        k_sig_insert<<<(n + tb - 1) / tb, tb, 0, s>>>(d_row.p, keys.p, d_len.p, d_hash.p, n, table.p, d_keymin.p, cap - 1, d_slot.p);
         k_sig_first<<<(n + 1 + tb - 1) / tb, tb, 0, s>>>(d_slot.p, d_keymin.p, n, d_rep.p, d_flag.p);
-        size_t b2 = scan_bytes;
-        GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, b2, d_flag.p, d_cls.p, n + 1, s), GEQ_ERR_INTERNAL);
+        geq_status cst = exclusive_scan_dev<uint32_t, SumU32>(d_flag.p, d_cls.p, n, scan_tmp, s, SumU32{});
+        if (cst != GEQ_OK) return cst;
         k_sig_assign<<<(n + tb - 1) / tb, tb, 0, s>>>(d_rep.p, d_cls.p, n, d_part.p);
         GEQ_HIP(hipMemcpyAsync(&cnt, d_cls.p + n, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);

@@ -20,8 +20,6 @@
 //                   order, which is the reference's numbering;
 //   6. k_finalize / k_edges - node records, hash entries, CSR rows, columns,
 //                   weights.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <cstring>
 #include <string>

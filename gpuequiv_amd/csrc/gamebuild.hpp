@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "runtime.hpp"
+#include "sort.hpp"
 
 namespace geq {
 namespace gb {
@@ -128,7 +129,10 @@ struct BuildScratch {
     DevBuf<uint32_t> hv_list, hv_count;
     HostWords rb;
     DevBuf<int32_t> l_lbl, l_enlbl;
-    DevBuf<uint8_t> d_insert, tmp, d_dead;
+    DevBuf<uint8_t> d_insert, d_dead;
+    DevBuf<uint32_t> scan32;          // block totals of the device scans
+    DevBuf<uint2> scan2;
+    RadixScratch sort;                // job extraction's sort
     DevBuf<Cand> cands;
     DevBuf<unsigned long long> table;
     DevBuf<uint32_t> job_of_state, key, key_s, val, order, dsort, escan, local, d_ps, d_es;

@@ -1,7 +1,5 @@
 // Host driver of the device gamebuild (level loop, capacities) and the C ABI
 // entry points geq_gamebuild / geq_built_* of include/gpuequiv.h.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -57,14 +55,9 @@ geq_status ensure(DevBuf<T>& b, size_t need, hipStream_t s, bool keep, const cha
     return GEQ_OK;
 }
 
-template <class T>
-geq_status exclusive_scan(const T* in, T* out, size_t n, DevBuf<uint8_t>& tmp, hipStream_t s) {
-    size_t bytes = 0;
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s), GEQ_ERR_INTERNAL);
-    geq_status st = ensure(tmp, bytes + 16, s, false, "scan scratch");
-    if (st != GEQ_OK) return st;
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, n, s), GEQ_ERR_INTERNAL);
-    return GEQ_OK;
+// Exclusive scan of in[0..n) into out[0..n); out[n] = the total.
+geq_status exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, DevBuf<uint32_t>& tmp, hipStream_t s) {
+    return exclusive_scan_dev<uint32_t, SumU32>(in, out, n, tmp, s, SumU32{});
 }
 
 // Sum of (candidates, set-pool words) bound pairs, saturating at `lim`
@@ -78,6 +71,7 @@ struct PairSum {
         const uint64_t x = (uint64_t)a.x + b.x, y = (uint64_t)a.y + b.y;
         return make_uint2(x < lim ? (uint32_t)x : lim, y < lim ? (uint32_t)y : lim);
     }
+    __host__ __device__ static uint2 identity() { return make_uint2(0u, 0u); }
 };
 
 // The saturation point of the bound scans: 2^32 - 1, or GEQ_BUILD_LEVEL_LIMIT
@@ -90,15 +84,10 @@ uint32_t level_limit() {
     return 0xffffffffu;
 }
 
-// Exclusive scan of (candidates, set-pool words) pairs: one scan for both.
-geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<uint8_t>& tmp, hipStream_t s) {
-    size_t bytes = 0;
-    const PairSum op{level_limit()};
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, in, out, op, make_uint2(0u, 0u), n, s), GEQ_ERR_INTERNAL);
-    geq_status st = ensure(tmp, bytes + 16, s, false, "scan scratch");
-    if (st != GEQ_OK) return st;
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveScan(tmp.p, bytes, in, out, op, make_uint2(0u, 0u), n, s), GEQ_ERR_INTERNAL);
-    return GEQ_OK;
+// Exclusive scan of (candidates, set-pool words) pairs, one scan for both:
+// out[0..n) and out[n] = the totals.
+geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<uint2>& tmp, hipStream_t s) {
+    return exclusive_scan_dev<uint2, PairSum>(in, out, n, tmp, s, PairSum{level_limit()});
 }
 
 // Words of the pinned readback block: a chunk's totals, the heavy positions
@@ -252,7 +241,7 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
     hipError_t e;
     DevBuf<uint32_t>& cnt = gm->rev_cnt;
     DevBuf<uint32_t>& fill = gm->rev_fill;
-    DevBuf<uint8_t>& tmp = gm->rev_tmp;
+    DevBuf<uint32_t>& tmp = gm->rev_tmp;
     if (cnt.n < V + 1ull && (e = cnt.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");
     if (fill.n < V + 1ull && (e = fill.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");
     const uint32_t tb = 256;
@@ -261,7 +250,7 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
     if (E) k_edges_u2<<<(E + tb - 1) / tb, tb, 0, s>>>(d_col, d_wt, E, gm->edges.p);
     k_fill_u32<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(cnt.p, V + 1, 0u);
     if (E) k_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(d_col, E, cnt.p);
-    geq_status st = exclusive_scan<uint32_t>(cnt.p, gm->rrow.p, V + 1, tmp, s);
+    geq_status st = exclusive_scan(cnt.p, gm->rrow.p, V, tmp, s);
     if (st != GEQ_OK) return st;
     GEQ_HIP(hipMemcpyAsync(fill.p, gm->rrow.p, 4ull * (V + 1), hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
     k_rev_scatter<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, V, d_col, fill.p, gm->rcol.p);
@@ -368,7 +357,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     auto& l_row = ws.l_row; auto& l_dst = ws.l_dst; auto& l_enrow = ws.l_enrow; auto& d_pairs = ws.d_pairs;
     auto& ub = ws.ub; auto& off = ws.off; auto& nsucc = ws.nsucc;
     auto& rowl = ws.rowl; auto& flag = ws.flag; auto& rank = ws.rank; auto& keymin = ws.keymin;
-    auto& l_lbl = ws.l_lbl; auto& l_enlbl = ws.l_enlbl; auto& d_insert = ws.d_insert; auto& tmp = ws.tmp;
+    auto& l_lbl = ws.l_lbl; auto& l_enlbl = ws.l_enlbl; auto& d_insert = ws.d_insert;
     auto& cands = ws.cands; auto& table = ws.table;
     geq_status st = GEQ_OK;
     auto bail = [&](geq_status e) { return e; };
@@ -551,7 +540,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             GB_TRY(ensure(ws.hv_list, std::max<uint32_t>(nf, 1), s, false, "heavy positions"));
             GB_HIP(hipMemsetAsync(ws.hv_count.p, 0, 4, s), GEQ_ERR_TRANSFER);
             k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub.p, heavy_args());
-            GB_TRY(exclusive_scan_pairs(ub.p, off.p, nf + 1, tmp, s));
+            GB_TRY(exclusive_scan_pairs(ub.p, off.p, nf, ws.scan2, s));
             k_readback<<<1, 64, 0, s>>>(&off.p[nf].x, &off.p[nf].y, ws.hv_count.p, ws.rb.d);
             GB_TRY(wait());
             n_cand = h_rb[RB_CAND];
@@ -577,14 +566,14 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (n_heavy)   // k_bounds listed them (their count came back with the totals)
             k_generate_heavy<<<std::min<uint32_t>(n_heavy, 1024), 256, 0, s>>>(nodes(), lo, L, off.p, B->spool_top, heavy_args(),
                                                                           cands.p, nsucc.p);
-        GB_TRY(exclusive_scan<uint32_t>(nsucc.p, rowl.p, nf + 1, tmp, s));
+        GB_TRY(exclusive_scan(nsucc.p, rowl.p, nf, ws.scan32, s));
         if (n_cand) {
             k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
             k_mark<<<(n_cand + 1 + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, keymin.p, flag.p);
         } else {
             GB_HIP(hipMemsetAsync(flag.p, 0, 4, s), GEQ_ERR_TRANSFER);
         }
-        GB_TRY(exclusive_scan<uint32_t>(flag.p, rank.p, n_cand + 1, tmp, s));
+        GB_TRY(exclusive_scan(flag.p, rank.p, n_cand, ws.scan32, s));
         if (n_cand) {
             k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, flag.p, rank.p, B->V, nodes(), table.p, keymin.p);
             k_edges<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p, B->wt.p);
@@ -610,7 +599,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             GB_HIP(hipMemsetAsync(ws.hv_count.p, 0, 4, s), GEQ_ERR_TRANSFER);
             k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots,
                                                                            chunk, L, ub.p, heavy_args());
-            GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots, tmp, s));
+            GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots - 1, ws.scan2, s));
         }
         // one synchronisation per chunk for the edge and position counts, the per-job
         // counts and the next level's first totals
@@ -775,7 +764,7 @@ geq_status geq::gamebuild_jobs(geq_ctx* ctx, const std::vector<geq_lts>& ltss, c
     hipStream_t s = ctx->stream;
     // stable sort of the positions by job: order[i] = merged id of the i-th position in job-major order
     auto& key = ws.key; auto& key_s = ws.key_s; auto& val = ws.val; auto& order = ws.order; auto& dsort = ws.dsort;
-    auto& escan = ws.escan; auto& local = ws.local; auto& d_ps = ws.d_ps; auto& d_es = ws.d_es; auto& tmp = ws.tmp;
+    auto& escan = ws.escan; auto& local = ws.local; auto& d_ps = ws.d_ps; auto& d_es = ws.d_es;
     for (auto* b : {&key, &key_s, &val, &order, &local})
         if ((st = ensure(*b, V, s, false, "job extraction")) != GEQ_OK) return st;
     for (auto* b : {&dsort, &escan})
@@ -785,12 +774,11 @@ geq_status geq::gamebuild_jobs(geq_ctx* ctx, const std::vector<geq_lts>& ltss, c
     k_pos_job<<<(V + tb - 1) / tb, tb, 0, s>>>(B->p.p, V, ws.job_of_state.p, key.p, val.p);
     int end_bit = 1;
     while ((1u << end_bit) < J) end_bit++;
-    size_t bytes = 0;
-    GEQ_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, key_s.p, val.p, order.p, V, 0, end_bit, s), GEQ_ERR_INTERNAL);
-    if ((st = ensure(tmp, bytes + 16, s, false, "sort scratch")) != GEQ_OK) return st;
-    GEQ_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, key_s.p, val.p, order.p, V, 0, end_bit, s), GEQ_ERR_INTERNAL);
+    // dsort and local are free until after the sort: they hold its middle passes
+    if ((st = radix_sort_pairs(key.p, val.p, key_s.p, order.p, dsort.p, local.p, V, (uint32_t)end_bit, ws.sort, s)) != GEQ_OK)
+        return st;
     k_sorted_deg<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(order.p, B->row.p, V, dsort.p);
-    st = exclusive_scan<uint32_t>(dsort.p, escan.p, V + 1, tmp, s);
+    st = exclusive_scan(dsort.p, escan.p, V, ws.scan32, s);
     if (st != GEQ_OK) return st;
     out->pos_start.assign(J + 1, 0);
     out->n_pos.assign(J, 0);

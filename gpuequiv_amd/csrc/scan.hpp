@@ -1,10 +1,14 @@
 // Device-wide exclusive scan (reduce-then-scan, three launches): used for the
-// energy download's offsets and for gamebuild's per-level offsets.
+// energy download's offsets, gamebuild's per-level offsets, the builder's and
+// the server's reverse CSRs, and the radix sort's digit offsets.
 //   k_scan_reduce : every block of SCAN_ITEMS consecutive items -> its total
 //   k_scan_tops   : one block scans the block totals in place (and the grand total)
 //   k_scan_apply  : every block scans its items again from its block's offset
-// Wave scans are lane shuffles; the combine operator is a template (plain sums,
-// or the saturating pair sums of the gamebuild bounds).
+// Global memory is read and written striped (lane-consecutive, coalesced); the
+// apply step transposes its tile through LDS so that each thread scans
+// SCAN_PER_THREAD consecutive items.  The combine operator is a template
+// (plain sums, or the saturating pair sums of the gamebuild bounds); it must
+// be associative and commutative (the block totals combine in striped order).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -67,10 +71,13 @@ __device__ __forceinline__ T block_scan_excl(T v, Op op, T& total) {
 
 template <class T, class Op>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const T* in, size_t n, T* tops, Op op) {
-    const size_t b0 = (size_t)blockIdx.x * SCAN_ITEMS + (size_t)threadIdx.x * SCAN_PER_THREAD;
+    const size_t b0 = (size_t)blockIdx.x * SCAN_ITEMS + threadIdx.x;
     T acc = Op::identity();
-    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++)
-        if (b0 + k < n) acc = op(acc, in[b0 + k]);
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
+        const size_t i = b0 + (size_t)k * SCAN_THREADS;
+        if (i < n) acc = op(acc, in[i]);
+    }
     T total;
     (void)detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total);
     if (threadIdx.x == 0) tops[blockIdx.x] = total;
@@ -93,25 +100,44 @@ __global__ __launch_bounds__(SCAN_TOP_THREADS) void k_scan_tops(T* tops, uint32_
     if (threadIdx.x == 0) tops[nb] = total;
 }
 
+// LDS slot of tile item i: one pad slot per 32 items spreads the per-thread
+// runs of SCAN_PER_THREAD consecutive items over the banks
+__device__ __forceinline__ uint32_t scan_slot(uint32_t i) { return i + (i >> 5); }
+
 template <class T, class Op>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T* in, T* out, size_t n, const T* tops, Op op) {
-    const size_t b0 = (size_t)blockIdx.x * SCAN_ITEMS + (size_t)threadIdx.x * SCAN_PER_THREAD;
+    __shared__ T tile[SCAN_ITEMS + SCAN_ITEMS / 32];
+    const size_t base = (size_t)blockIdx.x * SCAN_ITEMS;
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
+        const uint32_t j = k * SCAN_THREADS + t;
+        tile[scan_slot(j)] = base + j < n ? in[base + j] : Op::identity();
+    }
+    __syncthreads();
     T v[SCAN_PER_THREAD];
     T acc = Op::identity();
 #pragma unroll
     for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
-        v[k] = b0 + k < n ? in[b0 + k] : Op::identity();
+        v[k] = tile[scan_slot(t * SCAN_PER_THREAD + k)];
         acc = op(acc, v[k]);
     }
     T total;
     T run = op(tops[blockIdx.x], detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total));
+    // each thread rewrites only the slots it read; the barrier inside the block
+    // scan has ordered every read of the tile before these writes
 #pragma unroll
-    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++)
-        if (b0 + k < n) {
-            out[b0 + k] = run;
-            run = op(run, v[k]);
-        }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tops[gridDim.x];   // the grand total
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
+        tile[scan_slot(t * SCAN_PER_THREAD + k)] = run;
+        run = op(run, v[k]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
+        const uint32_t j = k * SCAN_THREADS + t;
+        if (base + j < n) out[base + j] = tile[scan_slot(j)];
+    }
+    if (blockIdx.x == gridDim.x - 1 && t == 0) out[n] = tops[gridDim.x];   // the grand total
 }
 
 // Exclusive scan of in[0..n) into out[0..n), the total into out[n] (out holds

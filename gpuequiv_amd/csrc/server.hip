@@ -12,8 +12,6 @@
 //
 // There is no counterpart in the reference, which solves one game per
 // TransitionSystem::compare / equivalences call (src/lib.rs:275-343).
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -211,7 +209,8 @@ struct geq_server {
     std::vector<geq_ctx*> bctx;        // builders' contexts: a stream each
     geq_game* pool = nullptr;          // pools of positions / edges / energies
     DevBuf<uint32_t> job_of, seed_cnt, place_cnt, place_fill, place_excl;
-    DevBuf<uint8_t> active, scan_tmp;
+    DevBuf<uint8_t> active;
+    DevBuf<uint32_t> scan_tmp;   // block totals of the placement scans
     DevBuf<uint32_t*> wl_ptrs;
     std::vector<uint8_t> h_active;
     uint32_t max_jobs = 0;
@@ -354,10 +353,7 @@ geq_status compact_arena(geq_server* S) {
     GEQ_TRY(ensure_buf(S->place_excl, n + 1, s));
     const uint32_t t = round_tag(g->h_ctl->round);
     k_arena_need<<<(n + 1 + tb - 1) / tb, tb, 0, s>>>(g->st.p, n, t, S->place_cnt.p);
-    size_t bytes = 0;
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, S->place_cnt.p, S->place_excl.p, n + 1, s), GEQ_ERR_INTERNAL);
-    GEQ_TRY(ensure_buf(S->scan_tmp, bytes + 16, s));
-    GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(S->scan_tmp.p, bytes, S->place_cnt.p, S->place_excl.p, n + 1, s), GEQ_ERR_INTERNAL);
+    GEQ_TRY((exclusive_scan_dev<uint32_t, SumU32>(S->place_cnt.p, S->place_excl.p, n, S->scan_tmp, s, SumU32{})));
     uint32_t live = 0;
     GEQ_HIP(hipMemcpyAsync(&live, S->place_excl.p + n, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -560,10 +556,7 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         GEQ_TRY(ensure_buf(S->place_fill, V + 1, s));
         GEQ_HIP(hipMemsetAsync(S->place_cnt.p, 0, 4ull * (V + 1), s), GEQ_ERR_TRANSFER);
         if (E) k_place_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, E, S->place_cnt.p);
-        size_t bytes = 0;
-        GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, S->place_cnt.p, S->place_excl.p, V + 1, s), GEQ_ERR_INTERNAL);
-        GEQ_TRY(ensure_buf(S->scan_tmp, bytes + 16, s));
-        GEQ_HIP(hipcub::DeviceScan::ExclusiveSum(S->scan_tmp.p, bytes, S->place_cnt.p, S->place_excl.p, V + 1, s), GEQ_ERR_INTERNAL);
+        GEQ_TRY((exclusive_scan_dev<uint32_t, SumU32>(S->place_cnt.p, S->place_excl.p, V, S->scan_tmp, s, SumU32{})));
         k_place_rrow<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(S->place_excl.p, V, J->eb, J->nb, g->rrow.p, S->place_fill.p);
         k_place_rcol<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, bcol, V, J->nb, J->eb, S->place_fill.p, g->rcol.p);
         k_node_hdr<<<(V + tb - 1) / tb, tb, 0, s>>>(g->row.p + J->nb, g->rrow.p + J->nb, g->atk.p + J->nb, V,

@@ -14,6 +14,22 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
+def _hubs():
+    """States whose transition lists take each sorting path of a refinement
+    round: up to 32 keys (one thread), up to 4096 (a block in LDS) and more (a
+    block in global memory), with repeated (label, target) pairs to drop."""
+    rng = np.random.default_rng(7)
+    n = 3000
+    fan = {0: 40, 1: 700, 2: 4096, 3: 4097, 4: 6000, 5: 30000, 6: 33}
+    src, dst, lbl = [], [], []
+    for s in range(n):
+        k = fan.get(s, int(rng.integers(0, 5)))
+        src += [s] * k
+        dst += [int(x) for x in rng.integers(0, n if s > 6 else 200, k)]
+        lbl += [int(x) for x in rng.integers(0, 3, k)]
+    return TransitionSystem.from_arrays(n, src, dst, lbl)
+
+
 def _cases():
     yield "empty transitions", TransitionSystem.from_arrays(5, [], [], [])
     yield "one state loop", TransitionSystem.from_edges([(0, 0, 0)])
@@ -23,6 +39,7 @@ def _cases():
     for seed, (n, m, l) in enumerate([(40, 80, 2), (300, 1200, 8), (2000, 4000, 3), (20000, 60000, 5)]):
         yield f"random {n}/{m}/{l}", TransitionSystem.from_arrays(*workloads.random_lts(n, m, l, 100 + seed))
     yield "power-law 50000", TransitionSystem.from_arrays(*workloads.powerlaw_lts(50000, 250000, 60, 1))
+    yield "hub states", _hubs()
     for f in ("peterson_mutex_weak.csv", "vasy_0_1.csv"):
         yield f, TransitionSystem.from_csv_file(os.path.join(GOLDEN, f))
 

@@ -184,3 +184,33 @@ def test_chunked_levels_and_the_position_cap(monkeypatch, heavy_min):
         assert (info["positions"], info["edges"]) == jobs[k][3], k
         assert H.canon_lists(*r) == jobs[k][2], k
     srv.close()
+
+
+def test_one_build_of_more_than_256_jobs(monkeypatch):
+    """A merged build of 300 jobs: the job extraction sorts positions by a
+    9-bit job id (two radix passes).  256 jobs are built first and left
+    unadmitted, so the builder waits (the ready queue is full) until the
+    first step, and then takes all 300 queued jobs at once."""
+    import time
+    print("serve: one merged build of 300 jobs", flush=True)
+    monkeypatch.setenv("GEQ_SERVER_MAX_BATCH", "1024")
+    fill = TransitionSystem.from_edges([(0, 1, 0), (1, 0, 0)])
+    jobs = _jobs(300, 11, 5, 14, 2, 4)
+    srv = Server(budget_bytes=512 << 20, rounds_per_step=4)
+    first = [srv.submit(fill, [(0, 1)]) for _ in range(256)]
+    deadline = time.time() + 60
+    while any(srv.poll(j, with_result=False)[0]["state_name"] != "built" for j in first):
+        assert time.time() < deadline, "the first 256 builds did not finish"
+        time.sleep(0.01)
+    builds0 = srv.stats()["builds"]
+    ids = [srv.submit(lts, pairs) for lts, pairs, _, _ in jobs]
+    res = srv.run_until_done(first + ids)
+    assert srv.stats()["builds"] == builds0 + 1
+    for j in first:
+        assert res[j][0]["state_name"] == "done"
+    for k, ((lts, pairs, alone, size), j) in enumerate(zip(jobs, ids)):
+        info, r = res[j]
+        assert info["state_name"] == "done", info
+        assert (info["positions"], info["edges"]) == size, k
+        assert H.canon_lists(*r) == alone, k
+    srv.close()
