@@ -135,6 +135,24 @@ def bench_batch(args) -> dict:
     batched_s = (time.perf_counter() - t0) / args.steps
     prof = b.solve(profile=True)
     print(f"bench: batched solve {batched_s*1e3:.2f} ms", file=sys.stderr, flush=True)
+    # the other solve mode on the same batch: replayed round graphs (1) or one
+    # persistent launch with a block per game (3); same lists and counts
+    mode = b.solve_mode()
+    want = b.raw_energies()
+    other = 3 if mode == 1 else 1
+    b.set_solve_mode(other)
+    b.solve()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        so = b.solve()
+    other_s = (time.perf_counter() - t0) / args.steps
+    same = all(np.array_equal(o0, o1) and np.array_equal(d0, d1) for (o0, d0), (o1, d1) in zip(want, b.raw_energies()))
+    same = same and (so["edge_updates"], so["node_evals"], so["rounds"]) == (s["edge_updates"], s["node_evals"], s["rounds"])
+    b.set_solve_mode(mode)
+    modes = {"used": mode, f"mode_{mode}_ms": batched_s * 1e3, f"mode_{other}_ms": other_s * 1e3,
+             "same_lists_and_counts": bool(same),
+             "modes": "1: replayed round graphs over the whole batch; 3: one persistent launch, a block per game"}
+    print(f"bench: solve modes {modes}", file=sys.stderr, flush=True)
 
     if args.no_plain:          # profiling runs: the batched solve alone
         kms = prof["eval_kernel_ms"]
@@ -211,6 +229,7 @@ def bench_batch(args) -> dict:
                        "latency_ms_p50": float(np.percentile(lat_ms, 50)),
                        "latency_ms_p99": float(np.percentile(lat_ms, 99))},
         "parity_sample": {"games_compared": len(sample), "mismatches": mismatches},
+        "solve_modes": modes,
         "autotune": tune,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_eval",
@@ -679,7 +698,7 @@ def main() -> int:
     # serving latency and throughput against the plain path
     if not args.no_batch:
         b = bench_batch(argparse.Namespace(games=2048, seed=4, steps=3, warmup=1, no_tune=True, no_plain=False))
-        line["batch"] = {k: b[k] for k in ("config", "games_per_s", "plain_path", "parity_sample", "ms_per_step")}
+        line["batch"] = {k: b[k] for k in ("config", "games_per_s", "plain_path", "parity_sample", "ms_per_step", "solve_modes")}
         line["batch"]["edge_updates_per_s"] = b["value"]
         line["batch"]["roofline"] = b["roofline"]
         print(f"bench: batch {b['games_per_s']}", file=sys.stderr, flush=True)

@@ -201,6 +201,8 @@ _SIGS = {
     "geq_signature_refinement_gpu": (C.c_int, [C.c_void_p, P(geq_lts), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
     "geq_bisimilar_minimize_gpu": (C.c_int, [C.c_void_p, P(geq_lts), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
                                              P(C.c_uint32), P(C.c_int32), P(C.c_uint32)]),
+    "geq_game_set_solve_mode": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "geq_game_get_solve_mode": (C.c_int, [C.c_void_p, P(C.c_uint32)]),
     "geq_game_get_tune": (C.c_int, [C.c_void_p, P(geq_tune_params)]),
     "geq_game_set_tune": (C.c_int, [C.c_void_p, P(geq_tune_params)]),
     "geq_game_autotune": (C.c_int, [C.c_void_p, P(C.c_uint32), C.c_uint32, C.c_uint32, P(geq_tune_report)]),

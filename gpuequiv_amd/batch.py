@@ -63,6 +63,16 @@ class GameBatch:
                                                  C.byref(rep)))
         return rep.as_dict()
 
+    def set_solve_mode(self, mode: int) -> None:
+        """1: replayed round graphs over the whole batch (default); 3: one
+        persistent launch with a block per game (geq_game_set_solve_mode)."""
+        _lib.check(_lib.lib().geq_game_set_solve_mode(self.handle, int(mode)))
+
+    def solve_mode(self) -> int:
+        m = C.c_uint32()
+        _lib.check(_lib.lib().geq_game_get_solve_mode(self.handle, C.byref(m)))
+        return int(m.value)
+
     def tune(self) -> dict:
         p = _lib.geq_tune_params()
         _lib.check(_lib.lib().geq_game_get_tune(self.handle, C.byref(p)))

@@ -1,7 +1,8 @@
 // Autotuner of the solver's launch parameters: workgroup size (waves per
 // block), grid sizes of k_eval / k_expand (blocks per CU; no k_expand grid:
-// the expansion runs inside k_eval), rounds per replayed hipGraph, and the
-// energy-arena capacity.  Candidates are timed on the game
+// the expansion runs inside k_eval), rounds per replayed hipGraph, the solve
+// mode (replayed rounds, or one persistent launch with a block per game) and
+// the energy-arena capacity.  Candidates are timed on the game
 // itself (median device time of repeated solves), searched coordinate by
 // coordinate from the current parameters.  Every candidate computes the same
 // fixpoint; only the launch shape changes.  No counterpart in the reference,
@@ -129,7 +130,28 @@ geq_status autotune(geq_game* g, const std::vector<uint32_t>* reach, uint32_t re
         return s;
     }
     Tune best = ms_b < ms_a ? best_b : best_a;
-    const double best_ms = std::min(ms_a, ms_b);
+    double best_ms = std::min(ms_a, ms_b);
+    // the persistent launch with a block per game (bitmap layout only): wins
+    // when the games are small next to the whole batch, loses when one game
+    // alone outlasts the rest on its single block
+    if (!g->conf.generic)
+        for (uint32_t w : {4u, 8u}) {
+            Tune c = best;
+            c.mode = SOLVE_MODE_GAMES;
+            c.waves_per_block = w;
+            apply(g, c);
+            double ms = 0;
+            s = measure(g, reach, repeats, &ms, &peak);
+            if (s != GEQ_OK) {
+                apply(g, start);
+                return s;
+            }
+            candidates++;
+            if (ms < 0.97 * best_ms) {
+                best = c;
+                best_ms = ms;
+            }
+        }
     apply(g, best);
     // antichain capacity: the arena holds the largest solve seen without regrowing
     const uint64_t want = peak + peak / 4 + 1024;
@@ -175,6 +197,22 @@ geq_status geq_game_set_tune(geq_game* g, const geq_tune_params* p) {
         geq_status s = grow_arena(g, (uint32_t)std::min<uint64_t>(p->arena_entries, 0xffffffffull));
         if (s != GEQ_OK) return s;
     }
+    return GEQ_OK;
+}
+
+geq_status geq_game_set_solve_mode(geq_game* g, uint32_t mode) {
+    if (!g) return fail(GEQ_ERR_INVALID_ARG, "game is NULL");
+    if (mode != SOLVE_MODE_GRAPH && mode != SOLVE_MODE_GAMES)
+        return fail(GEQ_ERR_INVALID_ARG, "solve mode: 1 (replayed round graphs) or 3 (one persistent launch, a block per game)");
+    if (mode == SOLVE_MODE_GAMES && g->conf.generic)
+        return fail(GEQ_ERR_UNSUPPORTED, "the per-game persistent solve covers the bitmap energy layout only");
+    g->tune.mode = mode;
+    return GEQ_OK;
+}
+
+geq_status geq_game_get_solve_mode(geq_game* g, uint32_t* mode) {
+    if (!g || !mode) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    *mode = g->tune.mode;
     return GEQ_OK;
 }
 

@@ -808,6 +808,83 @@ geq_status maybe_retag(geq_game* g) {
     return GEQ_OK;
 }
 
+geq_status solve_games(geq_game* g, EvalArgs ea, const uint32_t* seed, uint32_t n_reach, uint64_t* launches) {
+    hipStream_t s = g->ctx->stream;
+    const bool batch = g->game_base.size() >= 2;
+    const uint32_t G = batch ? (uint32_t)g->game_base.size() - 1 : 1u;
+    hipError_t e;
+    if (!g->pg_ready) {
+        std::vector<uint32_t> base = batch ? g->game_base : std::vector<uint32_t>{0u, g->V};
+        std::vector<uint32_t> order(G);
+        for (uint32_t i = 0; i < G; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t x, uint32_t y) { return base[x + 1] - base[x] > base[y + 1] - base[y]; });
+        if ((e = g->pg_base.alloc(G + 1)) != hipSuccess || (e = g->pg_order.alloc(G)) != hipSuccess ||
+            (e = g->pg_soff.alloc(G + 1)) != hipSuccess || (e = g->pg_cnt.alloc(G)) != hipSuccess ||
+            (e = g->pg_next.alloc(1)) != hipSuccess || (e = g->pg_ctl.alloc(G)) != hipSuccess)
+            return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("per-game solve state: ") + hipGetErrorString(e));
+        GEQ_HIP(hipMemcpyAsync(g->pg_base.p, base.data(), 4ull * (G + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        GEQ_HIP(hipMemcpyAsync(g->pg_order.p, order.data(), 4ull * G, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+        g->pg_ready = true;
+    }
+    if (n_reach && (!g->pg_seed.p || g->pg_seed.n < n_reach)) {
+        g->pg_seed.release();
+        if ((e = g->pg_seed.alloc(n_reach)) != hipSuccess)
+            return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("per-game seeds: ") + hipGetErrorString(e));
+    }
+    // the seeds grouped by game (in any order within a game: a list's order
+    // changes neither the values nor the counts of a round)
+    const uint32_t tb = 256;
+    GEQ_HIP(hipMemsetAsync(g->pg_cnt.p, 0, 4ull * G, s), GEQ_ERR_TRANSFER);
+    if (n_reach) k_seed_count<<<(n_reach + tb - 1) / tb, tb, 0, s>>>(seed, n_reach, g->pg_base.p, G, g->pg_cnt.p);
+    geq_status st = exclusive_scan_dev<uint32_t, SumU32>(g->pg_cnt.p, g->pg_soff.p, G, g->pg_scan, s, SumU32{});
+    if (st != GEQ_OK) return st;
+    if (n_reach) {
+        GEQ_HIP(hipMemsetAsync(g->pg_cnt.p, 0, 4ull * G, s), GEQ_ERR_TRANSFER);
+        k_seed_scatter<<<(n_reach + tb - 1) / tb, tb, 0, s>>>(seed, n_reach, g->pg_base.p, G, g->pg_soff.p, g->pg_cnt.p,
+                                                              g->pg_seed.p);
+    }
+    static_assert(GAME_NEW == 0, "the per-game state starts zeroed");
+    GEQ_HIP(hipMemsetAsync(g->pg_ctl.p, 0, sizeof(GameCtl) * G, s), GEQ_ERR_TRANSFER);
+    const uint32_t w = std::max(1u, std::min<uint32_t>(g->tune.waves_per_block, MAX_WAVES));
+    const size_t lds = (size_t)w * EVAL_FUSED_LDS_PER_WAVE;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_solve_games), (int)(64 * w), lds) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    // no block waits for another, so the grid need not be resident at once;
+    // more blocks than games would find the queue empty
+    const uint32_t grid = std::max(1u, std::min({G, (uint32_t)std::max(1, g->ctx->n_cu) * (uint32_t)per_cu, STAT_SLOTS}));
+    ea.ordered = 0;   // the expansion appends; no k_collect
+    GamesArgs ga{g->pg_base.p, g->pg_order.p, g->pg_seed.p ? g->pg_seed.p : g->pg_soff.p, g->pg_soff.p, g->pg_ctl.p,
+                 g->pg_next.p, G, retag_interval()};
+    while (true) {
+        GEQ_HIP(hipMemsetAsync(g->pg_next.p, 0, 4, s), GEQ_ERR_TRANSFER);
+        k_solve_games<<<grid, 64 * w, lds, s>>>(ea, ga);
+        GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
+        (*launches)++;
+        k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, grid, g->ctl.p->totals);
+        GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
+        GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+        if (!g->h_ctl->overflow) {
+            // every game ran rounds of its own: the download reads each node's
+            // younger slot at round min(rounds, retag_interval) (games that
+            // passed retag_interval re-tagged themselves to round 0)
+            g->h_ctl->round = std::min<uint32_t>(g->h_ctl->last_nonempty, retag_interval());
+            return GEQ_OK;
+        }
+        // the games stopped at a round boundary (the round that saw the
+        // overflow runs again); a larger arena, cleared queued bitmaps, and
+        // the games not done yet go on
+        st = recover_overflow(g);
+        if (st != GEQ_OK) return st;
+        ea.arena = g->arena.p;
+        ea.arena_cap = g->arena_cap;
+    }
+}
+
 uint32_t retag_interval() {
     // GEQ_RETAG_ROUNDS (1 .. RETAG_ROUNDS) re-tags more often, for tests
     if (const char* env = std::getenv("GEQ_RETAG_ROUNDS")) {
@@ -996,8 +1073,11 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         if (st != GEQ_OK) return st;
     }
 
+    const bool games_mode = g->tune.mode == SOLVE_MODE_GAMES && !profile_kernels;
+    if (games_mode && g->conf.generic)
+        return fail(GEQ_ERR_UNSUPPORTED, "the per-game persistent solve (GEQ_SOLVE_MODE=3) covers the bitmap energy layout only");
     // seed: predecessors of the nodes to reach (initialize_visit_lists, :1461-1465)
-    if (n_reach) {
+    if (n_reach && !games_mode) {
         ExpandArgs sx = xa;
         sx.seed = seed;
         sx.n_seed = n_reach;
@@ -1012,7 +1092,12 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     uint64_t launches = 0;
     geq_status result = GEQ_OK;
     bool stats_in_ctl = false;   // h_ctl->totals hold the statistics of every round that ran
-    while (g->h_ctl->wl_n[g->h_ctl->round % 3u] != 0) {
+    if (games_mode) {
+        result = solve_games(g, ea, seed, n_reach, &launches);
+        GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
+        stats_in_ctl = result == GEQ_OK;
+    }
+    while (!games_mode && g->h_ctl->wl_n[g->h_ctl->round % 3u] != 0) {
         stats_in_ctl = false;
         if (profile_kernels) {
             // one round per launch; each kernel's dispatch carries its own

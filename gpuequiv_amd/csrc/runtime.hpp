@@ -131,6 +131,11 @@ geq_status recover_overflow(geq_game* g);
 // between batches of rounds: k_retag when due (slot tags stay unambiguous)
 geq_status maybe_retag(geq_game* g);
 uint32_t retag_interval();
+// The persistent batched solve (solve mode 3, k_solve_games): the seeds
+// grouped by game, then launches until no arena overflow is left.  *h_ctl
+// holds the statistics and the round count afterwards.
+geq_status solve_games(geq_game* g, EvalArgs ea, const uint32_t* seed, uint32_t n_reach, uint64_t* launches);
+constexpr uint32_t SOLVE_MODE_GRAPH = 1, SOLVE_MODE_PROFILED = 2, SOLVE_MODE_GAMES = 3;
 
 }  // namespace geq
 
@@ -197,7 +202,8 @@ struct geq_game {
     std::vector<uint32_t> inv;
     geq::DevBuf<uint32_t> inv_map;
     struct Tune {                     // launch shape, set by the autotuner or defaults
-        uint32_t mode = 1;            // 1: hipGraph of captured rounds (0, the cooperative launch, is switched off)
+        uint32_t mode = 1;            // 1: hipGraph of captured rounds; 3: one persistent launch, a block per game
+                                      // (0, the cooperative launch, is switched off)
         uint32_t eval_blocks_per_cu = 8;     // capped at the blocks a CU holds at once
         uint32_t expand_blocks_per_cu = 0;   // 0: the expansion runs inside k_eval
         uint32_t rounds_per_launch = 8;
@@ -207,6 +213,11 @@ struct geq_game {
         uint32_t ordered = 2;
     } tune;
     std::vector<uint32_t> game_base;      // batched games: first node of every game (+ total)
+    // solve mode 3: game bounds, queue order (largest first), seeds grouped by
+    // game, per-game state; pg_base / pg_order are uploaded once
+    geq::DevBuf<uint32_t> pg_base, pg_order, pg_seed, pg_soff, pg_cnt, pg_next, pg_scan;
+    geq::DevBuf<geq::GameCtl> pg_ctl;
+    bool pg_ready = false;
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds
     uint64_t solves = 0;                  // solves finished (the first launches its rounds directly)
     hipEvent_t ev_start = nullptr, ev_end = nullptr;   // around each solve's rounds

@@ -313,9 +313,10 @@ __device__ __forceinline__ void flush_appends(const uint32_t* stage, uint32_t cn
 // appends the predecessors of its changed nodes to the next worklist.
 // r: the round; n: its worklist size (0 after an arena overflow).
 // lds: the block's dynamic LDS, EVAL_(FUSED_)LDS_PER_WAVE bytes per wave.
+// P: the round's list, where its appends go and which waves share it.
 template <bool FUSED>
 __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds, uint32_t* s_cnt, StatAcc& acc, uint32_t r,
-                                           uint32_t n) {
+                                           uint32_t n, const PhaseList& P) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = threadIdx.x >> 6;
     unsigned char* mine = lds + (size_t)wid * (FUSED ? EVAL_FUSED_LDS_PER_WAVE : EVAL_LDS_PER_WAVE);
@@ -333,20 +334,19 @@ __device__ __forceinline__ void eval_phase(const EvalArgs& a, unsigned char* lds
     // whole bitmap with plain stores (grid-stride), a short one clears its
     // nodes' bits one atomic each.
     uint32_t* const qcur = a.queued + (r & 1u) * a.qwords;
-    const bool clear_all = (uint64_t)a.qwords < 8ull * n;
+    const bool clear_all = P.grid_wide && (uint64_t)a.qwords < 8ull * n;
     if (clear_all)
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.qwords; i += gridDim.x * blockDim.x) qcur[i] = 0u;
-    uint32_t* out = a.wl[(r + 1u) & 1u];
-    uint32_t* out_n = &a.ctl->wl_n[(r + 1u) % 3u];
+    uint32_t* out = P.out;
+    uint32_t* out_n = P.out_n;
     uint32_t app = 0;             // FUSED: appends staged by this wave
     uint64_t st_bytes_lane = 0;
-    const uint32_t nw = blockDim.x >> 6;
-    const uint32_t* __restrict__ wl = a.wl[r & 1u];
+    const uint32_t* __restrict__ wl = P.wl;
     // entries per wave visit: up to 64, fewer when the worklist is too short
-    // to give every wave of the grid a full set
-    const uint32_t waves = gridDim.x * nw;
+    // to give every wave that shares it a full set
+    const uint32_t waves = P.waves;
     const uint32_t chunk = max(1u, min(64u, (n + waves - 1) / max(waves, 1u)));
-    for (uint32_t base = uniform((blockIdx.x * nw + wid) * chunk); base < n; base += waves * chunk) {
+    for (uint32_t base = uniform(P.wave0 * chunk); base < n; base += waves * chunk) {
         const uint32_t cnt = min(chunk, n - base);
         // lane k: worklist entry base + k
         uint32_t u = 0, r0 = 0, deg = 0, slot = 0, p0 = 0, pc = 0;
@@ -454,11 +454,165 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_eval(EvalArgs a, uint32_t k) {
         a.ctl->wl_n[(r + 2u) % 3u] = 0u;
         if (n) a.ctl->last_nonempty = r + 1u;
     }
-    eval_phase<FUSED>(a, dyn_lds, s_cnt, acc, r, n);
+    const uint32_t nw = blockDim.x >> 6;
+    const PhaseList P{a.wl[r & 1u], a.wl[(r + 1u) & 1u], &a.ctl->wl_n[(r + 1u) % 3u], blockIdx.x * nw + (threadIdx.x >> 6),
+                      gridDim.x * nw, true};
+    eval_phase<FUSED>(a, dyn_lds, s_cnt, acc, r, n, P);
     flush_stats(acc, a.stats);
 }
 template __global__ void k_eval<false>(EvalArgs, uint32_t);
 template __global__ void k_eval<true>(EvalArgs, uint32_t);
+
+namespace {
+
+// thread 0 reads a word that other blocks may write; every thread gets its
+// value (one barrier on each side, so the block's control flow stays uniform)
+__device__ __forceinline__ uint32_t block_read(const uint32_t* p, uint32_t* s_word) {
+    if (threadIdx.x == 0) *s_word = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t v = *s_word;
+    __syncthreads();
+    return v;
+}
+
+// A game's first worklist: the predecessors of its reach seeds (as k_expand's
+// seeding mode), queued on the parity-0 bitmap.  Uses each wave's expansion
+// scratch and append stage of the fused k_eval layout.
+__device__ void seed_game(const EvalArgs& a, unsigned char* lds, uint32_t* s_cnt, const uint32_t* seeds, uint32_t ns,
+                          uint32_t* out, uint32_t* out_n) {
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    unsigned char* mine = lds + (size_t)wid * EVAL_FUSED_LDS_PER_WAVE;
+    uint32_t* ex = reinterpret_cast<uint32_t*>(mine + STAGE * 16) + STAGE;
+    uint32_t* rs = ex + 64;
+    uint32_t* stage = rs + 64 + 3 * 128;
+    uint32_t app = 0;
+    for (uint32_t b = wid * 64u; b < ns; b += nw * 64u) {
+        uint32_t p0 = 0, pc = 0;
+        if (b + lane < ns) {
+            const uint4 hdr = a.st[(size_t)NODE_U4 * seeds[b + lane]];
+            p0 = hdr.z;
+            pc = hdr.w;
+        }
+        expand_nodes(a.rcol, a.queued, p0, pc, lane, ex, rs, stage, app, out, out_n);
+    }
+    flush_appends(stage, app, s_cnt, out, out_n);
+}
+
+}  // namespace
+
+// One launch for a batch: blocks take games from the queue (ga.order) and run
+// each game's rounds to its fixpoint, one block barrier between rounds.  The
+// rounds of a game are the Jacobi rounds of the game alone, which are also its
+// rounds inside the disjoint union that k_eval runs: same lists, same
+// evaluations.  After an arena overflow (set by any block) a block stops its
+// game at the end of the current round, records that round to run again, and
+// starts no further game; the host grows the arena and launches again.
+__global__ __launch_bounds__(MAX_BLOCK) void k_solve_games(EvalArgs a, GamesArgs ga) {
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    __shared__ uint32_t s_cnt[MAX_WAVES + 1];
+    __shared__ uint32_t s_word, s_next_n;
+    StatAcc acc;
+    const uint32_t nw = blockDim.x >> 6, wid = threadIdx.x >> 6;
+    uint32_t last = 0;   // 1 + the last non-empty round of this block's games
+    while (true) {
+        if (threadIdx.x == 0) s_word = atomicAdd(ga.next, 1u);
+        __syncthreads();
+        const uint32_t q = s_word;
+        __syncthreads();
+        if (q >= ga.n_games) break;
+        const uint32_t g = ga.order[q];
+        GameCtl c = ga.gctl[g];
+        if (c.state == GAME_DONE) continue;
+        if (block_read(&a.ctl->overflow, &s_word)) continue;   // left for the next launch
+        const uint32_t nb = ga.base[g], ne = ga.base[g + 1];
+        uint32_t* const wl0 = a.wl[0] + nb;
+        uint32_t* const wl1 = a.wl[1] + nb;
+        uint32_t r = 0, n = 0;
+        if (c.state == GAME_NEW) {
+            if (threadIdx.x == 0) s_next_n = 0u;
+            __syncthreads();
+            seed_game(a, dyn_lds, s_cnt, ga.seed + ga.soff[g], ga.soff[g + 1] - ga.soff[g], wl0, &s_next_n);
+            n = s_next_n;   // flush_appends ended with a barrier
+            __syncthreads();   // every thread has n before s_next_n is reset
+            c.retag = 0;
+        } else {
+            r = c.round;
+            n = c.n;
+        }
+        bool paused = false;
+        while (n) {
+            if (r - c.retag >= ga.retag_interval) {
+                // k_retag over the game's nodes, between two of its rounds
+                const uint32_t t = round_tag(r);
+                for (uint32_t v = nb + threadIdx.x; v < ne; v += blockDim.x) {
+                    const uint4 x = a.st[slot_index(v, 0)], y = a.st[slot_index(v, 1)];
+                    const uint32_t cs = cur_slot(x, y, t);
+                    a.st[slot_index(v, cs)] = nl_with_tag(cs ? y : x, t - 1u);
+                    a.st[slot_index(v, cs ^ 1u)] = nl_with_tag(cs ? x : y, t - 2u);
+                }
+                c.retag = r;
+            }
+            if (threadIdx.x == 0) s_next_n = 0u;
+            __syncthreads();
+            const PhaseList P{(r & 1u) ? wl1 : wl0, (r & 1u) ? wl0 : wl1, &s_next_n, wid, nw, false};
+            eval_phase<true>(a, dyn_lds, s_cnt, acc, r, n, P);   // ends with a barrier
+            last = max(last, r + 1u);
+            const uint32_t next_n = s_next_n;   // (block_read's barriers order this read before the reset)
+            if (block_read(&a.ctl->overflow, &s_word)) {
+                paused = true;   // round r runs again
+                break;
+            }
+            n = next_n;
+            r++;
+        }
+        if (!paused && r >= ga.retag_interval) {
+            // a long game: its slots re-tagged to the epoch of round 0 (current
+            // slot tag 0, the other 0xffff), so that the download's round tag,
+            // min(rounds, retag_interval) + 1, reads every game's last lists
+            const uint32_t t = round_tag(0);
+            for (uint32_t v = nb + threadIdx.x; v < ne; v += blockDim.x) {
+                const uint4 x = a.st[slot_index(v, 0)], y = a.st[slot_index(v, 1)];
+                const uint32_t cs = cur_slot(x, y, round_tag(r));
+                a.st[slot_index(v, cs)] = nl_with_tag(cs ? y : x, t - 1u);
+                a.st[slot_index(v, cs ^ 1u)] = nl_with_tag(cs ? x : y, t - 2u);
+            }
+        }
+        if (threadIdx.x == 0) {
+            if (paused) {
+                c.state = GAME_PAUSED;
+                c.round = r;
+                c.n = n;
+            } else {
+                c.state = GAME_DONE;
+            }
+            ga.gctl[g] = c;
+        }
+    }
+    if (threadIdx.x == 0 && last) atomicMax(&a.ctl->last_nonempty, last);
+    flush_stats(acc, a.stats);
+}
+
+__device__ __forceinline__ uint32_t game_of(const uint32_t* base, uint32_t n_games, uint32_t v) {
+    uint32_t lo = 0, hi = n_games - 1;   // the last game whose first node is <= v
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (base[mid] <= v) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+__global__ void k_seed_count(const uint32_t* seed, uint32_t n, const uint32_t* base, uint32_t n_games, uint32_t* cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[game_of(base, n_games, seed[i])], 1u);
+}
+
+__global__ void k_seed_scatter(const uint32_t* seed, uint32_t n, const uint32_t* base, uint32_t n_games, const uint32_t* soff,
+                               uint32_t* fill, uint32_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = game_of(base, n_games, seed[i]);
+    out[soff[g] + atomicAdd(&fill[g], 1u)] = seed[i];
+}
 
 // The predecessors of round r's changed nodes form round r + 1's worklist; in
 // seeding mode the seeds' predecessors join round r's own worklist.  A wave

@@ -56,6 +56,45 @@ struct EvalArgs {
     uint32_t ordered;               // 1: the expansion only sets queued bits; k_collect builds the next list in id order
 };
 
+// The list a round of k_eval's evaluation phase reads, where its appends go,
+// and the waves that share it: the whole grid (k_eval), or one block that runs
+// one game's rounds (k_solve_games).
+struct PhaseList {
+    const uint32_t* wl;             // the round's worklist
+    uint32_t* out;                  // the next round's worklist
+    uint32_t* out_n;                // its size (global memory or LDS)
+    uint32_t wave0, waves;          // this wave's index among the sharing waves, their number
+    bool grid_wide;                 // the grid shares the list: it may clear the queued bitmap with plain stores
+};
+
+// Persistent batched solve (k_solve_games): one launch for a whole batch of
+// games.  Each block takes games from a queue and runs all of a game's rounds
+// itself, with block barriers between the rounds; no block ever waits for
+// another, so the grid may be of any size.  Game g: nodes [base[g],
+// base[g + 1]), its worklists in the same ranges of wl[0] and wl[1], its
+// reach seeds at seed[soff[g] .. soff[g + 1]).
+struct GameCtl {                    // per game, 16 bytes
+    uint32_t state;                 // GAME_NEW, GAME_PAUSED (before `round`) or GAME_DONE
+    uint32_t round;                 // paused: the round that runs next
+    uint32_t n;                     // paused: its worklist size
+    uint32_t retag;                 // the round of the game's last re-tag
+};
+constexpr uint32_t GAME_NEW = 0, GAME_PAUSED = 1, GAME_DONE = 2;
+struct GamesArgs {
+    const uint32_t* base;           // n_games + 1 entries
+    const uint32_t* order;          // the queue: game indices, largest game first
+    const uint32_t* seed;
+    const uint32_t* soff;           // n_games + 1 entries
+    GameCtl* gctl;
+    uint32_t* next;                 // the queue's head (zero before the launch)
+    uint32_t n_games;
+    uint32_t retag_interval;        // a game re-tags its nodes every this many rounds
+};
+__global__ void k_solve_games(EvalArgs a, GamesArgs ga);
+__global__ void k_seed_count(const uint32_t* seed, uint32_t n, const uint32_t* base, uint32_t n_games, uint32_t* cnt);
+__global__ void k_seed_scatter(const uint32_t* seed, uint32_t n, const uint32_t* base, uint32_t n_games, const uint32_t* soff,
+                               uint32_t* fill, uint32_t* out);
+
 // The next round's worklist from its queued bitmap, in id order (each block a
 // contiguous range of bitmap words, one atomic per block).
 struct CollectArgs {

@@ -99,7 +99,7 @@ typedef struct {
     double expand_kernel_ms;/* sum of k_expand durations, when profiling was asked */
     uint64_t eval_launches;
     double solve_kernel_ms; /* reserved (0) */
-    uint32_t mode;          /* 1 hipGraph rounds, 2 profiled per-kernel launches */
+    uint32_t mode;          /* 1 hipGraph rounds, 2 profiled per-kernel launches, 3 one persistent launch (a block per game) */
     uint32_t pad;
 } geq_stats;
 
@@ -178,8 +178,10 @@ geq_status geq_batch_energies(geq_game* batch, geq_energies* outs);
  * hipGraph, and the energy-arena capacity (u16 entries; it only grows).
  * geq_*_autotune times candidate shapes on the game itself (median device time
  * of `repeats` solves each, coordinate search from the current shape with a
- * separate k_expand and from the same shape with the expansion inside k_eval),
- * keeps the fastest and sizes the arena for the largest solve seen.  Every
+ * separate k_expand and from the same shape with the expansion inside k_eval,
+ * then the persistent solve mode 3 with 4 and 8 waves per block for the bitmap
+ * layout), keeps the fastest (geq_game_get_solve_mode tells the mode) and
+ * sizes the arena for the largest solve seen.  Every
  * shape computes the same result.  No counterpart in the reference (fixed
  * WGSL workgroup sizes). */
 typedef struct {
@@ -195,6 +197,19 @@ typedef struct {
     uint32_t candidates, repeats;
     uint64_t arena_peak;            /* arena entries the solves used */
 } geq_tune_report;
+/* How geq_game_solve / geq_batch_solve run the rounds.  1 (default): batches
+ * of rounds captured in a hipGraph and replayed, every round over the whole
+ * batch's worklist.  3: one persistent launch; each block takes whole games
+ * from a device queue (largest first) and runs all of a game's rounds with
+ * block barriers in between, so no block waits for another.  Both give the
+ * same energies, node evaluations and edge updates (a game's rounds inside
+ * the disjoint union are its rounds alone); the round count is the largest
+ * game's.  Mode 3 covers the bitmap energy layout (elements <= 6, max <= 3)
+ * and returns GEQ_ERR_UNSUPPORTED for other confs.  GEQ_SOLVE_MODE sets the
+ * mode of new games.  No counterpart in the reference (one dispatch per
+ * shader pass, src/energygame.rs:1481-1523). */
+geq_status geq_game_set_solve_mode(geq_game* game, uint32_t mode);
+geq_status geq_game_get_solve_mode(geq_game* game, uint32_t* mode);
 geq_status geq_game_get_tune(geq_game* game, geq_tune_params* out);
 geq_status geq_game_set_tune(geq_game* game, const geq_tune_params* params);
 geq_status geq_game_autotune(geq_game* game, const uint32_t* to_reach, uint32_t n_reach, uint32_t repeats,
