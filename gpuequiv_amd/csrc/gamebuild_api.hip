@@ -434,6 +434,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
 
     // node, edge and set-pool capacities grow level by level
     auto node_cap = [&](size_t need) -> geq_status {
+        B->peak_nodes = std::max<uint64_t>(B->peak_nodes, need);
         geq_status e;
         if ((e = ensure(B->kind, need, s, true, "positions")) != GEQ_OK) return e;
         if ((e = ensure(B->p, need, s, true, "positions")) != GEQ_OK) return e;
@@ -475,11 +476,11 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     const uint64_t hv = jt ? 0 : std::min<uint64_t>(ctx->hint_positions.load(), hint_cap);
     const uint64_t he = jt ? 0 : std::min<uint64_t>(ctx->hint_edges.load(), hint_cap);
     const uint64_t hw = jt ? 0 : std::min<uint64_t>(ctx->hint_words.load(), hint_cap);
-    GB_TRY(node_cap(std::max<uint64_t>(std::max<uint32_t>(n_pairs, 1), hv + 1)));
-    GB_TRY(ensure(B->spool, std::max<uint64_t>((uint64_t)std::max<uint32_t>(n_pairs, 1) + 1024, hw + 1), s, false, "set pool"));
+    GB_TRY(node_cap(std::max<uint64_t>(std::max<uint32_t>(n_pairs, 1), hv)));
+    GB_TRY(ensure(B->spool, std::max<uint64_t>((uint64_t)std::max<uint32_t>(n_pairs, 1) + 1024, hw), s, false, "set pool"));
     if (he) {
-        GB_TRY(ensure(B->col, he + 1, s, false, "edges"));
-        GB_TRY(ensure(B->wt, he + 1, s, false, "edges"));
+        GB_TRY(ensure(B->col, he, s, false, "edges"));
+        GB_TRY(ensure(B->wt, he, s, false, "edges"));
     }
     GB_TRY(table_cap(std::max<uint32_t>(n_pairs, 1)));
     if (n_pairs) {
@@ -552,6 +553,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: a level of " + std::to_string(nf) +
                                                         " positions needs 2^32 or more candidates or set-pool words"));
         if ((uint64_t)B->spool_top + n_words >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: set pool exceeds 2^32 words"));
+        B->peak_words = std::max<uint64_t>(B->peak_words, B->spool_top + n_words + 1);
         GB_TRY(ensure(B->spool, B->spool_top + n_words + 1, s, true, "set pool"));
         GB_TRY(ensure(cands, std::max<uint32_t>(n_cand, 1), s, false, "candidates"));
         GB_TRY(ensure(flag, n_cand + 1, s, false, "candidates"));
@@ -559,6 +561,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         GB_TRY(node_cap((size_t)B->V + n_cand + 1));
         GB_TRY(table_cap((uint64_t)B->V + n_cand));
         if (E + n_cand >= 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 edges"));
+        B->peak_edges = std::max<uint64_t>(B->peak_edges, E + n_cand + 1);
         GB_TRY(ensure(B->col, E + n_cand + 1, s, true, "edges"));
         GB_TRY(ensure(B->wt, E + n_cand + 1, s, true, "edges"));
         k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, off.p, ws.hv_flag.p, B->spool_top, cands.p,
@@ -642,9 +645,8 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     }
     if (truncated) {
         // the last level's new positions get no successors: close their rows
-        const uint32_t rest = B->V - lo;
-        std::vector<uint32_t> rows(rest + 1, (uint32_t)E);
-        GB_HIP(hipMemcpyAsync(B->row.p + lo, rows.data(), 4ull * (rest + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+        const uint64_t rest = (uint64_t)B->V - lo + 1;
+        k_fill_u32<<<(uint32_t)((rest + tb - 1) / tb), tb, 0, s>>>(B->row.p + lo, rest, (uint32_t)E);
     } else {
         const uint32_t e32 = (uint32_t)E;
         GB_HIP(hipMemcpyAsync(B->row.p + B->V, &e32, 4, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
@@ -832,9 +834,9 @@ geq_status geq_gamebuild(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs
         delete B;
         return st;
     }
-    ctx->hint_positions = B->V;
-    ctx->hint_edges = B->E;
-    ctx->hint_words = B->spool_top;
+    ctx->hint_positions = B->peak_nodes;
+    ctx->hint_edges = B->peak_edges;
+    ctx->hint_words = B->peak_words;
     *out = B;
     return GEQ_OK;
 }

@@ -240,6 +240,10 @@ struct geq_built {
     // never expanded (their rows are empty although they may have successors)
     bool truncated = false;
     uint32_t frontier = 0;
+    // the largest position, edge and set-pool capacities the build asked for
+    // (above the final counts: a level's candidates include its duplicates);
+    // the next build of the context starts at these sizes
+    uint64_t peak_nodes = 0, peak_edges = 0, peak_words = 0;
     ~geq_built() {
         if (game) geq::release_built_game(game);
     }
