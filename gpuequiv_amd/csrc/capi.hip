@@ -808,7 +808,7 @@ geq_status maybe_retag(geq_game* g) {
     return GEQ_OK;
 }
 
-geq_status solve_games(geq_game* g, EvalArgs ea, const uint32_t* seed, uint32_t n_reach, uint64_t* launches) {
+geq_status solve_games(geq_game* g, EvalArgs ea, const uint32_t* seed, uint32_t n_reach, uint64_t* launches, hipEvent_t ev_end) {
     hipStream_t s = g->ctx->stream;
     const bool batch = g->game_base.size() >= 2;
     const uint32_t G = batch ? (uint32_t)g->game_base.size() - 1 : 1u;
@@ -865,6 +865,8 @@ geq_status solve_games(geq_game* g, EvalArgs ea, const uint32_t* seed, uint32_t 
         k_solve_games<<<grid, 64 * w, lds, s>>>(ea, ga);
         GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
         (*launches)++;
+        // (the solve's end, waited for with the control block just below)
+        GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
         k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, grid, g->ctl.p->totals);
         GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
@@ -1093,8 +1095,7 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
     geq_status result = GEQ_OK;
     bool stats_in_ctl = false;   // h_ctl->totals hold the statistics of every round that ran
     if (games_mode) {
-        result = solve_games(g, ea, seed, n_reach, &launches);
-        GEQ_HIP(hipEventRecord(ev_end, s), GEQ_ERR_NO_DEVICE);
+        result = solve_games(g, ea, seed, n_reach, &launches, ev_end);
         stats_in_ctl = result == GEQ_OK;
     }
     while (!games_mode && g->h_ctl->wl_n[g->h_ctl->round % 3u] != 0) {

@@ -134,7 +134,7 @@ uint32_t retag_interval();
 // The persistent batched solve (solve mode 3, k_solve_games): the seeds
 // grouped by game, then launches until no arena overflow is left.  *h_ctl
 // holds the statistics and the round count afterwards.
-geq_status solve_games(geq_game* g, EvalArgs ea, const uint32_t* seed, uint32_t n_reach, uint64_t* launches);
+geq_status solve_games(geq_game* g, EvalArgs ea, const uint32_t* seed, uint32_t n_reach, uint64_t* launches, hipEvent_t ev_end);
 constexpr uint32_t SOLVE_MODE_GRAPH = 1, SOLVE_MODE_PROFILED = 2, SOLVE_MODE_GAMES = 3;
 
 }  // namespace geq

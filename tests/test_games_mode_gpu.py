@@ -62,6 +62,17 @@ def test_games_mode_single_game():
     want, s1 = _solve([g], 1)
     got, s3 = _solve([g], 3)
     assert got == want and _counts(s3) == _counts(s1)
+    # the device time covers the persistent launch (one block runs the game)
+    assert s3["solve_ms"] > 0 and s1["solve_ms"] > 0
+
+
+def test_autotune_keeps_replayed_rounds_for_one_large_game():
+    # one block per game cannot beat the whole GPU on a single large game
+    g = workloads.random_game(100000, 43)
+    b = GameBatch([g])
+    b.autotune(repeats=2)
+    assert b.solve_mode() == 1
+    b.close()
 
 
 def test_games_mode_arena_overflow_relaunches(monkeypatch):
