@@ -400,10 +400,12 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
         }
         eb += graphs[i].n_edges;
     }
-    // internal ids in reverse breadth-first order (GEQ_RENUMBER=0 keeps the caller's)
+    // GEQ_RENUMBER=1: internal ids in reverse breadth-first order (off by
+    // default: measured within noise on batches and 3% slower on single
+    // games, whose consecutive predecessor ids then share queued-bitmap words)
     std::vector<uint32_t> inv;
     const char* renum_env = std::getenv("GEQ_RENUMBER");
-    if (!(renum_env && std::atoi(renum_env) == 0) && V > 1)
+    if (renum_env && std::atoi(renum_env) != 0 && V > 1)
         renumber(base, ci.generic ? ci.usz : 0, row, edges, rrow, rcol, atkv, gwt, inv);
 
     (void)hipSetDevice(ctx->device);

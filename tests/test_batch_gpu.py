@@ -109,6 +109,7 @@ def test_batch_in_id_order_with_explicit_reach(monkeypatch):
     # a batch whose rounds are listed from the queued bitmap, seeded with
     # explicit per-game reach lists (mapped through the upload's renumbering)
     monkeypatch.setenv("GEQ_WORKLIST_ORDER", "1")
+    monkeypatch.setenv("GEQ_RENUMBER", "1")
     print("batch, id-ordered rounds, explicit reach", flush=True)
     games = [workloads.random_game(int(v), 300 + i) for i, v in enumerate([800, 3000, 120, 9000])]
     reach = [np.array(H.standard_reach(g), np.uint32) for g in games]

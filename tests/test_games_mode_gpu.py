@@ -56,6 +56,16 @@ def test_games_mode_block_sizes(waves):
     assert got == want and _counts(s3) == _counts(s1)
 
 
+def test_games_mode_renumbered_batch(monkeypatch):
+    # games renumbered in reverse breadth-first order at upload keep their
+    # contiguous ranges, which are what a block of mode 3 works on
+    monkeypatch.setenv("GEQ_RENUMBER", "1")
+    graphs = _graphs(20, 13)
+    want, s1 = _solve(graphs, 1)
+    got, s3 = _solve(graphs, 3)
+    assert got == want and _counts(s3) == _counts(s1)
+
+
 def test_games_mode_single_game():
     g = workloads.random_game(100000, 42)
     print("solve mode 3, one game of 100000 positions", flush=True)

@@ -348,8 +348,8 @@ def test_json_example_game_matches_oracle():
 def test_worklist_order_and_renumbering_keep_results(monkeypatch, order, renumber):
     # Rounds listed in id order from the queued bitmap (GEQ_WORKLIST_ORDER=1,
     # the default from 2^20 positions) or in append order (0), positions
-    # renumbered in reverse breadth-first order at upload (the default) or not
-    # (GEQ_RENUMBER=0): the same sets, and the same Jacobi schedule counters.
+    # renumbered in reverse breadth-first order at upload (GEQ_RENUMBER=1) or
+    # not (the default): the same sets, and the same Jacobi schedule counters.
     monkeypatch.setenv("GEQ_WORKLIST_ORDER", order)
     monkeypatch.setenv("GEQ_RENUMBER", renumber)
     for V, seed in [(5000, 21), (30000, 22)]:
