@@ -157,7 +157,8 @@ def bench_batch(args) -> dict:
     if args.no_plain:          # profiling runs: the batched solve alone
         kms = prof["eval_kernel_ms"]
         return {"metric": "batched solve only (profiling run)", "ms_per_step": batched_s * 1e3,
-                "edge_updates_per_step": int(s["edge_updates"]), "rounds": int(s["rounds"]),
+                "edge_updates_per_step": int(s["edge_updates"]), "node_evals_per_step": int(s["node_evals"]),
+                "rounds": int(s["rounds"]), "k_eval_solves": args.warmup + args.steps + 1,
                 "k_eval_ms": kms, "alg_bytes": int(prof["alg_bytes"])}
     lib = _lib.lib()
     ctx = _lib.Context.get()

@@ -830,22 +830,29 @@ geq_status solve_games(geq_game* g, EvalArgs ea, const uint32_t* seed, uint32_t 
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
         g->pg_ready = true;
     }
-    if (n_reach && (!g->pg_seed.p || g->pg_seed.n < n_reach)) {
-        g->pg_seed.release();
-        if ((e = g->pg_seed.alloc(n_reach)) != hipSuccess)
-            return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("per-game seeds: ") + hipGetErrorString(e));
-    }
     // the seeds grouped by game (in any order within a game: a list's order
-    // changes neither the values nor the counts of a round)
-    const uint32_t tb = 256;
-    GEQ_HIP(hipMemsetAsync(g->pg_cnt.p, 0, 4ull * G, s), GEQ_ERR_TRANSFER);
-    if (n_reach) k_seed_count<<<(n_reach + tb - 1) / tb, tb, 0, s>>>(seed, n_reach, g->pg_base.p, G, g->pg_cnt.p);
-    geq_status st = exclusive_scan_dev<uint32_t, SumU32>(g->pg_cnt.p, g->pg_soff.p, G, g->pg_scan, s, SumU32{});
-    if (st != GEQ_OK) return st;
-    if (n_reach) {
+    // changes neither the values nor the counts of a round); the grouping of
+    // the game's standard reach is kept for its next solves
+    const bool std_seeds = n_reach && seed == g->std_seed.p && n_reach == g->std_n;
+    geq_status st = GEQ_OK;
+    if (!(std_seeds && g->pg_std_grouped)) {
+        g->pg_std_grouped = false;
+        if (n_reach && (!g->pg_seed.p || g->pg_seed.n < n_reach)) {
+            g->pg_seed.release();
+            if ((e = g->pg_seed.alloc(n_reach)) != hipSuccess)
+                return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("per-game seeds: ") + hipGetErrorString(e));
+        }
+        const uint32_t tb = 256;
         GEQ_HIP(hipMemsetAsync(g->pg_cnt.p, 0, 4ull * G, s), GEQ_ERR_TRANSFER);
-        k_seed_scatter<<<(n_reach + tb - 1) / tb, tb, 0, s>>>(seed, n_reach, g->pg_base.p, G, g->pg_soff.p, g->pg_cnt.p,
-                                                              g->pg_seed.p);
+        if (n_reach) k_seed_count<<<(n_reach + tb - 1) / tb, tb, 0, s>>>(seed, n_reach, g->pg_base.p, G, g->pg_cnt.p);
+        st = exclusive_scan_dev<uint32_t, SumU32>(g->pg_cnt.p, g->pg_soff.p, G, g->pg_scan, s, SumU32{});
+        if (st != GEQ_OK) return st;
+        if (n_reach) {
+            GEQ_HIP(hipMemsetAsync(g->pg_cnt.p, 0, 4ull * G, s), GEQ_ERR_TRANSFER);
+            k_seed_scatter<<<(n_reach + tb - 1) / tb, tb, 0, s>>>(seed, n_reach, g->pg_base.p, G, g->pg_soff.p, g->pg_cnt.p,
+                                                                  g->pg_seed.p);
+        }
+        g->pg_std_grouped = std_seeds;
     }
     static_assert(GAME_NEW == 0, "the per-game state starts zeroed");
     GEQ_HIP(hipMemsetAsync(g->pg_ctl.p, 0, sizeof(GameCtl) * G, s), GEQ_ERR_TRANSFER);
@@ -961,6 +968,7 @@ __global__ __launch_bounds__(256) void k_standard_reach(const uint32_t* row, con
 
 geq_status geq::standard_seed(geq_game* g) {
     if (g->std_ready) return GEQ_OK;
+    g->pg_std_grouped = false;   // a new list: solve mode 3 groups it again
     hipStream_t s = g->ctx->stream;
     const uint32_t V = g->V;
     if (g->std_seed.n < std::max<uint32_t>(V, 1) + 1ull)

@@ -221,6 +221,8 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
         gm->tune.ordered = worklist_order_default();
         gm->std_ready = false;
         gm->std_n = 0;
+        gm->pg_ready = false;         // solve mode 3's game bounds and seed grouping
+        gm->pg_std_grouped = false;
         gm->reach_limit = 0xffffffffu;
         gm->retag_round = 0;
         gm->solves = 0;

@@ -218,6 +218,7 @@ struct geq_game {
     geq::DevBuf<uint32_t> pg_base, pg_order, pg_seed, pg_soff, pg_cnt, pg_next, pg_scan;
     geq::DevBuf<geq::GameCtl> pg_ctl;
     bool pg_ready = false;
+    bool pg_std_grouped = false;          // pg_seed / pg_soff hold std_seed grouped by game
     hipGraphExec_t graph_exec = nullptr;  // rounds_per_launch captured rounds
     uint64_t solves = 0;                  // solves finished (the first launches its rounds directly)
     hipEvent_t ev_start = nullptr, ev_end = nullptr;   // around each solve's rounds

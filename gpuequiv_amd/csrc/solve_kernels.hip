@@ -601,17 +601,41 @@ __device__ __forceinline__ uint32_t game_of(const uint32_t* base, uint32_t n_gam
     return lo;
 }
 
+// Seeds of one game tend to sit next to each other: lanes group into runs of
+// equal games (a run starts where the game differs from the previous lane's)
+// and the first lane of a run does the run's one atomic.  Lanes past n get
+// game ~0u, a run of their own that does nothing.
+struct SeedRun {
+    uint32_t g, head, len;   // this lane's game, its run's first lane and length
+};
+__device__ __forceinline__ SeedRun seed_run(const uint32_t* seed, uint32_t n, const uint32_t* base, uint32_t n_games, uint32_t i) {
+    const uint32_t lane = threadIdx.x & 63u;
+    SeedRun s;
+    s.g = i < n ? game_of(base, n_games, seed[i]) : ~0u;
+    const uint32_t prev = (uint32_t)__shfl_up((int)s.g, 1u, 64);
+    const uint64_t heads = __ballot(lane == 0u || prev != s.g);
+    const uint64_t upto = heads & (lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull));
+    s.head = 63u - (uint32_t)__builtin_clzll(upto);
+    const uint64_t after = lane == 63u ? 0ull : heads >> (s.head + 1u);
+    s.len = after ? (uint32_t)__builtin_ctzll(after) + 1u : 64u - s.head;
+    return s;
+}
+
 __global__ void k_seed_count(const uint32_t* seed, uint32_t n, const uint32_t* base, uint32_t n_games, uint32_t* cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[game_of(base, n_games, seed[i])], 1u);
+    const SeedRun s = seed_run(seed, n, base, n_games, i);
+    if (s.g != ~0u && (threadIdx.x & 63u) == s.head) atomicAdd(&cnt[s.g], s.len);
 }
 
 __global__ void k_seed_scatter(const uint32_t* seed, uint32_t n, const uint32_t* base, uint32_t n_games, const uint32_t* soff,
                                uint32_t* fill, uint32_t* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t g = game_of(base, n_games, seed[i]);
-    out[soff[g] + atomicAdd(&fill[g], 1u)] = seed[i];
+    const uint32_t lane = threadIdx.x & 63u;
+    const SeedRun s = seed_run(seed, n, base, n_games, i);
+    uint32_t at = 0;
+    if (s.g != ~0u && lane == s.head) at = soff[s.g] + atomicAdd(&fill[s.g], s.len);
+    at = (uint32_t)__shfl((int)at, s.head, 64);   // the run's first slot, from its head
+    if (s.g != ~0u) out[at + (lane - s.head)] = seed[i];
 }
 
 // The predecessors of round r's changed nodes form round r + 1's worklist; in

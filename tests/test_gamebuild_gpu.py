@@ -197,11 +197,14 @@ def test_repeated_start_pair_survives_table_growth():
     _assert_same_game(b, lts, pairs)
 
 
-def test_built_games_reuse_freed_buffers():
+@pytest.mark.parametrize("mode", ["1", "3"])
+def test_built_games_reuse_freed_buffers(monkeypatch, mode):
     # A freed built game's device buffers serve the next built game that fits
     # in them: bigger, smaller and equal games in a row each give the oracle's
-    # game and energies, also when the reused buffers still hold a larger game.
-    print("built games in a row (buffer reuse)", flush=True)
+    # game and energies, also when the reused buffers still hold a larger game
+    # (and, in solve mode 3, the previous game's bounds and seed grouping).
+    print("built games in a row (buffer reuse), solve mode", mode, flush=True)
+    monkeypatch.setenv("GEQ_SOLVE_MODE", mode)
     shapes = [(60, 200, 4, 3), (12, 30, 2, 1), (60, 200, 4, 3), (30, 90, 3, 2), (60, 200, 4, 3), (12, 30, 2, 1)]
     for n, m, labels, seed in shapes:
         n2, src, dst, lbl = workloads.random_lts(n, m, labels, seed)
