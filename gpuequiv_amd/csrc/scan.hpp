@@ -4,6 +4,7 @@
 //   k_scan_reduce : every block of SCAN_ITEMS consecutive items -> its total
 //   k_scan_tops   : one block scans the block totals in place (and the grand total)
 //   k_scan_apply  : every block scans its items again from its block's offset
+// An input of at most SCAN_ITEMS items takes k_scan_apply alone (one launch).
 // Global memory is read and written striped (lane-consecutive, coalesced); the
 // apply step transposes its tile through LDS so that each thread scans
 // SCAN_PER_THREAD consecutive items.  The combine operator is a template
@@ -123,7 +124,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T* in, T* out
         acc = op(acc, v[k]);
     }
     T total;
-    T run = op(tops[blockIdx.x], detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total));
+    // tops NULL: a one-block scan (no block offsets; the block's total is the grand total)
+    const T pre = tops ? tops[blockIdx.x] : Op::identity();
+    T run = op(pre, detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total));
     // each thread rewrites only the slots it read; the barrier inside the block
     // scan has ordered every read of the tile before these writes
 #pragma unroll
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T* in, T* out
         const uint32_t j = k * SCAN_THREADS + t;
         if (base + j < n) out[base + j] = tile[scan_slot(j)];
     }
-    if (blockIdx.x == gridDim.x - 1 && t == 0) out[n] = tops[gridDim.x];   // the grand total
+    if (blockIdx.x == gridDim.x - 1 && t == 0) out[n] = tops ? tops[gridDim.x] : total;   // the grand total
 }
 
 // Exclusive scan of in[0..n) into out[0..n), the total into out[n] (out holds
@@ -146,6 +149,11 @@ template <class T, class Op>
 geq_status exclusive_scan_dev(const T* in, T* out, size_t n, DevBuf<T>& tmp, hipStream_t s, Op op) {
     const size_t nb = std::max<size_t>(1, (n + SCAN_ITEMS - 1) / SCAN_ITEMS);
     if (nb > 0x7fffffffull) return fail(GEQ_ERR_INVALID_ARG, "scan of more than 2^31 blocks");
+    if (nb == 1) {   // one block holds the input: one launch
+        k_scan_apply<T, Op><<<1, SCAN_THREADS, 0, s>>>(in, out, n, nullptr, op);
+        GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
+        return GEQ_OK;
+    }
     if (!tmp.p || tmp.n < nb + 1) {
         tmp.release();
         GEQ_HIP(tmp.alloc(nb + 1), GEQ_ERR_OUT_OF_MEMORY);
