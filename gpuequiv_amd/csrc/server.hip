@@ -136,6 +136,7 @@ struct Job {
     uint32_t positions = 0, edges = 0;   // game size once built
     uint32_t admit_round = 0, done_round = 0;
     double t_submit = 0, t_built = 0, t_admit = 0, t_done = 0;
+    bool alone = false;                 // its merged build failed: it is built by itself
     std::atomic<uint32_t> state{QUEUED};
     std::vector<uint32_t> offs, data;
 };
@@ -233,6 +234,7 @@ struct geq_server {
     uint64_t next_id = 1;
     std::chrono::steady_clock::time_point t0;
     uint64_t steps = 0, rounds_total = 0, compactions = 0, jobs_done = 0, builds = 0;
+    uint64_t split_builds = 0;          // merged builds that failed and were split into one-job builds
     uint64_t arena_min = 0;            // arena entries the pools start with
     uint64_t compact_entries = 0;      // compaction threshold (0: 3/4 of the arena)
     uint64_t max_positions = 0;        // per-job game size limit (0: the node pool)
@@ -254,10 +256,15 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv.wait(lk, [&] { return S->stop || (!S->build_q.empty() && S->ready_q.size() < S->max_ready); });
             if (S->stop) return;
+            // a job whose merged build failed is built alone, so that it fails
+            // (or succeeds) by itself
             while (!S->build_q.empty() && take.size() < S->max_batch) {
-                take.push_back(S->build_q.front());
+                Job* J = S->build_q.front();
+                if (J->alone && !take.empty()) break;
+                take.push_back(J);
                 S->build_q.pop_front();
-                take.back()->state = BUILDING;
+                J->state = BUILDING;
+                if (J->alone) break;
             }
         }
         // invalid jobs fail alone; jobs without pairs have nothing to solve
@@ -321,6 +328,19 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
         std::lock_guard<std::mutex> lk(S->mu);
         const double now = S->now_ms();
         S->builds++;
+        if (st != GEQ_OK && build.size() > 1) {
+            // the merged build failed (a level past the 32-bit totals, device
+            // memory): its jobs go back to the front of the queue, each to be
+            // built alone, so only the jobs that fail by themselves fail
+            for (size_t k = build.size(); k-- > 0;) {
+                build[k]->alone = true;
+                build[k]->state = QUEUED;
+                S->build_q.push_front(build[k]);
+            }
+            S->split_builds++;
+            S->cv.notify_all();
+            continue;
+        }
         for (uint32_t k = 0; k < build.size(); k++) {
             Job* J = build[k];
             J->t_built = now;
@@ -702,6 +722,7 @@ geq_status geq_server_get_stats(geq_server* S, geq_server_stats* out) {
     out->jobs_waiting = S->build_q.size() + S->ready_q.size();
     out->jobs_done = S->jobs_done;
     out->builds = S->builds;
+    out->split_builds = S->split_builds;
     return GEQ_OK;
 }
 

@@ -342,6 +342,7 @@ typedef struct {
     uint64_t arena_entries, arena_top;        /* energy arena: size, entries in use (u16) */
     uint64_t jobs_resident, jobs_waiting, jobs_done;
     uint64_t builds;                          /* merged builds (jobs built together) so far */
+    uint64_t split_builds;                    /* merged builds that failed and were rebuilt one job at a time */
 } geq_server_stats;
 geq_status geq_server_get_stats(geq_server* server, geq_server_stats* out);
 

@@ -214,3 +214,42 @@ def test_one_build_of_more_than_256_jobs(monkeypatch):
         assert (info["positions"], info["edges"]) == size, k
         assert H.canon_lists(*r) == alone, k
     srv.close()
+
+
+def test_failed_merged_build_is_split_per_job(monkeypatch):
+    """A merged build that fails (here: a level past a lowered per-level total,
+    as a level past 2^32 candidates would) is rebuilt one job at a time, so
+    only the jobs that fail by themselves fail; the others equal their
+    standalone results."""
+    print("serve: a failed merged build is split into one-job builds", flush=True)
+    jobs = _jobs(12, 23, 20, 60, 3, 6)
+    # the smallest limit at which standalone builds of a third or fewer of the jobs fail
+    limit, alone_fails = None, None
+    for lim in (1000, 2000, 5000, 10000, 20000, 50000, 100000, 400000):
+        monkeypatch.setenv("GEQ_BUILD_LEVEL_LIMIT", str(lim))
+        fails = []
+        for lts, pairs, _, _ in jobs:
+            try:
+                GameBuild(lts, pairs).close()
+                fails.append(False)
+            except Exception as e:   # the refusal: GEQ_ERR_OUT_OF_MEMORY
+                assert "2^32 or more" in str(e), e
+                fails.append(True)
+        if sum(fails) <= len(jobs) // 3:
+            limit, alone_fails = lim, fails
+            break
+    assert limit is not None
+    print("level limit", limit, "jobs failing alone", sum(alone_fails), flush=True)
+    srv = Server(budget_bytes=256 << 20, rounds_per_step=2)
+    ids = [srv.submit(lts, pairs) for lts, pairs, _, _ in jobs]
+    res = srv.run_until_done(ids)
+    for k, ((lts, pairs, alone, size), j, fails) in enumerate(zip(jobs, ids, alone_fails)):
+        info, r = res[j]
+        if fails:
+            assert info["state_name"] == "failed", (k, info)
+        else:
+            assert info["state_name"] == "done", (k, info)
+            assert H.canon_lists(*r) == alone, k
+    st = srv.stats()
+    assert st["split_builds"] >= 1, st
+    srv.close()
