@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -710,10 +711,17 @@ geq_status ensure_scratch(geq_game* g, const RoundShape& shape, uint64_t words) 
     return GEQ_OK;
 }
 
+// GEQ_SHORT_LISTS=0: every node takes the bitmap path of eval_node.  Read
+// when a solve sets up its kernel arguments (a game's captured rounds keep
+// the value of their capture).
+uint32_t short_lists_default() {
+    const char* env = std::getenv("GEQ_SHORT_LISTS");
+    return (env && std::atoi(env) == 0) ? 0u : 1u;
+}
 EvalArgs eval_args(geq_game* g) {
     return EvalArgs{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->st.p, g->arena.p, g->ctl.p, g->arena_cap,
                     g->chg.p, g->conf.max, g->stats.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V),
-                    round_shape(g).ordered ? 1u : 0u};
+                    round_shape(g).ordered ? 1u : 0u, short_lists_default()};
 }
 
 uint32_t worklist_order_default() {
@@ -1108,8 +1116,13 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         result = solve_games(g, ea, seed, n_reach, &launches, ev_end);
         stats_in_ctl = result == GEQ_OK;
     }
+    // GEQ_SOLVE_TRACE=1 with profiled kernels: one stderr line per round
+    // (worklist size, k_eval and expansion times)
+    const char* trace_env = std::getenv("GEQ_SOLVE_TRACE");
+    const bool trace = profile_kernels && trace_env && std::atoi(trace_env) != 0;
     while (!games_mode && g->h_ctl->wl_n[g->h_ctl->round % 3u] != 0) {
         stats_in_ctl = false;
+        const uint32_t trace_round = g->h_ctl->round, trace_n = g->h_ctl->wl_n[g->h_ctl->round % 3u];
         if (profile_kernels) {
             // one round per launch; each kernel's dispatch carries its own
             // start / end events (hipExtLaunchKernelGGL), so the times are the
@@ -1175,6 +1188,13 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
         }
         GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+        if (trace && prof.size() >= 4) {
+            float te = 0, tx = 0;
+            (void)hipEventElapsedTime(&te, prof[prof.size() - 4], prof[prof.size() - 3]);
+            (void)hipEventElapsedTime(&tx, prof[prof.size() - 2], prof[prof.size() - 1]);
+            std::fprintf(stderr, "solve: round %u: %u nodes, eval %.1f us, expand %.1f us\n", trace_round, trace_n, 1e3 * te,
+                         1e3 * tx);
+        }
         if (g->h_ctl->overflow) {
             result = recover_overflow(g);
             if (result != GEQ_OK) break;

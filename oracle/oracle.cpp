@@ -335,6 +335,8 @@ static bool lists_equal_ref(const Ctx& X, const List& a_new, const List& b_old) 
 // Jacobi, so a round's evaluations are independent and any split over threads
 // gives the same result; the merge stays sequential, in worklist order.
 static int g_threads = 1;
+// ORACLE_TRACE=1: one stderr line per round of the fixpoint loop (worklist sizes)
+static const bool g_trace = std::getenv("ORACLE_TRACE") && std::atoi(std::getenv("ORACLE_TRACE")) != 0;
 
 struct SolveStats {
     uint64_t rounds = 0, node_evals = 0, atk_evals = 0, def_evals = 0, edge_updates = 0, bytes = 0,
@@ -404,6 +406,9 @@ static void solve(const Ctx& X, uint32_t V, const uint32_t* row, const uint32_t*
             st.def_evals++;
         }
         st.rounds++;
+        if (g_trace)
+            std::fprintf(stderr, "oracle: round %llu: %zu attacker, %zu defender evaluations\n",
+                         (unsigned long long)st.rounds, atk_jobs.size(), def_jobs.size());
         // process attacker results, then defender results (:1510-1513)
         std::vector<uint32_t> changed;
         for (size_t i = 0; i < atk_jobs.size(); i++)
