@@ -711,17 +711,10 @@ geq_status ensure_scratch(geq_game* g, const RoundShape& shape, uint64_t words) 
     return GEQ_OK;
 }
 
-// GEQ_SHORT_LISTS=0: every node takes the bitmap path of eval_node.  Read
-// when a solve sets up its kernel arguments (a game's captured rounds keep
-// the value of their capture).
-uint32_t short_lists_default() {
-    const char* env = std::getenv("GEQ_SHORT_LISTS");
-    return (env && std::atoi(env) == 0) ? 0u : 1u;
-}
 EvalArgs eval_args(geq_game* g) {
     return EvalArgs{{g->wl0.p, g->wl1.p}, g->row.p, g->edges.p, g->atk.p, g->st.p, g->arena.p, g->ctl.p, g->arena_cap,
                     g->chg.p, g->conf.max, g->stats.p, g->rrow.p, g->rcol.p, g->queued.p, queue_words(g->V),
-                    round_shape(g).ordered ? 1u : 0u, short_lists_default()};
+                    round_shape(g).ordered ? 1u : 0u};
 }
 
 uint32_t worklist_order_default() {
