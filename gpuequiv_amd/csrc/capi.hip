@@ -41,13 +41,76 @@ hipError_t device_malloc(void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);
     if (e == hipErrorOutOfMemory) {
         int dev = 0;
-        if (hipGetDevice(&dev) == hipSuccess && drop_spare_games(dev)) {
+        const bool games = hipGetDevice(&dev) == hipSuccess && drop_spare_games(dev);
+        if (spare_drop() || games) {
             (void)hipGetLastError();   // the failed attempt's error is handled here
             e = hipMalloc(p, bytes);
         }
     }
     return e;
 }
+
+void* SpareBufs::take(size_t bytes, size_t* got) {
+    size_t best = bufs.size();
+    for (size_t i = 0; i < bufs.size(); i++)
+        if (bufs[i].bytes >= bytes && bufs[i].bytes / 4 <= bytes && (best == bufs.size() || bufs[i].bytes < bufs[best].bytes))
+            best = i;
+    if (best == bufs.size()) return nullptr;
+    void* p = bufs[best].p;
+    *got = bufs[best].bytes;
+    bufs.erase(bufs.begin() + best);
+    return p;
+}
+bool SpareBufs::drop() {
+    for (const Buf& b : bufs) (void)hipFree(b.p);
+    const bool any = !bufs.empty();
+    bufs.clear();
+    return any;
+}
+
+namespace {
+struct RetireList {
+    int depth = 0;
+    SpareBufs* pool = nullptr;   // the outermost open scope's
+    std::vector<SpareBufs::Buf> bufs;
+};
+thread_local RetireList t_retired;
+}  // namespace
+
+RetireScope::RetireScope(hipStream_t s, SpareBufs* pool) : s_(s), outer_(t_retired.depth == 0) {
+    if (outer_) t_retired.pool = pool;
+    t_retired.depth++;
+}
+RetireScope::~RetireScope() {
+    t_retired.depth--;
+    if (!outer_) return;
+    SpareBufs* pool = t_retired.pool;
+    t_retired.pool = nullptr;
+    if (t_retired.bufs.empty()) return;
+    if (hipStreamSynchronize(s_) != hipSuccess || !pool) {   // their kernels' state is unknown: free them
+        for (const SpareBufs::Buf& b : t_retired.bufs) (void)hipFree(b.p);
+        t_retired.bufs.clear();
+        return;
+    }
+    auto& v = pool->bufs;
+    v.insert(v.end(), t_retired.bufs.begin(), t_retired.bufs.end());
+    t_retired.bufs.clear();
+    if (v.size() > SPARE_BUFS) {   // keep the largest
+        std::sort(v.begin(), v.end(), [](const SpareBufs::Buf& a, const SpareBufs::Buf& b) { return a.bytes > b.bytes; });
+        for (size_t i = SPARE_BUFS; i < v.size(); i++) (void)hipFree(v[i].p);
+        v.resize(SPARE_BUFS);
+    }
+}
+bool RetireScope::active() { return t_retired.depth > 0; }
+bool RetireScope::take(void* p, size_t bytes) {
+    if (t_retired.depth == 0) return false;
+    t_retired.bufs.push_back(SpareBufs::Buf{p, bytes});
+    return true;
+}
+void* geq::spare_take(size_t bytes, size_t* got) {
+    return t_retired.depth && t_retired.pool ? t_retired.pool->take(bytes, got) : nullptr;
+}
+bool geq::spare_drop() { return t_retired.depth && t_retired.pool ? t_retired.pool->drop() : false; }
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 geq_status fail(geq_status s, const std::string& msg) {
@@ -1313,16 +1376,22 @@ geq_status download_lists(geq_game* g, uint32_t first, uint32_t V, uint32_t t, g
 geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_energies* out) {
     if (!g || !out || (uint64_t)first + V > g->V) return fail(GEQ_ERR_INVALID_ARG, "node range out of bounds");
     (void)hipSetDevice(g->ctx->device);
-    hipStream_t s = g->ctx->stream;
     if (!g->lens.p || g->lens.n < V) GEQ_HIP(g->lens.alloc(std::max<uint32_t>(V, 1)), GEQ_ERR_OUT_OF_MEMORY);
     const uint32_t t = round_tag(g->h_ctl->round);   // the lists after the last round that ran
     if (g->conf.generic) return download_lists(g, first, V, t, out);
-    // lengths -> offsets (device scan) -> lists gathered, ascending; offsets and
-    // lists come back through the game's pinned staging (two waits)
-    if (!g->offs.p || g->offs.n < V + 1ull) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
     // renumbered games: output entry v is internal node inv_map[first + v]
     const uint4* stb = g->inv_map.p ? g->st.p : g->st.p + (size_t)NODE_U4 * first;
     const uint32_t* map = g->inv_map.p ? g->inv_map.p + first : nullptr;
+    return download_mapped(g, stb, map, V, t, out);
+}
+
+// The lists of nodes map[0 .. V) (of nodes stb[0 .. V) without a map) at round tag t.
+// Lengths -> offsets (device scan) -> lists gathered, ascending; offsets and
+// lists come back through the game's mapped pinned staging, after one wait.
+geq_status geq::download_mapped(geq_game* g, const uint4* stb, const uint32_t* map, uint32_t V, uint32_t t, geq_energies* out) {
+    hipStream_t s = g->ctx->stream;
+    if (!g->lens.p || g->lens.n < V) GEQ_HIP(g->lens.alloc(std::max<uint32_t>(V, 1)), GEQ_ERR_OUT_OF_MEMORY);
+    if (!g->offs.p || g->offs.n < V + 1ull) GEQ_HIP(g->offs.alloc(V + 1), GEQ_ERR_OUT_OF_MEMORY);
     if (V) k_list_lens<<<(V + 255) / 256, 256, 0, s>>>(stb, V, t, g->lens.p, map);
     geq_status sst = exclusive_scan_dev<uint32_t, SumU32>(g->lens.p, g->offs.p, V, g->scan_tmp, s, SumU32{});
     if (sst != GEQ_OK) return sst;

@@ -710,9 +710,15 @@ __global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl
     wt[e] = c.weight;
 }
 
-__global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row) {
+__global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row, RowsTail T) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_front) row[lo + i] = e_base + rowl[i];
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0 && T.heavy_count) *T.heavy_count = 0u;
+        if (T.dead)
+            for (uint32_t j = threadIdx.x; j < T.n_jobs; j += blockDim.x)
+                if (T.cnt[j] > T.cap) T.dead[j] = 1;
+    }
 }
 
 // start positions Attack(p, [q]) at ids [0, n): node records + hash entries

@@ -94,7 +94,17 @@ __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* f
                            NodeArgs N, unsigned long long* table, uint32_t* keymin);
 __global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
                         uint32_t* col, uint32_t* wt);
-__global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row);
+// Work that follows a chunk's rows, done by k_rows' first block: the heavy
+// count is cleared for the next k_bounds_next (NULL: not), and merged builds
+// mark the jobs whose position counts passed the cap (dead NULL: none).
+struct RowsTail {
+    uint32_t* heavy_count;
+    const uint32_t* cnt;
+    uint32_t n_jobs;
+    uint64_t cap;
+    uint8_t* dead;
+};
+__global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row, RowsTail tail);
 __global__ void k_starts(const uint32_t* pairs, uint32_t n, const uint8_t* insert, NodeArgs N, unsigned long long* table,
                          uint32_t mask);
 __global__ void k_rehash(const uint32_t* hash, uint32_t n, const uint8_t* insert, uint32_t n_start, unsigned long long* table,
@@ -136,6 +146,7 @@ struct BuildScratch {
     DevBuf<Cand> cands;
     DevBuf<unsigned long long> table;
     DevBuf<uint32_t> job_of_state, key, key_s, val, order, dsort, escan, local, d_ps, d_es;
+    SpareBufs spare;                  // buffers outgrown by earlier builds (RetireScope)
 };
 
 }  // namespace gb
@@ -152,6 +163,8 @@ struct BuiltJobs {
     std::vector<uint8_t> dead;       // grew past the position cap
     uint32_t levels = 0;
     double build_ms = 0;
+    double prep_ms = 0;       // the merged LTS on the host
+    double extract_ms = 0;    // cutting the per-job games out of the merged build
 };
 // Builds the games of jobs (validated, each with at least one pair) in one
 // level loop over the disjoint union of their LTSs.  `ws` and `store` (the

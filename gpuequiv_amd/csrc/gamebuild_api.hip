@@ -39,17 +39,26 @@ geq_status ensure(DevBuf<T>& b, size_t need, hipStream_t s, bool keep, const cha
     if (b.p && b.n >= need) return GEQ_OK;
     size_t cap = std::max<size_t>(need, b.n ? 2 * b.n : 1024);
     T* p = nullptr;
-    hipError_t e = device_malloc(reinterpret_cast<void**>(&p), sizeof(T) * cap);
-    if (e != hipSuccess) return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("gamebuild: allocating ") + what + ": " + hipGetErrorString(e));
+    size_t got = 0;
+    if ((p = static_cast<T*>(spare_take(sizeof(T) * cap, &got))) != nullptr) {
+        cap = got / sizeof(T);   // a spare buffer of an earlier build on this thread
+    } else {
+        hipError_t e = device_malloc(reinterpret_cast<void**>(&p), sizeof(T) * cap);
+        if (e != hipSuccess)
+            return fail(GEQ_ERR_OUT_OF_MEMORY, std::string("gamebuild: allocating ") + what + ": " + hipGetErrorString(e));
+    }
+    hipError_t e = hipSuccess;
     if (keep && b.p && b.n) {
+        // stream-ordered: the old buffer is retired below, not freed, when a
+        // RetireScope is open; otherwise the copy must finish before the free
         e = hipMemcpyAsync(p, b.p, sizeof(T) * b.n, hipMemcpyDeviceToDevice, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess && !RetireScope::active()) e = hipStreamSynchronize(s);
         if (e != hipSuccess) {
             (void)hipFree(p);
             return fail(GEQ_ERR_TRANSFER, std::string("gamebuild: growing ") + what);
         }
     }
-    b.release();
+    b.retire();
     b.p = p;
     b.n = cap;
     return GEQ_OK;
@@ -128,12 +137,6 @@ __global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, u
         }
         out[i] = v;
     }
-}
-
-// Jobs whose position count passed the cap get no further successors.
-__global__ void k_mark_dead(const uint32_t* cnt, uint32_t n_jobs, uint64_t cap, uint8_t* dead) {
-    for (uint32_t j = threadIdx.x; j < n_jobs; j += blockDim.x)
-        if (cnt[j] > cap) dead[j] = 1;
 }
 
 }  // namespace
@@ -351,6 +354,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
 
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
+    RetireScope retire_scope(s, &ws.spare);   // outgrown buffers are kept for later builds
     auto t0 = std::chrono::steady_clock::now();
     B->ctx = ctx;
     B->n_start = n_pairs;
@@ -588,11 +592,14 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
                 k_job_count_cands<<<nb, tb, 0, s>>>(cands.p, flag.p, n_cand, d_job_of_state.p, n_jobs, d_jcnt.p);
             }
         }
-        k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p);
-        if (jt && jt->cap) k_mark_dead<<<1, 256, 0, s>>>(d_jcnt.p, n_jobs, jt->cap, d_dead.p);
         // After a level's last chunk, the next level is the positions the level
         // created: `add` by its earlier chunks and *(rank + n_cand) by this one.
         const bool last_chunk = c_hi == hi;
+        // the rows; the heavy count cleared for k_bounds_next; jobs past the cap marked
+        const bool mark = jt && jt->cap;
+        k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p,
+                                                 RowsTail{last_chunk ? ws.hv_count.p : nullptr, mark ? d_jcnt.p : nullptr,
+                                                          mark ? n_jobs : 0u, mark ? jt->cap : 0ull, mark ? d_dead.p : nullptr});
         const uint32_t add = B->V - hi;
         if (last_chunk) {
             const uint64_t slots = (uint64_t)add + n_cand + 1;
@@ -601,7 +608,6 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             GB_TRY(ensure(off, slots, s, false, "frontier"));
             GB_TRY(ensure(ws.hv_flag, slots, s, false, "heavy positions"));
             GB_TRY(ensure(ws.hv_list, slots, s, false, "heavy positions"));
-            GB_HIP(hipMemsetAsync(ws.hv_count.p, 0, 4, s), GEQ_ERR_TRANSFER);
             k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots,
                                                                            chunk, L, ub.p, heavy_args());
             GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots - 1, ws.scan2, s));
@@ -635,7 +641,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         }
         if (jt) {
             // positions of every job so far (read with the counts above); jobs past
-            // the cap stop growing (k_mark_dead set the same flags on the device)
+            // the cap stop growing (k_rows set the same flags on the device)
             for (uint32_t j = 0; j < n_jobs; j++) {
                 jt->count[j] = h_rb[RB_JOBS + j];
                 if (jt->cap && jt->count[j] > jt->cap) jt->dead[j] = 1;
@@ -724,6 +730,7 @@ geq_status geq::gamebuild_jobs(geq_ctx* ctx, const std::vector<geq_lts>& ltss, c
                                BuiltJobs* out) {
     const uint32_t J = (uint32_t)ltss.size();
     if (J == 0 || J > 1024) return fail(GEQ_ERR_INVALID_ARG, "gamebuild_jobs: 1..1024 jobs");
+    const auto tp0 = std::chrono::steady_clock::now();
     // disjoint union of the jobs' LTSs; pairs shifted to the merged state ids
     uint64_t n = 0, m = 0, np = 0;
     for (uint32_t j = 0; j < J; j++) {
@@ -757,6 +764,8 @@ geq_status geq::gamebuild_jobs(geq_ctx* ctx, const std::vector<geq_lts>& ltss, c
         pb += n_pairs[j];
     }
     geq_lts merged{(uint32_t)n, (uint32_t)m, row.data(), dst.data(), lbl.data()};
+    RetireScope retire_scope(ctx->stream, &ws.spare);   // outgrown buffers are kept for later builds
+    const auto tp1 = std::chrono::steady_clock::now();
     geq_build_stats bs{};
     geq_built* B = &store;
     geq_status st = build_core(ctx, &merged, mp.data(), (uint32_t)np, 0, &jt, B, ws, &bs);
@@ -808,6 +817,8 @@ geq_status geq::gamebuild_jobs(geq_ctx* ctx, const std::vector<geq_lts>& ltss, c
     out->dead = jt.dead;
     out->levels = (uint32_t)bs.levels;
     out->build_ms = bs.build_ms;
+    out->prep_ms = std::chrono::duration<double, std::milli>(tp1 - tp0).count();
+    out->extract_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp1).count() - bs.build_ms;
     return GEQ_OK;
 }
 

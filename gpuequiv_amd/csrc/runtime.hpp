@@ -56,6 +56,51 @@ bool drop_spare_games(int device);
 // the device address of its static scratch array, once per device.
 geq_status noalloc_setup(int device);
 
+// Device buffers kept for reuse by later builds of one build scratch.
+struct SpareBufs {
+    struct Buf {
+        void* p;
+        size_t bytes;
+    };
+    std::vector<Buf> bufs;
+    SpareBufs() = default;
+    SpareBufs(const SpareBufs&) = delete;
+    SpareBufs& operator=(const SpareBufs&) = delete;
+    ~SpareBufs() { drop(); }
+    // a buffer with room for `bytes` (and at most 4x as much), or NULL; its size in *got
+    void* take(size_t bytes, size_t* got);
+    bool drop();   // frees them all; true if there were any
+};
+constexpr size_t SPARE_BUFS = 24;   // spares kept at most (the largest)
+
+// Buffers outgrown inside a build.  hipFree waits for every stream of the
+// device, so a buffer that a build's queued kernels may still read is not
+// freed when it is outgrown: while a RetireScope is open on the thread it is
+// kept, and when the scope closes (after a wait on the scope's stream) it
+// joins the scope's spare buffers, which the build allocations of later
+// scopes on the same pool take before calling hipMalloc (spare_take).  Device
+// memory exhaustion frees the open scope's spares.  Scopes nest: an inner
+// scope hands its buffers to the outermost one.
+class RetireScope {
+  public:
+    RetireScope(hipStream_t s, SpareBufs* pool);
+    ~RetireScope();
+    RetireScope(const RetireScope&) = delete;
+    RetireScope& operator=(const RetireScope&) = delete;
+    // true: p (of `bytes` bytes) is kept until the outermost scope closes;
+    // false: no scope is open
+    static bool take(void* p, size_t bytes);
+    static bool active();
+
+  private:
+    hipStream_t s_;
+    bool outer_;
+};
+// A spare buffer of the open scope's pool with room for `bytes` (NULL: none, or no scope).
+void* spare_take(size_t bytes, size_t* got);
+// Frees the open scope's spare buffers; true if there were any.
+bool spare_drop();
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -65,6 +110,14 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
+    }
+    // release for a buffer that queued kernels may still use (RetireScope)
+    void retire() {
+        if (p && RetireScope::take(p, sizeof(T) * n)) {
+            p = nullptr;
+            n = 0;
+        }
+        release();
     }
     hipError_t alloc(size_t count) {
         release();
@@ -85,6 +138,10 @@ geq_status standard_seed(geq_game* g);
 // built game, or are freed when the cache is taken.
 void release_built_game(geq_game* g);
 geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_energies* out);
+// The same for any nodes of a game without renumbering and in the bitmap
+// layout: entry v of the result is node map[v] (device ids); t is the tag of
+// the round after the last one that ran.
+geq_status download_mapped(geq_game* g, const uint4* stb, const uint32_t* map, uint32_t V, uint32_t t, geq_energies* out);
 // Device-resident game from device arrays (STANDARD conf): rows, columns,
 // packed updates, attacker flags; the reverse CSR is derived on the device.
 geq_status game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uint32_t* d_row, const uint32_t* d_col,

@@ -155,7 +155,7 @@ geq_status exclusive_scan_dev(const T* in, T* out, size_t n, DevBuf<T>& tmp, hip
         return GEQ_OK;
     }
     if (!tmp.p || tmp.n < nb + 1) {
-        tmp.release();
+        tmp.retire();
         GEQ_HIP(tmp.alloc(nb + 1), GEQ_ERR_OUT_OF_MEMORY);
     }
     k_scan_reduce<T, Op><<<(uint32_t)nb, SCAN_THREADS, 0, s>>>(in, n, tmp.p, op);

@@ -210,6 +210,7 @@ struct geq_server {
     std::vector<geq_ctx*> bctx;        // builders' contexts: a stream each
     geq_game* pool = nullptr;          // pools of positions / edges / energies
     DevBuf<uint32_t> job_of, seed_cnt, place_cnt, place_fill, place_excl;
+    DevBuf<uint32_t> ret_nodes;        // start positions of the jobs retired in a step
     DevBuf<uint8_t> active;
     DevBuf<uint32_t> scan_tmp;   // block totals of the placement scans
     DevBuf<uint32_t*> wl_ptrs;
@@ -321,9 +322,9 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
             }
             std::fprintf(stderr,
                          "[geq server] t=%.1f ms build of %zu jobs: %llu positions, %u levels, %llu over the cap, %.1f ms "
-                         "(level loop %.1f ms; status %d)\n",
+                         "(merge %.1f, level loop %.1f, extraction %.1f ms; status %d)\n",
                          tb0, build.size(), (unsigned long long)pos, B->levels, (unsigned long long)dead, S->now_ms() - tb0,
-                         B->build_ms, (int)st);
+                         B->prep_ms, B->build_ms, B->extract_ms, (int)st);
         }
         std::lock_guard<std::mutex> lk(S->mu);
         const double now = S->now_ms();
@@ -558,47 +559,61 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         }
     }
     S->cv.notify_all();
-    for (Job* J : admit) {
-        const uint32_t V = J->V, E = J->E;
-        const BuiltJobs& Bj = *J->batch;
-        const uint32_t* brow = Bj.row.p + Bj.pos_start[J->bj] + J->bj;
-        const uint32_t* bcol = Bj.col.p + Bj.edge_start[J->bj];
-        const uint32_t* bwt = Bj.wt.p + Bj.edge_start[J->bj];
-        const uint8_t* batk = Bj.atk.p + Bj.pos_start[J->bj];
-        // the builder's stream finished this game before it was queued as BUILT
-        const uint32_t t0 = round_tag(g->h_ctl->round);   // the next round to run
-        k_place_nodes<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, J->eb, J->slot, t0, g->row.p, g->atk.p, g->st.p,
-                                                           S->job_of.p);
-        if (E) k_place_edges<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, bwt, E, J->nb, J->eb, g->edges.p);
-        // reverse CSR of the job's range
-        GEQ_TRY(ensure_buf(S->place_cnt, V + 1, s));
-        GEQ_TRY(ensure_buf(S->place_excl, V + 1, s));
-        GEQ_TRY(ensure_buf(S->place_fill, V + 1, s));
-        GEQ_HIP(hipMemsetAsync(S->place_cnt.p, 0, 4ull * (V + 1), s), GEQ_ERR_TRANSFER);
-        if (E) k_place_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, E, S->place_cnt.p);
-        GEQ_TRY((exclusive_scan_dev<uint32_t, SumU32>(S->place_cnt.p, S->place_excl.p, V, S->scan_tmp, s, SumU32{})));
-        k_place_rrow<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(S->place_excl.p, V, J->eb, J->nb, g->rrow.p, S->place_fill.p);
-        k_place_rcol<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, bcol, V, J->nb, J->eb, S->place_fill.p, g->rcol.p);
-        k_node_hdr<<<(V + tb - 1) / tb, tb, 0, s>>>(g->row.p + J->nb, g->rrow.p + J->nb, g->atk.p + J->nb, V,
-                                                    g->st.p + (size_t)NODE_U4 * J->nb);
-        // nodes to reach: zero energy + seeds of the current round
-        GEQ_TRY(ensure_buf(g->seed, V, s));
+    // Placement of the admitted jobs: their kernels back to back, their seeds
+    // in one list, then one wait and one seeding expansion for all of them.
+    if (!admit.empty()) {
+        uint64_t sum_v = 0;
+        uint32_t max_v = 0;
+        for (Job* J : admit) {
+            sum_v += J->V;
+            max_v = std::max(max_v, J->V);
+        }
+        GEQ_TRY(ensure_buf(g->seed, sum_v, s));
+        GEQ_TRY(ensure_buf(S->place_cnt, max_v + 1ull, s));
+        GEQ_TRY(ensure_buf(S->place_excl, max_v + 1ull, s));
+        GEQ_TRY(ensure_buf(S->place_fill, max_v + 1ull, s));
         GEQ_HIP(hipMemsetAsync(S->seed_cnt.p, 0, 4, s), GEQ_ERR_TRANSFER);
-        k_place_reach<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, t0, g->st.p, g->seed.p, S->seed_cnt.p);
+        // the next round to run; the seeds' slots are tagged relative to it
+        const uint32_t t0 = round_tag(g->h_ctl->round);
+        for (Job* J : admit) {
+            const uint32_t V = J->V, E = J->E;
+            const BuiltJobs& Bj = *J->batch;
+            const uint32_t* brow = Bj.row.p + Bj.pos_start[J->bj] + J->bj;
+            const uint32_t* bcol = Bj.col.p + Bj.edge_start[J->bj];
+            const uint32_t* bwt = Bj.wt.p + Bj.edge_start[J->bj];
+            const uint8_t* batk = Bj.atk.p + Bj.pos_start[J->bj];
+            // the builder's stream finished this game before it was queued as BUILT
+            k_place_nodes<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, J->eb, J->slot, t0, g->row.p, g->atk.p,
+                                                               g->st.p, S->job_of.p);
+            if (E) k_place_edges<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, bwt, E, J->nb, J->eb, g->edges.p);
+            // reverse CSR of the job's range (the scratch is reused by the next job, in stream order)
+            GEQ_HIP(hipMemsetAsync(S->place_cnt.p, 0, 4ull * (V + 1), s), GEQ_ERR_TRANSFER);
+            if (E) k_place_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(bcol, E, S->place_cnt.p);
+            GEQ_TRY((exclusive_scan_dev<uint32_t, SumU32>(S->place_cnt.p, S->place_excl.p, V, S->scan_tmp, s, SumU32{})));
+            k_place_rrow<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(S->place_excl.p, V, J->eb, J->nb, g->rrow.p, S->place_fill.p);
+            k_place_rcol<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, bcol, V, J->nb, J->eb, S->place_fill.p, g->rcol.p);
+            k_node_hdr<<<(V + tb - 1) / tb, tb, 0, s>>>(g->row.p + J->nb, g->rrow.p + J->nb, g->atk.p + J->nb, V,
+                                                        g->st.p + (size_t)NODE_U4 * J->nb);
+            // nodes to reach: zero energy, and the seed list of this step
+            k_place_reach<<<(V + tb - 1) / tb, tb, 0, s>>>(brow, batk, V, J->nb, t0, g->st.p, g->seed.p, S->seed_cnt.p);
+        }
         uint32_t n_seed = 0;
         GEQ_HIP(hipMemcpyAsync(&n_seed, S->seed_cnt.p, 4, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
         if (n_seed) {
+            // the predecessors of every admitted job's seeds join the current round's list
             ExpandArgs sx = expand_args(g);
             sx.seed = g->seed.p;
             sx.n_seed = n_seed;
             k_expand<<<std::max(1u, std::min(1024u, (n_seed + 64u * WAVES - 1) / (64u * WAVES))), BLOCK, 0, s>>>(sx, 0u);
         }
-        J->batch.reset();   // the pools hold the game now (the build is freed with its last job)
         std::lock_guard<std::mutex> lk(S->mu);
-        J->t_admit = S->now_ms();
-        J->state = ACTIVE;
-        S->resident.push_back(J);
+        for (Job* J : admit) {
+            J->batch.reset();   // the pools hold the game now (the build is freed with its last job)
+            J->t_admit = S->now_ms();
+            J->state = ACTIVE;
+            S->resident.push_back(J);
+        }
     }
     // ---- rounds over every resident job
     if (!S->resident.empty()) {
@@ -633,34 +648,45 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
         k_mark_active<<<256, 256, 0, s>>>(g->ctl.p, S->wl_ptrs.p, S->job_of.p, S->active.p);
         GEQ_HIP(hipMemcpyAsync(S->h_active.data(), S->active.p, S->max_jobs, hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
         GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
-        std::vector<Job*> still;
-        uint32_t retired = 0;
-        for (Job* J : S->resident) {
-            if (S->h_active[J->slot]) {
-                still.push_back(J);
-                continue;
-            }
+        std::vector<Job*> still, done_jobs;
+        for (Job* J : S->resident) (S->h_active[J->slot] ? still : done_jobs).push_back(J);
+        const uint32_t retired = (uint32_t)done_jobs.size();
+        if (retired) {
+            // the start positions of every retired job, downloaded together
+            std::vector<uint32_t> nodes;
+            for (Job* J : done_jobs)
+                for (uint32_t k = 0; k < J->n_start; k++) nodes.push_back(J->nb + k);
             geq_energies res{};
-            geq_status st = download_energies(g, J->nb, J->n_start, &res);
+            geq_status st = ensure_buf(S->ret_nodes, nodes.size(), s);
+            if (st == GEQ_OK && hipMemcpyAsync(S->ret_nodes.p, nodes.data(), 4 * nodes.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+                st = fail(GEQ_ERR_TRANSFER, "server: uploading the retired jobs' positions");
+            if (st == GEQ_OK)
+                st = download_mapped(g, g->st.p, S->ret_nodes.p, (uint32_t)nodes.size(), round_tag(g->h_ctl->round), &res);
+            const std::string err = st != GEQ_OK ? std::string(geq_last_error()) : std::string();
             std::lock_guard<std::mutex> lk(S->mu);
-            if (st != GEQ_OK) {
-                J->status = st;
-                J->error = geq_last_error();
-                J->state = FAILED;
-            } else {
-                J->offs.assign(res.offsets, res.offsets + J->n_start + 1);
-                J->data.assign(res.data, res.data + res.offsets[J->n_start]);
-                geq_energies_free(&res);
-                J->state = DONE;
-                S->jobs_done++;
+            size_t at = 0;
+            for (Job* J : done_jobs) {
+                if (st != GEQ_OK) {
+                    J->status = st;
+                    J->error = err;
+                    J->state = FAILED;
+                } else {
+                    const uint32_t b = res.offsets[at], e = res.offsets[at + J->n_start];
+                    J->offs.resize(J->n_start + 1);
+                    for (uint32_t k = 0; k <= J->n_start; k++) J->offs[k] = res.offsets[at + k] - b;
+                    J->data.assign(res.data + b, res.data + e);
+                    J->state = DONE;
+                    S->jobs_done++;
+                }
+                at += J->n_start;
+                J->t_done = S->now_ms();
+                S->finished.push_back(J->id);
+                k_clear_nodes<<<(J->V + 1 + tb - 1) / tb, tb, 0, s>>>(g->st.p, J->nb, J->V + 1);
+                S->nodes.release(J->nb, J->V + 1);
+                S->edges.release(J->eb, J->E);
+                S->free_slots.push_back(J->slot);
             }
-            J->t_done = S->now_ms();
-            S->finished.push_back(J->id);
-            k_clear_nodes<<<(J->V + 1 + tb - 1) / tb, tb, 0, s>>>(g->st.p, J->nb, J->V + 1);
-            S->nodes.release(J->nb, J->V + 1);
-            S->edges.release(J->eb, J->E);
-            S->free_slots.push_back(J->slot);
-            retired++;
+            if (st == GEQ_OK) geq_energies_free(&res);
         }
         S->resident.swap(still);
         const uint64_t top = g->h_ctl->arena_top;

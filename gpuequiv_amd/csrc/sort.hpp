@@ -108,7 +108,7 @@ static inline geq_status radix_sort_pairs(const uint32_t* key_in, const uint32_t
     const size_t nh = nb * RADIX_THREADS;
     if (nh >= 0xffffffffull) return fail(GEQ_ERR_INVALID_ARG, "radix sort of more than 2^32 items");
     if (!sc.hist.p || sc.hist.n < nh + 1) {
-        sc.hist.release();
+        sc.hist.retire();
         GEQ_HIP(sc.hist.alloc(nh + 1), GEQ_ERR_OUT_OF_MEMORY);
     }
     const uint32_t* sk = key_in;
