@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "gamebuild.hpp"
+#include "scan.hpp"
 
 namespace geq {
 namespace gb {
@@ -195,17 +196,22 @@ __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, u
     }
 }
 
-__global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, uint32_t add, uint32_t n_slots, uint32_t chunk,
-                              LtsArgs L, uint2* ub, HeavyArgs H) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_slots) return;
-    const uint32_t n_front = *n_new + add;
+// bound entry i of the next level (frontier of n_front positions from lo)
+__device__ __forceinline__ void bounds_next_at(const NodeArgs& N, uint32_t lo, uint32_t n_front, uint32_t chunk, const LtsArgs& L,
+                                               uint2* ub, const HeavyArgs& H, uint32_t i) {
     bool heavy = false;
     ub[i] = i < n_front ? bounds_of(N, L, lo + i, H.heavy_min, heavy) : make_uint2(0u, 0u);
     if (i < min(n_front, chunk)) {      // the flags and list cover the first chunk
         H.flag[i] = heavy ? 1 : 0;
         if (heavy) H.list[atomicAdd(H.count, 1u)] = i;
     }
+}
+
+__global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, uint32_t add, uint32_t n_slots, uint32_t chunk,
+                              LtsArgs L, uint2* ub, HeavyArgs H) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    bounds_next_at(N, lo, *n_new + add, chunk, L, ub, H, i);
 }
 
 // Dynamic LDS: 0, or 4 * lts_words(...) bytes for a copy of the LTS, which
@@ -638,9 +644,9 @@ __device__ bool cand_equals_node(const Cand& a, const NodeArgs& N, uint32_t id) 
            sets_equal(N.spool, a.q_off, N.q_off[id], a.q_len) && sets_equal(N.spool, a.x_off, N.x_off[id], a.x_len);
 }
 
-__global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_cand) return;
+// candidate j finds its position in the table, or claims a pending entry
+__device__ __forceinline__ void insert_cand(Cand* cands, uint32_t j, const NodeArgs& N, unsigned long long* table, uint32_t* keymin,
+                                            uint32_t mask) {
     Cand c = cands[j];
     if (c.kind_p == INVALID) return;
     const uint32_t h = c.hash;
@@ -674,18 +680,26 @@ __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long
     }
 }
 
-__global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag) {
+__global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j > n_cand) return;
-    if (j == n_cand) { flag[j] = 0; return; }
-    const Cand& c = cands[j];
-    flag[j] = (c.kind_p != INVALID && (c.res & PENDING) && keymin[c.res & ~PENDING] == j) ? 1u : 0u;
+    if (j < n_cand) insert_cand(cands, j, N, table, keymin, mask);
 }
 
-__global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,
-                           NodeArgs N, unsigned long long* table, uint32_t* keymin) {
+// 1 if candidate j is the first of a new position (j == n_cand: the trailing 0)
+__device__ __forceinline__ uint32_t mark_cand(const Cand* cands, uint32_t j, uint32_t n_cand, const uint32_t* keymin) {
+    if (j == n_cand) return 0u;
+    const Cand& c = cands[j];
+    return (c.kind_p != INVALID && (c.res & PENDING) && keymin[c.res & ~PENDING] == j) ? 1u : 0u;
+}
+
+__global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_cand || !flag[j]) return;
+    if (j <= n_cand) flag[j] = mark_cand(cands, j, n_cand, keymin);
+}
+
+// a new position's record and table entry (candidate j, flagged)
+__device__ __forceinline__ void finalize_cand(const Cand* cands, uint32_t j, const uint32_t* rank, uint32_t v_base, const NodeArgs& N,
+                                              unsigned long long* table, uint32_t* keymin) {
     const Cand c = cands[j];
     const uint32_t id = v_base + rank[j];
     const uint32_t s = c.res & ~PENDING;
@@ -698,16 +712,27 @@ __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* f
     keymin[s] = 0xffffffffu;
 }
 
-__global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
-                        uint32_t* col, uint32_t* wt) {
+__global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,
+                           NodeArgs N, unsigned long long* table, uint32_t* keymin) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_cand) return;
+    if (j < n_cand && flag[j]) finalize_cand(cands, j, rank, v_base, N, table, keymin);
+}
+
+// candidate j's edge: its parent's row, its slot, the target's id
+__device__ __forceinline__ void edge_cand(const Cand* cands, uint32_t j, const uint32_t* rowl, uint32_t e_base,
+                                          const unsigned long long* table, uint32_t* col, uint32_t* wt) {
     const Cand c = cands[j];
     if (c.kind_p == INVALID) return;
     const uint32_t target = (c.res & PENDING) ? (uint32_t)table[c.res & ~PENDING] : c.res;
     const uint32_t e = e_base + rowl[c.parent] + c.slot;
     col[e] = target;
     wt[e] = c.weight;
+}
+
+__global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
+                        uint32_t* col, uint32_t* wt) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n_cand) edge_cand(cands, j, rowl, e_base, table, col, wt);
 }
 
 __global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row, RowsTail T) {
@@ -719,6 +744,83 @@ __global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint
             for (uint32_t j = threadIdx.x; j < T.n_jobs; j += blockDim.x)
                 if (T.cnt[j] > T.cap) T.dead[j] = 1;
     }
+}
+
+// count word i of a chunk (see k_chunk_counts)
+__device__ __forceinline__ uint32_t chunk_word(uint32_t i, const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk,
+                                               const uint2* next, const uint32_t* heavy, const uint32_t* cnt) {
+    if (i == RB_EDGES) return *n_edges;
+    if (i == RB_NEW) return *n_new;
+    if (i == RB_CAND || i == RB_WORDS) {
+        if (!next) return 0u;
+        const uint2 t = next[min(*n_new + add, chunk)];
+        return i == RB_CAND ? t.x : t.y;
+    }
+    if (i == RB_HEAVY) return next ? *heavy : 0u;
+    return cnt[i - RB_JOBS];
+}
+
+__global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
+                               const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out) {
+    for (uint32_t i = threadIdx.x; i < RB_JOBS + n_cnt; i += blockDim.x) out[i] = chunk_word(i, n_edges, n_new, add, chunk, next, heavy, cnt);
+}
+
+// Exclusive scan of in[0..n) into out[0..n], out[n] the total, by one block of
+// TAIL_BLOCK threads (four consecutive items each; n <= TAIL_MAX).
+template <class T, class Op>
+__device__ void tail_scan(const T* in, T* out, uint32_t n, Op op) {
+    const uint32_t b = 4u * threadIdx.x;
+    T v[4];
+    T acc = Op::identity();
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+        v[k] = b + k < n ? in[b + k] : Op::identity();
+        acc = op(acc, v[k]);
+    }
+    T total;
+    T run = detail::block_scan_excl<T, Op, TAIL_BLOCK>(acc, op, total);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+        if (b + k < n) out[b + k] = run;
+        run = op(run, v[k]);
+    }
+    if (threadIdx.x == 0) out[n] = total;
+}
+
+__global__ __launch_bounds__(TAIL_BLOCK) void k_level_tail(TailArgs T) {
+    const uint32_t t = threadIdx.x;
+    // edge offsets of the chunk's positions (k_generate wrote their successor counts)
+    tail_scan<uint32_t, SumU32>(T.nsucc, T.rowl, T.nf, SumU32{});
+    for (uint32_t j = t; j < T.n_cand; j += TAIL_BLOCK) insert_cand(T.cands, j, T.N, T.table, T.keymin, T.mask);
+    __syncthreads();
+    for (uint32_t j = t; j <= T.n_cand; j += TAIL_BLOCK) T.flag[j] = mark_cand(T.cands, j, T.n_cand, T.keymin);
+    __syncthreads();
+    tail_scan<uint32_t, SumU32>(T.flag, T.rank, T.n_cand, SumU32{});
+    __syncthreads();
+    for (uint32_t j = t; j < T.n_cand; j += TAIL_BLOCK)
+        if (T.flag[j]) finalize_cand(T.cands, j, T.rank, T.v_base, T.N, T.table, T.keymin);
+    __syncthreads();   // edges read the table entries finalize wrote, and the row offsets
+    for (uint32_t j = t; j < T.n_cand; j += TAIL_BLOCK) {
+        edge_cand(T.cands, j, T.rowl, T.e_base, T.table, T.col, T.wt);
+        if (T.job_cnt && T.flag[j]) atomicAdd(&T.job_cnt[T.L.job_of_state[T.cands[j].kind_p & 0x3fffffffu]], 1u);
+    }
+    for (uint32_t i = t; i < T.nf; i += TAIL_BLOCK) T.row[T.lo + i] = T.e_base + T.rowl[i];
+    __syncthreads();   // the per-job counts are complete
+    if (T.job_dead && T.cap)
+        for (uint32_t j = t; j < T.n_jobs; j += TAIL_BLOCK)
+            if (T.job_cnt[j] > T.cap) T.job_dead[j] = 1;
+    if (T.last) {
+        if (t == 0) *T.H.count = 0u;
+        __syncthreads();   // the dead jobs and the cleared heavy count before the bounds
+        const uint32_t n_front = T.rank[T.n_cand] + T.add;   // the positions this level created
+        for (uint32_t i = t; i < T.slots; i += TAIL_BLOCK) bounds_next_at(T.N, T.hi, n_front, T.chunk, T.L, T.ub, T.H, i);
+        __syncthreads();
+        tail_scan<uint2, PairSum>(T.ub, T.off, T.slots - 1u, PairSum{T.lim});
+    }
+    __syncthreads();
+    const uint32_t n_cnt = T.job_cnt ? T.n_jobs : 0u;
+    for (uint32_t i = t; i < RB_JOBS + n_cnt; i += TAIL_BLOCK)
+        T.rb[i] = chunk_word(i, T.rowl + T.nf, T.rank + T.n_cand, T.add, T.chunk, T.last ? T.off : nullptr, T.H.count, T.job_cnt);
 }
 
 // start positions Attack(p, [q]) at ids [0, n): node records + hash entries

@@ -113,6 +113,65 @@ __global__ void k_fill_u64(unsigned long long* a, uint64_t n, unsigned long long
 __global__ void k_fill_u32(uint32_t* a, uint64_t n, uint32_t v);
 __global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk);
 
+// Words of the pinned readback block: a chunk's totals, the heavy positions
+// of the chunk bounded last, then the per-job position counts.
+constexpr uint32_t RB_EDGES = 0, RB_NEW = 1, RB_CAND = 2, RB_WORDS = 3, RB_HEAVY = 4, RB_JOBS = 5;
+
+// Sum of (candidates, set-pool words) bound pairs, saturating at `lim`
+// (saturating addition of non-negative values is associative, so the scan
+// stays exact below lim).  A level whose running total reaches lim is refused
+// before any candidate is written: its offsets no longer fit 32 bits, and a
+// wrapped total would size the buffers below the offsets k_generate writes at.
+struct PairSum {
+    uint32_t lim;
+    __host__ __device__ uint2 operator()(const uint2& a, const uint2& b) const {
+        const uint64_t x = (uint64_t)a.x + b.x, y = (uint64_t)a.y + b.y;
+        return make_uint2(x < lim ? (uint32_t)x : lim, y < lim ? (uint32_t)y : lim);
+    }
+    __host__ __device__ static uint2 identity() { return make_uint2(0u, 0u); }
+};
+
+// The count words at the end of a chunk, in pinned host memory: its edges, its
+// new positions and the per-job position counts cnt.  With `next` (the offsets
+// k_bounds_next scanned for the next level, whose *n_new + add positions are
+// taken in chunks of `chunk`), also the candidate and set-pool totals and the
+// heavy positions (*heavy) of the next level's first chunk.
+__global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
+                               const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out);
+
+// A chunk whose frontier, candidates and next-level bounds each fit TAIL_MAX
+// items runs everything after k_generate(_heavy) as one block of TAIL_BLOCK
+// threads, its phases separated by block barriers: the row scan, inserts,
+// marks, the rank scan, finalize, edges, per-job counts, rows, dead jobs, the
+// next level's bounds and their scan, and the count words.  The same per-item
+// steps as the kernels of larger chunks, so the same positions, ids and edges.
+constexpr uint32_t TAIL_BLOCK = 1024, TAIL_MAX = 4 * TAIL_BLOCK;
+struct TailArgs {
+    NodeArgs N;
+    LtsArgs L;
+    HeavyArgs H;
+    Cand* cands;
+    const uint32_t* nsucc;
+    uint32_t* rowl;
+    uint32_t* flag;
+    uint32_t* rank;
+    unsigned long long* table;
+    uint32_t* keymin;
+    uint32_t* col;
+    uint32_t* wt;
+    uint32_t* row;
+    uint2* ub;                 // last chunk: the next level's bounds ...
+    uint2* off;                // ... and their scan
+    uint32_t* job_cnt;         // merged builds: per-job counts (NULL otherwise)
+    uint8_t* job_dead;
+    uint32_t n_jobs;
+    uint64_t cap;              // merged builds: the position cap per job (0: none)
+    uint32_t* rb;              // pinned count words
+    uint32_t lo, nf, n_cand, v_base, e_base, mask;
+    uint32_t hi, add, slots, chunk, last, lim;
+};
+__global__ void k_level_tail(TailArgs T);
+
 // Pinned host words that kernels write directly (the level loop's counts).
 struct HostWords {
     uint32_t* h = nullptr;   // host view

@@ -69,20 +69,6 @@ geq_status exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, DevBuf<ui
     return exclusive_scan_dev<uint32_t, SumU32>(in, out, n, tmp, s, SumU32{});
 }
 
-// Sum of (candidates, set-pool words) bound pairs, saturating at `lim`
-// (saturating addition of non-negative values is associative, so the scan
-// stays exact below lim).  A level whose running total reaches lim is refused
-// before any candidate is written: its offsets no longer fit 32 bits, and a
-// wrapped total would size the buffers below the offsets k_generate writes at.
-struct PairSum {
-    uint32_t lim;
-    __host__ __device__ uint2 operator()(const uint2& a, const uint2& b) const {
-        const uint64_t x = (uint64_t)a.x + b.x, y = (uint64_t)a.y + b.y;
-        return make_uint2(x < lim ? (uint32_t)x : lim, y < lim ? (uint32_t)y : lim);
-    }
-    __host__ __device__ static uint2 identity() { return make_uint2(0u, 0u); }
-};
-
 // The saturation point of the bound scans: 2^32 - 1, or GEQ_BUILD_LEVEL_LIMIT
 // (lower values exercise the refusal in tests).
 uint32_t level_limit() {
@@ -99,10 +85,6 @@ geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<ui
     return exclusive_scan_dev<uint2, PairSum>(in, out, n, tmp, s, PairSum{level_limit()});
 }
 
-// Words of the pinned readback block: a chunk's totals, the heavy positions
-// of the chunk bounded last, then the per-job position counts.
-constexpr uint32_t RB_EDGES = 0, RB_NEW = 1, RB_CAND = 2, RB_WORDS = 3, RB_HEAVY = 4, RB_JOBS = 5;
-
 // Writes a chunk's candidate and set-pool totals (*a, *b) and its heavy
 // positions (*h) to pinned host memory: they reach the host with one launch
 // and one wait.
@@ -110,33 +92,6 @@ __global__ void k_readback(const uint32_t* a, const uint32_t* b, const uint32_t*
     if (threadIdx.x == 0) out[RB_CAND] = *a;
     if (threadIdx.x == 1) out[RB_WORDS] = *b;
     if (threadIdx.x == 2) out[RB_HEAVY] = *h;
-}
-
-// The counts at the end of a chunk, in pinned host memory: its edges, its new
-// positions and the per-job position counts cnt.  With `next` (the offsets
-// k_bounds_next scanned for the next level, whose *n_new + add positions are
-// taken in chunks of `chunk`), also the candidate and set-pool totals and the
-// heavy positions (*heavy) of the next level's first chunk.
-__global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
-                               const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out) {
-    for (uint32_t i = threadIdx.x; i < RB_JOBS + n_cnt; i += blockDim.x) {
-        uint32_t v = 0;
-        if (i == RB_EDGES) {
-            v = *n_edges;
-        } else if (i == RB_NEW) {
-            v = *n_new;
-        } else if (i == RB_CAND || i == RB_WORDS) {
-            if (next) {
-                const uint2 t = next[min(*n_new + add, chunk)];
-                v = i == RB_CAND ? t.x : t.y;
-            }
-        } else if (i == RB_HEAVY) {
-            v = next ? *heavy : 0u;
-        } else {
-            v = cnt[i - RB_JOBS];
-        }
-        out[i] = v;
-    }
 }
 
 }  // namespace
@@ -528,6 +483,14 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         const long long v = std::atoll(env);
         if (v >= 0) heavy_min = (uint32_t)std::min<long long>(v, 0xffffffffll);
     }
+    // Chunks whose frontier, candidates and next bounds fit tail_max items
+    // finish in one block (k_level_tail); GEQ_BUILD_TAIL_MAX lowers the limit
+    // (0: never), so tests take both paths.
+    uint32_t tail_max = TAIL_MAX;
+    if (const char* env = std::getenv("GEQ_BUILD_TAIL_MAX")) {
+        const long long v = std::atoll(env);
+        if (v >= 0) tail_max = (uint32_t)std::min<long long>(v, TAIL_MAX);
+    }
     GB_TRY(ensure(ws.hv_count, 1, s, false, "heavy positions"));
     auto heavy_args = [&]() { return HeavyArgs{ws.hv_flag.p, ws.hv_list.p, ws.hv_count.p, heavy_min}; };
     // The last chunk of a level also bounds and scans the next level before
@@ -575,48 +538,60 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (n_heavy)   // k_bounds listed them (their count came back with the totals)
             k_generate_heavy<<<std::min<uint32_t>(n_heavy, 1024), 256, 0, s>>>(nodes(), lo, L, off.p, B->spool_top, heavy_args(),
                                                                           cands.p, nsucc.p);
-        GB_TRY(exclusive_scan(nsucc.p, rowl.p, nf, ws.scan32, s));
-        if (n_cand) {
-            k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
-            k_mark<<<(n_cand + 1 + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, keymin.p, flag.p);
-        } else {
-            GB_HIP(hipMemsetAsync(flag.p, 0, 4, s), GEQ_ERR_TRANSFER);
-        }
-        GB_TRY(exclusive_scan(flag.p, rank.p, n_cand, ws.scan32, s));
-        if (n_cand) {
-            k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, flag.p, rank.p, B->V, nodes(), table.p, keymin.p);
-            k_edges<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p, B->wt.p);
-            if (jt) {
-                // this level's new positions per job (the winners among the candidates)
-                const uint32_t nb = std::min<uint32_t>(1024, (n_cand + tb - 1) / tb);
-                k_job_count_cands<<<nb, tb, 0, s>>>(cands.p, flag.p, n_cand, d_job_of_state.p, n_jobs, d_jcnt.p);
-            }
-        }
         // After a level's last chunk, the next level is the positions the level
         // created: `add` by its earlier chunks and *(rank + n_cand) by this one.
         const bool last_chunk = c_hi == hi;
-        // the rows; the heavy count cleared for k_bounds_next; jobs past the cap marked
-        const bool mark = jt && jt->cap;
-        k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p,
-                                                 RowsTail{last_chunk ? ws.hv_count.p : nullptr, mark ? d_jcnt.p : nullptr,
-                                                          mark ? n_jobs : 0u, mark ? jt->cap : 0ull, mark ? d_dead.p : nullptr});
         const uint32_t add = B->V - hi;
+        const uint64_t slots = last_chunk ? (uint64_t)add + n_cand + 1 : 0;
+        if (slots > 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 positions"));
         if (last_chunk) {
-            const uint64_t slots = (uint64_t)add + n_cand + 1;
-            if (slots > 0xffffffffull) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: more than 2^32 positions"));
             GB_TRY(ensure(ub, slots, s, false, "frontier"));
             GB_TRY(ensure(off, slots, s, false, "frontier"));
             GB_TRY(ensure(ws.hv_flag, slots, s, false, "heavy positions"));
             GB_TRY(ensure(ws.hv_list, slots, s, false, "heavy positions"));
-            k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots,
-                                                                           chunk, L, ub.p, heavy_args());
-            GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots - 1, ws.scan2, s));
         }
-        // one synchronisation per chunk for the edge and position counts, the per-job
-        // counts and the next level's first totals
-        k_chunk_counts<<<1, 256, 0, s>>>(rowl.p + nf, rank.p + n_cand, add, chunk, last_chunk ? off.p : nullptr,
-                                         ws.hv_count.p, jt ? d_jcnt.p : nullptr, n_jobs, ws.rb.d);
-        GB_TRY(wait());
+        const bool mark = jt && jt->cap;
+        if (nf <= tail_max && n_cand <= tail_max && (!last_chunk || slots - 1 <= tail_max)) {
+            // a small chunk: the rest of it in one block
+            const TailArgs T{nodes(), L, heavy_args(), cands.p, nsucc.p, rowl.p, flag.p, rank.p, table.p, keymin.p, B->col.p,
+                             B->wt.p, B->row.p, ub.p, off.p, jt ? d_jcnt.p : nullptr, mark ? d_dead.p : nullptr, n_jobs,
+                             mark ? jt->cap : 0ull, ws.rb.d, lo, nf, n_cand, B->V, (uint32_t)E, table_mask, hi, add,
+                             (uint32_t)slots, chunk, last_chunk ? 1u : 0u, level_limit()};
+            k_level_tail<<<1, TAIL_BLOCK, 0, s>>>(T);
+            GB_TRY(wait());
+        } else {
+            GB_TRY(exclusive_scan(nsucc.p, rowl.p, nf, ws.scan32, s));
+            if (n_cand) {
+                k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
+                k_mark<<<(n_cand + 1 + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, keymin.p, flag.p);
+            } else {
+                GB_HIP(hipMemsetAsync(flag.p, 0, 4, s), GEQ_ERR_TRANSFER);
+            }
+            GB_TRY(exclusive_scan(flag.p, rank.p, n_cand, ws.scan32, s));
+            if (n_cand) {
+                k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, flag.p, rank.p, B->V, nodes(), table.p, keymin.p);
+                k_edges<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p, B->wt.p);
+                if (jt) {
+                    // this level's new positions per job (the winners among the candidates)
+                    const uint32_t nb = std::min<uint32_t>(1024, (n_cand + tb - 1) / tb);
+                    k_job_count_cands<<<nb, tb, 0, s>>>(cands.p, flag.p, n_cand, d_job_of_state.p, n_jobs, d_jcnt.p);
+                }
+            }
+            // the rows; the heavy count cleared for k_bounds_next; jobs past the cap marked
+            k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p,
+                                                     RowsTail{last_chunk ? ws.hv_count.p : nullptr, mark ? d_jcnt.p : nullptr,
+                                                              mark ? n_jobs : 0u, mark ? jt->cap : 0ull, mark ? d_dead.p : nullptr});
+            if (last_chunk) {
+                k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots,
+                                                                               chunk, L, ub.p, heavy_args());
+                GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots - 1, ws.scan2, s));
+            }
+            // one synchronisation per chunk for the edge and position counts, the per-job
+            // counts and the next level's first totals
+            k_chunk_counts<<<1, 256, 0, s>>>(rowl.p + nf, rank.p + n_cand, add, chunk, last_chunk ? off.p : nullptr,
+                                             ws.hv_count.p, jt ? d_jcnt.p : nullptr, n_jobs, ws.rb.d);
+            GB_TRY(wait());
+        }
         const uint32_t n_edges = h_rb[RB_EDGES], n_new = h_rb[RB_NEW];
         if (last_chunk) {
             have_next = true;

@@ -144,6 +144,30 @@ def test_block_generated_positions_match_oracle(monkeypatch, heavy_min, label_ma
         assert np.array_equal(h[k], og2[k]), k
 
 
+@pytest.mark.parametrize("tail_max", ["0", "100", "4096"])
+def test_one_block_chunks_match_oracle(monkeypatch, tail_max):
+    # Chunks whose frontier, candidates and next bounds fit GEQ_BUILD_TAIL_MAX
+    # items finish in one block (0: none; 100: the small levels of this build;
+    # 4096, the default: most of them).  Same game and energies every way,
+    # a capped build included.
+    print("one-block chunks up to", tail_max, flush=True)
+    monkeypatch.setenv("GEQ_BUILD_TAIL_MAX", tail_max)
+    n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    lts, _ = lts.bisimilar_minimize()
+    pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q]
+    b = GameBuild(lts, pairs)
+    og = _assert_same_game(b, lts, pairs)
+    _assert_same_energies(b, og)
+    b2 = GameBuild(lts, pairs[:40], max_positions=3000)
+    og2 = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs[:40], max_positions=3000,
+                        positions_wanted=False)
+    h = b2._download()
+    assert bool(b2.build_stats["truncated"]) == og2["truncated"] and b2.n_positions() == og2["V"]
+    for k in ("row", "col", "wt"):
+        assert np.array_equal(h[k], og2[k]), k
+
+
 @pytest.mark.parametrize("heavy_min,label_mask", [("1024", "1"), ("1024", "0"), ("1", "1")])
 def test_large_lts_capped_build_matches_oracle(monkeypatch, heavy_min, label_mask):
     # An LTS too large for k_generate's LDS copy: its walks read global memory,
