@@ -409,14 +409,28 @@ def bench_serve(args) -> dict:
 
     # plain path: each job alone (build, solve, read the starting budgets back)
     service, sizes, plain_res, plain_edges = [], [], [], 0
+    def build_alone(lts, pairs):
+        # None: the job fails as the server fails it (a level past the 32-bit
+        # totals, or device memory)
+        try:
+            return GameBuild(lts, pairs, max_positions=max_pos)
+        except _lib.Error:
+            return None
+
     for lts, pairs in jobs[:8]:                       # warm-up
-        b = GameBuild(lts, pairs, max_positions=max_pos)
-        if not b.build_stats["truncated"]:
+        b = build_alone(lts, pairs)
+        if b is not None and not b.build_stats["truncated"]:
             b.solve()
-        b.close()
+        if b is not None:
+            b.close()
     for lts, pairs in jobs:
         t0 = time.perf_counter()
-        b = GameBuild(lts, pairs, max_positions=max_pos)
+        b = build_alone(lts, pairs)
+        if b is None:
+            service.append(time.perf_counter() - t0)
+            sizes.append((0, 0))
+            plain_res.append(None)
+            continue
         if b.build_stats["truncated"]:
             res = None
         else:

@@ -138,7 +138,8 @@ class geq_job_info(C.Structure):
 class geq_server_stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("steps", "rounds", "compactions", "node_slots", "node_slots_used", "edge_slots",
                                           "edge_slots_used", "arena_entries", "arena_top", "jobs_resident",
-                                          "jobs_waiting", "jobs_done", "builds", "split_builds")]
+                                          "jobs_waiting", "jobs_done", "builds", "split_builds",
+                                          "deferred_admissions")]
 
     def as_dict(self) -> dict:
         return {name: int(getattr(self, name)) for name, _ in self._fields_}

@@ -47,6 +47,9 @@ hipError_t device_malloc(void** p, size_t bytes) {
             e = hipMalloc(p, bytes);
         }
     }
+    // the caller gets the error from the return value: it must not stay behind
+    // for the next hipGetLastError (a launch check) to report
+    if (e != hipSuccess) (void)hipGetLastError();
     return e;
 }
 
@@ -95,11 +98,18 @@ RetireScope::~RetireScope() {
     auto& v = pool->bufs;
     v.insert(v.end(), t_retired.bufs.begin(), t_retired.bufs.end());
     t_retired.bufs.clear();
-    if (v.size() > SPARE_BUFS) {   // keep the largest
-        std::sort(v.begin(), v.end(), [](const SpareBufs::Buf& a, const SpareBufs::Buf& b) { return a.bytes > b.bytes; });
-        for (size_t i = SPARE_BUFS; i < v.size(); i++) (void)hipFree(v[i].p);
-        v.resize(SPARE_BUFS);
+    // keep the largest buffers that fit SPARE_BUFS and SPARE_BYTES; free the rest
+    std::sort(v.begin(), v.end(), [](const SpareBufs::Buf& a, const SpareBufs::Buf& b) { return a.bytes > b.bytes; });
+    size_t bytes = 0, kept = 0;
+    for (const SpareBufs::Buf& b : v) {
+        if (kept < SPARE_BUFS && bytes + b.bytes <= SPARE_BYTES) {
+            v[kept++] = b;
+            bytes += b.bytes;
+        } else {
+            (void)hipFree(b.p);
+        }
     }
+    v.resize(kept);
 }
 bool RetireScope::active() { return t_retired.depth > 0; }
 bool RetireScope::take(void* p, size_t bytes) {

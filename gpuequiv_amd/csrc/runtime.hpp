@@ -71,7 +71,8 @@ struct SpareBufs {
     void* take(size_t bytes, size_t* got);
     bool drop();   // frees them all; true if there were any
 };
-constexpr size_t SPARE_BUFS = 24;   // spares kept at most (the largest)
+constexpr size_t SPARE_BUFS = 24;                  // spares kept at most ...
+constexpr size_t SPARE_BYTES = 4ull << 30;         // ... and their bytes at most (the largest that fit)
 
 // Buffers outgrown inside a build.  hipFree waits for every stream of the
 // device, so a buffer that a build's queued kernels may still read is not

@@ -23,6 +23,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -137,6 +138,7 @@ struct Job {
     uint32_t admit_round = 0, done_round = 0;
     double t_submit = 0, t_built = 0, t_admit = 0, t_done = 0;
     bool alone = false;                 // its merged build failed: it is built by itself
+    bool exclusive = false;             // its own build ran out of device memory: built again once, with no other build running
     std::atomic<uint32_t> state{QUEUED};
     std::vector<uint32_t> offs, data;
 };
@@ -231,11 +233,15 @@ struct geq_server {
     std::vector<std::thread> builders;
     std::vector<uint64_t> finished;    // ids of jobs done or failed, not yet taken
     std::mutex spare_mu;
+    std::shared_mutex build_rw;        // builds hold it shared; a job built again after running out of memory, exclusively
+    bool exclusive_pending = false;    // such a job is taken: the other builders take no new job until it is built
     std::vector<BuiltJobs*> spare_outs;  // job-game buffers of placed builds, for reuse
     uint64_t next_id = 1;
     std::chrono::steady_clock::time_point t0;
     uint64_t steps = 0, rounds_total = 0, compactions = 0, jobs_done = 0, builds = 0;
     uint64_t split_builds = 0;          // merged builds that failed and were split into one-job builds
+    uint64_t deferred_admissions = 0;   // admitted jobs sent back to wait: no device memory for their placement
+    uint64_t force_defer = 0;           // GEQ_SERVER_DEFER_PLACEMENTS: placements treated as short of memory (tests)
     uint64_t arena_min = 0;            // arena entries the pools start with
     uint64_t compact_entries = 0;      // compaction threshold (0: 3/4 of the arena)
     uint64_t max_positions = 0;        // per-job game size limit (0: the node pool)
@@ -245,17 +251,26 @@ struct geq_server {
 
 namespace {
 
+// A builder's scratch is replaced after a build that needed more positions
+// than this, so that a rare very large job does not keep its device memory.
+constexpr uint64_t BIG_BUILD_POSITIONS = 64ull << 20;
+
 // Builder thread: takes every queued job (up to max_batch) and builds their
 // games in one level loop over the union of their LTSs (gamebuild_jobs).
 void builder_main(geq_server* S, geq_ctx* bctx) {
     (void)hipSetDevice(bctx->device);
-    gb::BuildScratch ws;   // reused by every build of this thread
-    geq_built store;
+    // reused by every build of this thread (replaced after a very large build,
+    // so that its peak device memory is not kept for the server's lifetime)
+    auto ws = std::make_unique<gb::BuildScratch>();
+    auto store = std::make_unique<geq_built>();
     while (true) {
         std::vector<Job*> take;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv.wait(lk, [&] { return S->stop || (!S->build_q.empty() && S->ready_q.size() < S->max_ready); });
+            // no new builds while a job waits to be built with no other build running
+            S->cv.wait(lk, [&] {
+                return S->stop || (!S->build_q.empty() && S->ready_q.size() < S->max_ready && !S->exclusive_pending);
+            });
             if (S->stop) return;
             // a job whose merged build failed is built alone, so that it fails
             // (or succeeds) by itself
@@ -265,6 +280,7 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
                 take.push_back(J);
                 S->build_q.pop_front();
                 J->state = BUILDING;
+                if (J->exclusive) S->exclusive_pending = true;
                 if (J->alone) break;
             }
         }
@@ -296,7 +312,15 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
             prs.push_back(J->pairs.data());
             nps.push_back(J->n_start);
         }
-        if (build.empty()) continue;
+        if (build.empty()) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (Job* J : take)
+                if (J->exclusive) {   // (it did not reach its build: the others may go on)
+                    S->exclusive_pending = false;
+                    S->cv.notify_all();
+                }
+            continue;
+        }
         BuiltJobs* raw = nullptr;
         {
             std::lock_guard<std::mutex> lk(S->spare_mu);
@@ -312,7 +336,18 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
             S->spare_outs.push_back(b);
         });
         const double tb0 = S->now_ms();
-        const geq_status st = gamebuild_jobs(bctx, ltss, prs, nps, S->max_positions, ws, store, B.get());
+        // A build that ran out of device memory by itself runs again alone:
+        // the other builders finish their builds and wait meanwhile.
+        const bool exclusive = build.size() == 1 && build[0]->exclusive;
+        std::shared_lock<std::shared_mutex> shared(S->build_rw, std::defer_lock);
+        std::unique_lock<std::shared_mutex> sole(S->build_rw, std::defer_lock);
+        if (exclusive) sole.lock(); else shared.lock();
+        const geq_status st = gamebuild_jobs(bctx, ltss, prs, nps, S->max_positions, *ws, *store, B.get());
+        if (store->peak_nodes > BIG_BUILD_POSITIONS) {   // a very large build: its scratch goes
+            ws = std::make_unique<gb::BuildScratch>();
+            store = std::make_unique<geq_built>();
+        }
+        if (exclusive) sole.unlock(); else shared.unlock();
         const std::string err = st != GEQ_OK ? std::string(geq_last_error()) : std::string();
         if (S->trace) {
             uint64_t pos = 0, dead = 0;
@@ -341,6 +376,20 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
             S->split_builds++;
             S->cv.notify_all();
             continue;
+        }
+        if (st == GEQ_ERR_OUT_OF_MEMORY && build.size() == 1 && !build[0]->exclusive &&
+            err.find("2^3") == std::string::npos) {
+            // device memory, not a 32-bit limit: once more with no other build running
+            build[0]->exclusive = true;
+            build[0]->alone = true;
+            build[0]->state = QUEUED;
+            S->build_q.push_front(build[0]);
+            S->cv.notify_all();
+            continue;
+        }
+        if (exclusive) {   // the other builders may take jobs again
+            S->exclusive_pending = false;
+            S->cv.notify_all();
         }
         for (uint32_t k = 0; k < build.size(); k++) {
             Job* J = build[k];
@@ -434,6 +483,7 @@ geq_status geq_server_create(geq_ctx* ctx, const geq_server_opts* opts, geq_serv
     S->rounds_per_step = opts && opts->rounds_per_step ? opts->rounds_per_step : 4;
     S->compact_entries = opts ? opts->compact_entries : 0;
     S->trace = std::getenv("GEQ_SERVER_TRACE") != nullptr;
+    if (const char* env = std::getenv("GEQ_SERVER_DEFER_PLACEMENTS")) S->force_defer = (uint64_t)std::max(0, std::atoi(env));
     if (const char* env = std::getenv("GEQ_SERVER_MAX_BATCH"))   // jobs per merged build (the per-job counters hold 1024)
         S->max_batch = (uint32_t)std::min(std::max(std::atoi(env), 1), 1024);
     S->max_positions = opts && opts->max_positions ? std::min<uint64_t>(opts->max_positions, V) : V;
@@ -568,10 +618,35 @@ geq_status geq_server_step(geq_server* S, uint32_t* n_resident, uint32_t* n_wait
             sum_v += J->V;
             max_v = std::max(max_v, J->V);
         }
-        GEQ_TRY(ensure_buf(g->seed, sum_v, s));
-        GEQ_TRY(ensure_buf(S->place_cnt, max_v + 1ull, s));
-        GEQ_TRY(ensure_buf(S->place_excl, max_v + 1ull, s));
-        GEQ_TRY(ensure_buf(S->place_fill, max_v + 1ull, s));
+        // scratch for the largest job (and the scans' block totals); when device
+        // memory runs short (the builders hold much of it) the jobs wait for a
+        // later step instead of failing
+        geq_status sst = ensure_buf(g->seed, sum_v, s);
+        if (sst == GEQ_OK) sst = ensure_buf(S->place_cnt, max_v + 1ull, s);
+        if (sst == GEQ_OK) sst = ensure_buf(S->place_excl, max_v + 1ull, s);
+        if (sst == GEQ_OK) sst = ensure_buf(S->place_fill, max_v + 1ull, s);
+        if (sst == GEQ_OK) sst = ensure_buf(S->scan_tmp, (max_v + SCAN_ITEMS - 1ull) / SCAN_ITEMS + 2ull, s);
+        if (sst == GEQ_OK && S->force_defer) {
+            S->force_defer--;
+            sst = GEQ_ERR_OUT_OF_MEMORY;
+        }
+        if (sst == GEQ_ERR_OUT_OF_MEMORY) {
+            (void)hipGetLastError();
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (auto it = admit.rbegin(); it != admit.rend(); ++it) {
+                Job* J = *it;
+                S->nodes.release(J->nb, J->V + 1);
+                S->edges.release(J->eb, J->E);
+                S->free_slots.push_back(J->slot);
+                S->ready_q.push_front(J);
+            }
+            S->deferred_admissions += admit.size();
+            admit.clear();
+        } else if (sst != GEQ_OK) {
+            return sst;
+        }
+    }
+    if (!admit.empty()) {
         GEQ_HIP(hipMemsetAsync(S->seed_cnt.p, 0, 4, s), GEQ_ERR_TRANSFER);
         // the next round to run; the seeds' slots are tagged relative to it
         const uint32_t t0 = round_tag(g->h_ctl->round);
@@ -749,6 +824,7 @@ geq_status geq_server_get_stats(geq_server* S, geq_server_stats* out) {
     out->jobs_done = S->jobs_done;
     out->builds = S->builds;
     out->split_builds = S->split_builds;
+    out->deferred_admissions = S->deferred_admissions;
     return GEQ_OK;
 }
 

@@ -343,6 +343,7 @@ typedef struct {
     uint64_t jobs_resident, jobs_waiting, jobs_done;
     uint64_t builds;                          /* merged builds (jobs built together) so far */
     uint64_t split_builds;                    /* merged builds that failed and were rebuilt one job at a time */
+    uint64_t deferred_admissions;             /* admissions sent back to wait: no device memory for the placement */
 } geq_server_stats;
 geq_status geq_server_get_stats(geq_server* server, geq_server_stats* out);
 

@@ -85,6 +85,25 @@ def test_jobs_same_alone_and_in_the_mix():
     srv.close()
 
 
+def test_placement_short_of_memory_waits(monkeypatch):
+    # GEQ_SERVER_DEFER_PLACEMENTS=3: the first three placements find no device
+    # memory for their scratch (forced on the host); their jobs go back to wait
+    # and are placed by later steps, with the same results.
+    print("serve: placements deferred three times", flush=True)
+    monkeypatch.setenv("GEQ_SERVER_DEFER_PLACEMENTS", "3")
+    jobs = _jobs(8, 5)
+    srv = Server(budget_bytes=256 << 20, rounds_per_step=2)
+    ids = [srv.submit(lts, pairs) for lts, pairs, _, _ in jobs]
+    res = srv.run_until_done(ids)
+    for k, j in enumerate(ids):
+        assert res[j][0]["state_name"] == "done", res[j][0]
+        assert H.canon_lists(*res[j][1]) == jobs[k][2], k
+    st = srv.stats()
+    assert st["deferred_admissions"] >= 3
+    assert st["jobs_done"] == len(jobs) and st["node_slots_used"] == 0 and st["edge_slots_used"] == 0
+    srv.close()
+
+
 def test_tiny_budget_defers_admission():
     print("serve: budget for about one game", flush=True)
     jobs = _jobs(10, 2)
