@@ -157,14 +157,21 @@ __device__ __forceinline__ bool contains_sorted(const uint32_t* a, uint32_t n, u
 
 __device__ __forceinline__ uint32_t deg_of(const LtsArgs& L, uint32_t s) { return L.row[s + 1] - L.row[s]; }
 
-// The bounds of position u; `heavy` tells whether k_generate_heavy takes it:
-// an attack position whose serial walk (one binary search per member of Q and
-// label group of p, the successor sets gathered and sorted) weighs at least
-// heavy_min (0: none), and whose sets fit the block's LDS.
-__device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, uint32_t u, uint32_t heavy_min, bool& heavy) {
+// The bounds of position u, and whether it is listed for a kernel of its own
+// (`heavy`; `wave`: k_generate_wave, else k_generate_heavy).  An attack
+// position with a non-empty Q is listed (none when heavy_min is 0):
+//   - under k_generate (GEN_BY_THREAD), when its serial walk (one binary search
+//     per member of Q and label group of p, the successor sets gathered and
+//     sorted) weighs at least heavy_min and its sets fit the block's LDS;
+//   - under k_generate_sub (GEN_BY_SUB), when it is past that kernel's LDS
+//     limits: for a wave when it fits the wave's, else for a block when it
+//     fits the block's (past both: the eight-lane kernel's first lane walks
+//     it); GEN_BY_WAVE / GEN_BY_BLOCK list every one that fits for a wave / block.
+__device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, uint32_t u, const HeavyArgs& H, bool& heavy,
+                                           bool& wave) {
     const uint32_t kind = N.kind[u], p = N.p[u], ql = N.q_len[u], xl = N.x_len[u];
     uint32_t c = 0, s = 0;
-    heavy = false;
+    heavy = wave = false;
     if (L.job_dead && L.job_dead[L.job_of_state[p]]) {
         // a job past its position cap: no successors
     } else if (kind == K_CLAUSE) {
@@ -178,8 +185,20 @@ __device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, 
         uint32_t sd = 0;
         for (uint32_t k = 0; k < ql; k++) sd += deg_of(L, N.spool[N.q_off[u] + k]);
         s = sd + 4 * ql;
-        const uint32_t ng = L.en_row[p + 1] - L.en_row[p];
-        heavy = heavy_min != 0u && sd <= HV_CAP && ql <= HV_CAP && (uint64_t)ql * (ng + 1u) + sd >= heavy_min;
+        const bool fits_block = sd <= HV_CAP && ql <= HV_CAP;
+        const bool fits_wave = sd <= WAVE_TMAX && ql <= WAVE_QMAX;
+        if (H.heavy_min == 0u) {
+        } else if (H.mode == GEN_BY_THREAD) {
+            const uint32_t ng = L.en_row[p + 1] - L.en_row[p];
+            heavy = fits_block && (uint64_t)ql * (ng + 1u) + sd >= H.heavy_min;
+        } else if (H.mode == GEN_BY_WAVE) {
+            heavy = wave = fits_wave;
+        } else if (H.mode == GEN_BY_BLOCK) {
+            heavy = fits_block;
+        } else if (sd > SUB_TMAX || ql > SUB_QMAX) {
+            wave = fits_wave;
+            heavy = fits_wave || fits_block;
+        }
     }
     return make_uint2(c, s);
 }
@@ -187,23 +206,23 @@ __device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, 
 __global__ void k_bounds(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, uint2* ub, HeavyArgs H) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n_front) return;
-    bool heavy = false;
+    bool heavy = false, wave = false;
     // trailing zero so the exclusive scan yields the totals
-    ub[i] = i == n_front ? make_uint2(0u, 0u) : bounds_of(N, L, lo + i, H.heavy_min, heavy);
+    ub[i] = i == n_front ? make_uint2(0u, 0u) : bounds_of(N, L, lo + i, H, heavy, wave);
     if (i < n_front) {
         H.flag[i] = heavy ? 1 : 0;
-        if (heavy) H.list[atomicAdd(H.count, 1u)] = i;
+        if (heavy) H.list[atomicAdd(H.count, 1u)] = i | (wave ? WAVE_TAG : 0u);
     }
 }
 
 // bound entry i of the next level (frontier of n_front positions from lo)
 __device__ __forceinline__ void bounds_next_at(const NodeArgs& N, uint32_t lo, uint32_t n_front, uint32_t chunk, const LtsArgs& L,
                                                uint2* ub, const HeavyArgs& H, uint32_t i) {
-    bool heavy = false;
-    ub[i] = i < n_front ? bounds_of(N, L, lo + i, H.heavy_min, heavy) : make_uint2(0u, 0u);
+    bool heavy = false, wave = false;
+    ub[i] = i < n_front ? bounds_of(N, L, lo + i, H, heavy, wave) : make_uint2(0u, 0u);
     if (i < min(n_front, chunk)) {      // the flags and list cover the first chunk
         H.flag[i] = heavy ? 1 : 0;
-        if (heavy) H.list[atomicAdd(H.count, 1u)] = i;
+        if (heavy) H.list[atomicAdd(H.count, 1u)] = i | (wave ? WAVE_TAG : 0u);
     }
 }
 
@@ -221,6 +240,9 @@ __global__ void k_bounds_next(NodeArgs N, uint32_t lo, const uint32_t* n_new, ui
 // sorted and duplicate-free there, probed and hashed there, and written to the
 // set pool once.  Larger sets take the same steps in the set pool.
 constexpr uint32_t GEN_BLOCK = 256, GEN_LDS = 16;
+
+__device__ __forceinline__ void gen_walk(const NodeArgs& N, uint32_t lo, uint32_t i, const LtsArgs& L, const uint2* off, uint64_t spool_base,
+                         Cand* cands, uint32_t* nsucc, uint32_t* qb, uint32_t st);
 
 __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L,
                                                         const uint2* off, const uint8_t* heavy, uint64_t spool_base,
@@ -249,6 +271,13 @@ __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo,
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == n_front) nsucc[i] = 0;   // trailing zero: the exclusive scan yields the edge total
     if (i >= n_front || heavy[i]) return;   // k_generate_heavy writes the heavy positions
+    gen_walk(N, lo, i, L, off, spool_base, cands, nsucc, qb, GEN_BLOCK);
+}
+
+// One position's successor candidates by one thread (the walk described above
+// k_generate); qb: GEN_LDS words of LDS at stride st.
+__device__ __forceinline__ void gen_walk(const NodeArgs& N, uint32_t lo, uint32_t i, const LtsArgs& L, const uint2* off, uint64_t spool_base,
+                         Cand* cands, uint32_t* nsucc, uint32_t* qb, uint32_t st) {
     const uint32_t u = lo + i;
     const uint32_t kind = N.kind[u], p = N.p[u], qo = N.q_off[u], ql = N.q_len[u], xo = N.x_off[u], xl = N.x_len[u];
     uint32_t* spool = N.spool;
@@ -309,32 +338,32 @@ __global__ __launch_bounds__(GEN_BLOCK) void k_generate(NodeArgs N, uint32_t lo,
                         continue;
                     }
                     uint32_t j = nq;
-                    while (j > 0 && qb[(j - 1) * GEN_BLOCK] > v) j--;
-                    if (j > 0 && qb[(j - 1) * GEN_BLOCK] == v) continue;
+                    while (j > 0 && qb[(j - 1) * st] > v) j--;
+                    if (j > 0 && qb[(j - 1) * st] == v) continue;
                     if (nq == GEN_LDS) {          // full: continue in the pool
-                        for (uint32_t y = 0; y < nq; y++) spool[sp + y] = qb[y * GEN_BLOCK];
+                        for (uint32_t y = 0; y < nq; y++) spool[sp + y] = qb[y * st];
                         T = nq;
                         spool[sp + T++] = v;
                         spill = true;
                         continue;
                     }
-                    for (uint32_t y = nq; y > j; y--) qb[y * GEN_BLOCK] = qb[(y - 1) * GEN_BLOCK];
-                    qb[j * GEN_BLOCK] = v;
+                    for (uint32_t y = nq; y > j; y--) qb[y * st] = qb[(y - 1) * st];
+                    qb[j * st] = v;
                     nq++;
                 }
             }
             if (!spill) {
-                for (uint32_t y = 0; y < nq; y++) spool[sp + y] = qb[y * GEN_BLOCK];
+                for (uint32_t y = 0; y < nq; y++) spool[sp + y] = qb[y * st];
                 for (uint32_t tt = t; tt < g_end; tt++) {
                     const uint32_t pp = L.dst[tt];
                     bool in = false;
-                    for (uint32_t y = 0; y < nq; y++) in |= qb[y * GEN_BLOCK] == pp;
+                    for (uint32_t y = 0; y < nq; y++) in |= qb[y * st] == pp;
                     if (!in) {
                         Cand c;
                         c.kind_p = (K_ATTACK << 30) | pp;
                         c.q_off = (uint32_t)sp; c.q_len = nq; c.x_off = 0; c.x_len = 0;
                         c.weight = W_OBS;
-                        c.hash = pos_hash_lds(K_ATTACK, pp, qb, nq, GEN_BLOCK);
+                        c.hash = pos_hash_lds(K_ATTACK, pp, qb, nq, st);
                         c.res = 0;
                         c.parent = i;
                         c.slot = slot;
@@ -484,6 +513,7 @@ __global__ __launch_bounds__(HV_BLOCK) void k_generate_heavy(NodeArgs N, uint32_
     uint32_t* spool = N.spool;
     for (uint32_t h = blockIdx.x; h < n_heavy; h += gridDim.x) {
         const uint32_t i = H.list[h];
+        if (i & WAVE_TAG) continue;   // k_generate_wave's (the same for the whole block)
         const uint32_t u = lo + i;
         const uint32_t p = N.p[u], qo = N.q_off[u], ql = N.q_len[u];
         const uint2 o0 = off[i];
@@ -630,6 +660,392 @@ __global__ __launch_bounds__(HV_BLOCK) void k_generate_heavy(NodeArgs N, uint32_
         if (tid == 0) nsucc[i] = slot;
         for (uint32_t j = cbase + slot + tid; j < cend; j += HV_BLOCK) cands[j].kind_p = INVALID;
         __syncthreads();   // qs and fl are rewritten for the next position
+    }
+}
+
+namespace {
+
+constexpr uint32_t SUB_RC = 8;   // transitions of a member kept in registers
+
+// Lanes [first, first + G) of a wave work on one position.
+template <uint32_t G>
+struct Coop {
+    static __device__ __forceinline__ uint32_t lane() { return threadIdx.x & (G - 1u); }
+    static __device__ __forceinline__ uint32_t first() { return threadIdx.x & 63u & ~(G - 1u); }
+    // exclusive prefix sum over the lanes; `total` gets their sum
+    static __device__ __forceinline__ uint32_t scan(uint32_t v, uint32_t& total) {
+        const uint32_t r = lane();
+        uint32_t x = v;
+        for (uint32_t d = 1; d < G; d <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, d, (int)G);
+            if (r >= d) x += y;
+        }
+        total = (uint32_t)__shfl((int)x, (int)G - 1, (int)G);
+        return x - v;
+    }
+    static __device__ __forceinline__ uint32_t sum(uint32_t v) {
+        uint32_t t;
+        (void)scan(v, t);
+        return t;
+    }
+    // the lanes' bits of a wave ballot (bit r: lane r)
+    static __device__ __forceinline__ uint64_t ballot(bool f) {
+        const uint64_t b = __ballot(f);
+        if constexpr (G == 64u) return b;
+        else return (b >> first()) & ((1ull << G) - 1ull);
+    }
+    static __device__ __forceinline__ uint32_t below(uint64_t m) { return (uint32_t)__popcll(m & ((1ull << lane()) - 1ull)); }
+};
+
+// One position's LDS: Q's members, their enabledness flags (bit 0 subset,
+// bit 1 superset), a label group's successors as gathered, and a second
+// buffer for the sort and for Q'.
+template <uint32_t QMAX, uint32_t TMAX>
+struct CoopLds {
+    uint32_t mem[QMAX];
+    uint32_t gath[TMAX];
+    uint32_t sort[TMAX];
+    uint8_t flag[QMAX];
+};
+
+}  // namespace
+
+// Position i's candidates by G lanes, with the candidates, set-pool layout and
+// hashes of gen_walk.  Attack positions with a non-empty Q of at most QMAX
+// members whose member degrees sum to at most TMAX:
+//   - lane r takes member r (and r + G, ...); member r's transitions stay in
+//     registers when it has at most SUB_RC of them, so the label groups of p
+//     read no further memory for it;
+//   - per label group of p, the members' successors are gathered in LDS
+//     (offsets from a scan over the lanes), sorted (rank counting, or a
+//     bitonic sort for large groups of a wave), deduplicated into Q' (written
+//     to the pool), and the group's transitions are tested and emitted in
+//     parallel, ranks from a ballot keeping the slot order;
+//   - the conjunction challenges partition Q with ballots.
+// Defend positions emit their clause moves in parallel.  Clause positions,
+// attack positions with an empty Q and attack positions past the limits run
+// gen_walk on the first lane.
+template <uint32_t G, uint32_t QMAX, uint32_t TMAX>
+__device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_t i, const LtsArgs& L, const uint2* off,
+                                         uint64_t spool_base, Cand* cands, uint32_t* nsucc, CoopLds<QMAX, TMAX>& S) {
+    using C = Coop<G>;
+    const uint32_t r = C::lane();
+    uint32_t* const sm = S.mem;
+    uint8_t* const sf = S.flag;
+    const uint32_t u = lo + i;
+    const uint32_t kind = N.kind[u], p = N.p[u], qo = N.q_off[u], ql = N.q_len[u], xl = N.x_len[u];
+    uint32_t* spool = N.spool;
+    const uint2 o0 = off[i], o1 = off[i + 1];
+    const uint32_t cbase = o0.x, cend = o1.x;
+    const uint64_t sbase = spool_base + o0.y;
+    if (L.job_dead && L.job_dead[L.job_of_state[p]]) {   // k_bounds gave it no room
+        if (r == 0) nsucc[i] = 0;
+        return;
+    }
+    if (kind == K_DEFEND) {
+        // Defend(p, Q, Q*) -> Clause(p, q) per q in Q [0,0,0,Min(3)], Attack(p, Q*) [Min(3)]
+        for (uint32_t k = r; k < ql; k += G) {
+            Cand c;
+            c.kind_p = (K_CLAUSE << 30) | p;
+            c.q_off = qo + k; c.q_len = 1; c.x_off = 0; c.x_len = 0;
+            c.weight = W_ANS;
+            c.hash = pos_hash(K_CLAUSE, p, spool, qo + k, 1, 0, 0);
+            c.res = 0;
+            c.parent = i;
+            c.slot = k;
+            cands[cbase + k] = c;
+        }
+        if (r == 0) {
+            if (xl) {
+                const uint32_t xo = N.x_off[u];
+                Cand c;
+                c.kind_p = (K_ATTACK << 30) | p;
+                c.q_off = xo; c.q_len = xl; c.x_off = 0; c.x_len = 0;
+                c.weight = W_REV;
+                c.hash = pos_hash(K_ATTACK, p, spool, xo, xl, 0, 0);
+                c.res = 0;
+                c.parent = i;
+                c.slot = ql;
+                cands[cbase + ql] = c;
+            }
+            nsucc[i] = ql + (xl ? 1u : 0u);   // all the room k_bounds gave it
+        }
+        return;
+    }
+    const uint32_t sd = (o1.y - o0.y) - 4u * ql;   // attack positions: set-pool words = sum of member degrees + 4 |Q|
+    if (kind == K_CLAUSE || ql == 0 || ql > QMAX || sd > TMAX) {
+        if (r == 0) gen_walk(N, lo, i, L, off, spool_base, cands, nsucc, S.sort, 1);
+        return;
+    }
+    // Q's members in LDS; member r and, when they fit, its transitions in registers
+    for (uint32_t k = r; k < ql; k += G) sm[k] = spool[qo + k];
+    const uint32_t q0 = r < ql ? spool[qo + r] : 0u;
+    uint32_t q0b = 0, q0d = 0;
+    if (r < ql) {
+        q0b = L.row[q0];
+        q0d = L.row[q0 + 1] - q0b;
+    }
+    const bool cached = r < ql && q0d <= SUB_RC;
+    int32_t cl[SUB_RC];
+    uint32_t cd[SUB_RC];
+#pragma unroll
+    for (uint32_t j = 0; j < SUB_RC; j++) {
+        const bool in = cached && j < q0d;
+        cl[j] = in ? L.lbl[q0b + j] : INT32_MIN;   // no label equals INT32_MIN
+        cd[j] = in ? L.dst[q0b + j] : 0u;
+    }
+    // p's label groups: from transitions held one per lane when p has at most
+    // G of them, else from its distinct labels and label ranges
+    const uint32_t pr0 = L.row[p], pdeg = L.row[p + 1] - pr0;
+    const bool pc = pdeg <= G;
+    int32_t plb = INT32_MIN;
+    uint32_t pdst = 0;
+    if (pc && r < pdeg) {
+        plb = L.lbl[pr0 + r];
+        pdst = L.dst[pr0 + r];
+    }
+    const int32_t prev = __shfl_up(plb, 1, (int)G);
+    uint64_t rest = C::ballot(pc && r < pdeg && (r == 0 || prev != plb));   // group starts
+    const uint32_t ng = pc ? (uint32_t)__popcll(rest) : L.en_row[p + 1] - L.en_row[p];
+    wave_sync();   // the members in LDS
+    uint32_t slot = 0;
+    uint64_t sp = sbase;
+    for (uint32_t g = 0; g < ng; g++) {
+        int32_t a;
+        uint32_t t, te;
+        if (pc) {
+            const uint32_t s0 = (uint32_t)__ffsll((unsigned long long)rest) - 1u;
+            rest &= rest - 1ull;
+            const uint32_t s1 = rest ? (uint32_t)__ffsll((unsigned long long)rest) - 1u : pdeg;
+            t = pr0 + s0;
+            te = pr0 + s1;
+            a = __shfl(plb, (int)s0, (int)G);
+        } else {
+            a = L.en_lbl[L.en_row[p] + g];
+            label_range(L, p, a, t, te);
+        }
+        // the members' label-a successors, gathered (in any order: they are sorted next)
+        uint32_t c = 0;
+        if (cached) {
+#pragma unroll
+            for (uint32_t j = 0; j < SUB_RC; j++) c += cl[j] == a ? 1u : 0u;
+        } else if (r < ql) {
+            uint32_t x0, x1;
+            label_range(L, q0, a, x0, x1);
+            c = x1 - x0;
+        }
+        for (uint32_t k = r + G; k < ql; k += G) {
+            uint32_t x0, x1;
+            label_range(L, sm[k], a, x0, x1);
+            c += x1 - x0;
+        }
+        uint32_t T;
+        uint32_t w = C::scan(c, T);   // T <= sd <= TMAX
+        uint32_t* const sg = S.gath;
+        if (cached) {
+#pragma unroll
+            for (uint32_t j = 0; j < SUB_RC; j++)
+                if (cl[j] == a) sg[w++] = cd[j];
+        } else if (r < ql) {
+            uint32_t x0, x1;
+            label_range(L, q0, a, x0, x1);
+            for (uint32_t x = x0; x < x1; x++) sg[w++] = L.dst[x];
+        }
+        for (uint32_t k = r + G; k < ql; k += G) {
+            uint32_t x0, x1;
+            label_range(L, sm[k], a, x0, x1);
+            for (uint32_t x = x0; x < x1; x++) sg[w++] = L.dst[x];
+        }
+        wave_sync();
+        // sorted: by rank counting (ties by gather index) into S.sort, or for a
+        // large group of a wave by a bitonic sort in place (padding sorts last:
+        // state ids are below 2^30)
+        const uint32_t* srt = S.sort;
+        uint32_t* qs = sg;   // Q'
+        if (G == 64u && T > 64u) {
+            uint32_t n2 = 1;
+            while (n2 < T) n2 <<= 1;   // <= TMAX, a power of two
+            for (uint32_t x = T + r; x < n2; x += G) sg[x] = 0xffffffffu;
+            wave_sync();
+            for (uint32_t k = 2; k <= n2; k <<= 1)
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t x = r; x < n2; x += G) {
+                        const uint32_t y = x ^ j;
+                        if (y > x) {
+                            const uint32_t va = sg[x], vb = sg[y];
+                            if ((va > vb) == ((x & k) == 0u)) {
+                                sg[x] = vb;
+                                sg[y] = va;
+                            }
+                        }
+                    }
+                    wave_sync();
+                }
+            srt = sg;
+            qs = S.sort;
+        } else {
+            for (uint32_t x = r; x < T; x += G) {
+                const uint32_t v = sg[x];
+                uint32_t rank = 0;
+                for (uint32_t y = 0; y < T; y++) {
+                    const uint32_t z = sg[y];
+                    rank += (z < v || (z == v && y < x)) ? 1u : 0u;
+                }
+                S.sort[rank] = v;
+            }
+            wave_sync();
+        }
+        // Q' = the distinct values in order: into qs and the pool at sp
+        uint32_t nq = 0;
+        for (uint32_t x0 = 0; x0 < T; x0 += G) {
+            const uint32_t x = x0 + r;
+            const bool keep = x < T && (x == 0u || srt[x] != srt[x - 1u]);
+            const uint64_t m = C::ballot(keep);
+            if (keep) {
+                const uint32_t d = nq + C::below(m);
+                qs[d] = srt[x];
+                spool[sp + d] = srt[x];
+            }
+            nq += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+        // an observation move to every target of the group outside Q', in transition order
+        for (uint32_t b0 = t; b0 < te; b0 += G) {
+            uint32_t pp = 0;
+            bool in;
+            if (pc) {   // lane r holds transition pr0 + r
+                in = r < pdeg && pr0 + r >= t && pr0 + r < te;
+                pp = pdst;
+            } else {
+                in = b0 + r < te;
+                if (in) pp = L.dst[b0 + r];
+            }
+            const bool e = in && !contains_sorted(qs, nq, pp);
+            const uint64_t m = C::ballot(e);
+            if (e) {
+                uint32_t h = mix(mix(mix(0x811c9dc5u, K_ATTACK), pp), nq);
+                for (uint32_t y = 0; y < nq; y++) h = mix(h, qs[y]);
+                Cand cc;
+                cc.kind_p = (K_ATTACK << 30) | pp;
+                cc.q_off = (uint32_t)sp; cc.q_len = nq; cc.x_off = 0; cc.x_len = 0;
+                cc.weight = W_OBS;
+                cc.hash = hash_finish(mix(h, 0u));
+                cc.res = 0;
+                cc.parent = i;
+                cc.slot = slot + C::below(m);
+                cands[cbase + cc.slot] = cc;
+            }
+            slot += (uint32_t)__popcll(m);
+            if (pc) break;   // one pass covers p's transitions
+        }
+        sp += nq;
+        wave_sync();   // the buffers are rewritten by the next group
+    }
+    if (r == 0) {   // Defend(p, Q, {})
+        uint32_t h = mix(mix(mix(0x811c9dc5u, K_DEFEND), p), ql);
+        for (uint32_t k = 0; k < ql; k++) h = mix(h, sm[k]);
+        Cand cc;
+        cc.kind_p = (K_DEFEND << 30) | p;
+        cc.q_off = qo; cc.q_len = ql; cc.x_off = 0; cc.x_len = 0;
+        cc.weight = W_CHAL;
+        cc.hash = hash_finish(mix(h, 0u));
+        cc.res = 0;
+        cc.parent = i;
+        cc.slot = slot;
+        cands[cbase + slot] = cc;
+    }
+    slot++;
+    // conjunction challenges with partitioned Q* (gamepos.rs:329-375)
+    for (uint32_t k = r; k < ql; k += G) {
+        bool sub, sup;
+        compare_enabled(L, p, sm[k], sub, sup);
+        sf[k] = (uint8_t)((sub ? 1u : 0u) | (sup ? 2u : 0u));
+    }
+    wave_sync();
+    uint64_t pb = sbase + sd;
+    uint32_t emitted[3];
+    uint32_t n_emitted = 0;
+    for (uint32_t which = 0; which < 3u; which++) {
+        uint32_t local = 0;
+        for (uint32_t k = r; k < ql; k += G) local += in_part(which, sf[k]) ? 1u : 0u;
+        const uint32_t cnt = C::sum(local);
+        if (cnt == 0u || cnt == ql) continue;
+        bool dup = false;
+        for (uint32_t e = 0; e < n_emitted; e++) {
+            uint32_t diff = 0;
+            for (uint32_t k = r; k < ql; k += G) diff += in_part(which, sf[k]) != in_part(emitted[e], sf[k]) ? 1u : 0u;
+            dup |= C::sum(diff) == 0u;
+        }
+        if (dup) continue;
+        // stable partition: the members outside the part, then those inside
+        const uint32_t na = ql - cnt;
+        uint32_t run = 0;
+        for (uint32_t k0 = 0; k0 < ql; k0 += G) {
+            const uint32_t k = k0 + r;
+            const bool f = k < ql && in_part(which, sf[k]);
+            const uint64_t m = C::ballot(f);
+            const uint32_t before = run + C::below(m);
+            if (k < ql) {
+                if (f) spool[pb + na + before] = sm[k];
+                else spool[pb + (k - before)] = sm[k];
+            }
+            run += (uint32_t)__popcll(m);
+        }
+        if (r == 0) {
+            uint32_t h = mix(mix(mix(0x811c9dc5u, K_DEFEND), p), na);
+            for (uint32_t k = 0; k < ql; k++)
+                if (!in_part(which, sf[k])) h = mix(h, sm[k]);
+            h = mix(h, cnt);
+            for (uint32_t k = 0; k < ql; k++)
+                if (in_part(which, sf[k])) h = mix(h, sm[k]);
+            Cand cc;
+            cc.kind_p = (K_DEFEND << 30) | p;
+            cc.q_off = (uint32_t)pb; cc.q_len = na; cc.x_off = (uint32_t)(pb + na); cc.x_len = cnt;
+            cc.weight = W_CHAL;
+            cc.hash = hash_finish(h);
+            cc.res = 0;
+            cc.parent = i;
+            cc.slot = slot;
+            cands[cbase + slot] = cc;
+        }
+        slot++;
+        emitted[n_emitted++] = which;
+        pb += ql;
+    }
+    if (r == 0) nsucc[i] = slot;
+    for (uint32_t j = cbase + slot + r; j < cend; j += G) cands[j].kind_p = INVALID;
+}
+
+// SUB_LANES lanes per frontier position; the listed (heavy) positions are left
+// to k_generate_wave and k_generate_heavy.
+__global__ __launch_bounds__(SUB_BLOCK) void k_generate_sub(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L,
+                                                            const uint2* off, const uint8_t* heavy, uint64_t spool_base,
+                                                            Cand* cands, uint32_t* nsucc) {
+    constexpr uint32_t PER_BLOCK = SUB_BLOCK / SUB_LANES;
+    __shared__ CoopLds<SUB_QMAX, SUB_TMAX> lds[PER_BLOCK];
+    const uint32_t gs = threadIdx.x / SUB_LANES;
+    const uint32_t i = blockIdx.x * PER_BLOCK + gs;
+    if (i > n_front) return;
+    if (i == n_front) {   // trailing zero: the exclusive scan yields the edge total
+        if ((threadIdx.x & (SUB_LANES - 1u)) == 0) nsucc[i] = 0;
+        return;
+    }
+    if (heavy[i]) return;
+    gen_coop<SUB_LANES, SUB_QMAX, SUB_TMAX>(N, lo, i, L, off, spool_base, cands, nsucc, lds[gs]);
+}
+
+// One wave per listed position tagged WAVE_TAG (grid-stride over the list).
+__global__ __launch_bounds__(SUB_BLOCK) void k_generate_wave(NodeArgs N, uint32_t lo, LtsArgs L, const uint2* off,
+                                                             uint64_t spool_base, HeavyArgs H, Cand* cands, uint32_t* nsucc) {
+    constexpr uint32_t PER_BLOCK = SUB_BLOCK / 64u;
+    __shared__ CoopLds<WAVE_QMAX, WAVE_TMAX> lds[PER_BLOCK];
+    const uint32_t w = threadIdx.x / 64u;
+    const uint32_t n = *H.count;
+    for (uint32_t h = blockIdx.x * PER_BLOCK + w; h < n; h += gridDim.x * PER_BLOCK) {
+        const uint32_t e = H.list[h];
+        if (!(e & WAVE_TAG)) continue;
+        gen_coop<64u, WAVE_QMAX, WAVE_TMAX>(N, lo, e & ~WAVE_TAG, L, off, spool_base, cands, nsucc, lds[w]);
+        wave_sync();   // the LDS is rewritten by the next position
     }
 }
 

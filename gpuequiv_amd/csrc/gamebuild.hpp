@@ -60,16 +60,31 @@ struct Cand {               // one successor candidate of a frontier position
     uint32_t slot;          // successor index within the parent
 };
 
-// Heavy frontier positions: attack positions whose serial successor walk is
-// long (see bounds_of) are generated by one block each (k_generate_heavy)
-// instead of one thread.  HV_CAP bounds their |Q| and sum of member degrees
+// Heavy frontier positions: attack positions too large for the kernel that
+// generates the level (see bounds_of) are generated by one block each
+// (k_generate_heavy) or one wave each (k_generate_wave).  HV_CAP bounds their |Q| and sum of member degrees
 // (the block's LDS).
 constexpr uint32_t HV_CAP = 8192;
+// k_generate_sub: lanes per position, threads per block, and the largest |Q|
+// and sum of member degrees its LDS holds; k_generate_wave (a wave per
+// position): the same limits.
+constexpr uint32_t SUB_LANES = 8, SUB_BLOCK = 256, SUB_QMAX = 32, SUB_TMAX = 64;
+constexpr uint32_t WAVE_QMAX = 256, WAVE_TMAX = 512;
+// Which kernels generate a level: k_generate with k_generate_heavy
+// (GEN_BY_THREAD), or k_generate_sub with k_generate_wave and k_generate_heavy
+// for the positions past its limits (GEN_BY_SUB); GEN_BY_WAVE and GEN_BY_BLOCK send
+// every attack position with a non-empty Q that fits to the wave / block
+// kernel (tests).
+constexpr uint32_t GEN_BY_THREAD = 0, GEN_BY_SUB = 1, GEN_BY_WAVE = 2, GEN_BY_BLOCK = 3;
+// List entries of k_generate_wave's positions carry this bit (frontier
+// indices stay below 2^31).
+constexpr uint32_t WAVE_TAG = 0x80000000u;
 struct HeavyArgs {
-    uint8_t* flag;       // per frontier position: 1 if heavy
-    uint32_t* list;      // the heavy positions' frontier indices (any order)
+    uint8_t* flag;       // per frontier position: 1 if listed
+    uint32_t* list;      // the listed positions' frontier indices (any order), WAVE_TAG on k_generate_wave's
     uint32_t* count;     // their number (zero before k_bounds)
-    uint32_t heavy_min;  // weight from which a position is heavy (0: none)
+    uint32_t heavy_min;  // GEN_BY_THREAD: weight from which a position is listed; any mode: 0 lists none
+    uint32_t mode;       // GEN_BY_*
 };
 
 // Per frontier position: upper bounds on its candidates (x) and on the set-pool
@@ -88,6 +103,13 @@ __global__ void k_generate(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L,
                            uint64_t spool_base, Cand* cands, uint32_t* nsucc);
 __global__ void k_generate_heavy(NodeArgs N, uint32_t lo, LtsArgs L, const uint2* off, uint64_t spool_base, HeavyArgs H,
                                  Cand* cands, uint32_t* nsucc);
+// The same candidates as k_generate, SUB_LANES lanes per position (grid of
+// (n_front + 1) * SUB_LANES threads in blocks of SUB_BLOCK).
+__global__ void k_generate_sub(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArgs L, const uint2* off, const uint8_t* heavy,
+                               uint64_t spool_base, Cand* cands, uint32_t* nsucc);
+// The listed positions tagged WAVE_TAG, a wave each (blocks of SUB_BLOCK, any grid).
+__global__ void k_generate_wave(NodeArgs N, uint32_t lo, LtsArgs L, const uint2* off, uint64_t spool_base, HeavyArgs H,
+                                Cand* cands, uint32_t* nsucc);
 __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask);
 __global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag);
 __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,

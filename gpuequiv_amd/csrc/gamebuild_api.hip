@@ -491,8 +491,18 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         const long long v = std::atoll(env);
         if (v >= 0) tail_max = (uint32_t)std::min<long long>(v, TAIL_MAX);
     }
+    // Positions are generated by SUB_LANES lanes each (k_generate_sub, with
+    // k_generate_wave and k_generate_heavy for the larger ones), or by one
+    // thread each with GEQ_BUILD_GEN=thread (k_generate); GEQ_BUILD_GEN=wave /
+    // block send every attack position that fits to the wave / block kernel.
+    uint32_t gen_mode = GEN_BY_SUB;
+    if (const char* env = std::getenv("GEQ_BUILD_GEN")) {
+        if (!std::strcmp(env, "thread")) gen_mode = GEN_BY_THREAD;
+        else if (!std::strcmp(env, "wave")) gen_mode = GEN_BY_WAVE;
+        else if (!std::strcmp(env, "block")) gen_mode = GEN_BY_BLOCK;
+    }
     GB_TRY(ensure(ws.hv_count, 1, s, false, "heavy positions"));
-    auto heavy_args = [&]() { return HeavyArgs{ws.hv_flag.p, ws.hv_list.p, ws.hv_count.p, heavy_min}; };
+    auto heavy_args = [&]() { return HeavyArgs{ws.hv_flag.p, ws.hv_list.p, ws.hv_count.p, heavy_min, gen_mode}; };
     // The last chunk of a level also bounds and scans the next level before
     // its wait, so the next level's first chunk starts without one.
     bool have_next = false;
@@ -533,11 +543,23 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         B->peak_edges = std::max<uint64_t>(B->peak_edges, E + n_cand + 1);
         GB_TRY(ensure(B->col, E + n_cand + 1, s, true, "edges"));
         GB_TRY(ensure(B->wt, E + n_cand + 1, s, true, "edges"));
-        k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, off.p, ws.hv_flag.p, B->spool_top, cands.p,
-                                                               nsucc.p);
-        if (n_heavy)   // k_bounds listed them (their count came back with the totals)
-            k_generate_heavy<<<std::min<uint32_t>(n_heavy, 1024), 256, 0, s>>>(nodes(), lo, L, off.p, B->spool_top, heavy_args(),
-                                                                          cands.p, nsucc.p);
+        if (gen_mode != GEN_BY_THREAD) {
+            if (nf >= WAVE_TAG) return bail(fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: a chunk of 2^31 or more positions"));
+            k_generate_sub<<<(uint32_t)(((uint64_t)nf + 1 + SUB_BLOCK / SUB_LANES - 1) / (SUB_BLOCK / SUB_LANES)), SUB_BLOCK, 0, s>>>(
+                nodes(), lo, nf, L, off.p, ws.hv_flag.p, B->spool_top, cands.p, nsucc.p);
+        } else {
+            k_generate<<<(nf + 1 + tb - 1) / tb, tb, gen_lds, s>>>(nodes(), lo, nf, L, off.p, ws.hv_flag.p, B->spool_top,
+                                                                   cands.p, nsucc.p);
+        }
+        if (n_heavy) {   // k_bounds listed them (their count came back with the totals)
+            if (gen_mode != GEN_BY_THREAD)
+                k_generate_wave<<<std::min<uint32_t>((n_heavy + 3) / 4, 2048), SUB_BLOCK, 0, s>>>(nodes(), lo, L, off.p,
+                                                                                               B->spool_top, heavy_args(),
+                                                                                               cands.p, nsucc.p);
+            if (gen_mode != GEN_BY_WAVE)
+                k_generate_heavy<<<std::min<uint32_t>(n_heavy, 1024), 256, 0, s>>>(nodes(), lo, L, off.p, B->spool_top,
+                                                                              heavy_args(), cands.p, nsucc.p);
+        }
         // After a level's last chunk, the next level is the positions the level
         // created: `add` by its earlier chunks and *(rank + n_cand) by this one.
         const bool last_chunk = c_hi == hi;

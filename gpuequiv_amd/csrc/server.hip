@@ -350,16 +350,17 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
         if (exclusive) sole.unlock(); else shared.unlock();
         const std::string err = st != GEQ_OK ? std::string(geq_last_error()) : std::string();
         if (S->trace) {
-            uint64_t pos = 0, dead = 0;
+            uint64_t pos = 0, dead = 0, dead_pos = 0;
             for (uint32_t k = 0; st == GEQ_OK && k < build.size(); k++) {
                 pos += B->n_pos[k];
                 dead += B->dead[k];
+                dead_pos += B->dead[k] ? B->n_pos[k] : 0;
             }
             std::fprintf(stderr,
-                         "[geq server] t=%.1f ms build of %zu jobs: %llu positions, %u levels, %llu over the cap, %.1f ms "
-                         "(merge %.1f, level loop %.1f, extraction %.1f ms; status %d)\n",
-                         tb0, build.size(), (unsigned long long)pos, B->levels, (unsigned long long)dead, S->now_ms() - tb0,
-                         B->prep_ms, B->build_ms, B->extract_ms, (int)st);
+                         "[geq server] t=%.1f ms build of %zu jobs: %llu positions, %u levels, %llu over the cap (%llu positions), "
+                         "%.1f ms (merge %.1f, level loop %.1f, extraction %.1f ms; status %d)\n",
+                         tb0, build.size(), (unsigned long long)pos, B->levels, (unsigned long long)dead,
+                         (unsigned long long)dead_pos, S->now_ms() - tb0, B->prep_ms, B->build_ms, B->extract_ms, (int)st);
         }
         std::lock_guard<std::mutex> lk(S->mu);
         const double now = S->now_ms();

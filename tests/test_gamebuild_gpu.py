@@ -118,13 +118,15 @@ def test_compare_all_matches_oracle(n, m, labels, seed):
 @pytest.mark.parametrize("heavy_min,label_mask", [("1", "1"), ("40", "1"), ("0", "1"), ("1", "0"), ("0", "0")])
 @pytest.mark.parametrize("n,m,labels,seed", [(60, 200, 4, 3), (300, 1200, 8, 1)])
 def test_block_generated_positions_match_oracle(monkeypatch, heavy_min, label_mask, n, m, labels, seed):
-    # Attack positions weighing at least GEQ_BUILD_HEAVY_MIN are generated by
-    # a block each (1: every one with a non-empty Q; 0: none).  label_mask 0
-    # turns off the per-state label bit sets (the label walks search the rows
-    # instead).  The game and its energies are the oracle's either way,
-    # capped builds included.
+    # With one thread per position (GEQ_BUILD_GEN=thread), attack positions
+    # weighing at least GEQ_BUILD_HEAVY_MIN are generated by a block each (1:
+    # every one with a non-empty Q; 0: none).  label_mask 0 turns off the
+    # per-state label bit sets (the label walks search the rows instead).  The
+    # game and its energies are the oracle's either way, capped builds
+    # included.
     print("block-generated positions, heavy_min", heavy_min, "label_mask", label_mask, "lts", n, m, labels, seed,
           flush=True)
+    monkeypatch.setenv("GEQ_BUILD_GEN", "thread")
     monkeypatch.setenv("GEQ_BUILD_HEAVY_MIN", heavy_min)
     monkeypatch.setenv("GEQ_BUILD_LABEL_MASK", label_mask)
     n, src, dst, lbl = workloads.random_lts(n, m, labels, seed)
@@ -141,6 +143,57 @@ def test_block_generated_positions_match_oracle(monkeypatch, heavy_min, label_ma
     h = b2._download()
     assert bool(b2.build_stats["truncated"]) == og2["truncated"] and b2.n_positions() == og2["V"]
     for k in ("row", "col", "wt"):
+        assert np.array_equal(h[k], og2[k]), k
+
+
+GEN_CASES = [   # (states, transitions, labels, seed, pairs: "part" = enabledness partitions or a random count, cap)
+    (60, 200, 4, 3, "part", 0),
+    (300, 1200, 8, 1, "part", 0),
+    (40, 480, 3, 5, 10, 0),             # 12 transitions per state, 3 labels: 238,657 positions
+    (100, 1300, 70, 4, 40, 0),          # labels past 63: no label bit sets
+    (150, 2000, 20, 9, 30, 200000),     # capped at 200,000 positions
+]
+
+
+@pytest.mark.parametrize("gen,heavy_min,label_mask", [("thread", "1024", "1"), ("sub", "1024", "1"), ("sub", "0", "1"),
+                                                      ("sub", "1024", "0"), ("wave", "1024", "1"), ("wave", "1024", "0"),
+                                                      ("block", "1024", "1")])
+@pytest.mark.parametrize("case", range(len(GEN_CASES)))
+def test_generators_match_oracle(monkeypatch, gen, heavy_min, label_mask, case):
+    # Positions are generated by one thread each (GEQ_BUILD_GEN=thread) or by
+    # eight lanes each (the default), positions past the eight-lane limits by
+    # a wave or a block each (heavy_min 0: by the eight-lane kernel's first
+    # lane); wave / block send every attack position that fits to the wave /
+    # block kernel.  The dense LTSs give states with more transitions than the
+    # lanes hold in registers, sets past the eight-lane limits and label groups
+    # large enough for the wave's bitonic sort.  Same game and energies every
+    # way; capped builds equal the oracle's capped walk.
+    n, m, labels, seed, pmode, cap = GEN_CASES[case]
+    print("generator", gen, "heavy_min", heavy_min, "label_mask", label_mask, "lts", n, m, labels, seed, pmode, cap,
+          flush=True)
+    monkeypatch.setenv("GEQ_BUILD_GEN", gen)
+    monkeypatch.setenv("GEQ_BUILD_HEAVY_MIN", heavy_min)
+    monkeypatch.setenv("GEQ_BUILD_LABEL_MASK", label_mask)
+    n, src, dst, lbl = workloads.random_lts(n, m, labels, seed)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    if pmode == "part":
+        if n >= 300:
+            lts, _ = lts.bisimilar_minimize()
+        pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q]
+    else:
+        rng = np.random.default_rng(seed)
+        pairs = [(int(p), int(q)) for p, q in rng.integers(0, n, (4 * pmode, 2)) if p != q][:pmode]
+    if cap == 0:
+        b = GameBuild(lts, pairs)
+        og = _assert_same_game(b, lts, pairs)
+        _assert_same_energies(b, og)
+        pairs, cap = pairs[:40], 3000
+    b2 = GameBuild(lts, pairs, max_positions=cap)
+    og2 = orc.gamebuild(lts.n_vertices(), lts.row_offsets, lts.targets, lts.labels, pairs, max_positions=cap,
+                        positions_wanted=False)
+    h = b2._download()
+    assert bool(b2.build_stats["truncated"]) == og2["truncated"] and b2.n_positions() == og2["V"]
+    for k in ("row", "col", "wt", "atk"):
         assert np.array_equal(h[k], og2[k]), k
 
 
@@ -168,12 +221,15 @@ def test_one_block_chunks_match_oracle(monkeypatch, tail_max):
         assert np.array_equal(h[k], og2[k]), k
 
 
-@pytest.mark.parametrize("heavy_min,label_mask", [("1024", "1"), ("1024", "0"), ("1", "1")])
-def test_large_lts_capped_build_matches_oracle(monkeypatch, heavy_min, label_mask):
+@pytest.mark.parametrize("gen,heavy_min,label_mask", [("sub", "1024", "1"), ("sub", "1024", "0"), ("thread", "1024", "1"),
+                                                      ("thread", "1", "1"), ("wave", "1024", "1")])
+def test_large_lts_capped_build_matches_oracle(monkeypatch, gen, heavy_min, label_mask):
     # An LTS too large for k_generate's LDS copy: its walks read global memory,
     # where the per-state label bit sets apply (label_mask 0 turns them off).
-    # A capped build of 300 pairs equals the oracle's capped breadth-first walk.
-    print("large lts, capped build, heavy_min", heavy_min, "label_mask", label_mask, flush=True)
+    # A capped build of 300 pairs equals the oracle's capped breadth-first walk,
+    # whichever kernels generate the positions.
+    print("large lts, capped build, generator", gen, "heavy_min", heavy_min, "label_mask", label_mask, flush=True)
+    monkeypatch.setenv("GEQ_BUILD_GEN", gen)
     monkeypatch.setenv("GEQ_BUILD_HEAVY_MIN", heavy_min)
     monkeypatch.setenv("GEQ_BUILD_LABEL_MASK", label_mask)
     n, src, dst, lbl = workloads.random_lts(3000, 12000, 4, 7)

@@ -168,17 +168,18 @@ def test_invalid_and_empty_jobs_finish_alone():
     srv.close()
 
 
-@pytest.mark.parametrize("heavy_min", ["1024", "1"])
-def test_chunked_levels_and_the_position_cap(monkeypatch, heavy_min):
+@pytest.mark.parametrize("gen", ["sub", "wave", "block"])
+def test_chunked_levels_and_the_position_cap(monkeypatch, gen):
     # Tiny chunks: every level of the merged builds runs in pieces of 37
     # frontier positions, with the per-job caps checked after each piece.  The
     # jobs within the cap give exactly their standalone results; a job past it
     # fails alone.
-    # With heavy_min 1 every attack position with a non-empty Q is generated by
-    # a block of its own (k_generate_heavy), in chunks and ahead of the waits too.
-    print("serve: chunked levels, one job over the position cap, heavy_min", heavy_min, flush=True)
+    # With gen wave / block every attack position with a non-empty Q is
+    # generated by a wave / block of its own (k_generate_wave /
+    # k_generate_heavy), in chunks and ahead of the waits too.
+    print("serve: chunked levels, one job over the position cap, generator", gen, flush=True)
     monkeypatch.setenv("GEQ_BUILD_CHUNK", "37")
-    monkeypatch.setenv("GEQ_BUILD_HEAVY_MIN", heavy_min)
+    monkeypatch.setenv("GEQ_BUILD_GEN", gen)
     jobs = _jobs(10, 8)
     cap = max(size[0] for *_, size in jobs)
     rng = np.random.default_rng(99)
