@@ -98,13 +98,18 @@ RetireScope::~RetireScope() {
     auto& v = pool->bufs;
     v.insert(v.end(), t_retired.bufs.begin(), t_retired.bufs.end());
     t_retired.bufs.clear();
-    // keep the largest buffers that fit SPARE_BUFS and SPARE_BYTES; free the rest
+    // keep the largest buffers that fit SPARE_BUFS and SPARE_BYTES, and up to
+    // SPARE_SMALL_BUFS more of at most SPARE_SMALL bytes; free the rest
     std::sort(v.begin(), v.end(), [](const SpareBufs::Buf& a, const SpareBufs::Buf& b) { return a.bytes > b.bytes; });
-    size_t bytes = 0, kept = 0;
+    size_t bytes = 0, kept = 0, large = 0, small = 0;
     for (const SpareBufs::Buf& b : v) {
-        if (kept < SPARE_BUFS && bytes + b.bytes <= SPARE_BYTES) {
+        if (large < SPARE_BUFS && bytes + b.bytes <= SPARE_BYTES) {
             v[kept++] = b;
             bytes += b.bytes;
+            large++;
+        } else if (b.bytes <= SPARE_SMALL && small < SPARE_SMALL_BUFS) {
+            v[kept++] = b;
+            small++;
         } else {
             (void)hipFree(b.p);
         }

@@ -71,8 +71,15 @@ struct SpareBufs {
     void* take(size_t bytes, size_t* got);
     bool drop();   // frees them all; true if there were any
 };
-constexpr size_t SPARE_BUFS = 24;                  // spares kept at most ...
-constexpr size_t SPARE_BYTES = 4ull << 30;         // ... and their bytes at most (the largest that fit)
+// Spares kept: the largest buffers whose bytes fit SPARE_BYTES (at most
+// SPARE_BUFS of them), and besides those up to SPARE_SMALL_BUFS buffers of at
+// most SPARE_SMALL bytes each.  Every buffer freed is a hipFree, which waits
+// for the whole device, so the many small buffers a growing build outgrows
+// are kept rather than freed.
+constexpr size_t SPARE_BUFS = 24;
+constexpr size_t SPARE_BYTES = 4ull << 30;
+constexpr size_t SPARE_SMALL_BUFS = 256;
+constexpr size_t SPARE_SMALL = 16ull << 20;
 
 // Buffers outgrown inside a build.  hipFree waits for every stream of the
 // device, so a buffer that a build's queued kernels may still read is not

@@ -342,7 +342,9 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
         std::shared_lock<std::shared_mutex> shared(S->build_rw, std::defer_lock);
         std::unique_lock<std::shared_mutex> sole(S->build_rw, std::defer_lock);
         if (exclusive) sole.lock(); else shared.lock();
+        const double tb1 = S->now_ms();
         const geq_status st = gamebuild_jobs(bctx, ltss, prs, nps, S->max_positions, *ws, *store, B.get());
+        const double tb2 = S->now_ms();
         if (store->peak_nodes > BIG_BUILD_POSITIONS) {   // a very large build: its scratch goes
             ws = std::make_unique<gb::BuildScratch>();
             store = std::make_unique<geq_built>();
@@ -358,9 +360,10 @@ void builder_main(geq_server* S, geq_ctx* bctx) {
             }
             std::fprintf(stderr,
                          "[geq server] t=%.1f ms build of %zu jobs: %llu positions, %u levels, %llu over the cap (%llu positions), "
-                         "%.1f ms (merge %.1f, level loop %.1f, extraction %.1f ms; status %d)\n",
+                         "%.1f ms (lock %.1f, merge %.1f, level loop %.1f, extraction %.1f, teardown %.1f ms; status %d)\n",
                          tb0, build.size(), (unsigned long long)pos, B->levels, (unsigned long long)dead,
-                         (unsigned long long)dead_pos, S->now_ms() - tb0, B->prep_ms, B->build_ms, B->extract_ms, (int)st);
+                         (unsigned long long)dead_pos, S->now_ms() - tb0, tb1 - tb0, B->prep_ms, B->build_ms, B->extract_ms,
+                         tb2 - tb1 - B->prep_ms - B->build_ms - B->extract_ms, (int)st);
         }
         std::lock_guard<std::mutex> lk(S->mu);
         const double now = S->now_ms();
