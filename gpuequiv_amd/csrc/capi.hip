@@ -24,17 +24,24 @@ static std::vector<geq_ctx*> g_ctxs;
 
 bool drop_spare_games(int device) {
     std::vector<geq_game*> drop;
+    bool bufs = false;
     {
         std::lock_guard<std::mutex> lk(g_ctx_mu);
         for (geq_ctx* c : g_ctxs) {
             if (c->device != device) continue;
-            std::lock_guard<std::mutex> gl(c->game_mu);
-            if (c->spare_game) drop.push_back(c->spare_game);
-            c->spare_game = nullptr;
+            {
+                std::lock_guard<std::mutex> gl(c->game_mu);
+                if (c->spare_game) drop.push_back(c->spare_game);
+                c->spare_game = nullptr;
+            }
+            // the spares of the context's build scratch, unless a build holds it
+            // (that build's own allocation drops them through its RetireScope)
+            std::unique_lock<std::mutex> bl(c->build_mu, std::try_to_lock);
+            if (bl.owns_lock() && c->build_ws) bufs = drop_build_spares(c->build_ws.get()) || bufs;
         }
     }
     for (geq_game* g : drop) delete g;
-    return !drop.empty();
+    return !drop.empty() || bufs;
 }
 
 hipError_t device_malloc(void** p, size_t bytes) {
@@ -95,11 +102,14 @@ RetireScope::~RetireScope() {
         t_retired.bufs.clear();
         return;
     }
-    auto& v = pool->bufs;
-    v.insert(v.end(), t_retired.bufs.begin(), t_retired.bufs.end());
+    pool->bufs.insert(pool->bufs.end(), t_retired.bufs.begin(), t_retired.bufs.end());
     t_retired.bufs.clear();
-    // keep the largest buffers that fit SPARE_BUFS and SPARE_BYTES, and up to
-    // SPARE_SMALL_BUFS more of at most SPARE_SMALL bytes; free the rest
+    pool->trim();
+}
+// keep the largest buffers that fit SPARE_BUFS and SPARE_BYTES, and up to
+// SPARE_SMALL_BUFS more of at most SPARE_SMALL bytes; free the rest
+void SpareBufs::trim() {
+    auto& v = bufs;
     std::sort(v.begin(), v.end(), [](const SpareBufs::Buf& a, const SpareBufs::Buf& b) { return a.bytes > b.bytes; });
     size_t bytes = 0, kept = 0, large = 0, small = 0;
     for (const SpareBufs::Buf& b : v) {

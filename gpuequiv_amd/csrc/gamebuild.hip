@@ -16,7 +16,7 @@
 //   4. k_insert   - a device hash table with full-equality checks finds known
 //                   positions or claims a pending entry; among duplicates of a
 //                   new position the smallest candidate index wins;
-//   5. k_mark + scan - winners get ranks, i.e. new ids in first-occurrence
+//   5. scan_marks - winners get ranks, i.e. new ids in first-occurrence
 //                   order, which is the reference's numbering;
 //   6. k_finalize / k_edges - node records, hash entries, CSR rows, columns,
 //                   weights.
@@ -1108,10 +1108,21 @@ __device__ __forceinline__ uint32_t mark_cand(const Cand* cands, uint32_t j, uin
     return (c.kind_p != INVALID && (c.res & PENDING) && keymin[c.res & ~PENDING] == j) ? 1u : 0u;
 }
 
-__global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j <= n_cand) flag[j] = mark_cand(cands, j, n_cand, keymin);
+namespace {
+// candidate j's winner flag, for the scan that ranks the winners
+struct MarkIn {
+    const Cand* cands;
+    uint32_t n_cand;
+    const uint32_t* keymin;
+    __device__ __forceinline__ uint32_t operator()(size_t j) const { return mark_cand(cands, (uint32_t)j, n_cand, keymin); }
+};
+}  // namespace
+
+geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, DevBuf<uint32_t>& tmp,
+                      hipStream_t s) {
+    return exclusive_scan_in<uint32_t, SumU32, MarkIn>(MarkIn{cands, n_cand, keymin}, rank, n_cand, tmp, s, SumU32{});
 }
+
 
 // a new position's record and table entry (candidate j, flagged)
 __device__ __forceinline__ void finalize_cand(const Cand* cands, uint32_t j, const uint32_t* rank, uint32_t v_base, const NodeArgs& N,
@@ -1128,10 +1139,11 @@ __device__ __forceinline__ void finalize_cand(const Cand* cands, uint32_t j, con
     keymin[s] = 0xffffffffu;
 }
 
-__global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,
-                           NodeArgs N, unsigned long long* table, uint32_t* keymin) {
+// rank: the exclusive scan of the winner flags (a winner is where it steps)
+__global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* rank, uint32_t v_base, NodeArgs N,
+                           unsigned long long* table, uint32_t* keymin) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n_cand && flag[j]) finalize_cand(cands, j, rank, v_base, N, table, keymin);
+    if (j < n_cand && rank[j + 1] != rank[j]) finalize_cand(cands, j, rank, v_base, N, table, keymin);
 }
 
 // candidate j's edge: its parent's row, its slot, the target's id
@@ -1177,8 +1189,10 @@ __device__ __forceinline__ uint32_t chunk_word(uint32_t i, const uint32_t* n_edg
 }
 
 __global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
-                               const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out) {
-    for (uint32_t i = threadIdx.x; i < RB_JOBS + n_cnt; i += blockDim.x) out[i] = chunk_word(i, n_edges, n_new, add, chunk, next, heavy, cnt);
+                               const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out, uint32_t seq) {
+    for (uint32_t i = threadIdx.x; i < RB_JOBS + n_cnt; i += blockDim.x)
+        if (i != RB_SEQ) out[i] = chunk_word(i, n_edges, n_new, add, chunk, next, heavy, cnt);
+    rb_publish(out, seq);
 }
 
 // Exclusive scan of in[0..n) into out[0..n], out[n] the total, by one block of
@@ -1236,7 +1250,9 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_level_tail(TailArgs T) {
     __syncthreads();
     const uint32_t n_cnt = T.job_cnt ? T.n_jobs : 0u;
     for (uint32_t i = t; i < RB_JOBS + n_cnt; i += TAIL_BLOCK)
-        T.rb[i] = chunk_word(i, T.rowl + T.nf, T.rank + T.n_cand, T.add, T.chunk, T.last ? T.off : nullptr, T.H.count, T.job_cnt);
+        if (i != RB_SEQ)
+            T.rb[i] = chunk_word(i, T.rowl + T.nf, T.rank + T.n_cand, T.add, T.chunk, T.last ? T.off : nullptr, T.H.count, T.job_cnt);
+    rb_publish(T.rb, T.seq);
 }
 
 // start positions Attack(p, [q]) at ids [0, n): node records + hash entries

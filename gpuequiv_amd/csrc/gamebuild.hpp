@@ -111,9 +111,12 @@ __global__ void k_generate_sub(NodeArgs N, uint32_t lo, uint32_t n_front, LtsArg
 __global__ void k_generate_wave(NodeArgs N, uint32_t lo, LtsArgs L, const uint2* off, uint64_t spool_base, HeavyArgs H,
                                 Cand* cands, uint32_t* nsucc);
 __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask);
-__global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag);
-__global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* flag, const uint32_t* rank, uint32_t v_base,
-                           NodeArgs N, unsigned long long* table, uint32_t* keymin);
+// rank[0..n_cand] := exclusive scan of the candidates' winner flags (the
+// smallest candidate index of each new position), computed on the fly.
+geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, DevBuf<uint32_t>& tmp,
+                      hipStream_t s);
+__global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* rank, uint32_t v_base, NodeArgs N,
+                           unsigned long long* table, uint32_t* keymin);
 __global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
                         uint32_t* col, uint32_t* wt);
 // Work that follows a chunk's rows, done by k_rows' first block: the heavy
@@ -137,7 +140,9 @@ __global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk);
 
 // Words of the pinned readback block: a chunk's totals, the heavy positions
 // of the chunk bounded last, then the per-job position counts.
-constexpr uint32_t RB_EDGES = 0, RB_NEW = 1, RB_CAND = 2, RB_WORDS = 3, RB_HEAVY = 4, RB_JOBS = 5;
+// RB_SEQ is written last, after a system-scope fence, with the number the
+// host waits for (see rb_publish).
+constexpr uint32_t RB_EDGES = 0, RB_NEW = 1, RB_CAND = 2, RB_WORDS = 3, RB_HEAVY = 4, RB_SEQ = 5, RB_JOBS = 6;
 
 // Sum of (candidates, set-pool words) bound pairs, saturating at `lim`
 // (saturating addition of non-negative values is associative, so the scan
@@ -157,9 +162,22 @@ struct PairSum {
 // new positions and the per-job position counts cnt.  With `next` (the offsets
 // k_bounds_next scanned for the next level, whose *n_new + add positions are
 // taken in chunks of `chunk`), also the candidate and set-pool totals and the
-// heavy positions (*heavy) of the next level's first chunk.
+// heavy positions (*heavy) of the next level's first chunk.  `seq` goes to
+// out[RB_SEQ] last.
 __global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
-                               const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out);
+                               const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out, uint32_t seq);
+
+// The last step of a kernel that fills count words for the host: each
+// calling thread has written its words; after a system-scope fence by every
+// thread and a barrier, thread 0 writes `seq` to out[RB_SEQ].  The host, which
+// waits for that number, then reads every word (the memory is coherent).
+__device__ __forceinline__ void rb_publish(uint32_t* out, uint32_t seq) {
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(out + RB_SEQ, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
 
 // A chunk whose frontier, candidates and next-level bounds each fit TAIL_MAX
 // items runs everything after k_generate(_heavy) as one block of TAIL_BLOCK
@@ -191,6 +209,7 @@ struct TailArgs {
     uint32_t* rb;              // pinned count words
     uint32_t lo, nf, n_cand, v_base, e_base, mask;
     uint32_t hi, add, slots, chunk, last, lim;
+    uint32_t seq;              // written to rb[RB_SEQ] last
 };
 __global__ void k_level_tail(TailArgs T);
 
@@ -210,23 +229,42 @@ struct HostWords {
     }
 };
 
+// Pinned host memory a build packs its uploads into.
+struct PinnedStage {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage&) = delete;
+    PinnedStage& operator=(const PinnedStage&) = delete;
+    ~PinnedStage() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
 // Device buffers of one build (level loop and job extraction), kept by a
 // caller that builds repeatedly so that steady-state builds allocate nothing.
 struct BuildScratch {
-    DevBuf<uint32_t> l_row, l_dst, l_enrow, d_pairs, nsucc, rowl, flag, rank, keymin, d_jcnt;
-    DevBuf<unsigned long long> l_mask;
+    DevBuf<uint32_t> nsucc, rowl, flag, rank, keymin, d_jcnt;
+    // the build's uploads in one buffer: the LTS (rows, targets, labels), its
+    // distinct labels per state, label bit sets, start pairs and their insert
+    // flags, and for merged builds the job of each state and the job flags
+    DevBuf<uint32_t> l_pack;
+    PinnedStage stage;                // their pinned copy on the host
+    const uint32_t* job_map = nullptr;   // merged builds: the job of each state (in l_pack)
     DevBuf<uint2> ub, off;
     DevBuf<uint8_t> hv_flag;          // heavy positions of a frontier (k_bounds)
     DevBuf<uint32_t> hv_list, hv_count;
     HostWords rb;
-    DevBuf<int32_t> l_lbl, l_enlbl;
-    DevBuf<uint8_t> d_insert, d_dead;
+    uint32_t rb_seq = 0;              // the last number written to rb[RB_SEQ]
     DevBuf<uint32_t> scan32;          // block totals of the device scans
     DevBuf<uint2> scan2;
     RadixScratch sort;                // job extraction's sort
     DevBuf<Cand> cands;
     DevBuf<unsigned long long> table;
-    DevBuf<uint32_t> job_of_state, key, key_s, val, order, dsort, escan, local, d_ps, d_es;
+    DevBuf<uint32_t> key, key_s, val, order, dsort, escan, local, d_ps, d_es;
     SpareBufs spare;                  // buffers outgrown by earlier builds (RetireScope)
 };
 

@@ -88,10 +88,13 @@ geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<ui
 // Writes a chunk's candidate and set-pool totals (*a, *b) and its heavy
 // positions (*h) to pinned host memory: they reach the host with one launch
 // and one wait.
-__global__ void k_readback(const uint32_t* a, const uint32_t* b, const uint32_t* h, uint32_t* out) {
-    if (threadIdx.x == 0) out[RB_CAND] = *a;
-    if (threadIdx.x == 1) out[RB_WORDS] = *b;
-    if (threadIdx.x == 2) out[RB_HEAVY] = *h;
+__global__ void k_readback(const uint32_t* a, const uint32_t* b, const uint32_t* h, uint32_t* out, uint32_t seq) {
+    if (threadIdx.x == 0) {
+        out[RB_CAND] = *a;
+        out[RB_WORDS] = *b;
+        out[RB_HEAVY] = *h;
+    }
+    rb_publish(out, seq);
 }
 
 }  // namespace
@@ -142,6 +145,8 @@ bool game_fits(const geq_game* g, uint32_t V, uint32_t E) {
            g->stats.p;
 }
 }  // namespace
+
+bool geq::drop_build_spares(gb::BuildScratch* ws) { return ws->spare.drop(); }
 
 void geq::release_built_game(geq_game* g) {
     geq_ctx* ctx = g->ctx;
@@ -253,15 +258,15 @@ struct JobTrack {
     uint64_t cap = 0;
 };
 
-// New positions per job among a level's candidates: the winners (flag set)
-// counted by the job of their process (LDS histogram per block).
-__global__ void k_job_count_cands(const Cand* cands, const uint32_t* flag, uint32_t n_cand, const uint32_t* job_of_state,
+// New positions per job among a level's candidates: the winners (where the
+// rank steps) counted by the job of their process (LDS histogram per block).
+__global__ void k_job_count_cands(const Cand* cands, const uint32_t* rank, uint32_t n_cand, const uint32_t* job_of_state,
                                   uint32_t n_jobs, uint32_t* cnt) {
     __shared__ uint32_t h[1024];
     for (uint32_t j = threadIdx.x; j < n_jobs; j += blockDim.x) h[j] = 0;
     __syncthreads();
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cand; c += gridDim.x * blockDim.x)
-        if (flag[c]) atomicAdd(&h[job_of_state[cands[c].kind_p & 0x3fffffffu]], 1u);
+        if (rank[c + 1] != rank[c]) atomicAdd(&h[job_of_state[cands[c].kind_p & 0x3fffffffu]], 1u);
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < n_jobs; j += blockDim.x)
         if (h[j]) atomicAdd(&cnt[j], h[j]);
@@ -315,13 +320,16 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     B->n_start = n_pairs;
     B->V = B->E = 0;
     B->spool_top = 0;
-    auto& l_row = ws.l_row; auto& l_dst = ws.l_dst; auto& l_enrow = ws.l_enrow; auto& d_pairs = ws.d_pairs;
     auto& ub = ws.ub; auto& off = ws.off; auto& nsucc = ws.nsucc;
     auto& rowl = ws.rowl; auto& flag = ws.flag; auto& rank = ws.rank; auto& keymin = ws.keymin;
-    auto& l_lbl = ws.l_lbl; auto& l_enlbl = ws.l_enlbl; auto& d_insert = ws.d_insert;
     auto& cands = ws.cands; auto& table = ws.table;
     geq_status st = GEQ_OK;
-    auto bail = [&](geq_status e) { return e; };
+    // a failed build waits for its queued work before it returns: the next
+    // build on this scratch rewrites the pinned stage its upload reads
+    auto bail = [&](geq_status e) {
+        (void)hipStreamSynchronize(s);
+        return e;
+    };
 #define GB_TRY(x)                      \
     do {                               \
         geq_status _s = (x);           \
@@ -332,45 +340,68 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         hipError_t _e = (x);                                                                     \
         if (_e != hipSuccess) return bail(fail((code), std::string(#x) + ": " + hipGetErrorString(_e))); \
     } while (0)
-    GB_TRY(ensure(l_row, n + 1, s, false, "LTS"));
-    GB_TRY(ensure(l_dst, std::max<uint32_t>(m, 1), s, false, "LTS"));
-    GB_TRY(ensure(l_lbl, std::max<uint32_t>(m, 1), s, false, "LTS"));
-    GB_TRY(ensure(l_enrow, n + 1, s, false, "LTS"));
-    GB_TRY(ensure(l_enlbl, std::max<size_t>(en_lbl.size(), 1), s, false, "LTS"));
-    GB_HIP(hipMemcpyAsync(l_row.p, lts->row_offsets, 4ull * (n + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    if (m) {
-        GB_HIP(hipMemcpyAsync(l_dst.p, lts->targets, 4ull * m, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-        GB_HIP(hipMemcpyAsync(l_lbl.p, lts->labels, 4ull * m, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    }
-    GB_HIP(hipMemcpyAsync(l_enrow.p, en_row.data(), 4ull * (n + 1), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    if (!en_lbl.empty())
-        GB_HIP(hipMemcpyAsync(l_enlbl.p, en_lbl.data(), 4ull * en_lbl.size(), hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     auto& d_jcnt = ws.d_jcnt;
-    auto& d_dead = ws.d_dead;
-    auto& d_job_of_state = ws.job_of_state;
     const uint32_t n_jobs = jt ? (uint32_t)jt->count.size() : 0u;
-    if (jt) {
-        GB_TRY(ensure(d_job_of_state, n, s, false, "job map"));
-        GB_TRY(ensure(d_jcnt, n_jobs, s, false, "job counters"));
-        GB_TRY(ensure(d_dead, n_jobs, s, false, "job flags"));
-        GB_HIP(hipMemcpyAsync(d_job_of_state.p, jt->job_of_state.data(), 4ull * n, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-        GB_HIP(hipMemsetAsync(d_jcnt.p, 0, 4ull * n_jobs, s), GEQ_ERR_TRANSFER);
-        GB_HIP(hipMemcpyAsync(d_dead.p, jt->dead.data(), n_jobs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-    }
     // label bit sets when every label is in [0, 64) (label_range, compare_enabled)
     bool use_mask = !en_lbl.empty();
     for (const int32_t a : en_lbl) use_mask = use_mask && a >= 0 && a < 64;
     if (const char* env = std::getenv("GEQ_BUILD_LABEL_MASK")) use_mask = use_mask && std::atoi(env) != 0;
-    std::vector<unsigned long long> mask;   // alive until the build's last wait
+    std::vector<unsigned long long> mask;
     if (use_mask) {
         mask.assign(n, 0ull);
         for (uint32_t st = 0; st < n; st++)
             for (uint32_t k = en_row[st]; k < en_row[st + 1]; k++) mask[st] |= 1ull << en_lbl[k];
-        GB_TRY(ensure(ws.l_mask, std::max<uint32_t>(n, 1), s, false, "LTS"));
-        GB_HIP(hipMemcpyAsync(ws.l_mask.p, mask.data(), 8ull * n, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
     }
-    LtsArgs L{l_row.p, l_dst.p, l_lbl.p, l_enrow.p, l_enlbl.p, jt ? d_job_of_state.p : nullptr, jt ? d_dead.p : nullptr,
-              n, m, (uint32_t)en_lbl.size(), use_mask ? ws.l_mask.p : nullptr};
+    // The LTS, its label index and bit sets, the start pairs and (merged
+    // builds) the job map and flags go up as one copy from pinned memory into
+    // one device buffer (a copy from pageable memory waits for the runtime's
+    // staging, each time).
+    struct Part {
+        const void* src;
+        size_t bytes, off;
+    };
+    std::vector<Part> parts;
+    size_t pack_bytes = 0;
+    auto part = [&](const void* src, size_t bytes) {
+        parts.push_back(Part{src, bytes, pack_bytes});
+        pack_bytes = (pack_bytes + bytes + 15) & ~(size_t)15;
+        return parts.back().off;
+    };
+    const size_t o_row = part(lts->row_offsets, 4ull * (n + 1));
+    const size_t o_dst = part(lts->targets, 4ull * m);
+    const size_t o_lbl = part(lts->labels, 4ull * m);
+    const size_t o_enrow = part(en_row.data(), 4ull * (n + 1));
+    const size_t o_enlbl = part(en_lbl.data(), 4ull * en_lbl.size());
+    const size_t o_mask = part(mask.data(), 8ull * mask.size());
+    const size_t o_pairs = part(pairs, 8ull * n_pairs);
+    const size_t o_insert = part(insert.data(), n_pairs);
+    const size_t o_job = part(jt ? jt->job_of_state.data() : nullptr, jt ? 4ull * n : 0);
+    const size_t o_dead = part(jt ? jt->dead.data() : nullptr, jt ? (size_t)n_jobs : 0);
+    if (ws.stage.bytes < pack_bytes) {
+        // the last build's copy from the old stage ended before that build returned
+        const size_t bytes = std::max<size_t>(pack_bytes, 2 * ws.stage.bytes);
+        ws.stage.release();
+        GB_HIP(hipHostMalloc(&ws.stage.p, bytes, hipHostMallocDefault), GEQ_ERR_OUT_OF_MEMORY);
+        ws.stage.bytes = bytes;
+    }
+    for (const Part& pt : parts)
+        if (pt.bytes) std::memcpy(static_cast<char*>(ws.stage.p) + pt.off, pt.src, pt.bytes);
+    GB_TRY(ensure(ws.l_pack, std::max<size_t>(pack_bytes / 4, 1), s, false, "LTS"));
+    if (pack_bytes) GB_HIP(hipMemcpyAsync(ws.l_pack.p, ws.stage.p, pack_bytes, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
+    char* const pack = reinterpret_cast<char*>(ws.l_pack.p);
+    auto at32 = [&](size_t off) { return reinterpret_cast<uint32_t*>(pack + off); };
+    uint32_t* const d_pairs = at32(o_pairs);
+    uint8_t* const d_insert = reinterpret_cast<uint8_t*>(pack + o_insert);
+    uint32_t* const d_job_of_state = at32(o_job);
+    uint8_t* const d_dead = reinterpret_cast<uint8_t*>(pack + o_dead);
+    ws.job_map = jt ? d_job_of_state : nullptr;
+    if (jt) {
+        GB_TRY(ensure(d_jcnt, n_jobs, s, false, "job counters"));
+        GB_HIP(hipMemsetAsync(d_jcnt.p, 0, 4ull * n_jobs, s), GEQ_ERR_TRANSFER);
+    }
+    LtsArgs L{at32(o_row), at32(o_dst), reinterpret_cast<const int32_t*>(pack + o_lbl), at32(o_enrow),
+              reinterpret_cast<const int32_t*>(pack + o_enlbl), jt ? d_job_of_state : nullptr, jt ? d_dead : nullptr,
+              n, m, (uint32_t)en_lbl.size(), use_mask ? reinterpret_cast<const unsigned long long*>(pack + o_mask) : nullptr};
     // a small LTS is read from LDS by k_generate (its walks are chains of dependent reads)
     const uint32_t gen_lds = lts_words(n, m, (uint32_t)en_lbl.size()) <= LTS_LDS_WORDS
                                  ? 4u * lts_words(n, m, (uint32_t)en_lbl.size()) : 0u;
@@ -386,10 +417,26 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             return bail(fail(GEQ_ERR_INTERNAL, "gamebuild: no device view of the pinned count words"));
         }
         ws.rb.n = words;
+        ws.rb.h[RB_SEQ] = ws.rb_seq;
     }
     const uint32_t* const h_rb = ws.rb.h;
-    auto wait = [&]() -> geq_status {
-        GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    // The kernel that ends a wait writes the count words and then, last, the
+    // wait's number into h_rb[RB_SEQ] (rb_publish).  The host spins on that
+    // word, which is quicker than the runtime's completion wait; after 2 ms it
+    // waits for the stream instead, so a failed kernel still ends the wait
+    // (with the stream's error).
+    auto next_seq = [&]() { return ++ws.rb_seq; };
+    auto wait = [&](uint32_t seq) -> geq_status {
+        const auto w0 = std::chrono::steady_clock::now();
+        while (__atomic_load_n(h_rb + RB_SEQ, __ATOMIC_ACQUIRE) != seq) {
+            if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(2)) {
+                GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+                if (__atomic_load_n(h_rb + RB_SEQ, __ATOMIC_ACQUIRE) != seq)
+                    return fail(GEQ_ERR_INTERNAL, "gamebuild: the level's count words did not arrive");
+                break;
+            }
+            __builtin_ia32_pause();
+        }
         return GEQ_OK;
     };
 
@@ -408,11 +455,14 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     };
     uint32_t table_mask = 0;
     uint64_t table_slots = 0;     // power of two in use (the buffers may be larger)
+    // The table grows eightfold (each growth refills it and rehashes every
+    // position), to at most 2^31 slots.
     auto table_cap = [&](uint64_t entries) -> geq_status {
+        if (table_slots >= 2 * entries) return GEQ_OK;
         uint64_t cap = 1024;
         while (cap < 2 * entries) cap <<= 1;
+        cap = std::max<uint64_t>(cap, std::min<uint64_t>(8 * table_slots, 1ull << 31));
         if (cap > (1ull << 31)) return fail(GEQ_ERR_OUT_OF_MEMORY, "gamebuild: position table exceeds 2^31 entries");
-        if (table_slots >= cap) return GEQ_OK;
         table_slots = cap;
         geq_status e;
         if ((e = ensure(table, cap, s, false, "position table")) != GEQ_OK) return e;
@@ -422,7 +472,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         k_fill_u64<<<(uint32_t)((cap + tb - 1) / tb), tb, 0, s>>>(table.p, cap, EMPTY_ENTRY);
         k_fill_u32<<<(uint32_t)((cap + tb - 1) / tb), tb, 0, s>>>(keymin.p, cap, 0xffffffffu);
         if (B->V)
-            k_rehash<<<(B->V + tb - 1) / tb, tb, 0, s>>>(B->hash.p, B->V, d_insert.p, std::min<uint32_t>(n_pairs, B->V), table.p,
+            k_rehash<<<(B->V + tb - 1) / tb, tb, 0, s>>>(B->hash.p, B->V, d_insert, std::min<uint32_t>(n_pairs, B->V), table.p,
                                                         table_mask);
         return GEQ_OK;
     };
@@ -443,19 +493,16 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         GB_TRY(ensure(B->col, he, s, false, "edges"));
         GB_TRY(ensure(B->wt, he, s, false, "edges"));
     }
-    GB_TRY(table_cap(std::max<uint32_t>(n_pairs, 1)));
-    if (n_pairs) {
-        GB_TRY(ensure(d_pairs, 2ull * n_pairs, s, false, "start pairs"));
-        GB_TRY(ensure(d_insert, n_pairs, s, false, "start pairs"));
-        GB_HIP(hipMemcpyAsync(d_pairs.p, pairs, 8ull * n_pairs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-        GB_HIP(hipMemcpyAsync(d_insert.p, insert.data(), n_pairs, hipMemcpyHostToDevice, s), GEQ_ERR_TRANSFER);
-        k_starts<<<(n_pairs + tb - 1) / tb, tb, 0, s>>>(d_pairs.p, n_pairs, d_insert.p, nodes(), table.p, table_mask);
-    }
+    // a standalone build's table starts at the size of the context's last
+    // build too (up to 2^24 positions: the fill costs as much as the table)
+    GB_TRY(table_cap(std::max<uint64_t>(std::max<uint32_t>(n_pairs, 1), std::min<uint64_t>(hv, 1ull << 24))));
+    if (n_pairs)   // the pairs and their insert flags arrived with the LTS
+        k_starts<<<(n_pairs + tb - 1) / tb, tb, 0, s>>>(d_pairs, n_pairs, d_insert, nodes(), table.p, table_mask);
     B->V = n_pairs;
     B->spool_top = n_pairs;
     if (jt && n_pairs) {
         const uint32_t nb = std::min<uint32_t>(1024, (n_pairs + tb - 1) / tb);
-        k_job_count<<<nb, tb, 0, s>>>(B->p.p, 0, n_pairs, d_job_of_state.p, n_jobs, d_jcnt.p);
+        k_job_count<<<nb, tb, 0, s>>>(B->p.p, 0, n_pairs, d_job_of_state, n_jobs, d_jcnt.p);
     }
     uint64_t E = 0, levels = 0;
     uint32_t lo = 0, hi = n_pairs;   // [lo, hi): the rest of the current level's frontier
@@ -507,6 +554,9 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     // its wait, so the next level's first chunk starts without one.
     bool have_next = false;
     uint32_t next_cand = 0, next_words = 0, next_heavy = 0;
+    if (trace)
+        std::fprintf(stderr, "gamebuild: setup %.3f ms\n",
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     while (lo < hi) {
         const uint32_t c_hi = hi - lo > chunk ? lo + chunk : hi;
         const uint32_t nf = c_hi - lo;
@@ -521,8 +571,9 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             GB_HIP(hipMemsetAsync(ws.hv_count.p, 0, 4, s), GEQ_ERR_TRANSFER);
             k_bounds<<<(nf + 1 + tb - 1) / tb, tb, 0, s>>>(nodes(), lo, nf, L, ub.p, heavy_args());
             GB_TRY(exclusive_scan_pairs(ub.p, off.p, nf, ws.scan2, s));
-            k_readback<<<1, 64, 0, s>>>(&off.p[nf].x, &off.p[nf].y, ws.hv_count.p, ws.rb.d);
-            GB_TRY(wait());
+            const uint32_t sq = next_seq();
+            k_readback<<<1, 64, 0, s>>>(&off.p[nf].x, &off.p[nf].y, ws.hv_count.p, ws.rb.d, sq);
+            GB_TRY(wait(sq));
             n_cand = h_rb[RB_CAND];
             n_words = h_rb[RB_WORDS];
             n_heavy = h_rb[RB_HEAVY];
@@ -575,34 +626,32 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         const bool mark = jt && jt->cap;
         if (nf <= tail_max && n_cand <= tail_max && (!last_chunk || slots - 1 <= tail_max)) {
             // a small chunk: the rest of it in one block
+            const uint32_t sq = next_seq();
             const TailArgs T{nodes(), L, heavy_args(), cands.p, nsucc.p, rowl.p, flag.p, rank.p, table.p, keymin.p, B->col.p,
-                             B->wt.p, B->row.p, ub.p, off.p, jt ? d_jcnt.p : nullptr, mark ? d_dead.p : nullptr, n_jobs,
+                             B->wt.p, B->row.p, ub.p, off.p, jt ? d_jcnt.p : nullptr, mark ? d_dead : nullptr, n_jobs,
                              mark ? jt->cap : 0ull, ws.rb.d, lo, nf, n_cand, B->V, (uint32_t)E, table_mask, hi, add,
-                             (uint32_t)slots, chunk, last_chunk ? 1u : 0u, level_limit()};
+                             (uint32_t)slots, chunk, last_chunk ? 1u : 0u, level_limit(), sq};
             k_level_tail<<<1, TAIL_BLOCK, 0, s>>>(T);
-            GB_TRY(wait());
+            GB_TRY(wait(sq));
         } else {
             GB_TRY(exclusive_scan(nsucc.p, rowl.p, nf, ws.scan32, s));
-            if (n_cand) {
+            if (n_cand)
                 k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
-                k_mark<<<(n_cand + 1 + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, keymin.p, flag.p);
-            } else {
-                GB_HIP(hipMemsetAsync(flag.p, 0, 4, s), GEQ_ERR_TRANSFER);
-            }
-            GB_TRY(exclusive_scan(flag.p, rank.p, n_cand, ws.scan32, s));
+            // the winners' ranks (their flags computed inside the scan)
+            GB_TRY(scan_marks(cands.p, n_cand, keymin.p, rank.p, ws.scan32, s));
             if (n_cand) {
-                k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, flag.p, rank.p, B->V, nodes(), table.p, keymin.p);
+                k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rank.p, B->V, nodes(), table.p, keymin.p);
                 k_edges<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p, B->wt.p);
                 if (jt) {
                     // this level's new positions per job (the winners among the candidates)
                     const uint32_t nb = std::min<uint32_t>(1024, (n_cand + tb - 1) / tb);
-                    k_job_count_cands<<<nb, tb, 0, s>>>(cands.p, flag.p, n_cand, d_job_of_state.p, n_jobs, d_jcnt.p);
+                    k_job_count_cands<<<nb, tb, 0, s>>>(cands.p, rank.p, n_cand, d_job_of_state, n_jobs, d_jcnt.p);
                 }
             }
             // the rows; the heavy count cleared for k_bounds_next; jobs past the cap marked
             k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p,
                                                      RowsTail{last_chunk ? ws.hv_count.p : nullptr, mark ? d_jcnt.p : nullptr,
-                                                              mark ? n_jobs : 0u, mark ? jt->cap : 0ull, mark ? d_dead.p : nullptr});
+                                                              mark ? n_jobs : 0u, mark ? jt->cap : 0ull, mark ? d_dead : nullptr});
             if (last_chunk) {
                 k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots,
                                                                                chunk, L, ub.p, heavy_args());
@@ -610,9 +659,10 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             }
             // one synchronisation per chunk for the edge and position counts, the per-job
             // counts and the next level's first totals
+            const uint32_t sq = next_seq();
             k_chunk_counts<<<1, 256, 0, s>>>(rowl.p + nf, rank.p + n_cand, add, chunk, last_chunk ? off.p : nullptr,
-                                             ws.hv_count.p, jt ? d_jcnt.p : nullptr, n_jobs, ws.rb.d);
-            GB_TRY(wait());
+                                             ws.hv_count.p, jt ? d_jcnt.p : nullptr, n_jobs, ws.rb.d, sq);
+            GB_TRY(wait(sq));
         }
         const uint32_t n_edges = h_rb[RB_EDGES], n_new = h_rb[RB_NEW];
         if (last_chunk) {
@@ -628,7 +678,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         const bool level_done = lo == hi;
         if (level_done) {        // the next level is every position this one created
             if (trace)
-                std::fprintf(stderr, "gamebuild: level %llu: %u frontier -> %u new positions, %llu positions, %llu edges, %.1f ms\n",
+                std::fprintf(stderr, "gamebuild: level %llu: %u frontier -> %u new positions, %llu positions, %llu edges, %.3f ms\n",
                              (unsigned long long)levels, hi - level_lo, B->V - hi, (unsigned long long)B->V,
                              (unsigned long long)E,
                              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
@@ -781,7 +831,7 @@ geq_status geq::gamebuild_jobs(geq_ctx* ctx, const std::vector<geq_lts>& ltss, c
         if ((st = ensure(*b, V + 1, s, false, "job extraction")) != GEQ_OK) return st;
     for (auto* b : {&d_ps, &d_es})
         if ((st = ensure(*b, J + 1, s, false, "job extraction")) != GEQ_OK) return st;
-    k_pos_job<<<(V + tb - 1) / tb, tb, 0, s>>>(B->p.p, V, ws.job_of_state.p, key.p, val.p);
+    k_pos_job<<<(V + tb - 1) / tb, tb, 0, s>>>(B->p.p, V, ws.job_map, key.p, val.p);
     int end_bit = 1;
     while ((1u << end_bit) < J) end_bit++;
     // dsort and local are free until after the sort: they hold its middle passes
@@ -959,7 +1009,26 @@ geq_status geq_built_energies(geq_built* B, uint32_t first, uint32_t count, geq_
 
 void geq_built_free(geq_built* B) {
     if (!B) return;
-    (void)hipSetDevice(B->ctx->device);
+    geq_ctx* ctx = B->ctx;
+    (void)hipSetDevice(ctx->device);
+    {
+        // The game's position, edge and set arrays become spares of the
+        // context's build scratch, which the next build takes instead of
+        // allocating (and each hipFree would wait for the whole device).  Work
+        // queued on the context's stream may still read them: it ends first.
+        std::unique_lock<std::mutex> lk(ctx->build_mu, std::try_to_lock);
+        if (lk.owns_lock() && ctx->build_ws && hipStreamSynchronize(ctx->stream) == hipSuccess) {
+            SpareBufs& pool = ctx->build_ws->spare;
+            auto give = [&](auto& b) {
+                if (b.p) pool.bufs.push_back(SpareBufs::Buf{b.p, sizeof(*b.p) * std::max<size_t>(b.n, 1)});
+                b.p = nullptr;
+                b.n = 0;
+            };
+            give(B->kind); give(B->atk); give(B->p); give(B->q_off); give(B->q_len); give(B->x_off); give(B->x_len);
+            give(B->hash); give(B->spool); give(B->row); give(B->col); give(B->wt);
+            pool.trim();
+        }
+    }
     delete B;
 }
 

@@ -70,6 +70,7 @@ struct SpareBufs {
     // a buffer with room for `bytes` (and at most 4x as much), or NULL; its size in *got
     void* take(size_t bytes, size_t* got);
     bool drop();   // frees them all; true if there were any
+    void trim();   // frees those past the limits below
 };
 // Spares kept: the largest buffers whose bytes fit SPARE_BYTES (at most
 // SPARE_BUFS of them), and besides those up to SPARE_SMALL_BUFS buffers of at
@@ -145,6 +146,11 @@ geq_status standard_seed(geq_game* g);
 // A built game's buffers go to its context's cache (one game) for the next
 // built game, or are freed when the cache is taken.
 void release_built_game(geq_game* g);
+namespace gb {
+struct BuildScratch;
+}
+// Frees the spare buffers of a build scratch (true if there were any).
+bool drop_build_spares(gb::BuildScratch* ws);
 geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_energies* out);
 // The same for any nodes of a game without renumbering and in the bitmap
 // layout: entry v of the result is node map[v] (device ids); t is the tag of

@@ -1,15 +1,19 @@
-// Device-wide exclusive scan (reduce-then-scan, three launches): used for the
-// energy download's offsets, gamebuild's per-level offsets, the builder's and
-// the server's reverse CSRs, and the radix sort's digit offsets.
+// Device-wide exclusive scan (reduce-then-scan): used for the energy
+// download's offsets, gamebuild's per-level offsets, the builder's and the
+// server's reverse CSRs, and the radix sort's digit offsets.
 //   k_scan_reduce : every block of SCAN_ITEMS consecutive items -> its total
 //   k_scan_tops   : one block scans the block totals in place (and the grand total)
 //   k_scan_apply  : every block scans its items again from its block's offset
-// An input of at most SCAN_ITEMS items takes k_scan_apply alone (one launch).
+// An input of at most SCAN_ITEMS items takes k_scan_apply alone (one launch);
+// one of at most SCAN_RAW_MAX blocks skips k_scan_tops, each k_scan_apply
+// block combining the totals of the blocks before it itself (two launches).
 // Global memory is read and written striped (lane-consecutive, coalesced); the
 // apply step transposes its tile through LDS so that each thread scans
 // SCAN_PER_THREAD consecutive items.  The combine operator is a template
 // (plain sums, or the saturating pair sums of the gamebuild bounds); it must
 // be associative and commutative (the block totals combine in striped order).
+// The input is read through an accessor (an array, or values computed from
+// other arrays, as gamebuild's candidate flags).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,6 +28,7 @@ constexpr uint32_t SCAN_THREADS = 256;
 constexpr uint32_t SCAN_PER_THREAD = 16;
 constexpr uint32_t SCAN_ITEMS = SCAN_THREADS * SCAN_PER_THREAD;   // items per block
 constexpr uint32_t SCAN_TOP_THREADS = 1024;
+constexpr uint32_t SCAN_RAW_MAX = 1024;   // block totals an apply block combines itself
 
 struct SumU32 {
     __host__ __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
@@ -70,14 +75,21 @@ __device__ __forceinline__ T block_scan_excl(T v, Op op, T& total) {
 
 }  // namespace detail
 
-template <class T, class Op>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const T* in, size_t n, T* tops, Op op) {
+// Input accessor of an array.
+template <class T>
+struct ScanPtr {
+    const T* p;
+    __device__ __forceinline__ T operator()(size_t i) const { return p[i]; }
+};
+
+template <class T, class Op, class In>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(In in, size_t n, T* tops, Op op) {
     const size_t b0 = (size_t)blockIdx.x * SCAN_ITEMS + threadIdx.x;
     T acc = Op::identity();
 #pragma unroll
     for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
         const size_t i = b0 + (size_t)k * SCAN_THREADS;
-        if (i < n) acc = op(acc, in[i]);
+        if (i < n) acc = op(acc, in(i));
     }
     T total;
     (void)detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total);
@@ -105,15 +117,27 @@ __global__ __launch_bounds__(SCAN_TOP_THREADS) void k_scan_tops(T* tops, uint32_
 // runs of SCAN_PER_THREAD consecutive items over the banks
 __device__ __forceinline__ uint32_t scan_slot(uint32_t i) { return i + (i >> 5); }
 
-template <class T, class Op>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T* in, T* out, size_t n, const T* tops, Op op) {
+// How k_scan_apply finds its block's offset: none (one block), from tops
+// scanned by k_scan_tops, or by combining the raw block totals before it.
+constexpr uint32_t SCAN_TOPS_NONE = 0, SCAN_TOPS_SCANNED = 1, SCAN_TOPS_RAW = 2;
+
+template <class T, class Op, class In>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(In in, T* out, size_t n, const T* tops, uint32_t mode, Op op) {
     __shared__ T tile[SCAN_ITEMS + SCAN_ITEMS / 32];
     const size_t base = (size_t)blockIdx.x * SCAN_ITEMS;
     const uint32_t t = threadIdx.x;
 #pragma unroll
     for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
         const uint32_t j = k * SCAN_THREADS + t;
-        tile[scan_slot(j)] = base + j < n ? in[base + j] : Op::identity();
+        tile[scan_slot(j)] = base + j < n ? in(base + j) : Op::identity();
+    }
+    T pre = Op::identity();
+    if (mode == SCAN_TOPS_SCANNED) {
+        pre = tops[blockIdx.x];
+    } else if (mode == SCAN_TOPS_RAW) {   // the totals of the blocks before this one
+        T a = Op::identity();
+        for (uint32_t k = t; k < blockIdx.x; k += SCAN_THREADS) a = op(a, tops[k]);
+        (void)detail::block_scan_excl<T, Op, SCAN_THREADS>(a, op, pre);
     }
     __syncthreads();
     T v[SCAN_PER_THREAD];
@@ -124,8 +148,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T* in, T* out
         acc = op(acc, v[k]);
     }
     T total;
-    // tops NULL: a one-block scan (no block offsets; the block's total is the grand total)
-    const T pre = tops ? tops[blockIdx.x] : Op::identity();
     T run = op(pre, detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total));
     // each thread rewrites only the slots it read; the barrier inside the block
     // scan has ordered every read of the tile before these writes
@@ -140,17 +162,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T* in, T* out
         const uint32_t j = k * SCAN_THREADS + t;
         if (base + j < n) out[base + j] = tile[scan_slot(j)];
     }
-    if (blockIdx.x == gridDim.x - 1 && t == 0) out[n] = tops ? tops[gridDim.x] : total;   // the grand total
+    if (blockIdx.x == gridDim.x - 1 && t == 0)   // the grand total
+        out[n] = mode == SCAN_TOPS_SCANNED ? tops[gridDim.x] : op(pre, total);
 }
 
-// Exclusive scan of in[0..n) into out[0..n), the total into out[n] (out holds
-// n + 1 entries; in == out is allowed).  tmp: block totals, grown as needed.
-template <class T, class Op>
-geq_status exclusive_scan_dev(const T* in, T* out, size_t n, DevBuf<T>& tmp, hipStream_t s, Op op) {
+// Exclusive scan of in(0..n) into out[0..n), the total into out[n] (out holds
+// n + 1 entries; an array input may be out itself).  tmp: block totals, grown
+// as needed.
+template <class T, class Op, class In>
+geq_status exclusive_scan_in(In in, T* out, size_t n, DevBuf<T>& tmp, hipStream_t s, Op op) {
     const size_t nb = std::max<size_t>(1, (n + SCAN_ITEMS - 1) / SCAN_ITEMS);
     if (nb > 0x7fffffffull) return fail(GEQ_ERR_INVALID_ARG, "scan of more than 2^31 blocks");
     if (nb == 1) {   // one block holds the input: one launch
-        k_scan_apply<T, Op><<<1, SCAN_THREADS, 0, s>>>(in, out, n, nullptr, op);
+        k_scan_apply<T, Op, In><<<1, SCAN_THREADS, 0, s>>>(in, out, n, nullptr, SCAN_TOPS_NONE, op);
         GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
         return GEQ_OK;
     }
@@ -158,11 +182,20 @@ geq_status exclusive_scan_dev(const T* in, T* out, size_t n, DevBuf<T>& tmp, hip
         tmp.retire();
         GEQ_HIP(tmp.alloc(nb + 1), GEQ_ERR_OUT_OF_MEMORY);
     }
-    k_scan_reduce<T, Op><<<(uint32_t)nb, SCAN_THREADS, 0, s>>>(in, n, tmp.p, op);
-    k_scan_tops<T, Op><<<1, SCAN_TOP_THREADS, 0, s>>>(tmp.p, (uint32_t)nb, op);
-    k_scan_apply<T, Op><<<(uint32_t)nb, SCAN_THREADS, 0, s>>>(in, out, n, tmp.p, op);
+    k_scan_reduce<T, Op, In><<<(uint32_t)nb, SCAN_THREADS, 0, s>>>(in, n, tmp.p, op);
+    if (nb <= SCAN_RAW_MAX) {
+        k_scan_apply<T, Op, In><<<(uint32_t)nb, SCAN_THREADS, 0, s>>>(in, out, n, tmp.p, SCAN_TOPS_RAW, op);
+    } else {
+        k_scan_tops<T, Op><<<1, SCAN_TOP_THREADS, 0, s>>>(tmp.p, (uint32_t)nb, op);
+        k_scan_apply<T, Op, In><<<(uint32_t)nb, SCAN_THREADS, 0, s>>>(in, out, n, tmp.p, SCAN_TOPS_SCANNED, op);
+    }
     GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
     return GEQ_OK;
+}
+
+template <class T, class Op>
+geq_status exclusive_scan_dev(const T* in, T* out, size_t n, DevBuf<T>& tmp, hipStream_t s, Op op) {
+    return exclusive_scan_in<T, Op, ScanPtr<T>>(ScanPtr<T>{in}, out, n, tmp, s, op);
 }
 
 }  // namespace geq
