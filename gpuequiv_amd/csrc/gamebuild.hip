@@ -47,27 +47,43 @@ __device__ __forceinline__ uint32_t mix(uint32_t h, uint32_t v) {
     return h * 0x85ebca6bu;
 }
 
+// Position hashes key the position table only: every match is confirmed by
+// comparing the positions, so no id depends on them.  A position's hash is
+// fmix32(head + S_Q + S_X): the head mixes kind, p and the two set sizes, and
+// each set adds one term per member (fmix32 of the member, salted apart for
+// Q and Q*).  The terms of a set sum in any order, so lanes or a block add a
+// set's part by a reduction, and all candidates of a label group share it.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+constexpr uint32_t SALT_Q = 0x2545f491u, SALT_X = 0x68e31da4u;
+__device__ __forceinline__ uint32_t set_term(uint32_t v, uint32_t salt) { return fmix32(v * 0x9e3779b1u + salt); }
+__device__ __forceinline__ uint32_t hash_head(uint32_t kind, uint32_t p, uint32_t ql, uint32_t xl) {
+    return mix(mix(mix(mix(0x811c9dc5u, kind), p), ql), xl);
+}
+__device__ __forceinline__ uint32_t hash_of(uint32_t head, uint32_t sets) {
+    const uint32_t h = fmix32(head + sets);
+    return h == 0xffffffffu ? 0x7fffffffu : h;
+}
+
 __device__ uint32_t pos_hash(uint32_t kind, uint32_t p, const uint32_t* __restrict__ spool, uint32_t qo, uint32_t ql,
                              uint32_t xo, uint32_t xl) {
-    uint32_t h = mix(0x811c9dc5u, kind);
-    h = mix(h, p);
-    h = mix(h, ql);
-    for (uint32_t k = 0; k < ql; k++) h = mix(h, spool[qo + k]);
-    h = mix(h, xl);
-    for (uint32_t k = 0; k < xl; k++) h = mix(h, spool[xo + k]);
-    h ^= h >> 16;
-    return h == 0xffffffffu ? 0x7fffffffu : h;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < ql; k++) sum += set_term(spool[qo + k], SALT_Q);
+    for (uint32_t k = 0; k < xl; k++) sum += set_term(spool[xo + k], SALT_X);
+    return hash_of(hash_head(kind, p, ql, xl), sum);
 }
 
 // The same hash of a position with set Q in LDS (entry k at q[k * st]) and no Q*.
 __device__ uint32_t pos_hash_lds(uint32_t kind, uint32_t p, const uint32_t* q, uint32_t ql, uint32_t st) {
-    uint32_t h = mix(0x811c9dc5u, kind);
-    h = mix(h, p);
-    h = mix(h, ql);
-    for (uint32_t k = 0; k < ql; k++) h = mix(h, q[k * st]);
-    h = mix(h, 0u);
-    h ^= h >> 16;
-    return h == 0xffffffffu ? 0x7fffffffu : h;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < ql; k++) sum += set_term(q[k * st], SALT_Q);
+    return hash_of(hash_head(kind, p, ql, 0), sum);
 }
 
 __device__ bool sets_equal(const uint32_t* __restrict__ spool, uint32_t ao, uint32_t bo, uint32_t len) {
@@ -439,10 +455,6 @@ namespace {
 
 constexpr uint32_t HV_BLOCK = 256;
 
-__device__ __forceinline__ uint32_t hash_finish(uint32_t h) {
-    h ^= h >> 16;
-    return h == 0xffffffffu ? 0x7fffffffu : h;
-}
 
 // Exclusive prefix sum of one value per thread over the block (threads in
 // order); `total` gets the block's sum.  ws: HV_BLOCK / 64 words of LDS.
@@ -554,10 +566,15 @@ __global__ __launch_bounds__(HV_BLOCK) void k_generate_heavy(NodeArgs N, uint32_
             // Q' = the distinct values in order, written to the pool at sp
             uint32_t b0, b1;
             share(T, b0, b1);
-            uint32_t nd = 0;
-            for (uint32_t x = b0; x < b1; x++) nd += (x == 0u || qs[x] != qs[x - 1]) ? 1u : 0u;
-            uint32_t Tu;
+            uint32_t nd = 0, part_sum = 0;
+            for (uint32_t x = b0; x < b1; x++)
+                if (x == 0u || qs[x] != qs[x - 1]) {
+                    nd++;
+                    part_sum += set_term(qs[x], SALT_Q);
+                }
+            uint32_t Tu, sq;
             uint32_t r = block_scan(nd, ws, Tu);
+            (void)block_scan(part_sum, ws, sq);   // Q''s part of its candidates' hashes
             for (uint32_t x = b0; x < b1; x++)
                 if (x == 0u || qs[x] != qs[x - 1]) spool[sp + r++] = qs[x];
             // an observation move to every target of the group outside Q', in transition order
@@ -570,14 +587,11 @@ __global__ __launch_bounds__(HV_BLOCK) void k_generate_heavy(NodeArgs N, uint32_
             for (uint32_t x = c0; x < c1; x++) {
                 const uint32_t pp = L.dst[t + x];
                 if (contains_sorted(qs, T, pp)) continue;
-                uint32_t hh = mix(mix(mix(0x811c9dc5u, K_ATTACK), pp), Tu);
-                for (uint32_t y = 0; y < T; y++)
-                    if (y == 0u || qs[y] != qs[y - 1]) hh = mix(hh, qs[y]);
                 Cand c;
                 c.kind_p = (K_ATTACK << 30) | pp;
                 c.q_off = (uint32_t)sp; c.q_len = Tu; c.x_off = 0; c.x_len = 0;
                 c.weight = W_OBS;
-                c.hash = hash_finish(mix(hh, 0u));
+                c.hash = hash_of(hash_head(K_ATTACK, pp, Tu, 0), sq);
                 c.res = 0;
                 c.parent = i;
                 c.slot = slot + rr;
@@ -589,22 +603,23 @@ __global__ __launch_bounds__(HV_BLOCK) void k_generate_heavy(NodeArgs N, uint32_
             __syncthreads();   // qs is rewritten by the next group
         }
         // Q and its enabledness flags (bit 0 subset, bit 1 superset) in LDS
+        uint32_t q_sum = 0;
         for (uint32_t k = a0; k < a1; k++) {
             const uint32_t q = spool[qo + k];
             bool sub, sup;
             compare_enabled(L, p, q, sub, sup);
             qs[k] = q;
             fl[k] = (sub ? 1u : 0u) | (sup ? 2u : 0u);
+            q_sum += set_term(q, SALT_Q);
         }
-        __syncthreads();
+        uint32_t sq;
+        (void)block_scan(q_sum, ws, sq);   // its barriers also order the writes above
         if (tid == 0) {                 // Defend(p, Q, {})
-            uint32_t hh = mix(mix(mix(0x811c9dc5u, K_DEFEND), p), ql);
-            for (uint32_t k = 0; k < ql; k++) hh = mix(hh, qs[k]);
             Cand c;
             c.kind_p = (K_DEFEND << 30) | p;
             c.q_off = qo; c.q_len = ql; c.x_off = 0; c.x_len = 0;
             c.weight = W_CHAL;
-            c.hash = hash_finish(mix(hh, 0u));
+            c.hash = hash_of(hash_head(K_DEFEND, p, ql, 0), sq);
             c.res = 0;
             c.parent = i;
             c.slot = slot;
@@ -632,22 +647,24 @@ __global__ __launch_bounds__(HV_BLOCK) void k_generate_heavy(NodeArgs N, uint32_
             if (dup) continue;
             // stable partition: the members outside the part, then those inside
             const uint32_t na = ql - cnt;
+            uint32_t h_sum = 0;   // the members outside the part are Q, those inside Q*
             for (uint32_t k = a0; k < a1; k++) {
-                if (in_part(which, fl[k])) spool[pb + na + rin++] = qs[k];
-                else spool[pb + (k - rin)] = qs[k];
+                if (in_part(which, fl[k])) {
+                    spool[pb + na + rin++] = qs[k];
+                    h_sum += set_term(qs[k], SALT_X);
+                } else {
+                    spool[pb + (k - rin)] = qs[k];
+                    h_sum += set_term(qs[k], SALT_Q);
+                }
             }
+            uint32_t sqx;
+            (void)block_scan(h_sum, ws, sqx);
             if (tid == 0) {
-                uint32_t hh = mix(mix(mix(0x811c9dc5u, K_DEFEND), p), na);
-                for (uint32_t k = 0; k < ql; k++)
-                    if (!in_part(which, fl[k])) hh = mix(hh, qs[k]);
-                hh = mix(hh, cnt);
-                for (uint32_t k = 0; k < ql; k++)
-                    if (in_part(which, fl[k])) hh = mix(hh, qs[k]);
                 Cand c;
                 c.kind_p = (K_DEFEND << 30) | p;
                 c.q_off = (uint32_t)pb; c.q_len = na; c.x_off = (uint32_t)(pb + na); c.x_len = cnt;
                 c.weight = W_CHAL;
-                c.hash = hash_finish(hh);
+                c.hash = hash_of(hash_head(K_DEFEND, p, na, cnt), sqx);
                 c.res = 0;
                 c.parent = i;
                 c.slot = slot;
@@ -895,8 +912,9 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
             }
             wave_sync();
         }
-        // Q' = the distinct values in order: into qs and the pool at sp
-        uint32_t nq = 0;
+        // Q' = the distinct values in order: into qs and the pool at sp; sq is
+        // its part of the group's candidate hashes
+        uint32_t nq = 0, part_sum = 0;
         for (uint32_t x0 = 0; x0 < T; x0 += G) {
             const uint32_t x = x0 + r;
             const bool keep = x < T && (x == 0u || srt[x] != srt[x - 1u]);
@@ -905,9 +923,11 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
                 const uint32_t d = nq + C::below(m);
                 qs[d] = srt[x];
                 spool[sp + d] = srt[x];
+                part_sum += set_term(srt[x], SALT_Q);
             }
             nq += (uint32_t)__popcll(m);
         }
+        const uint32_t sq = C::sum(part_sum);
         wave_sync();
         // an observation move to every target of the group outside Q', in transition order
         for (uint32_t b0 = t; b0 < te; b0 += G) {
@@ -923,13 +943,11 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
             const bool e = in && !contains_sorted(qs, nq, pp);
             const uint64_t m = C::ballot(e);
             if (e) {
-                uint32_t h = mix(mix(mix(0x811c9dc5u, K_ATTACK), pp), nq);
-                for (uint32_t y = 0; y < nq; y++) h = mix(h, qs[y]);
                 Cand cc;
                 cc.kind_p = (K_ATTACK << 30) | pp;
                 cc.q_off = (uint32_t)sp; cc.q_len = nq; cc.x_off = 0; cc.x_len = 0;
                 cc.weight = W_OBS;
-                cc.hash = hash_finish(mix(h, 0u));
+                cc.hash = hash_of(hash_head(K_ATTACK, pp, nq, 0), sq);
                 cc.res = 0;
                 cc.parent = i;
                 cc.slot = slot + C::below(m);
@@ -941,14 +959,15 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
         sp += nq;
         wave_sync();   // the buffers are rewritten by the next group
     }
+    uint32_t q_sum = 0;
+    for (uint32_t k = r; k < ql; k += G) q_sum += set_term(sm[k], SALT_Q);
+    const uint32_t sq_all = C::sum(q_sum);
     if (r == 0) {   // Defend(p, Q, {})
-        uint32_t h = mix(mix(mix(0x811c9dc5u, K_DEFEND), p), ql);
-        for (uint32_t k = 0; k < ql; k++) h = mix(h, sm[k]);
         Cand cc;
         cc.kind_p = (K_DEFEND << 30) | p;
         cc.q_off = qo; cc.q_len = ql; cc.x_off = 0; cc.x_len = 0;
         cc.weight = W_CHAL;
-        cc.hash = hash_finish(mix(h, 0u));
+        cc.hash = hash_of(hash_head(K_DEFEND, p, ql, 0), sq_all);
         cc.res = 0;
         cc.parent = i;
         cc.slot = slot;
@@ -979,7 +998,7 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
         if (dup) continue;
         // stable partition: the members outside the part, then those inside
         const uint32_t na = ql - cnt;
-        uint32_t run = 0;
+        uint32_t run = 0, h_sum = 0;   // the members outside the part are Q, those inside Q*
         for (uint32_t k0 = 0; k0 < ql; k0 += G) {
             const uint32_t k = k0 + r;
             const bool f = k < ql && in_part(which, sf[k]);
@@ -988,21 +1007,17 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
             if (k < ql) {
                 if (f) spool[pb + na + before] = sm[k];
                 else spool[pb + (k - before)] = sm[k];
+                h_sum += set_term(sm[k], f ? SALT_X : SALT_Q);
             }
             run += (uint32_t)__popcll(m);
         }
+        const uint32_t sqx = C::sum(h_sum);
         if (r == 0) {
-            uint32_t h = mix(mix(mix(0x811c9dc5u, K_DEFEND), p), na);
-            for (uint32_t k = 0; k < ql; k++)
-                if (!in_part(which, sf[k])) h = mix(h, sm[k]);
-            h = mix(h, cnt);
-            for (uint32_t k = 0; k < ql; k++)
-                if (in_part(which, sf[k])) h = mix(h, sm[k]);
             Cand cc;
             cc.kind_p = (K_DEFEND << 30) | p;
             cc.q_off = (uint32_t)pb; cc.q_len = na; cc.x_off = (uint32_t)(pb + na); cc.x_len = cnt;
             cc.weight = W_CHAL;
-            cc.hash = hash_finish(h);
+            cc.hash = hash_of(hash_head(K_DEFEND, p, na, cnt), sqx);
             cc.res = 0;
             cc.parent = i;
             cc.slot = slot;
