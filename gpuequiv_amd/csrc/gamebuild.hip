@@ -715,13 +715,15 @@ struct Coop {
 };
 
 // One position's LDS: Q's members, their enabledness flags (bit 0 subset,
-// bit 1 superset), a label group's successors as gathered, and a second
-// buffer for the sort and for Q'.
+// bit 1 superset), a label group's successors as gathered, and the rank
+// sort's output for groups of at most SUB_SORT successors (larger groups of
+// a wave are sorted in place).
+constexpr uint32_t SUB_SORT = 64;
 template <uint32_t QMAX, uint32_t TMAX>
 struct CoopLds {
     uint32_t mem[QMAX];
     uint32_t gath[TMAX];
-    uint32_t sort[TMAX];
+    uint32_t sort[SUB_SORT];
     uint8_t flag[QMAX];
 };
 
@@ -876,10 +878,11 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
         wave_sync();
         // sorted: by rank counting (ties by gather index) into S.sort, or for a
         // large group of a wave by a bitonic sort in place (padding sorts last:
-        // state ids are below 2^30)
+        // state ids are below 2^30); Q' then goes to sg (in place for the
+        // bitonic sort: each round reads its values before it moves them down)
         const uint32_t* srt = S.sort;
         uint32_t* qs = sg;   // Q'
-        if (G == 64u && T > 64u) {
+        if (G == 64u && T > SUB_SORT) {
             uint32_t n2 = 1;
             while (n2 < T) n2 <<= 1;   // <= TMAX, a power of two
             for (uint32_t x = T + r; x < n2; x += G) sg[x] = 0xffffffffu;
@@ -899,7 +902,6 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
                     wave_sync();
                 }
             srt = sg;
-            qs = S.sort;
         } else {
             for (uint32_t x = r; x < T; x += G) {
                 const uint32_t v = sg[x];
@@ -917,15 +919,18 @@ __device__ __forceinline__ void gen_coop(const NodeArgs& N, uint32_t lo, uint32_
         uint32_t nq = 0, part_sum = 0;
         for (uint32_t x0 = 0; x0 < T; x0 += G) {
             const uint32_t x = x0 + r;
-            const bool keep = x < T && (x == 0u || srt[x] != srt[x - 1u]);
+            const uint32_t v = x < T ? srt[x] : 0u;
+            const bool keep = x < T && (x == 0u || v != srt[x - 1u]);
             const uint64_t m = C::ballot(keep);
+            wave_sync();   // every read of this round before the moves (qs may be srt)
             if (keep) {
-                const uint32_t d = nq + C::below(m);
-                qs[d] = srt[x];
-                spool[sp + d] = srt[x];
-                part_sum += set_term(srt[x], SALT_Q);
+                const uint32_t d = nq + C::below(m);   // d <= x
+                qs[d] = v;
+                spool[sp + d] = v;
+                part_sum += set_term(v, SALT_Q);
             }
             nq += (uint32_t)__popcll(m);
+            wave_sync();
         }
         const uint32_t sq = C::sum(part_sum);
         wave_sync();
@@ -1133,9 +1138,18 @@ struct MarkIn {
 };
 }  // namespace
 
-geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, DevBuf<uint32_t>& tmp,
-                      hipStream_t s) {
-    return exclusive_scan_in<uint32_t, SumU32, MarkIn>(MarkIn{cands, n_cand, keymin}, rank, n_cand, tmp, s, SumU32{});
+__global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n_cand) flag[j] = mark_cand(cands, j, n_cand, keymin);
+}
+
+geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, uint32_t* flag,
+                      DevBuf<uint32_t>& tmp, hipStream_t s) {
+    if (n_cand < SCAN_MARK_FLAGS)   // the flags computed inside the scan (one launch less)
+        return exclusive_scan_in<uint32_t, SumU32, MarkIn>(MarkIn{cands, n_cand, keymin}, rank, n_cand, tmp, s, SumU32{});
+    // many candidates: the scan's two passes would each read the 40-byte records
+    k_mark<<<(n_cand + 255) / 256, 256, 0, s>>>(cands, n_cand, keymin, flag);
+    return exclusive_scan_dev<uint32_t, SumU32>(flag, rank, n_cand, tmp, s, SumU32{});
 }
 
 

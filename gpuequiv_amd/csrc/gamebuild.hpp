@@ -69,7 +69,7 @@ constexpr uint32_t HV_CAP = 8192;
 // and sum of member degrees its LDS holds; k_generate_wave (a wave per
 // position): the same limits.
 constexpr uint32_t SUB_LANES = 8, SUB_BLOCK = 256, SUB_QMAX = 32, SUB_TMAX = 64;
-constexpr uint32_t WAVE_QMAX = 256, WAVE_TMAX = 512;
+constexpr uint32_t WAVE_QMAX = 256, WAVE_TMAX = 1024;
 // Which kernels generate a level: k_generate with k_generate_heavy
 // (GEN_BY_THREAD), or k_generate_sub with k_generate_wave and k_generate_heavy
 // for the positions past its limits (GEN_BY_SUB); GEN_BY_WAVE and GEN_BY_BLOCK send
@@ -112,9 +112,11 @@ __global__ void k_generate_wave(NodeArgs N, uint32_t lo, LtsArgs L, const uint2*
                                 Cand* cands, uint32_t* nsucc);
 __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask);
 // rank[0..n_cand] := exclusive scan of the candidates' winner flags (the
-// smallest candidate index of each new position), computed on the fly.
-geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, DevBuf<uint32_t>& tmp,
-                      hipStream_t s);
+// smallest candidate index of each new position): computed inside the scan
+// below SCAN_MARK_FLAGS candidates, else written to flag[0..n_cand) first.
+constexpr uint32_t SCAN_MARK_FLAGS = 1u << 20;
+geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, uint32_t* flag,
+                      DevBuf<uint32_t>& tmp, hipStream_t s);
 __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* rank, uint32_t v_base, NodeArgs N,
                            unsigned long long* table, uint32_t* keymin);
 __global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,

@@ -638,7 +638,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
             if (n_cand)
                 k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
             // the winners' ranks (their flags computed inside the scan)
-            GB_TRY(scan_marks(cands.p, n_cand, keymin.p, rank.p, ws.scan32, s));
+            GB_TRY(scan_marks(cands.p, n_cand, keymin.p, rank.p, flag.p, ws.scan32, s));
             if (n_cand) {
                 k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rank.p, B->V, nodes(), table.p, keymin.p);
                 k_edges<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p, B->wt.p);
