@@ -16,7 +16,7 @@
 //   4. k_insert   - a device hash table with full-equality checks finds known
 //                   positions or claims a pending entry; among duplicates of a
 //                   new position the smallest candidate index wins;
-//   5. scan_marks - winners get ranks, i.e. new ids in first-occurrence
+//   5. k_mark + scan - winners get ranks, i.e. new ids in first-occurrence
 //                   order, which is the reference's numbering;
 //   6. k_finalize / k_edges - node records, hash entries, CSR rows, columns,
 //                   weights.
@@ -1128,16 +1128,6 @@ __device__ __forceinline__ uint32_t mark_cand(const Cand* cands, uint32_t j, uin
     return (c.kind_p != INVALID && (c.res & PENDING) && keymin[c.res & ~PENDING] == j) ? 1u : 0u;
 }
 
-namespace {
-// candidate j's winner flag, for the scan that ranks the winners
-struct MarkIn {
-    const Cand* cands;
-    uint32_t n_cand;
-    const uint32_t* keymin;
-    __device__ __forceinline__ uint32_t operator()(size_t j) const { return mark_cand(cands, (uint32_t)j, n_cand, keymin); }
-};
-}  // namespace
-
 __global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n_cand) flag[j] = mark_cand(cands, j, n_cand, keymin);
@@ -1145,10 +1135,9 @@ __global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymi
 
 geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, uint32_t* flag,
                       DevBuf<uint32_t>& tmp, hipStream_t s) {
-    if (n_cand < SCAN_MARK_FLAGS)   // the flags computed inside the scan (one launch less)
-        return exclusive_scan_in<uint32_t, SumU32, MarkIn>(MarkIn{cands, n_cand, keymin}, rank, n_cand, tmp, s, SumU32{});
-    // many candidates: the scan's two passes would each read the 40-byte records
-    k_mark<<<(n_cand + 255) / 256, 256, 0, s>>>(cands, n_cand, keymin, flag);
+    // the flags first: one candidate per thread (computing them inside the
+    // scan made each thread chase 16 records and their table slots in turn)
+    if (n_cand) k_mark<<<(n_cand + 255) / 256, 256, 0, s>>>(cands, n_cand, keymin, flag);
     return exclusive_scan_dev<uint32_t, SumU32>(flag, rank, n_cand, tmp, s, SumU32{});
 }
 

@@ -112,9 +112,8 @@ __global__ void k_generate_wave(NodeArgs N, uint32_t lo, LtsArgs L, const uint2*
                                 Cand* cands, uint32_t* nsucc);
 __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long long* table, uint32_t* keymin, uint32_t mask);
 // rank[0..n_cand] := exclusive scan of the candidates' winner flags (the
-// smallest candidate index of each new position): computed inside the scan
-// below SCAN_MARK_FLAGS candidates, else written to flag[0..n_cand) first.
-constexpr uint32_t SCAN_MARK_FLAGS = 1u << 20;
+// smallest candidate index of each new position), written to flag[0..n_cand)
+// first.
 geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, uint32_t* flag,
                       DevBuf<uint32_t>& tmp, hipStream_t s);
 __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* rank, uint32_t v_base, NodeArgs N,
