@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -42,6 +43,42 @@ bool drop_spare_games(int device) {
     }
     for (geq_game* g : drop) delete g;
     return !drop.empty() || bufs;
+}
+
+hipError_t alloc_host_ctl(geq_game* g) {
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, 128, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
+    void* d = nullptr;
+    if ((e = hipHostGetDevicePointer(&d, p, 0)) != hipSuccess || !d) {
+        (void)hipHostFree(p);
+        (void)hipGetLastError();
+        return e != hipSuccess ? e : hipErrorInvalidValue;
+    }
+    std::memset(p, 0, 128);
+    g->h_ctl = static_cast<Ctl*>(p);
+    *g->h_ctl = Ctl{};
+    g->d_ctl_host = static_cast<Ctl*>(d);
+    g->sig_seq = 0;
+    return hipSuccess;
+}
+
+geq_status wait_signal(geq_game* g, uint32_t seq) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(g->h_ctl) + 64);
+    const auto t0 = std::chrono::steady_clock::now();
+    while (__atomic_load_n(w, __ATOMIC_ACQUIRE) != seq) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+            // a long batch, or a failed kernel: the stream's wait (and its error)
+            GEQ_HIP(hipStreamSynchronize(g->ctx->stream), GEQ_ERR_TRANSFER);
+            if (__atomic_load_n(w, __ATOMIC_ACQUIRE) != seq) return fail(GEQ_ERR_INTERNAL, "the device's signal did not arrive");
+            return GEQ_OK;
+        }
+        __builtin_ia32_pause();
+    }
+    return GEQ_OK;
 }
 
 hipError_t device_malloc(void** p, size_t bytes) {
@@ -537,9 +574,7 @@ geq_status upload_games(geq_ctx* ctx, uint32_t n_games, const geq_graph* graphs,
         gm->arena_cap = initial_arena_entries(E, 0);
         if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
     }
-    if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
-        return oom(e, "pinned control block");
-    *gm->h_ctl = Ctl{};
+    if ((e = alloc_host_ctl(gm)) != hipSuccess) return oom(e, "pinned control block");
     hipStream_t s = ctx->stream;
     if (ci.generic) {
         if ((E && hipMemcpyAsync(gm->gwt.p, gwt.data(), 4ull * E * ci.usz, hipMemcpyHostToDevice, s) != hipSuccess) ||
@@ -622,6 +657,23 @@ __global__ void k_stage_out(const uint32_t* offs, uint32_t n, const uint32_t* da
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i <= n; i += stride) h_offs[i] = offs[i];
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += stride) h_data[i] = data[i];
 }
+
+// The signal word after the control block: written last, at system scope.
+__device__ __forceinline__ uint32_t* signal_word(Ctl* host_ctl) {
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(host_ctl) + 64);
+}
+
+// The control block into the host's pinned copy, then `seq` into the signal
+// word (one wave: its system-scope fence orders its own stores).
+__global__ void k_ctl_out(const Ctl* ctl, Ctl* host_ctl, uint32_t seq) {
+    static_assert(sizeof(Ctl) % 4 == 0 && sizeof(Ctl) / 4 <= 64, "one word per thread");
+    const uint32_t t = threadIdx.x;
+    if (t < sizeof(Ctl) / 4) reinterpret_cast<uint32_t*>(host_ctl)[t] = reinterpret_cast<const uint32_t*>(ctl)[t];
+    __threadfence_system();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(signal_word(host_ctl), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 
 // slot 0 (tag 0, current while both slots are untouched) of every node to reach
 __global__ void k_seed_zero(uint4* st, const uint32_t* ids, uint32_t n) {
@@ -1267,8 +1319,12 @@ geq_status geq::solve_core(geq_game* g, uint32_t n_reach, const geq_energies* in
             k_reduce_stats<<<1, 256, 0, s>>>(g->stats.p, shape.eval_blocks, g->ctl.p->totals);
             stats_in_ctl = true;
         }
-        GEQ_HIP(hipMemcpyAsync(g->h_ctl, g->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s), GEQ_ERR_TRANSFER);
-        GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+        // the control block comes back through the mapped copy and its signal word
+        const uint32_t seq = ++g->sig_seq;
+        k_ctl_out<<<1, 64, 0, s>>>(g->ctl.p, g->d_ctl_host, seq);
+        GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
+        result = wait_signal(g, seq);
+        if (result != GEQ_OK) break;
         if (trace && prof.size() >= 4) {
             float te = 0, tx = 0;
             (void)hipEventElapsedTime(&te, prof[prof.size() - 4], prof[prof.size() - 3]);

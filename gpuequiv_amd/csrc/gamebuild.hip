@@ -1208,7 +1208,8 @@ __device__ __forceinline__ uint32_t chunk_word(uint32_t i, const uint32_t* n_edg
 
 __global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
                                const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out, uint32_t seq) {
-    for (uint32_t i = threadIdx.x; i < RB_JOBS + n_cnt; i += blockDim.x)
+    if (threadIdx.x >= 64) return;   // the first wave writes and publishes every word
+    for (uint32_t i = threadIdx.x; i < RB_JOBS + n_cnt; i += 64)
         if (i != RB_SEQ) out[i] = chunk_word(i, n_edges, n_new, add, chunk, next, heavy, cnt);
     rb_publish(out, seq);
 }
@@ -1267,7 +1268,8 @@ __global__ __launch_bounds__(TAIL_BLOCK) void k_level_tail(TailArgs T) {
     }
     __syncthreads();
     const uint32_t n_cnt = T.job_cnt ? T.n_jobs : 0u;
-    for (uint32_t i = t; i < RB_JOBS + n_cnt; i += TAIL_BLOCK)
+    if (t >= 64) return;   // the first wave writes and publishes every word
+    for (uint32_t i = t; i < RB_JOBS + n_cnt; i += 64)
         if (i != RB_SEQ)
             T.rb[i] = chunk_word(i, T.rowl + T.nf, T.rank + T.n_cand, T.add, T.chunk, T.last ? T.off : nullptr, T.H.count, T.job_cnt);
     rb_publish(T.rb, T.seq);

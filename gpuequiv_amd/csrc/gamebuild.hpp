@@ -168,16 +168,14 @@ struct PairSum {
 __global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
                                const uint32_t* heavy, const uint32_t* cnt, uint32_t n_cnt, uint32_t* out, uint32_t seq);
 
-// The last step of a kernel that fills count words for the host: each
-// calling thread has written its words; after a system-scope fence by every
-// thread and a barrier, thread 0 writes `seq` to out[RB_SEQ].  The host, which
+// The last step of a kernel that fills count words for the host, called by
+// the block's first wave, which has written all of them: its system-scope
+// fence orders those stores before `seq` goes to out[RB_SEQ].  The host, which
 // waits for that number, then reads every word (the memory is coherent).
+// One wave fences: each fence writes back the L2.
 __device__ __forceinline__ void rb_publish(uint32_t* out, uint32_t seq) {
     __threadfence_system();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(out + RB_SEQ, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    if (threadIdx.x == 0) __hip_atomic_store(out + RB_SEQ, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // A chunk whose frontier, candidates and next-level bounds each fit TAIL_MAX

@@ -89,11 +89,10 @@ geq_status exclusive_scan_pairs(const uint2* in, uint2* out, size_t n, DevBuf<ui
 // positions (*h) to pinned host memory: they reach the host with one launch
 // and one wait.
 __global__ void k_readback(const uint32_t* a, const uint32_t* b, const uint32_t* h, uint32_t* out, uint32_t seq) {
-    if (threadIdx.x == 0) {
-        out[RB_CAND] = *a;
-        out[RB_WORDS] = *b;
-        out[RB_HEAVY] = *h;
-    }
+    if (threadIdx.x != 0) return;
+    out[RB_CAND] = *a;
+    out[RB_WORDS] = *b;
+    out[RB_HEAVY] = *h;
     rb_publish(out, seq);
 }
 
@@ -127,9 +126,7 @@ geq_status geq::alloc_game_buffers(geq_ctx* ctx, uint32_t V, uint32_t E, geq_gam
     if ((e = gm->stats.alloc(4ull * STAT_SLOTS + 4)) != hipSuccess) return oom(e, "stats");
     gm->arena_cap = initial_arena_entries(E, arena_entries);
     if ((e = gm->arena.alloc(gm->arena_cap)) != hipSuccess) return oom(e, "energy arena");
-    if ((e = hipHostMalloc(reinterpret_cast<void**>(&gm->h_ctl), sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
-        return oom(e, "pinned control block");
-    *gm->h_ctl = Ctl{};
+    if ((e = alloc_host_ctl(gm)) != hipSuccess) return oom(e, "pinned control block");
     if (const char* env = std::getenv("GEQ_SOLVE_MODE")) gm->tune.mode = (uint32_t)std::atoi(env);
     gm->tune.ordered = worklist_order_default();
     *out = gm;

@@ -146,6 +146,11 @@ geq_status standard_seed(geq_game* g);
 // A built game's buffers go to its context's cache (one game) for the next
 // built game, or are freed when the cache is taken.
 void release_built_game(geq_game* g);
+// The game's pinned control block with its device view (alloc_game_buffers).
+hipError_t alloc_host_ctl(geq_game* g);
+// Waits until the game's signal word holds `seq` (written last by k_ctl_out
+// on the game's stream): a spin of up to 2 ms, then a stream synchronisation.
+geq_status wait_signal(geq_game* g, uint32_t seq);
 namespace gb {
 struct BuildScratch;
 }
@@ -259,7 +264,12 @@ struct geq_game {
     uint32_t graph_scr_words = 0;
     geq::DevBuf<geq::Ctl> ctl;
     geq::DevBuf<unsigned long long> stats;
-    geq::Ctl* h_ctl = nullptr;        // pinned
+    // pinned, mapped and coherent: the control block (read back after each
+    // batch of rounds), then at byte 64 the number the last readback kernel
+    // wrote (the host spins on it); d_ctl_host is the device view
+    geq::Ctl* h_ctl = nullptr;
+    geq::Ctl* d_ctl_host = nullptr;
+    uint32_t sig_seq = 0;             // the last number asked for
     geq::DevBuf<uint32_t> scan_tmp;   // block totals of the download's offset scan
     uint32_t* h_stage = nullptr;      // mapped pinned staging of the energy download (host view)
     uint32_t* d_stage = nullptr;      // its device view
