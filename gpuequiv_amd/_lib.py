@@ -190,6 +190,7 @@ _SIGS = {
     "geq_built_solve_capped": (C.c_int, [C.c_void_p, C.c_int, P(geq_stats)]),
     "geq_built_frontier": (C.c_int, [C.c_void_p, P(C.c_uint32), P(C.c_uint32)]),
     "geq_built_energies": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(geq_energies)]),
+    "geq_built_preorders": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_uint32), C.c_uint32, P(C.c_uint8)]),
     "geq_built_free": (None, [C.c_void_p]),
     "geq_signature_refinement": (C.c_int, [P(geq_lts), P(C.c_uint32), P(C.c_uint32)]),
     "geq_server_create": (C.c_int, [C.c_void_p, P(geq_server_opts), P(C.c_void_p)]),

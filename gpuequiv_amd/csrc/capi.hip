@@ -1454,6 +1454,56 @@ geq_status download_lists(geq_game* g, uint32_t first, uint32_t V, uint32_t t, g
 }  // namespace
 
 // Energies of the nodes [first, first + V) as packed words, ascending per node.
+// The game's mapped pinned staging, with room for `words` words.
+geq_status geq::ensure_stage(geq_game* g, size_t words) {
+    if (g->h_stage && g->h_stage_n >= words) return GEQ_OK;
+    if (g->h_stage) (void)hipHostFree(g->h_stage);
+    g->h_stage = g->d_stage = nullptr;
+    g->h_stage_n = 0;
+    const size_t cap = std::max<size_t>(words, 1024);
+    GEQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->h_stage), 4 * cap, hipHostMallocMapped | hipHostMallocCoherent),
+            GEQ_ERR_OUT_OF_MEMORY);
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&g->d_stage), g->h_stage, 0) != hipSuccess || !g->d_stage) {
+        (void)hipHostFree(g->h_stage);
+        g->h_stage = g->d_stage = nullptr;
+        return fail(GEQ_ERR_INTERNAL, "no device view of the download staging");
+    }
+    g->h_stage_n = cap;
+    return GEQ_OK;
+}
+
+// Preorder verdicts of nodes [first, first + count) against energies[0 .. n_eqs)
+// (EnergyArray::test_equivalence, src/energy.rs:307-318, on each node's list),
+// computed on the device: only the verdict bytes come back.
+geq_status geq::game_preorders(geq_game* g, uint32_t first, uint32_t count, const uint32_t* energies, uint32_t n_eqs,
+                               uint8_t* out) {
+    if (!g || (count && n_eqs && !out) || (n_eqs && !energies)) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    if ((uint64_t)first + count > g->V) return fail(GEQ_ERR_INVALID_ARG, "node range out of bounds");
+    if (n_eqs > EQ_WORDS_MAX) return fail(GEQ_ERR_INVALID_ARG, "at most 64 energies per call");
+    if (g->conf.generic)
+        return fail(GEQ_ERR_UNSUPPORTED, "preorder verdicts on the device take confs of at most 6 elements and max 3");
+    EqWords ew{};
+    ew.n = n_eqs;
+    for (uint32_t k = 0; k < n_eqs; k++)
+        if (!energy_valid(g->conf, energies + k) || !energy_to_canon(g->conf, energies[k], ew.w[k]))
+            return fail(GEQ_ERR_INVALID_ARG, "energy " + std::to_string(k) + " is not packed for the game's EnergyConf");
+    if (!count || !n_eqs) return GEQ_OK;
+    (void)hipSetDevice(g->ctx->device);
+    hipStream_t s = g->ctx->stream;
+    const uint32_t t = round_tag(g->h_ctl->round);   // the lists after the last round that ran
+    const uint4* stb = g->inv_map.p ? g->st.p : g->st.p + (size_t)NODE_U4 * first;
+    const uint32_t* map = g->inv_map.p ? g->inv_map.p + first : nullptr;
+    const size_t bytes = (size_t)count * n_eqs;
+    geq_status est = ensure_stage(g, (bytes + 3) / 4);
+    if (est != GEQ_OK) return est;
+    k_preorders<<<(count + 255) / 256, 256, 0, s>>>(stb, g->arena.p, count, t, map, ew, g->conf.elements,
+                                                   reinterpret_cast<uint8_t*>(g->d_stage));
+    GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
+    GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
+    std::memcpy(out, g->h_stage, bytes);
+    return GEQ_OK;
+}
+
 geq_status geq::download_energies(geq_game* g, uint32_t first, uint32_t V, geq_energies* out) {
     if (!g || !out || (uint64_t)first + V > g->V) return fail(GEQ_ERR_INVALID_ARG, "node range out of bounds");
     (void)hipSetDevice(g->ctx->device);
@@ -1483,20 +1533,8 @@ geq_status geq::download_mapped(geq_game* g, const uint4* stb, const uint32_t* m
     // offsets and lists written by one copy kernel into mapped pinned memory
     // (room for the bound), then one wait
     const size_t words = V + 1ull + bound;
-    if (!g->h_stage || g->h_stage_n < words) {
-        if (g->h_stage) (void)hipHostFree(g->h_stage);
-        g->h_stage = g->d_stage = nullptr;
-        g->h_stage_n = 0;
-        const size_t cap = std::max<size_t>(words, 1024);
-        GEQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->h_stage), 4 * cap, hipHostMallocMapped | hipHostMallocCoherent),
-                GEQ_ERR_OUT_OF_MEMORY);
-        if (hipHostGetDevicePointer(reinterpret_cast<void**>(&g->d_stage), g->h_stage, 0) != hipSuccess || !g->d_stage) {
-            (void)hipHostFree(g->h_stage);
-            g->h_stage = g->d_stage = nullptr;
-            return fail(GEQ_ERR_INTERNAL, "no device view of the download staging");
-        }
-        g->h_stage_n = cap;
-    }
+    geq_status est = ensure_stage(g, words);
+    if (est != GEQ_OK) return est;
     k_stage_out<<<std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((words + 255) / 256), 2048)), 256, 0, s>>>(
         g->offs.p, V, g->out.p, bound, g->d_stage, g->d_stage + V + 1);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);

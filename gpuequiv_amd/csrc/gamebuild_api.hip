@@ -1004,6 +1004,13 @@ geq_status geq_built_energies(geq_built* B, uint32_t first, uint32_t count, geq_
     return download_energies(B->game, first, count, out);
 }
 
+geq_status geq_built_preorders(geq_built* B, uint32_t first, uint32_t count, const uint32_t* energies, uint32_t n_energies,
+                               uint8_t* out) {
+    if (!B) return fail(GEQ_ERR_INVALID_ARG, "NULL argument");
+    if (!B->game) return fail(GEQ_ERR_INVALID_ARG, "the built game was not solved");
+    return game_preorders(B->game, first, count, energies, n_energies, out);
+}
+
 void geq_built_free(geq_built* B) {
     if (!B) return;
     geq_ctx* ctx = B->ctx;

@@ -157,6 +157,13 @@ struct BuildScratch;
 // Frees the spare buffers of a build scratch (true if there were any).
 bool drop_build_spares(gb::BuildScratch* ws);
 geq_status download_energies(geq_game* g, uint32_t first, uint32_t count, geq_energies* out);
+// The game's mapped pinned staging with room for `words` words.
+geq_status ensure_stage(geq_game* g, size_t words);
+// Preorder verdicts of nodes [first, first + count) against n_eqs packed
+// energies (bitmap-kernel confs): out[i * n_eqs + k] = 1 when no energy of
+// node first + i lies below energy k, else 0.
+geq_status game_preorders(geq_game* g, uint32_t first, uint32_t count, const uint32_t* energies, uint32_t n_eqs,
+                          uint8_t* out);
 // The same for any nodes of a game without renumbering and in the bitmap
 // layout: entry v of the result is node map[v] (device ids); t is the tag of
 // the round after the last one that ran.

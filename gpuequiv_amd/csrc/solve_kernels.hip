@@ -768,6 +768,29 @@ __global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* l
     if (v < n) lens[v] = nl_len(load_cur(st, map ? map[v] : v, t));
 }
 
+// Preorder verdicts (src/equivalence.rs:125-132, EnergyGame's budgets of the
+// starting points against each equivalence's energy): out[v * n_eqs + k] = 1
+// when no energy of node v's list lies below eq.w[k] in every field (2-bit
+// fields, `fields` of them), else 0.  Entry v is node map[v] (v without a map).
+__global__ void k_preorders(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* map, EqWords eq,
+                            uint32_t fields, uint8_t* out) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint4 d = load_cur(st, map ? map[v] : v, t);
+    const uint32_t len = nl_len(d);
+    for (uint32_t k = 0; k < eq.n; k++) {
+        const uint32_t e = eq.w[k];
+        uint32_t holds = 1;
+        for (uint32_t j = 0; j < len && holds; j++) {
+            const uint32_t b = nl_get(d, arena, j);
+            bool below = true;
+            for (uint32_t f = 0; f < fields; f++) below = below && ((b >> (2 * f)) & 3u) <= ((e >> (2 * f)) & 3u);
+            if (below) holds = 0;
+        }
+        out[(size_t)v * eq.n + k] = (uint8_t)holds;
+    }
+}
+
 // Copy every node's list into the packed output, ascending.  Lists produced
 // by the single-successor defender copy keep the successor's order and are
 // sorted here (insertion sort; such lists are short antichain images).

@@ -207,6 +207,14 @@ __global__ void k_node_hdr(const uint32_t* row, const uint32_t* rrow, const uint
 __global__ void k_reduce_stats(const unsigned long long* slots, uint32_t nslots, unsigned long long* out);
 // Output download: entry v of the output is node map[v] (node v when map is NULL).
 __global__ void k_list_lens(const uint4* st, uint32_t n, uint32_t t, uint32_t* lens, const uint32_t* map);
+// The equivalences' energies of a preorder check, packed (2-bit fields).
+constexpr uint32_t EQ_WORDS_MAX = 64;
+struct EqWords {
+    uint32_t w[EQ_WORDS_MAX];
+    uint32_t n;
+};
+__global__ void k_preorders(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* map, EqWords eq,
+                            uint32_t fields, uint8_t* out);
 // out holds cap words; a list that would pass it is skipped (the caller has
 // checked offs[n] <= cap)
 __global__ void k_gather(const uint4* st, const uint16_t* arena, uint32_t n, uint32_t t, const uint32_t* offs, uint32_t* out,

@@ -61,12 +61,10 @@ class Equivalence:
         vectorised over the packed budgets of all seeded pairs (a pair's
         preorder holds unless one of its budgets lies below e, :125-132)."""
         import numpy as np
-        n, k = self.start_info.n_processes, len(equivalences)
+        k = len(equivalences)
         conf = equivalences[0].conf if k else None
         eqs = np.array([e.to_vec() for e in equivalences], dtype=np.int64).reshape(k, -1)
         sp = self.start_info.starting_points
-        m = (max(self.minimization) + 1) if self.minimization is not None else n
-        red = np.zeros((max(m, 1), max(m, 1), k), dtype=bool)
         below_any = np.zeros((len(sp), k), dtype=bool)
         if sp and k:
             for e in self.energies:
@@ -88,17 +86,7 @@ class Equivalence:
                 starts = np.concatenate(([0], np.cumsum(lens)[:-1]))
                 has = lens > 0
                 below_any[has] = np.maximum.reduceat(below, starts[has], axis=0).astype(bool)
-        if sp:
-            P = np.fromiter((pt.p for pt in sp), dtype=np.int64, count=len(sp))
-            Q = np.fromiter((pt.q[0] for pt in sp), dtype=np.int64, count=len(sp))
-            # a repeated pair takes its last entry, as the per-pair loop would
-            _, first_rev = np.unique((P * max(m, 1) + Q)[::-1], return_index=True)
-            last = len(sp) - 1 - first_rev
-            red[P[last], Q[last]] = ~below_any[last]
-        idx = np.arange(m)
-        red[idx, idx] = True
-        part = np.asarray(self.minimization if self.minimization is not None else range(n), dtype=np.int64)
-        return red.take(part, axis=0).take(part, axis=1)
+        return preorder_matrix_of(self.start_info, self.minimization, ~below_any)
 
     def relation(self, equivalence: Energy) -> "EquivalenceRelation":
         """Classes of processes equivalent under `equivalence` (:154-170)."""
@@ -162,3 +150,33 @@ class EquivalenceRelation:
             else:
                 rep[part] = proc
         return out
+
+
+def preorder_matrix_of(start_info: StartInfo, minimization, holds):
+    """preorder(p, q, e) for every ordered pair of processes and every
+    equivalence e, as Equivalence.preorder_matrix returns it, from the
+    starting points' verdicts: holds[i, k] is True when no budget of starting
+    point i lies below equivalence k (src/equivalence.rs:125-132;
+    GameBuild.preorder_bits computes them on the device).  A pair that is not
+    a starting point is not in the preorder unless p and q map to the same
+    process."""
+    import numpy as np
+    holds = np.asarray(holds, dtype=bool)
+    sp = start_info.starting_points
+    n = start_info.n_processes
+    k = holds.shape[1] if holds.ndim == 2 else 0
+    if holds.shape[0] != len(sp):
+        raise ValueError("one row of verdicts per starting point")
+    m = (max(minimization) + 1) if minimization is not None else n
+    red = np.zeros((max(m, 1), max(m, 1), k), dtype=bool)
+    if sp:
+        P = np.fromiter((pt.p for pt in sp), dtype=np.int64, count=len(sp))
+        Q = np.fromiter((pt.q[0] for pt in sp), dtype=np.int64, count=len(sp))
+        # a repeated pair takes its last entry, as the per-pair loop would
+        _, first_rev = np.unique((P * max(m, 1) + Q)[::-1], return_index=True)
+        last = len(sp) - 1 - first_rev
+        red[P[last], Q[last]] = holds[last]
+    idx = np.arange(m)
+    red[idx, idx] = True
+    part = np.asarray(minimization if minimization is not None else range(n), dtype=np.int64)
+    return red.take(part, axis=0).take(part, axis=1)

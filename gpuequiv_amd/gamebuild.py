@@ -141,6 +141,24 @@ class GameBuild:
         o, d = self.energies_raw(first, count)
         return energies_from_flat(o, d, STD)
 
+    def preorder_bits(self, equivalences, first: int = 0, count: int = None):
+        """After solve(): bool array [count, len(equivalences)], True where no
+        budget of position first + i lies below equivalence k
+        (EnergyArray::test_equivalence, src/energy.rs:307-318), computed on the
+        device; with first 0 and count n_starting_points these are the
+        starting points' verdicts (equivalence.preorder_matrix_of)."""
+        import numpy as np
+        if count is None:
+            count = self.n_starting_points - first
+        eqs = np.array([e.raw_data()[0] for e in equivalences], dtype=np.uint32)
+        for e in equivalences:
+            if len(e.raw_data()) != 1:
+                raise ValueError("one word per energy")
+        out = np.zeros(max(int(count) * len(eqs), 1), dtype=np.uint8)
+        _lib.check(_lib.lib().geq_built_preorders(self.handle, int(first), int(count), _lib.u32p(eqs), len(eqs),
+                                                  out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out[:int(count) * len(eqs)].reshape(int(count), len(eqs)).astype(bool)
+
     # -------------------------------------------------------------- host copies
     def _download(self):
         if self._host is None:

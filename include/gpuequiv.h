@@ -286,6 +286,16 @@ geq_status geq_built_solve_capped(geq_built* built, int profile_kernels, geq_sta
 geq_status geq_built_frontier(geq_built* built, uint32_t* truncated, uint32_t* frontier);
 /* energies of positions [first, first + count) after geq_built_solve */
 geq_status geq_built_energies(geq_built* built, uint32_t first, uint32_t count, geq_energies* out);
+/* Replaces Equivalence::preorder's test (src/equivalence.rs:125-132,
+ * EnergyArray::test_equivalence src/energy.rs:307-318) for positions
+ * [first, first + count) after geq_built_solve, on the device: out[i * n_energies
+ * + k] = 1 when no budget of position first + i lies below energies[k] in every
+ * component (the preorder holds for that starting point), else 0.  energies:
+ * n_energies (at most 64) energies packed in the game's EnergyConf.  Only the
+ * verdict bytes cross to the host.  GEQ_ERR_UNSUPPORTED for confs of more than
+ * 6 elements or max above 3 (download the energies instead). */
+geq_status geq_built_preorders(geq_built* built, uint32_t first, uint32_t count, const uint32_t* energies,
+                               uint32_t n_energies, uint8_t* out);
 void geq_built_free(geq_built* built);
 
 /* ---- serving: continuous batching of spectroscopy jobs ----

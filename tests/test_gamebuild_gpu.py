@@ -373,6 +373,37 @@ def test_full_spectroscopy_verdict_matrix():
     assert want.sum() > n * 13          # verdicts beyond the diagonal
 
 
+def test_device_verdicts_equal_the_oracle_matrix():
+    """The starting points' 13 verdicts computed on the device
+    (geq_built_preorders), assembled into the 300 x 300 x 13 matrix, equal the
+    oracle's; per point they equal test_equivalence on the downloaded budgets,
+    also over a sub-range; energies not packed for the conf are refused."""
+    from gpuequiv_amd._lib import Error
+    from gpuequiv_amd.equivalence import preorder_matrix_of
+    print("device verdicts, spectroscopy 300/1200/8 seed 1", flush=True)
+    n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
+    lts = TransitionSystem.from_arrays(n, src, dst, lbl)
+    want = H.oracle_verdicts(n, lts.row_offsets, lts.targets, lts.labels)
+    red, part = lts.bisimilar_minimize()
+    b, info = GameBuild.compare_all(red)
+    b.solve()
+    eqs = [getattr(std_equivalences, nm)() for nm in H.EQ_NAMES]
+    bits = b.preorder_bits(eqs)
+    assert bits.shape == (b.n_starting_points, len(eqs))
+    assert np.array_equal(preorder_matrix_of(info, part, bits), want)
+    ens = b.energies(0, b.n_starting_points)
+    for i, a in enumerate(ens):
+        assert [a.test_equivalence(e) for e in eqs] == bits[i].tolist(), i
+    assert np.array_equal(b.preorder_bits(eqs, first=3, count=5), bits[3:8])
+    assert 0 < bits.sum() < bits.size      # both verdicts occur
+
+    class Bad:
+        def raw_data(self):
+            return (0xFFFFFFFF,)
+    with pytest.raises(Error):
+        b.preorder_bits([Bad()])
+
+
 def test_level_past_the_scan_limit_is_refused(monkeypatch):
     # A level whose candidate or set-pool-word bounds add up past 32 bits is
     # refused with OUT_OF_MEMORY before any candidate is written (the bound

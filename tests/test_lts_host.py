@@ -112,3 +112,29 @@ def test_preorder_matrix_random_budgets_and_repeated_pairs():
         for q in range(50):
             for k, e in enumerate(eqs):
                 assert mat[p, q, k] == eq.preorder(p, q, e), (p, q, k)
+
+
+def test_preorder_matrix_from_verdicts():
+    # preorder_matrix_of (the assembly the device verdicts take) given each
+    # starting point's test_equivalence results equals preorder_matrix, with
+    # empty budgets, a merging minimisation and a pair seeded twice
+    import numpy as np
+    from gpuequiv_amd import EnergyArray, EnergyConf, std_equivalences
+    from gpuequiv_amd.equivalence import Equivalence, StartInfo, preorder_matrix_of
+    from gpuequiv_amd.gamebuild import AttackPosition
+    STD = EnergyConf.STANDARD
+    rng = np.random.default_rng(12)
+    m = 30
+    pairs = [(int(p), int(q)) for p, q in rng.integers(0, m, (200, 2)) if p != q]
+    pairs.append(pairs[3])
+    budgets = [rng.integers(0, 4, (int(rng.integers(0, 5)), 6)).tolist() for _ in pairs]
+    info = StartInfo([AttackPosition(p, (q,)) for p, q in pairs], 36)
+    arrays = [EnergyArray.from_conf(b, STD) for b in budgets]
+    eq = Equivalence(info, arrays)
+    part = list(range(m)) + [int(x) for x in rng.integers(0, m, 36 - m)]
+    eq.set_minimization(part)
+    eqs = [getattr(std_equivalences, nm)() for nm in std_equivalences.NAMES]
+    holds = np.array([[a.test_equivalence(e) for e in eqs] for a in arrays], dtype=bool)
+    assert np.array_equal(preorder_matrix_of(info, part, holds), eq.preorder_matrix(eqs))
+    with pytest.raises(ValueError):
+        preorder_matrix_of(info, part, holds[:-1])
