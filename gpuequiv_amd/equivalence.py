@@ -7,11 +7,34 @@ from .energy import Energy, EnergyArray
 
 
 class StartInfo:
-    """Which pairs the game was seeded with (src/equivalence.rs:17-38)."""
+    """Which pairs the game was seeded with (src/equivalence.rs:17-38).  Given
+    the pairs as an array ([k, 2] of (p, q)), the AttackPosition list is made
+    when first asked for."""
 
-    def __init__(self, starting_points: Sequence, n_processes: int):
-        self.starting_points = list(starting_points)
+    def __init__(self, starting_points, n_processes: int, pairs=None):
+        import numpy as np
+        self._sp = None if starting_points is None else list(starting_points)
+        self._pairs = None if pairs is None else np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+        if self._sp is None and self._pairs is None:
+            raise ValueError("starting points or pairs")
         self.n_processes = int(n_processes)
+
+    @property
+    def starting_points(self) -> list:
+        if self._sp is None:
+            from .gamebuild import AttackPosition
+            self._sp = [AttackPosition(p, (q,)) for p, q in self._pairs.tolist()]
+        return self._sp
+
+    def pairs(self):
+        """The starting points' (p, q) as an int64 array [k, 2]."""
+        import numpy as np
+        if self._pairs is None:
+            self._pairs = np.array([(pt.p, pt.q[0]) for pt in self._sp], dtype=np.int64).reshape(-1, 2)
+        return self._pairs
+
+    def __len__(self) -> int:
+        return len(self._sp) if self._sp is not None else len(self._pairs)
 
     def equivalence(self, energies: Sequence[EnergyArray]) -> "Equivalence":
         return Equivalence(self, energies)
@@ -162,21 +185,28 @@ def preorder_matrix_of(start_info: StartInfo, minimization, holds):
     process."""
     import numpy as np
     holds = np.asarray(holds, dtype=bool)
-    sp = start_info.starting_points
+    pr = start_info.pairs()
+    n_sp = pr.shape[0]
     n = start_info.n_processes
     k = holds.shape[1] if holds.ndim == 2 else 0
-    if holds.shape[0] != len(sp):
+    if holds.shape[0] != n_sp:
         raise ValueError("one row of verdicts per starting point")
-    m = (max(minimization) + 1) if minimization is not None else n
-    red = np.zeros((max(m, 1), max(m, 1), k), dtype=bool)
-    if sp:
-        P = np.fromiter((pt.p for pt in sp), dtype=np.int64, count=len(sp))
-        Q = np.fromiter((pt.q[0] for pt in sp), dtype=np.int64, count=len(sp))
+    part = np.asarray(minimization if minimization is not None else np.arange(n), dtype=np.int64)
+    m = int(part.max()) + 1 if minimization is not None and part.size else n
+    if k == 0:
+        return np.zeros((part.size, part.size, 0), dtype=bool)
+    # the verdicts of a pair as k bits in nb bytes: the two gathers over the
+    # minimisation move nb bytes per pair instead of k
+    nb = (k + 7) // 8
+    red = np.zeros((max(m, 1), max(m, 1), nb), dtype=np.uint8)
+    if n_sp:
+        P, Q = pr[:, 0], pr[:, 1]
         # a repeated pair takes its last entry, as the per-pair loop would
         _, first_rev = np.unique((P * max(m, 1) + Q)[::-1], return_index=True)
-        last = len(sp) - 1 - first_rev
-        red[P[last], Q[last]] = holds[last]
+        last = n_sp - 1 - first_rev
+        red[P[last], Q[last]] = np.packbits(holds[last], axis=1, bitorder="little")
     idx = np.arange(m)
-    red[idx, idx] = True
-    part = np.asarray(minimization if minimization is not None else range(n), dtype=np.int64)
-    return red.take(part, axis=0).take(part, axis=1)
+    red[idx, idx] = np.packbits(np.ones(k, dtype=bool), bitorder="little")
+    full = red.take(part, axis=0).take(part, axis=1)
+    bits = np.unpackbits(full.reshape(-1), bitorder="little").view(bool)
+    return bits.reshape(part.size, part.size, nb * 8)[:, :, :k]

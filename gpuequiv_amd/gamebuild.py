@@ -66,9 +66,12 @@ class GameBuild:
 
     def __init__(self, lts, pairs: Sequence, max_positions: int = 0):
         self.ctx = _lib.Context.get()
-        self.n_starting_points = len(pairs)
-        self._pairs = [(int(p), int(q)) for p, q in pairs]
-        arr = np.ascontiguousarray(np.asarray(self._pairs, dtype=np.uint32).reshape(-1), dtype=np.uint32)
+        pr = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+        if pr.size and (pr.min() < 0 or pr.max() > 0xFFFFFFFF):
+            raise ValueError("process ids out of range")
+        self.n_starting_points = pr.shape[0]
+        self._pairs = pr                      # [k, 2] of (p, q)
+        arr = np.ascontiguousarray(pr.astype(np.uint32).reshape(-1))
         if arr.size == 0:
             arr = np.zeros(2, np.uint32)
         s = lts._c_struct()
@@ -86,22 +89,42 @@ class GameBuild:
     def compare(cls, lts, p: int, q: int) -> "GameBuild":
         return cls(lts, [(p, q)])
 
+    @staticmethod
+    def _class_pairs(members, sizes):
+        """Every ordered pair (p, q), p != q, within each group (members group
+        after group, `sizes` of them), in the order of `for p in group for q in
+        group` (src/gamebuild.rs:182-194)."""
+        members = np.asarray(members, dtype=np.int64)
+        sizes = np.asarray(sizes, dtype=np.int64)
+        sq = sizes * sizes
+        total = int(sq.sum())
+        if total == 0:
+            return np.zeros((0, 2), np.int64)
+        start = np.concatenate(([0], np.cumsum(sizes)[:-1]))          # first member of each group
+        pstart = np.concatenate(([0], np.cumsum(sq)[:-1]))           # first pair of each group
+        sz = np.repeat(sizes, sq)
+        t = np.arange(total, dtype=np.int64) - np.repeat(pstart, sq)  # pair index within its group
+        a, b = t // sz, t % sz
+        base = np.repeat(start, sq)
+        keep = a != b
+        return np.stack([members[base + a][keep], members[base + b][keep]], axis=1)
+
     @classmethod
     def compare_multiple(cls, lts, processes: Sequence[int]):
         from .equivalence import StartInfo
-        pairs = [(p, q) for part in lts.enabledness_partitions(processes) for p in part for q in part if p != q]
+        pairs = cls._class_pairs(*lts._enabledness_groups(processes))
         b = cls(lts, pairs)
-        return b, StartInfo(b.starting_points(), lts.n_vertices())
+        return b, StartInfo(None, lts.n_vertices(), pairs=pairs)
 
     @classmethod
     def compare_all(cls, lts):
         from .equivalence import StartInfo
-        pairs = [(p, q) for part in lts.enabledness_partitions() for p in part for q in part if p != q]
+        pairs = cls._class_pairs(*lts._enabledness_groups())
         b = cls(lts, pairs)
-        return b, StartInfo(b.starting_points(), lts.n_vertices())
+        return b, StartInfo(None, lts.n_vertices(), pairs=pairs)
 
     def starting_points(self) -> list:
-        return [AttackPosition(p, (q,)) for p, q in self._pairs]
+        return [AttackPosition(p, (q,)) for p, q in self._pairs.tolist()]
 
     def n_positions(self) -> int:
         return int(self.build_stats["positions"])

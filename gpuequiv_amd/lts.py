@@ -165,13 +165,44 @@ class TransitionSystem:
     def enabledness_partitions(self, processes: Sequence[int] | None = None) -> list:
         """Processes grouped by their set of enabled labels (src/gamebuild.rs:104-133).
         Groups are ordered by their first member (the reference iterates a hash map)."""
-        procs = range(self.n_vertices()) if processes is None else processes
+        members, sizes = self._enabledness_groups(processes)
+        out, k = [], 0
+        ml = members.tolist()
+        for sz in sizes.tolist():
+            out.append(ml[k:k + sz])
+            k += sz
+        return out
+
+    def _enabledness_groups(self, processes: Sequence[int] | None = None):
+        """enabledness_partitions as arrays: the members, group after group, and
+        the group sizes."""
+        import numpy as np
+        n = self.n_vertices()
+        procs = np.arange(n, dtype=np.int64) if processes is None else np.asarray(list(processes), dtype=np.int64)
+        lab = np.asarray(self.labels)
+        if procs.size and (lab.size == 0 or (lab.min() >= 0 and lab.max() < 64)):
+            # the label set of a state as a bit mask; groups in order of their
+            # first member, members in process order (as the dict below)
+            row = np.asarray(self.row_offsets, dtype=np.int64)
+            mask = np.zeros(n, dtype=np.uint64)
+            if lab.size:
+                deg = np.diff(row)
+                full = deg > 0   # rows with transitions: their segments of `bits` in order
+                bits = np.left_shift(np.uint64(1), lab.astype(np.uint64))
+                mask[full] = np.bitwise_or.reduceat(bits, row[:-1][full])
+            _, first, inv = np.unique(mask[procs], return_index=True, return_inverse=True)
+            rank = np.empty(len(first), dtype=np.int64)
+            rank[np.argsort(first, kind="stable")] = np.arange(len(first))
+            g = rank[inv.reshape(-1)]
+            return procs[np.argsort(g, kind="stable")], np.bincount(g, minlength=len(first))
         groups: dict = {}
-        for p in procs:
+        for p in procs.tolist():
             a, b = int(self.row_offsets[p]), int(self.row_offsets[p + 1])
             key = tuple(sorted(set(self.labels[a:b].tolist())))
             groups.setdefault(key, []).append(int(p))
-        return list(groups.values())
+        parts = list(groups.values())
+        members = np.array([p for part in parts for p in part], dtype=np.int64)
+        return members, np.array([len(part) for part in parts], dtype=np.int64)
 
     def compare(self, p: int, q: int) -> EnergyArray:
         """Budgets of the attack position (p, {q}) after bisimulation
