@@ -4,9 +4,11 @@
 //   k_scan_reduce : every block of SCAN_ITEMS consecutive items -> its total
 //   k_scan_tops   : one block scans the block totals in place (and the grand total)
 //   k_scan_apply  : every block scans its items again from its block's offset
-// An input of at most SCAN_ITEMS items takes k_scan_apply alone (one launch);
-// one of at most SCAN_RAW_MAX blocks skips k_scan_tops, each k_scan_apply
-// block combining the totals of the blocks before it itself (two launches).
+// An input of at most SCAN_ONE_ITEMS items takes k_scan_apply alone, in one
+// block (of SCAN_THREADS threads, or SCAN_ONE_THREADS past SCAN_ITEMS items):
+// one launch; one of at most SCAN_RAW_MAX blocks skips k_scan_tops, each
+// k_scan_apply block combining the totals of the blocks before it itself (two
+// launches).
 // Global memory is read and written striped (lane-consecutive, coalesced); the
 // apply step transposes its tile through LDS so that each thread scans
 // SCAN_PER_THREAD consecutive items.  The combine operator is a template
@@ -29,6 +31,10 @@ constexpr uint32_t SCAN_PER_THREAD = 16;
 constexpr uint32_t SCAN_ITEMS = SCAN_THREADS * SCAN_PER_THREAD;   // items per block
 constexpr uint32_t SCAN_TOP_THREADS = 1024;
 constexpr uint32_t SCAN_RAW_MAX = 1024;   // block totals an apply block combines itself
+// the one-block scan: 1,024 threads, 16,384 items (an LDS tile of up to 132 KB
+// for 8-byte items; a gfx950 workgroup may hold 160 KB)
+constexpr uint32_t SCAN_ONE_THREADS = 1024;
+constexpr uint32_t SCAN_ONE_ITEMS = SCAN_ONE_THREADS * SCAN_PER_THREAD;
 
 struct SumU32 {
     __host__ __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
@@ -121,14 +127,15 @@ __device__ __forceinline__ uint32_t scan_slot(uint32_t i) { return i + (i >> 5);
 // scanned by k_scan_tops, or by combining the raw block totals before it.
 constexpr uint32_t SCAN_TOPS_NONE = 0, SCAN_TOPS_SCANNED = 1, SCAN_TOPS_RAW = 2;
 
-template <class T, class Op, class In>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(In in, T* out, size_t n, const T* tops, uint32_t mode, Op op) {
-    __shared__ T tile[SCAN_ITEMS + SCAN_ITEMS / 32];
-    const size_t base = (size_t)blockIdx.x * SCAN_ITEMS;
+template <class T, class Op, class In, uint32_t NT = SCAN_THREADS>
+__global__ __launch_bounds__(NT) void k_scan_apply(In in, T* out, size_t n, const T* tops, uint32_t mode, Op op) {
+    constexpr uint32_t ITEMS = NT * SCAN_PER_THREAD;
+    __shared__ T tile[ITEMS + ITEMS / 32];
+    const size_t base = (size_t)blockIdx.x * ITEMS;
     const uint32_t t = threadIdx.x;
 #pragma unroll
     for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
-        const uint32_t j = k * SCAN_THREADS + t;
+        const uint32_t j = k * NT + t;
         tile[scan_slot(j)] = base + j < n ? in(base + j) : Op::identity();
     }
     T pre = Op::identity();
@@ -136,8 +143,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(In in, T* out, size
         pre = tops[blockIdx.x];
     } else if (mode == SCAN_TOPS_RAW) {   // the totals of the blocks before this one
         T a = Op::identity();
-        for (uint32_t k = t; k < blockIdx.x; k += SCAN_THREADS) a = op(a, tops[k]);
-        (void)detail::block_scan_excl<T, Op, SCAN_THREADS>(a, op, pre);
+        for (uint32_t k = t; k < blockIdx.x; k += NT) a = op(a, tops[k]);
+        (void)detail::block_scan_excl<T, Op, NT>(a, op, pre);
     }
     __syncthreads();
     T v[SCAN_PER_THREAD];
@@ -148,7 +155,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(In in, T* out, size
         acc = op(acc, v[k]);
     }
     T total;
-    T run = op(pre, detail::block_scan_excl<T, Op, SCAN_THREADS>(acc, op, total));
+    T run = op(pre, detail::block_scan_excl<T, Op, NT>(acc, op, total));
     // each thread rewrites only the slots it read; the barrier inside the block
     // scan has ordered every read of the tile before these writes
 #pragma unroll
@@ -159,7 +166,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(In in, T* out, size
     __syncthreads();
 #pragma unroll
     for (uint32_t k = 0; k < SCAN_PER_THREAD; k++) {
-        const uint32_t j = k * SCAN_THREADS + t;
+        const uint32_t j = k * NT + t;
         if (base + j < n) out[base + j] = tile[scan_slot(j)];
     }
     if (blockIdx.x == gridDim.x - 1 && t == 0)   // the grand total
@@ -175,6 +182,11 @@ geq_status exclusive_scan_in(In in, T* out, size_t n, DevBuf<T>& tmp, hipStream_
     if (nb > 0x7fffffffull) return fail(GEQ_ERR_INVALID_ARG, "scan of more than 2^31 blocks");
     if (nb == 1) {   // one block holds the input: one launch
         k_scan_apply<T, Op, In><<<1, SCAN_THREADS, 0, s>>>(in, out, n, nullptr, SCAN_TOPS_NONE, op);
+        GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
+        return GEQ_OK;
+    }
+    if (n <= SCAN_ONE_ITEMS) {   // one large block: one launch instead of two
+        k_scan_apply<T, Op, In, SCAN_ONE_THREADS><<<1, SCAN_ONE_THREADS, 0, s>>>(in, out, n, nullptr, SCAN_TOPS_NONE, op);
         GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
         return GEQ_OK;
     }
