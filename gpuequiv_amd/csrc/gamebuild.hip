@@ -19,7 +19,8 @@
 //   5. k_mark + scan - winners get ranks, i.e. new ids in first-occurrence
 //                   order, which is the reference's numbering;
 //   6. k_finalize / k_edges - node records, hash entries, CSR rows, columns,
-//                   weights.
+//                   weights (k_level_out: the edges, rows and the next level's
+//                   bounds in one launch when no per-job counts are kept).
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -86,9 +87,20 @@ __device__ uint32_t pos_hash_lds(uint32_t kind, uint32_t p, const uint32_t* q, u
     return hash_of(hash_head(kind, p, ql, 0), sum);
 }
 
+// the sets of len words at ao and bo in the pool are equal; eight words of each
+// per step, read at clamped (in-range) indices so the sixteen loads are issued
+// together instead of one compare waiting on the next pair
 __device__ bool sets_equal(const uint32_t* __restrict__ spool, uint32_t ao, uint32_t bo, uint32_t len) {
-    for (uint32_t k = 0; k < len; k++)
-        if (spool[ao + k] != spool[bo + k]) return false;
+    for (uint32_t k = 0; k < len; k += 8) {
+        uint32_t d = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) {
+            const uint32_t x = min(k + u, len - 1u);
+            const uint32_t v = spool[ao + x] ^ spool[bo + x];
+            d |= k + u < len ? v : 0u;
+        }
+        if (d) return false;
+    }
     return true;
 }
 
@@ -1075,9 +1087,10 @@ __device__ bool cand_equals_cand(const Cand& a, const Cand& b, const uint32_t* s
 }
 
 __device__ bool cand_equals_node(const Cand& a, const NodeArgs& N, uint32_t id) {
-    const uint32_t kp = (uint32_t(N.kind[id]) << 30) | N.p[id];
-    return a.kind_p == kp && a.q_len == N.q_len[id] && a.x_len == N.x_len[id] &&
-           sets_equal(N.spool, a.q_off, N.q_off[id], a.q_len) && sets_equal(N.spool, a.x_off, N.x_off[id], a.x_len);
+    // every field read before the first comparison (no load waits on another)
+    const uint32_t kind = N.kind[id], p = N.p[id], ql = N.q_len[id], xl = N.x_len[id], qo = N.q_off[id], xo = N.x_off[id];
+    if (a.kind_p != ((kind << 30) | p) || a.q_len != ql || a.x_len != xl) return false;
+    return sets_equal(N.spool, a.q_off, qo, ql) && sets_equal(N.spool, a.x_off, xo, xl);
 }
 
 // candidate j finds its position in the table, or claims a pending entry
@@ -1158,9 +1171,11 @@ __device__ __forceinline__ void finalize_cand(const Cand* cands, uint32_t j, con
 }
 
 // rank: the exclusive scan of the winner flags (a winner is where it steps)
+// (clear: the heavy count, zeroed for a following k_level_out; NULL: none)
 __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* rank, uint32_t v_base, NodeArgs N,
-                           unsigned long long* table, uint32_t* keymin) {
+                           unsigned long long* table, uint32_t* keymin, uint32_t* clear) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear && j == 0) *clear = 0u;
     if (j < n_cand && rank[j + 1] != rank[j]) finalize_cand(cands, j, rank, v_base, N, table, keymin);
 }
 
@@ -1179,6 +1194,15 @@ __global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl
                         uint32_t* col, uint32_t* wt) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n_cand) edge_cand(cands, j, rowl, e_base, table, col, wt);
+}
+
+__global__ void k_level_out(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
+                            uint32_t* col, uint32_t* wt, uint32_t nf, uint32_t lo, uint32_t* row, NodeArgs N, uint32_t hi,
+                            const uint32_t* n_new, uint32_t add, uint32_t n_slots, uint32_t chunk, LtsArgs L, uint2* ub, HeavyArgs H) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n_cand) edge_cand(cands, g, rowl, e_base, table, col, wt);
+    if (g < nf) row[lo + g] = e_base + rowl[g];
+    if (g < n_slots) bounds_next_at(N, hi, *n_new + add, chunk, L, ub, H, g);
 }
 
 __global__ void k_rows(const uint32_t* rowl, uint32_t n_front, uint32_t lo, uint32_t e_base, uint32_t* row, RowsTail T) {

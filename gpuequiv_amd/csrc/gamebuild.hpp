@@ -117,9 +117,15 @@ __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long
 geq_status scan_marks(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* rank, uint32_t* flag,
                       DevBuf<uint32_t>& tmp, hipStream_t s);
 __global__ void k_finalize(const Cand* cands, uint32_t n_cand, const uint32_t* rank, uint32_t v_base, NodeArgs N,
-                           unsigned long long* table, uint32_t* keymin);
+                           unsigned long long* table, uint32_t* keymin, uint32_t* clear);
 __global__ void k_edges(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
                         uint32_t* col, uint32_t* wt);
+// k_edges, the rows of k_rows and (n_slots > 0) k_bounds_next in one launch of
+// max(n_cand, nf, n_slots) threads, for builds without per-job counts; the
+// heavy count was cleared by k_finalize.
+__global__ void k_level_out(const Cand* cands, uint32_t n_cand, const uint32_t* rowl, uint32_t e_base, const unsigned long long* table,
+                            uint32_t* col, uint32_t* wt, uint32_t nf, uint32_t lo, uint32_t* row, NodeArgs N, uint32_t hi,
+                            const uint32_t* n_new, uint32_t add, uint32_t n_slots, uint32_t chunk, LtsArgs L, uint2* ub, HeavyArgs H);
 // Work that follows a chunk's rows, done by k_rows' first block: the heavy
 // count is cleared for the next k_bounds_next (NULL: not), and merged builds
 // mark the jobs whose position counts passed the cap (dead NULL: none).

@@ -636,8 +636,19 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
                 k_insert<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, nodes(), table.p, keymin.p, table_mask);
             // the winners' ranks (their flags computed inside the scan)
             GB_TRY(scan_marks(cands.p, n_cand, keymin.p, rank.p, flag.p, ws.scan32, s));
-            if (n_cand) {
-                k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rank.p, B->V, nodes(), table.p, keymin.p);
+            // without per-job counts, the edges, rows and next bounds are one launch
+            const bool fused = !jt && n_cand;
+            if (fused) {
+                k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rank.p, B->V, nodes(), table.p, keymin.p,
+                                                                 last_chunk ? ws.hv_count.p : nullptr);
+                const uint64_t nt = std::max<uint64_t>(std::max<uint64_t>(n_cand, nf), slots);
+                k_level_out<<<(uint32_t)((nt + tb - 1) / tb), tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p,
+                                                                          B->wt.p, nf, lo, B->row.p, nodes(), hi, rank.p + n_cand,
+                                                                          add, (uint32_t)slots, chunk, L, ub.p, heavy_args());
+                if (last_chunk) GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots - 1, ws.scan2, s));
+            } else if (n_cand) {
+                k_finalize<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rank.p, B->V, nodes(), table.p, keymin.p,
+                                                                 nullptr);
                 k_edges<<<(n_cand + tb - 1) / tb, tb, 0, s>>>(cands.p, n_cand, rowl.p, (uint32_t)E, table.p, B->col.p, B->wt.p);
                 if (jt) {
                     // this level's new positions per job (the winners among the candidates)
@@ -646,10 +657,11 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
                 }
             }
             // the rows; the heavy count cleared for k_bounds_next; jobs past the cap marked
-            k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p,
-                                                     RowsTail{last_chunk ? ws.hv_count.p : nullptr, mark ? d_jcnt.p : nullptr,
-                                                              mark ? n_jobs : 0u, mark ? jt->cap : 0ull, mark ? d_dead : nullptr});
-            if (last_chunk) {
+            if (!fused)
+                k_rows<<<(nf + tb - 1) / tb, tb, 0, s>>>(rowl.p, nf, lo, (uint32_t)E, B->row.p,
+                                                         RowsTail{last_chunk ? ws.hv_count.p : nullptr, mark ? d_jcnt.p : nullptr,
+                                                                  mark ? n_jobs : 0u, mark ? jt->cap : 0ull, mark ? d_dead : nullptr});
+            if (last_chunk && !fused) {
                 k_bounds_next<<<(uint32_t)((slots + tb - 1) / tb), tb, 0, s>>>(nodes(), hi, rank.p + n_cand, add, (uint32_t)slots,
                                                                                chunk, L, ub.p, heavy_args());
                 GB_TRY(exclusive_scan_pairs(ub.p, off.p, slots - 1, ws.scan2, s));
