@@ -210,8 +210,20 @@ __device__ __forceinline__ uint2 bounds_of(const NodeArgs& N, const LtsArgs& L, 
         c = 1;
     } else {
         c = deg_of(L, p) + 4;
+        // the members' degrees, eight members a step: their ids, then their rows,
+        // each read at clamped (in-range) indices so the loads go out together
         uint32_t sd = 0;
-        for (uint32_t k = 0; k < ql; k++) sd += deg_of(L, N.spool[N.q_off[u] + k]);
+        const uint32_t qo = N.q_off[u];
+        for (uint32_t k = 0; k < ql; k += 8) {
+            uint32_t m[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) m[j] = N.spool[qo + min(k + j, ql - 1u)];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) {
+                const uint32_t d = deg_of(L, m[j]);
+                sd += k + j < ql ? d : 0u;
+            }
+        }
         s = sd + 4 * ql;
         const bool fits_block = sd <= HV_CAP && ql <= HV_CAP;
         const bool fits_wave = sd <= WAVE_TMAX && ql <= WAVE_QMAX;
