@@ -1150,7 +1150,12 @@ __global__ void k_insert(Cand* cands, uint32_t n_cand, NodeArgs N, unsigned long
 __device__ __forceinline__ uint32_t mark_cand(const Cand* cands, uint32_t j, uint32_t n_cand, const uint32_t* keymin) {
     if (j == n_cand) return 0u;
     const Cand& c = cands[j];
-    return (c.kind_p != INVALID && (c.res & PENDING) && keymin[c.res & ~PENDING] == j) ? 1u : 0u;
+    // keymin was set by atomics, which act in the L2: read it there (a block
+    // that runs several levels may hold an older copy of its line in the L1)
+    return (c.kind_p != INVALID && (c.res & PENDING) &&
+            __hip_atomic_load(&keymin[c.res & ~PENDING], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == j)
+               ? 1u
+               : 0u;
 }
 
 __global__ void k_mark(const Cand* cands, uint32_t n_cand, const uint32_t* keymin, uint32_t* flag) {
@@ -1238,7 +1243,8 @@ __device__ __forceinline__ uint32_t chunk_word(uint32_t i, const uint32_t* n_edg
         const uint2 t = next[min(*n_new + add, chunk)];
         return i == RB_CAND ? t.x : t.y;
     }
-    if (i == RB_HEAVY) return next ? *heavy : 0u;
+    if (i == RB_HEAVY) return next ? __hip_atomic_load(heavy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;   // counted by atomics
+    if (i < RB_JOBS) return 0u;   // the words of k_level_tail's own levels
     return cnt[i - RB_JOBS];
 }
 
@@ -1272,43 +1278,113 @@ __device__ void tail_scan(const T* in, T* out, uint32_t n, Op op) {
     if (threadIdx.x == 0) out[n] = total;
 }
 
-__global__ __launch_bounds__(TAIL_BLOCK) void k_level_tail(TailArgs T) {
+// One chunk after its generation (the body of k_level_tail).
+__device__ void tail_chunk(const TailArgs& T, uint32_t lo, uint32_t nf, uint32_t n_cand, uint32_t v_base, uint32_t e_base,
+                           uint32_t hi, uint32_t add, uint32_t slots, uint32_t last) {
     const uint32_t t = threadIdx.x;
-    // edge offsets of the chunk's positions (k_generate wrote their successor counts)
-    tail_scan<uint32_t, SumU32>(T.nsucc, T.rowl, T.nf, SumU32{});
-    for (uint32_t j = t; j < T.n_cand; j += TAIL_BLOCK) insert_cand(T.cands, j, T.N, T.table, T.keymin, T.mask);
+    // edge offsets of the chunk's positions (the generation wrote their successor counts)
+    tail_scan<uint32_t, SumU32>(T.nsucc, T.rowl, nf, SumU32{});
+    for (uint32_t j = t; j < n_cand; j += TAIL_BLOCK) insert_cand(T.cands, j, T.N, T.table, T.keymin, T.mask);
     __syncthreads();
-    for (uint32_t j = t; j <= T.n_cand; j += TAIL_BLOCK) T.flag[j] = mark_cand(T.cands, j, T.n_cand, T.keymin);
+    for (uint32_t j = t; j <= n_cand; j += TAIL_BLOCK) T.flag[j] = mark_cand(T.cands, j, n_cand, T.keymin);
     __syncthreads();
-    tail_scan<uint32_t, SumU32>(T.flag, T.rank, T.n_cand, SumU32{});
+    tail_scan<uint32_t, SumU32>(T.flag, T.rank, n_cand, SumU32{});
     __syncthreads();
-    for (uint32_t j = t; j < T.n_cand; j += TAIL_BLOCK)
-        if (T.flag[j]) finalize_cand(T.cands, j, T.rank, T.v_base, T.N, T.table, T.keymin);
+    for (uint32_t j = t; j < n_cand; j += TAIL_BLOCK)
+        if (T.flag[j]) finalize_cand(T.cands, j, T.rank, v_base, T.N, T.table, T.keymin);
     __syncthreads();   // edges read the table entries finalize wrote, and the row offsets
-    for (uint32_t j = t; j < T.n_cand; j += TAIL_BLOCK) {
-        edge_cand(T.cands, j, T.rowl, T.e_base, T.table, T.col, T.wt);
+    for (uint32_t j = t; j < n_cand; j += TAIL_BLOCK) {
+        edge_cand(T.cands, j, T.rowl, e_base, T.table, T.col, T.wt);
         if (T.job_cnt && T.flag[j]) atomicAdd(&T.job_cnt[T.L.job_of_state[T.cands[j].kind_p & 0x3fffffffu]], 1u);
     }
-    for (uint32_t i = t; i < T.nf; i += TAIL_BLOCK) T.row[T.lo + i] = T.e_base + T.rowl[i];
+    for (uint32_t i = t; i < nf; i += TAIL_BLOCK) T.row[lo + i] = e_base + T.rowl[i];
     __syncthreads();   // the per-job counts are complete
     if (T.job_dead && T.cap)
         for (uint32_t j = t; j < T.n_jobs; j += TAIL_BLOCK)
             if (T.job_cnt[j] > T.cap) T.job_dead[j] = 1;
-    if (T.last) {
+    if (last) {
         if (t == 0) *T.H.count = 0u;
         __syncthreads();   // the dead jobs and the cleared heavy count before the bounds
-        const uint32_t n_front = T.rank[T.n_cand] + T.add;   // the positions this level created
-        for (uint32_t i = t; i < T.slots; i += TAIL_BLOCK) bounds_next_at(T.N, T.hi, n_front, T.chunk, T.L, T.ub, T.H, i);
+        const uint32_t n_front = T.rank[n_cand] + add;   // the positions this level created
+        for (uint32_t i = t; i < slots; i += TAIL_BLOCK) bounds_next_at(T.N, hi, n_front, T.chunk, T.L, T.ub, T.H, i);
         __syncthreads();
-        tail_scan<uint2, PairSum>(T.ub, T.off, T.slots - 1u, PairSum{T.lim});
+        tail_scan<uint2, PairSum>(T.ub, T.off, slots - 1u, PairSum{T.lim});
     }
     __syncthreads();
+}
+
+// The count words of the last level done (first wave), RB_SEQ published last.
+__device__ void tail_publish(const TailArgs& T, uint32_t done, uint32_t lo, uint32_t nf, uint32_t n_cand, uint32_t n_words,
+                             uint32_t v_base, uint32_t e_base, uint32_t sp, uint32_t add, uint32_t last) {
+    const uint32_t t = threadIdx.x;
     const uint32_t n_cnt = T.job_cnt ? T.n_jobs : 0u;
     if (t >= 64) return;   // the first wave writes and publishes every word
-    for (uint32_t i = t; i < RB_JOBS + n_cnt; i += 64)
-        if (i != RB_SEQ)
-            T.rb[i] = chunk_word(i, T.rowl + T.nf, T.rank + T.n_cand, T.add, T.chunk, T.last ? T.off : nullptr, T.H.count, T.job_cnt);
+    for (uint32_t i = t; i < RB_JOBS + n_cnt; i += 64) {
+        uint32_t w;
+        switch (i) {
+            case RB_SEQ: continue;
+            case RB_LOOP: w = done; break;
+            case RB_LOOP_LO: w = lo; break;
+            case RB_LOOP_NF: w = nf; break;
+            case RB_LOOP_CAND: w = n_cand; break;
+            case RB_LOOP_WORDS: w = n_words; break;
+            case RB_LOOP_V: w = v_base; break;
+            case RB_LOOP_E: w = e_base; break;
+            case RB_LOOP_SP: w = sp; break;
+            default:
+                w = chunk_word(i, T.rowl + nf, T.rank + n_cand, add, T.chunk, last ? T.off : nullptr, T.H.count, T.job_cnt);
+        }
+        T.rb[i] = w;
+    }
     rb_publish(T.rb, T.seq);
+}
+
+__global__ __launch_bounds__(TAIL_BLOCK) void k_level_tail(TailArgs T) {
+    tail_chunk(T, T.lo, T.nf, T.n_cand, T.v_base, T.e_base, T.hi, T.add, T.slots, T.last);
+    tail_publish(T, 0u, T.lo, T.nf, T.n_cand, T.room.n_words, T.v_base, T.e_base, T.room.spool_top, T.add, T.last);
+}
+
+// The launched chunk (a level's only chunk), then up to T.loop further levels
+// in the same block: when the next level has at most T.room.gen_max positions
+// (at most TAIL_GEN) and TAIL_MAX candidates, lists no position for
+// k_generate_wave / k_generate_heavy, and fits every buffer's capacity and the
+// table's load (T.room), the block generates it itself (SUB_LANES lanes per
+// position, as k_generate_sub) and goes on.  The count words describe the
+// last level done; RB_LOOP* give how many levels were added and where the
+// last one started.
+__global__ __launch_bounds__(TAIL_BLOCK) void k_level_tail_loop(TailArgs T) {
+    constexpr uint32_t GROUPS = TAIL_BLOCK / SUB_LANES;   // positions generated at once
+    __shared__ CoopLds<SUB_QMAX, SUB_TMAX> glds[GROUPS];
+    const uint32_t t = threadIdx.x;
+    uint32_t lo = T.lo, nf = T.nf, n_cand = T.n_cand, v_base = T.v_base, e_base = T.e_base, hi = T.hi;
+    uint32_t sp = T.room.spool_top, n_words = T.room.n_words, done = 0;
+    for (;;) {
+        tail_chunk(T, lo, nf, n_cand, v_base, e_base, hi, 0u, n_cand + 1u, 1u);
+        if (done >= T.loop) break;
+        // the next level: the positions this one created, [hi, hi + n_new)
+        const uint32_t n_new = T.rank[n_cand];
+        const uint32_t e_next = e_base + T.rowl[nf];
+        const uint2 tot = T.off[min(n_new, T.chunk)];
+        const uint32_t v_next = hi + n_new;
+        const uint64_t sp_next = (uint64_t)sp + n_words;
+        const TailRoom& R = T.room;
+        const bool go = n_new > 0 && n_new <= min(TAIL_GEN, R.gen_max) && n_new <= T.chunk && tot.x <= min(TAIL_MAX, R.tail_max) &&
+                        __hip_atomic_load(T.H.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u &&
+                        (!R.max_positions || v_next <= R.max_positions) && n_new + 1u <= R.front &&
+                        tot.x + 1u <= R.cand && tot.x + 1u <= R.slots && (uint64_t)v_next + tot.x + 1u <= R.nodes &&
+                        2ull * ((uint64_t)v_next + tot.x) <= R.table && (uint64_t)e_next + tot.x + 1u <= R.edges &&
+                        sp_next + tot.y + 1u <= R.words && tot.x < T.lim && tot.y < T.lim;
+        if (!go) break;
+        __syncthreads();   // every thread has read the values above before the generation rewrites them
+        for (uint32_t i = t / SUB_LANES; i < n_new; i += GROUPS)
+            gen_coop<SUB_LANES, SUB_QMAX, SUB_TMAX>(T.N, hi, i, T.L, T.off, sp_next, T.cands, T.nsucc, glds[t / SUB_LANES]);
+        if (t == 0) T.nsucc[n_new] = 0u;   // trailing zero: the scan yields the edge total
+        __syncthreads();
+        lo = hi; nf = n_new; n_cand = tot.x; v_base = v_next; e_base = e_next; hi = v_next;
+        sp = (uint32_t)sp_next; n_words = tot.y;
+        done++;
+    }
+    tail_publish(T, done, lo, nf, n_cand, n_words, v_base, e_base, sp, 0u, 1u);
 }
 
 // start positions Attack(p, [q]) at ids [0, n): node records + hash entries

@@ -149,7 +149,12 @@ __global__ void k_attacker_flags(const uint8_t* kind, uint32_t n, uint8_t* atk);
 // of the chunk bounded last, then the per-job position counts.
 // RB_SEQ is written last, after a system-scope fence, with the number the
 // host waits for (see rb_publish).
-constexpr uint32_t RB_EDGES = 0, RB_NEW = 1, RB_CAND = 2, RB_WORDS = 3, RB_HEAVY = 4, RB_SEQ = 5, RB_JOBS = 6;
+constexpr uint32_t RB_EDGES = 0, RB_NEW = 1, RB_CAND = 2, RB_WORDS = 3, RB_HEAVY = 4, RB_SEQ = 5;
+// k_level_tail's further levels: how many, and the last one's frontier start
+// and size, candidates, set-pool words, first new id, first edge and set-pool base
+constexpr uint32_t RB_LOOP = 6, RB_LOOP_LO = 7, RB_LOOP_NF = 8, RB_LOOP_CAND = 9, RB_LOOP_WORDS = 10, RB_LOOP_V = 11,
+                   RB_LOOP_E = 12, RB_LOOP_SP = 13;
+constexpr uint32_t RB_JOBS = 14;   // then the per-job counts
 
 // Sum of (candidates, set-pool words) bound pairs, saturating at `lim`
 // (saturating addition of non-negative values is associative, so the scan
@@ -191,12 +196,27 @@ __device__ __forceinline__ void rb_publish(uint32_t* out, uint32_t seq) {
 // next level's bounds and their scan, and the count words.  The same per-item
 // steps as the kernels of larger chunks, so the same positions, ids and edges.
 constexpr uint32_t TAIL_BLOCK = 1024, TAIL_MAX = 4 * TAIL_BLOCK;
+// What k_level_tail's further levels may use: the capacities (entries) of the
+// frontier arrays (nsucc, rowl), candidate arrays (cands, flag, rank), bound
+// arrays (ub, off, heavy flags and list), position records, edges and set
+// pool; the table's slots (a level may fill half); the position cap (0: none);
+// the launched level's set-pool base and words; the largest candidate count
+// of a chunk the block runs (at most TAIL_MAX); and the largest level it
+// generates itself (at most TAIL_GEN positions).
+struct TailRoom {
+    uint32_t front, cand, slots;
+    uint64_t nodes, edges, words, table, max_positions;
+    uint32_t spool_top, n_words;
+    uint32_t tail_max, gen_max;
+};
+constexpr uint32_t TAIL_GEN = 512;   // positions of a level k_level_tail_loop generates itself
+
 struct TailArgs {
     NodeArgs N;
     LtsArgs L;
     HeavyArgs H;
     Cand* cands;
-    const uint32_t* nsucc;
+    uint32_t* nsucc;
     uint32_t* rowl;
     uint32_t* flag;
     uint32_t* rank;
@@ -215,8 +235,11 @@ struct TailArgs {
     uint32_t lo, nf, n_cand, v_base, e_base, mask;
     uint32_t hi, add, slots, chunk, last, lim;
     uint32_t seq;              // written to rb[RB_SEQ] last
+    uint32_t loop;             // further levels the block may build (0: the launched chunk only)
+    TailRoom room;
 };
 __global__ void k_level_tail(TailArgs T);
+__global__ void k_level_tail_loop(TailArgs T);
 
 // Pinned host words that kernels write directly (the level loop's counts).
 struct HostWords {

@@ -530,6 +530,19 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     // Chunks whose frontier, candidates and next bounds fit tail_max items
     // finish in one block (k_level_tail); GEQ_BUILD_TAIL_MAX lowers the limit
     // (0: never), so tests take both paths.
+    // further small levels k_level_tail may build in its block (GEQ_BUILD_TAIL_LOOP=0: none)
+    uint32_t tail_loop = 4096;
+    if (const char* env = std::getenv("GEQ_BUILD_TAIL_LOOP")) {
+        const long long v = std::atoll(env);
+        if (v >= 0) tail_loop = (uint32_t)std::min<long long>(v, 1ll << 20);
+    }
+    // the largest level it generates itself (GEQ_BUILD_TAIL_GEN, at most TAIL_GEN positions)
+    uint32_t tail_gen = 128;
+    if (const char* env = std::getenv("GEQ_BUILD_TAIL_GEN")) {
+        const long long v = std::atoll(env);
+        if (v >= 0) tail_gen = (uint32_t)std::min<long long>(v, TAIL_GEN);
+    }
+    if (!tail_gen) tail_loop = 0;
     uint32_t tail_max = TAIL_MAX;
     if (const char* env = std::getenv("GEQ_BUILD_TAIL_MAX")) {
         const long long v = std::atoll(env);
@@ -557,6 +570,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
     while (lo < hi) {
         const uint32_t c_hi = hi - lo > chunk ? lo + chunk : hi;
         const uint32_t nf = c_hi - lo;
+        uint32_t end = c_hi;   // where the next chunk starts (after k_level_tail's own levels: their last one's end)
         GB_TRY(ensure(nsucc, nf + 1, s, false, "frontier"));
         GB_TRY(ensure(rowl, nf + 1, s, false, "frontier"));
         uint32_t n_cand = next_cand, n_words = next_words, n_heavy = next_heavy;
@@ -624,12 +638,53 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (nf <= tail_max && n_cand <= tail_max && (!last_chunk || slots - 1 <= tail_max)) {
             // a small chunk: the rest of it in one block
             const uint32_t sq = next_seq();
+            // a standalone build's one-chunk level may go on into the following
+            // small levels in the same block, within the buffers as they are now
+            const bool loop = !jt && gen_mode == GEN_BY_SUB && last_chunk && add == 0 && tail_loop;
+            auto cap3 = [](size_t a, size_t b, size_t c) { return std::min(std::min(a, b), c); };
+            TailRoom room{};
+            if (loop) {
+                room.front = (uint32_t)std::min<size_t>(std::min(nsucc.n, rowl.n), 0xffffffffu);
+                room.cand = (uint32_t)std::min<size_t>(cap3(cands.n, flag.n, rank.n), 0xffffffffu);
+                room.slots = (uint32_t)std::min<size_t>(std::min(cap3(ub.n, off.n, ws.hv_flag.n), ws.hv_list.n), 0xffffffffu);
+                room.nodes = std::min(std::min(cap3(B->kind.n, B->p.n, B->q_off.n), cap3(B->q_len.n, B->x_off.n, B->x_len.n)),
+                                      std::min<size_t>(B->hash.n, B->row.n ? B->row.n - 1 : 0));
+                room.edges = std::min<size_t>(std::min(B->col.n, B->wt.n), 0xffffffffull);
+                room.words = std::min<size_t>(B->spool.n, 0xffffffffull);
+                room.table = table_slots;
+                room.max_positions = max_positions;
+                room.spool_top = (uint32_t)B->spool_top;
+                room.n_words = n_words;
+                room.tail_max = tail_max;
+                room.gen_max = tail_gen;
+            }
             const TailArgs T{nodes(), L, heavy_args(), cands.p, nsucc.p, rowl.p, flag.p, rank.p, table.p, keymin.p, B->col.p,
                              B->wt.p, B->row.p, ub.p, off.p, jt ? d_jcnt.p : nullptr, mark ? d_dead : nullptr, n_jobs,
                              mark ? jt->cap : 0ull, ws.rb.d, lo, nf, n_cand, B->V, (uint32_t)E, table_mask, hi, add,
-                             (uint32_t)slots, chunk, last_chunk ? 1u : 0u, level_limit(), sq};
-            k_level_tail<<<1, TAIL_BLOCK, 0, s>>>(T);
+                             (uint32_t)slots, chunk, last_chunk ? 1u : 0u, level_limit(), sq, loop ? tail_loop : 0u, room};
+            if (loop) k_level_tail_loop<<<1, TAIL_BLOCK, 0, s>>>(T);
+            else k_level_tail<<<1, TAIL_BLOCK, 0, s>>>(T);
             GB_TRY(wait(sq));
+            if (const uint32_t more = h_rb[RB_LOOP]) {
+                // the block built `more` further levels: the launched one is done, and the
+                // count words below describe the last of them, which started here
+                if (trace)
+                    std::fprintf(stderr, "gamebuild: levels %llu-%llu in one block: %u frontier -> ... -> %u frontier\n",
+                                 (unsigned long long)levels, (unsigned long long)(levels + more), nf, h_rb[RB_LOOP_NF]);
+                levels += more;
+                level_lo = h_rb[RB_LOOP_LO];
+                lo = level_lo;
+                hi = h_rb[RB_LOOP_V];
+                end = hi;
+                B->V = hi;
+                E = h_rb[RB_LOOP_E];
+                B->spool_top = h_rb[RB_LOOP_SP];
+                n_words = h_rb[RB_LOOP_WORDS];
+                const uint64_t lc = h_rb[RB_LOOP_CAND];
+                B->peak_words = std::max<uint64_t>(B->peak_words, B->spool_top + n_words + 1);
+                B->peak_edges = std::max<uint64_t>(B->peak_edges, E + lc + 1);
+                B->peak_nodes = std::max<uint64_t>(B->peak_nodes, B->V + lc + 1);
+            }
         } else {
             GB_TRY(exclusive_scan(nsucc.p, rowl.p, nf, ws.scan32, s));
             if (n_cand)
@@ -683,7 +738,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         E += n_edges;
         B->spool_top += n_words;
         B->V += n_new;
-        lo = c_hi;
+        lo = end;
         const bool level_done = lo == hi;
         if (level_done) {        // the next level is every position this one created
             if (trace)

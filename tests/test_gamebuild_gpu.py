@@ -197,14 +197,18 @@ def test_generators_match_oracle(monkeypatch, gen, heavy_min, label_mask, case):
         assert np.array_equal(h[k], og2[k]), k
 
 
-@pytest.mark.parametrize("tail_max", ["0", "100", "4096"])
-def test_one_block_chunks_match_oracle(monkeypatch, tail_max):
+@pytest.mark.parametrize("tail_max,tail_gen", [("0", "128"), ("100", "512"), ("4096", "0"), ("4096", "16"), ("4096", "128"),
+                                               ("4096", "512")])
+def test_one_block_chunks_match_oracle(monkeypatch, tail_max, tail_gen):
     # Chunks whose frontier, candidates and next bounds fit GEQ_BUILD_TAIL_MAX
     # items finish in one block (0: none; 100: the small levels of this build;
-    # 4096, the default: most of them).  Same game and energies every way,
-    # a capped build included.
-    print("one-block chunks up to", tail_max, flush=True)
+    # 4096, the default: most of them), which then builds the following levels
+    # of at most GEQ_BUILD_TAIL_GEN positions itself (0: none; 128 the
+    # default; 512 the most).  Same game and energies every way, a capped
+    # build (whose cap stops the block's levels) included.
+    print("one-block chunks up to", tail_max, "levels generated in the block up to", tail_gen, flush=True)
     monkeypatch.setenv("GEQ_BUILD_TAIL_MAX", tail_max)
+    monkeypatch.setenv("GEQ_BUILD_TAIL_GEN", tail_gen)
     n, src, dst, lbl = workloads.random_lts(300, 1200, 8, 1)
     lts = TransitionSystem.from_arrays(n, src, dst, lbl)
     lts, _ = lts.bisimilar_minimize()
