@@ -1245,7 +1245,7 @@ __device__ __forceinline__ uint32_t chunk_word(uint32_t i, const uint32_t* n_edg
     }
     if (i == RB_HEAVY) return next ? __hip_atomic_load(heavy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;   // counted by atomics
     if (i < RB_JOBS) return 0u;   // the words of k_level_tail's own levels
-    return cnt[i - RB_JOBS];
+    return __hip_atomic_load(&cnt[i - RB_JOBS], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // counted by atomics
 }
 
 __global__ void k_chunk_counts(const uint32_t* n_edges, const uint32_t* n_new, uint32_t add, uint32_t chunk, const uint2* next,
@@ -1299,9 +1299,9 @@ __device__ void tail_chunk(const TailArgs& T, uint32_t lo, uint32_t nf, uint32_t
     }
     for (uint32_t i = t; i < nf; i += TAIL_BLOCK) T.row[lo + i] = e_base + T.rowl[i];
     __syncthreads();   // the per-job counts are complete
-    if (T.job_dead && T.cap)
+    if (T.job_dead && T.cap)   // the counts come from atomics: read in the L2
         for (uint32_t j = t; j < T.n_jobs; j += TAIL_BLOCK)
-            if (T.job_cnt[j] > T.cap) T.job_dead[j] = 1;
+            if (__hip_atomic_load(&T.job_cnt[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > T.cap) T.job_dead[j] = 1;
     if (last) {
         if (t == 0) *T.H.count = 0u;
         __syncthreads();   // the dead jobs and the cleared heavy count before the bounds

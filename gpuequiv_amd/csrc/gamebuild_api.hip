@@ -638,9 +638,9 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (nf <= tail_max && n_cand <= tail_max && (!last_chunk || slots - 1 <= tail_max)) {
             // a small chunk: the rest of it in one block
             const uint32_t sq = next_seq();
-            // a standalone build's one-chunk level may go on into the following
-            // small levels in the same block, within the buffers as they are now
-            const bool loop = !jt && gen_mode == GEN_BY_SUB && last_chunk && add == 0 && tail_loop;
+            // a one-chunk level may go on into the following small levels in the
+            // same block, within the buffers as they are now
+            const bool loop = gen_mode == GEN_BY_SUB && last_chunk && add == 0 && tail_loop;
             auto cap3 = [](size_t a, size_t b, size_t c) { return std::min(std::min(a, b), c); };
             TailRoom room{};
             if (loop) {
@@ -652,7 +652,7 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
                 room.edges = std::min<size_t>(std::min(B->col.n, B->wt.n), 0xffffffffull);
                 room.words = std::min<size_t>(B->spool.n, 0xffffffffull);
                 room.table = table_slots;
-                room.max_positions = max_positions;
+                room.max_positions = jt ? 0 : max_positions;   // merged builds cap each job instead
                 room.spool_top = (uint32_t)B->spool_top;
                 room.n_words = n_words;
                 room.tail_max = tail_max;
