@@ -638,9 +638,11 @@ geq_status build_core(geq_ctx* ctx, const geq_lts* lts, const uint32_t* pairs, u
         if (nf <= tail_max && n_cand <= tail_max && (!last_chunk || slots - 1 <= tail_max)) {
             // a small chunk: the rest of it in one block
             const uint32_t sq = next_seq();
-            // a one-chunk level may go on into the following small levels in the
-            // same block, within the buffers as they are now
-            const bool loop = gen_mode == GEN_BY_SUB && last_chunk && add == 0 && tail_loop;
+            // a one-chunk level no larger than the levels the block generates may go
+            // on into the following small levels in the same block, within the
+            // buffers as they are now (k_level_tail_loop spills registers, so a
+            // larger level that will not go on takes k_level_tail)
+            const bool loop = gen_mode == GEN_BY_SUB && last_chunk && add == 0 && tail_loop && nf <= tail_gen;
             auto cap3 = [](size_t a, size_t b, size_t c) { return std::min(std::min(a, b), c); };
             TailRoom room{};
             if (loop) {

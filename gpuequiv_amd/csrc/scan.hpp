@@ -4,9 +4,10 @@
 //   k_scan_reduce : every block of SCAN_ITEMS consecutive items -> its total
 //   k_scan_tops   : one block scans the block totals in place (and the grand total)
 //   k_scan_apply  : every block scans its items again from its block's offset
-// An input of at most SCAN_ONE_ITEMS items takes k_scan_apply alone, in one
-// block (of SCAN_THREADS threads, or SCAN_ONE_THREADS past SCAN_ITEMS items):
-// one launch; one of at most SCAN_RAW_MAX blocks skips k_scan_tops, each
+// An input of at most SCAN_ITEMS items (4-byte items: SCAN_ONE_ITEMS) takes
+// k_scan_apply alone, in one block (of SCAN_THREADS threads, or
+// SCAN_ONE_THREADS past SCAN_ITEMS items): one launch; one of at most
+// SCAN_RAW_MAX blocks skips k_scan_tops, each
 // k_scan_apply block combining the totals of the blocks before it itself (two
 // launches).
 // Global memory is read and written striped (lane-consecutive, coalesced); the
@@ -31,8 +32,8 @@ constexpr uint32_t SCAN_PER_THREAD = 16;
 constexpr uint32_t SCAN_ITEMS = SCAN_THREADS * SCAN_PER_THREAD;   // items per block
 constexpr uint32_t SCAN_TOP_THREADS = 1024;
 constexpr uint32_t SCAN_RAW_MAX = 1024;   // block totals an apply block combines itself
-// the one-block scan: 1,024 threads, 16,384 items (an LDS tile of up to 132 KB
-// for 8-byte items; a gfx950 workgroup may hold 160 KB)
+// the one-block scan of 4-byte items: 1,024 threads, 16,384 items (a 66 KB LDS
+// tile; for 8-byte items this block took longer than the two launches)
 constexpr uint32_t SCAN_ONE_THREADS = 1024;
 constexpr uint32_t SCAN_ONE_ITEMS = SCAN_ONE_THREADS * SCAN_PER_THREAD;
 
@@ -185,7 +186,7 @@ geq_status exclusive_scan_in(In in, T* out, size_t n, DevBuf<T>& tmp, hipStream_
         GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
         return GEQ_OK;
     }
-    if (n <= SCAN_ONE_ITEMS) {   // one large block: one launch instead of two
+    if (sizeof(T) == 4 && n <= SCAN_ONE_ITEMS) {   // one large block: one launch instead of two (8-byte pairs: slower)
         k_scan_apply<T, Op, In, SCAN_ONE_THREADS><<<1, SCAN_ONE_THREADS, 0, s>>>(in, out, n, nullptr, SCAN_TOPS_NONE, op);
         GEQ_HIP(hipGetLastError(), GEQ_ERR_NO_DEVICE);
         return GEQ_OK;
