@@ -23,14 +23,19 @@ __global__ void k_edges_u2(const uint32_t* col, const uint32_t* wt, uint32_t E, 
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < E) edges[e] = make_uint2(col[e], wt[e]);
 }
-__global__ void k_indeg(const uint32_t* col, uint32_t E, uint32_t* cnt) {
+// in-degrees, and each edge's rank among its target's predecessors (the count
+// its atomic saw), so the placement below needs no atomics
+__global__ void k_indeg(const uint32_t* col, uint32_t E, uint32_t* cnt, uint32_t* rank) {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < E) atomicAdd(&cnt[col[e]], 1u);
+    if (e < E) rank[e] = atomicAdd(&cnt[col[e]], 1u);
 }
-__global__ void k_rev_scatter(const uint32_t* row, uint32_t V, const uint32_t* col, uint32_t* fill, uint32_t* rcol) {
+// predecessor v of each edge's target at its rank (the edges of v are
+// independent: no iteration waits on the one before)
+__global__ void k_rev_scatter(const uint32_t* row, uint32_t V, const uint32_t* col, const uint32_t* rank, const uint32_t* rrow,
+                              uint32_t* rcol) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    for (uint32_t e = row[v]; e < row[v + 1]; e++) rcol[atomicAdd(&fill[col[e]], 1u)] = v;
+    for (uint32_t e = row[v]; e < row[v + 1]; e++) rcol[rrow[col[e]] + rank[e]] = v;
 }
 
 
@@ -202,20 +207,20 @@ geq_status geq::game_from_device(geq_ctx* ctx, uint32_t V, uint32_t E, const uin
     };
     hipError_t e;
     DevBuf<uint32_t>& cnt = gm->rev_cnt;
-    DevBuf<uint32_t>& fill = gm->rev_fill;
+    DevBuf<uint32_t>& rank = gm->rev_fill;   // one entry per edge
     DevBuf<uint32_t>& tmp = gm->rev_tmp;
     if (cnt.n < V + 1ull && (e = cnt.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");
-    if (fill.n < V + 1ull && (e = fill.alloc(V + 1)) != hipSuccess) return oom(e, "reverse CSR scratch");
+    if (rank.n < std::max<uint64_t>(E, 1) && (e = rank.alloc(std::max<uint64_t>(E, 1))) != hipSuccess)
+        return oom(e, "reverse CSR scratch");
     const uint32_t tb = 256;
     GEQ_HIP(hipMemcpyAsync(gm->row.p, d_row, 4ull * (V + 1), hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
     GEQ_HIP(hipMemcpyAsync(gm->atk.p, d_atk, V, hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
     if (E) k_edges_u2<<<(E + tb - 1) / tb, tb, 0, s>>>(d_col, d_wt, E, gm->edges.p);
     k_fill_u32<<<(V + 1 + tb - 1) / tb, tb, 0, s>>>(cnt.p, V + 1, 0u);
-    if (E) k_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(d_col, E, cnt.p);
+    if (E) k_indeg<<<(E + tb - 1) / tb, tb, 0, s>>>(d_col, E, cnt.p, rank.p);
     geq_status st = exclusive_scan(cnt.p, gm->rrow.p, V, tmp, s);
     if (st != GEQ_OK) return st;
-    GEQ_HIP(hipMemcpyAsync(fill.p, gm->rrow.p, 4ull * (V + 1), hipMemcpyDeviceToDevice, s), GEQ_ERR_TRANSFER);
-    k_rev_scatter<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, V, d_col, fill.p, gm->rcol.p);
+    if (V) k_rev_scatter<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, V, d_col, rank.p, gm->rrow.p, gm->rcol.p);
     if (V) k_node_hdr<<<(V + tb - 1) / tb, tb, 0, s>>>(gm->row.p, gm->rrow.p, gm->atk.p, V, gm->st.p);
     GEQ_HIP(hipStreamSynchronize(s), GEQ_ERR_TRANSFER);
     *out = own.release();

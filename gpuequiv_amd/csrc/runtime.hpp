@@ -259,7 +259,7 @@ struct geq_game {
     geq::DevBuf<uint2> edges;
     geq::DevBuf<uint8_t> atk, chg;
     geq::DevBuf<uint4> st;            // two list slots per node
-    geq::DevBuf<uint32_t> rev_cnt, rev_fill;   // reverse-CSR scratch of game_from_device
+    geq::DevBuf<uint32_t> rev_cnt, rev_fill;   // reverse-CSR scratch of game_from_device (in-degrees; edge ranks)
     geq::DevBuf<uint32_t> rev_tmp;
     geq::DevBuf<uint16_t> arena;
     // generic confs (list kernels): update words per edge, the word arena (its
