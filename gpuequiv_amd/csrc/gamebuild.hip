@@ -87,14 +87,15 @@ __device__ uint32_t pos_hash_lds(uint32_t kind, uint32_t p, const uint32_t* q, u
     return hash_of(hash_head(kind, p, ql, 0), sum);
 }
 
-// the sets of len words at ao and bo in the pool are equal; eight words of each
-// per step, read at clamped (in-range) indices so the sixteen loads are issued
-// together instead of one compare waiting on the next pair
+// the sets of len words at ao and bo in the pool are equal; four words of each
+// per step, read at clamped (in-range) indices so the eight loads are issued
+// together instead of one compare waiting on the next pair (eight words a step
+// made the large levels' inserts slower: most sets have one or two members)
 __device__ bool sets_equal(const uint32_t* __restrict__ spool, uint32_t ao, uint32_t bo, uint32_t len) {
-    for (uint32_t k = 0; k < len; k += 8) {
+    for (uint32_t k = 0; k < len; k += 4) {
         uint32_t d = 0;
 #pragma unroll
-        for (uint32_t u = 0; u < 8; u++) {
+        for (uint32_t u = 0; u < 4; u++) {
             const uint32_t x = min(k + u, len - 1u);
             const uint32_t v = spool[ao + x] ^ spool[bo + x];
             d |= k + u < len ? v : 0u;
